@@ -1,0 +1,200 @@
+/*
+ * snoo_hip.h -- C ABI of libsnoo_hip.so, the MI355X (gfx950) kernels behind the
+ * nonlinear reduced-vector-potential solve.
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is a device pointer (HBM); sizes are
+ *     element counts.  Host pointers are named *_host.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *     calls are asynchronous with respect to the host unless stated.
+ *   - Every function returns 0 on success or an SN_ERR_* code; the message of
+ *     the last failure on the calling thread is returned by sn_last_error().
+ *   - Ownership: the caller owns every buffer it passes.  Handles
+ *     (sn_pattern, sn_cg) own their internal device memory and are released
+ *     with their destroy call.
+ *   - Indices are int32, 0-based (the reference C kernels take 0-based int32
+ *     ids: fem_c/interface_c.pyx:248-256 after curl_curl_assembler.py:916-917).
+ */
+#ifndef SNOO_HIP_H
+#define SNOO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SN_OK 0
+#define SN_ERR_ARG 1            /* bad size, null pointer or unsupported option   */
+#define SN_ERR_HIP 2            /* a HIP runtime call failed                      */
+#define SN_ERR_NOT_CONVERGED 3  /* iteration cap reached                          */
+#define SN_ERR_BREAKDOWN 4      /* CG breakdown: p.Ap <= 0 or non-finite scalar   */
+
+/* reluctivity model codes */
+#define SN_NU_CONST 0  /* ConstantReluctance (snoopy/materials.py:486-536)        */
+#define SN_NU_PCHIP 1  /* Reluctance, PCHIP in |B| (snoopy/materials.py:271-437) */
+#define SN_NU_BSQ 2    /* cubic B-spline in |B|^2, piecewise-polynomial form     */
+
+const char *sn_last_error(void);
+int sn_version(void);
+
+/* ---------------------------------------------------------------- sources */
+
+/* Biot-Savart of straight segments with core radius; B (+)= I*1e-7*sum.
+ * Replaces compute_B_line_segs (fem_c/ctools.c:1761-1923) as bound by
+ * compute_B_line_segs_cpp (fem_c/interface_c.pyx:928-991).
+ * src_dev [num_src*6], tar_dev [num_tar*3], B_dev [num_tar*3].
+ * accumulate=0 overwrites B, 1 adds (sum over coils without a host copy). */
+int sn_biot_savart_f64(const double *src_dev, int num_src, const double *tar_dev,
+                       int num_tar, double current, double radius, double *B_dev,
+                       int accumulate, void *stream);
+
+/* Brick-field source: piecewise-constant B in 3 bricks.
+ * Replaces BrickField.compute_B (snoopy/brick_field.py:400-433).
+ * faces_host [3*6*6] = (centre, unit normal) per face, fields_host [9]. */
+int sn_brick_field_f64(const double *faces_host, const double *fields_host,
+                       const double *tar_dev, int num_tar, double *B_dev,
+                       int accumulate, void *stream);
+
+/* Physical quadrature points sum_a phi_a(xi_q) X_a, out_dev [num_el*Q*3].
+ * Replaces the element loop of get_quadrature_points
+ * (snoopy/curl_curl_assembler.py:1521-1593, evaluate_finite_element at
+ * fem_c/ctools.c:480-504).  phi_dev [Q*4]. */
+int sn_quad_points_f64(int num_el, const double *nodes_dev, const int *cells_dev,
+                       const double *phi_dev, int Q, double *out_dev, void *stream);
+
+/* Points and curl A at the quadrature points, points_dev/B_dev [num_el*Q*3].
+ * Replaces compute_B_Hcurl (fem_c/ctools.c:1500-1563) as bound by
+ * fem_c/interface_c.pyx:555-696.  curls_dev [64*Q*6*3], phi_dev [Q*4].
+ * The element is the affine tetrahedron: the q = 0 slice of the curl table is
+ * used for every q (lowest-order curls are constant on the element). */
+int sn_B_Hcurl_f64(int num_el, const double *nodes_dev, const int *cells_dev,
+                   const int *glob_ids_dev, const int *orient_dev,
+                   const double *curls_dev, const double *phi_dev, int Q,
+                   const double *x_dev, double *points_dev, double *B_dev,
+                   void *stream);
+
+/* ---------------------------------------------------- pattern and scatter */
+
+typedef struct sn_pattern sn_pattern;
+
+/* Build, on the device, the CSR pattern of the union of the per-element 6x6
+ * blocks (rows and columns sorted, as scipy's canonical CSR) plus the ordered
+ * contribution lists that make the scatter a deterministic gather:
+ *   nonzero p <- sum of local pair values, in ascending element order;
+ *   DOF i     <- sum of local DOF values, in ascending element order.
+ * glob_ids_dev [num_el*6] covers every element of every domain. */
+int sn_pattern_create(sn_pattern **out, int num_dofs, int num_el,
+                      const int *glob_ids_dev, void *stream);
+void sn_pattern_destroy(sn_pattern *pat);
+/* nnz and device views of indptr [num_dofs+1], indices [nnz], diag [num_dofs]
+ * (position of (i,i)); the views stay owned by the pattern. */
+int sn_pattern_info(const sn_pattern *pat, long long *nnz, const int **indptr_dev,
+                    const int **indices_dev, const int **diag_dev);
+/* Copy indptr [num_dofs+1], indices [nnz] and diag [num_dofs] into caller
+ * buffers (device); any output may be NULL. */
+int sn_pattern_copy(const sn_pattern *pat, int *indptr_dev, int *indices_dev, int *diag_dev,
+                    void *stream);
+
+/* Local element data layout (structure of arrays, element fastest):
+ *   pair values   loc[pr*num_el_total + e], pr = index of (j<=k) in row-major
+ *                 upper-triangle order, 21 pairs;
+ *   DOF values    loc6[j*num_el_total + e], 6 entries. */
+
+/* Element kernel of the drop-in assembly: local K, dK (21 pairs each) and
+ * rhs (6) for elements [el_begin, el_begin+el_count) of one domain.
+ * Replaces compute_K_dK_Hcurl_red (fem_c/ctools.c:1195-1388, Bs_dev != NULL)
+ * and compute_K_dK_Hcurl (fem_c/ctools.c:960-1193, Bs_dev == NULL) as called
+ * from compute_stiffness_and_jacobi_matrix_curl_curl
+ * (fem_c/interface_c.pyx:162-374).  Bs_dev is indexed from the domain's
+ * first element: [el_count*Q*3].  curls_dev [64*Q*6*3], d_phi_dev [Q*4*3],
+ * w_dev [Q]; coeffs_dev [4*n_int] (or [1] for SN_NU_CONST), knots_dev
+ * [n_int+1].  The element is the affine tetrahedron: the q = 0 slices of the
+ * curl and gradient tables are used for every q (both are constant on the
+ * lowest-order element); B_s, nu and nu' vary with q. */
+int sn_element_local_f64(int num_el_total, int el_begin, int el_count,
+                         const double *nodes_dev, const int *cells_dev,
+                         const int *glob_ids_dev, const int *orient_dev,
+                         const double *curls_dev, const double *d_phi_dev,
+                         const double *w_dev, int Q, int type_spec, int n_int,
+                         const double *coeffs_dev, const double *knots_dev,
+                         const double *x_dev, const double *Bs_dev,
+                         double *locK_dev, double *locdK_dev, double *loc6_dev,
+                         void *stream);
+
+/* CSR values of K, dK and J = dK + K from the local pair buffers (the COO->CSR
+ * sums of fem_c/interface_c.pyx:367-368 and the K += ..., dK + K of
+ * snoopy/curl_curl_assembler.py:928-932).  Any output may be NULL. */
+int sn_csr_gather_f64(const sn_pattern *pat, const double *locK_dev,
+                      const double *locdK_dev, double *K_dev, double *dK_dev,
+                      double *J_dev, void *stream);
+
+/* out[i] = sum of the local DOF values of DOF i (the rhs scatter of
+ * fem_c/ctools.c:1332-1335 made deterministic). */
+int sn_dof_gather_f64(const sn_pattern *pat, const double *loc6_dev,
+                      double *out_dev, void *stream);
+
+/* Fused Newton element kernel: local Jacobian J_e = K_e + dK_e (21 pairs,
+ * float64 or float32 output) and one local vector in float64 (6):
+ *   mode 0: b_e = dK_e x_e + rhs_e  (b = Jx - Kx + rhs, red_mvp_solver.py:186)
+ *   mode 1: R_e = K_e x_e - rhs_e   (Newton residual R(x) = K(x)x - rhs(x))
+ * Element arithmetic is float64 in both precisions.  Bs_dev == NULL marks a
+ * source-free (air) domain. */
+int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin,
+                      int el_count, const double *nodes_dev, const int *cells_dev,
+                      const int *glob_ids_dev, const int *orient_dev,
+                      const double *curls_dev, const double *d_phi_dev,
+                      const double *w_dev, int Q, int type_spec, int n_int,
+                      const double *coeffs_dev, const double *knots_dev,
+                      const double *x_dev, const double *Bs_dev,
+                      void *locJ_dev, double *loc6_dev, void *stream);
+
+/* CSR values from one local pair buffer, float64 or float32. */
+int sn_csr_gather_one(const sn_pattern *pat, int f32, const void *loc_dev,
+                      void *vals_dev, void *stream);
+
+/* ------------------------------------------------------- sparse algebra */
+
+/* y = A x over rows with mask != 0 and columns with mask != 0 (mask may be
+ * NULL); float64. */
+int sn_spmv_f64(const sn_pattern *pat, const double *vals_dev,
+                const unsigned char *mask_dev, const double *x_dev, double *y_dev,
+                void *stream);
+
+/* Reductions over entries with mask != 0 (mask may be NULL), float64 result
+ * written to the host. */
+int sn_dot_f64(int n, const double *a_dev, const double *b_dev,
+               const unsigned char *mask_dev, double *out_host, void *stream);
+int sn_absmax_f64(int n, const double *a_dev, const unsigned char *mask_dev,
+                  double *out_host, void *stream);
+
+/* ------------------------------------------------------------------- PCG */
+
+typedef struct sn_cg sn_cg;
+
+/* Workspace for Jacobi-PCG on one pattern.  precision 64: float64 matrix and
+ * vectors; precision 32: float32 matrix and vectors with float64 dot-product
+ * accumulation.  lanes_per_row in {0 (auto), 4, 8, 16}; chunk = iterations
+ * per captured hipGraph (0 = default). */
+int sn_cg_create(sn_cg **out, const sn_pattern *pat, int precision,
+                 int lanes_per_row, int chunk, void *stream);
+void sn_cg_destroy(sn_cg *cg);
+
+/* Jacobi-PCG on the system restricted to mask != 0, scipy 1.15.3 cg
+ * semantics as driven by snoopy/red_mvp_solver.py:195-205:
+ *   M = diag(A_c)^-1; stop when ||r||_2 < max(atol, rtol*||b_c||_2) is seen at
+ *   the top of an iteration; at most maxiter iterations (maxiter < 0:
+ *   10 * number of free rows).
+ * precision 64: vals_dev is double*, x_dev holds x0 on entry and the solution
+ *   on exit (entries with mask == 0 are left untouched).
+ * precision 32: vals_dev is float*, the float64 b is rounded to float32, CG
+ *   starts from 0 and the float32 result is ADDED to the float64 x_dev
+ *   (one refinement step: x += A^-1 b).
+ * iters_host, rnorm_host (final recursive-residual norm, float64) may be NULL.
+ * Returns SN_OK, SN_ERR_NOT_CONVERGED or SN_ERR_BREAKDOWN. */
+int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
+                const double *b_dev, double *x_dev, double rtol, double atol,
+                int maxiter, int *iters_host, double *rnorm_host, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

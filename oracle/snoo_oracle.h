@@ -1,0 +1,97 @@
+/*
+ * CPU FP64 oracle for the nonlinear reduced-vector-potential (RVP) solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the CPU baseline that bench.py times beside it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (snoo.py_amd) never links, imports or falls back to it.
+ *
+ * Every function restates one piece of the reference algorithm in plain C,
+ * in the reference's own loop order, and cites the file:line it follows.
+ * All arithmetic is double precision; indices are int32 (0-based), as in the
+ * reference's C kernels.
+ */
+#ifndef SNOO_ORACLE_H
+#define SNOO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reluctivity model codes (type_spec) */
+#define OR_NU_CONST 0   /* ConstantReluctance: c[0]                          */
+#define OR_NU_PCHIP 1   /* Reluctance: piecewise cubic in |B| (scipy PCHIP)  */
+#define OR_NU_BSQ   2   /* piecewise cubic in s=|B|^2 (cubic B-spline in B^2) */
+
+/* Piecewise-cubic evaluation in power basis, fem_c/ctools.c:130-204.
+ * c is [4, n_int] row-major (c[j*n_int+i]), knots x[n_int+1]. */
+double or_pchip_eval(int n_int, const double *c, const double *x, double B);
+double or_pchip_deriv(int n_int, const double *c, const double *x, double B);
+
+/* nu(|B|) and dnu/d|B| for a model (type 0/1 follow ctools.c:130-204;
+ * type 2 evaluates the piecewise cubic at s=B^2 and applies the chain rule). */
+void or_nu_eval(int type_spec, int n_int, const double *c, const double *x,
+                double B, double *nu, double *dnu);
+
+/* Iron element kernel with source field, fem_c/ctools.c:1195-1388.
+ * Emits 36 triplets per element in the reference order (j-major, (j,k) then
+ * mirrored (k,j)) into rows/cols/vK/vdK (each 36*num_el long) and adds the
+ * rhs into rhs[num_dofs] (not cleared here).  Bs is [num_el*Q*3]. */
+void or_K_dK_red(int num_el, const double *nodes, const int *cells,
+                 const int *glob_ids, const int *orient,
+                 const double *curls, const double *d_phi,
+                 const double *w, int Q,
+                 int type_spec, int n_int, const double *c, const double *xk,
+                 const double *x, const double *Bs,
+                 int *rows, int *cols, double *vK, double *vdK, double *rhs);
+
+/* Air / source-free element kernel, fem_c/ctools.c:960-1193 (B_s = 0, no rhs). */
+void or_K_dK_air(int num_el, const double *nodes, const int *cells,
+                 const int *glob_ids, const int *orient,
+                 const double *curls, const double *d_phi,
+                 const double *w, int Q,
+                 int type_spec, int n_int, const double *c, const double *xk,
+                 const double *x,
+                 int *rows, int *cols, double *vK, double *vdK);
+
+/* Quadrature-point positions and curl A, fem_c/ctools.c:1500-1563.
+ * points, B are [num_el*Q*3]. phi is [Q,4]. */
+void or_B_Hcurl(int num_el, const double *nodes, const int *cells,
+                const int *glob_ids, const int *orient,
+                const double *curls, const double *phi, const double *d_phi,
+                int Q, const double *x, double *points, double *B);
+
+/* Biot-Savart of straight segments with a core radius,
+ * fem_c/ctools.c:1761-1923.  src [num_src*6], tar [num_tar*3], B [num_tar*3]
+ * (overwritten).  Per target the sum runs over segments in order, which is
+ * the reference's single-thread order; threads split targets only. */
+void or_biot_savart(const double *src, int num_src, const double *tar,
+                    int num_tar, double current, double rad, double *B,
+                    int num_threads);
+
+/* COO -> CSR with duplicates summed and column indices sorted, the semantics
+ * of scipy csr_array((v,(i,j))) used at fem_c/interface_c.pyx:367-368.
+ * Duplicates are summed in input order.  indptr [n+1]; indices/data must hold
+ * nt entries; returns nnz. */
+long long or_coo_to_csr(int n, long long nt, const int *ri, const int *ci,
+                        const double *v, int *indptr, int *indices,
+                        double *data);
+
+/* y = A x for CSR A. */
+void or_spmv(int n, const int *indptr, const int *indices, const double *data,
+             const double *x, double *y);
+
+/* scipy 1.15.3 cg with Jacobi M = diag(A)^-1 on the rows with mask[i]!=0
+ * (Dirichlet rows/columns removed as J[mask][:,mask] does,
+ * snoopy/red_mvp_solver.py:195-205).  Stops when ||r||_2 < max(atol,
+ * rtol*||b_free||_2), at most maxiter iterations.  x (full length) holds x0 on
+ * entry; masked entries are left untouched.  Returns the iteration count; info
+ * is 0 on convergence, else maxiter. */
+int or_pcg(int n, const int *indptr, const int *indices, const double *data,
+           const unsigned char *mask, const double *b, double *x,
+           double rtol, double atol, int maxiter, int *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,21 @@
+"""snoopy_amd: the nonlinear reduced-vector-potential solve of snoo.py on
+AMD Instinct MI355X (gfx950).
+
+Drop-in classes with the reference's names and signatures:
+RedMVPSolver, CurlCurlAssembler, Reluctance, ConstantReluctance,
+RacetrackCoil, BrickField.  Behind them the hot path runs as hand-written HIP
+kernels in libsnoo_hip.so (C ABI: include/snoo_hip.h).
+"""
+from .brick_field import BrickField
+from .curl_curl_assembler import CurlCurlAssembler
+from .finite_element import FiniteElement
+from .materials import (BSplineReluctance, ConstantReluctance, Reluctance, tanh_reluctance)
+from .mesh import TetMesh, TetModel, kuhn_mesh
+from .mesh_tools import get_cotree_dofs, get_edge_boundary_dofs
+from .racetrack_coil import RacetrackCoil
+from .red_mvp_solver import NewtonDriver, RedMVPSolver
+
+__all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctance',
+           'ConstantReluctance', 'Reluctance', 'tanh_reluctance', 'TetMesh', 'TetModel',
+           'kuhn_mesh', 'get_cotree_dofs', 'get_edge_boundary_dofs', 'RacetrackCoil',
+           'NewtonDriver', 'RedMVPSolver']

@@ -1,0 +1,107 @@
+"""ctypes binding of libsnoo_hip.so (declarations in include/snoo_hip.h).
+
+The library is loaded after torch so that it binds to the HIP runtime torch
+has already loaded.  There is no fallback: if the library is missing or fails
+to load, every GPU entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libsnoo_hip.so')
+
+_c_int, _c_dbl, _c_ll, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
+_P_int = ctypes.POINTER(ctypes.c_int)
+_P_dbl = ctypes.POINTER(ctypes.c_double)
+_P_ll = ctypes.POINTER(ctypes.c_longlong)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    'sn_last_error': (ctypes.c_char_p, []),
+    'sn_version': (_c_int, []),
+    'sn_biot_savart_f64': (_c_int, [_vp, _c_int, _vp, _c_int, _c_dbl, _c_dbl, _vp, _c_int, _vp]),
+    'sn_brick_field_f64': (_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_int, _vp]),
+    'sn_quad_points_f64': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    'sn_B_Hcurl_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pattern_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _vp, _vp]),
+    'sn_pattern_destroy': (None, [_vp]),
+    'sn_pattern_info': (_c_int, [_vp, _P_ll, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                 ctypes.POINTER(_vp)]),
+    'sn_pattern_copy': (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'sn_element_local_f64': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                      _vp]),
+    'sn_csr_gather_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_dof_gather_f64': (_c_int, [_vp, _vp, _vp, _vp]),
+    'sn_element_newton': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp]),
+    'sn_csr_gather_one': (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
+    'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),
+    'sn_absmax_f64': (_c_int, [_c_int, _vp, _vp, _P_dbl, _vp]),
+    'sn_cg_create': (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int, _c_int, _vp]),
+    'sn_cg_destroy': (None, [_vp]),
+    'sn_cg_solve': (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_int, _P_int, _P_dbl,
+                             _vp]),
+}
+
+SN_OK, SN_ERR_ARG, SN_ERR_HIP, SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN = 0, 1, 2, 3, 4
+
+
+class SnooError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__('snoo_hip error {}: {}'.format(code, msg))
+        self.code = code
+
+
+class NotConverged(SnooError):
+    pass
+
+
+class Breakdown(SnooError):
+    pass
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load (once) and return the ctypes library object."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError('libsnoo_hip.so not built ({}); run __graft_entry__.build()'.format(path))
+    import torch  # noqa: F401  (bind to torch's HIP runtime)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc == SN_OK:
+        return
+    msg = load().sn_last_error().decode(errors='replace')
+    if rc == SN_ERR_NOT_CONVERGED:
+        raise NotConverged(rc, msg)
+    if rc == SN_ERR_BREAKDOWN:
+        raise Breakdown(rc, msg)
+    raise SnooError(rc, msg)
+
+
+def ptr(t):
+    """Device (or host) address of a torch tensor, None for None."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,569 @@
+// Jacobi-preconditioned CG on the CSR Jacobian, scipy 1.15.3 `cg` semantics
+// (scipy/sparse/linalg/_isolve/iterative.py: r = b - A x0; loop: stop when
+// ||r|| < max(atol, rtol ||b||); z = M r; rho = r.z; p = z + (rho/rho_prev) p;
+// q = A p; alpha = rho/(p.q); x += alpha p; r -= alpha q), as driven by
+// snoopy/red_mvp_solver.py:195-205 with Dirichlet rows/columns removed by a
+// mask instead of a fancy-indexed copy of the matrix.
+//
+// Two kernels per iteration, each ending in a deterministic grid reduction
+// (per-block partials summed in block order by the last block to finish):
+//   A: p_new = z + beta p_old (formed on the fly for every column read),
+//      q = A p_new, p.q
+//   B: x += alpha p_new, r -= alpha q, z = dinv r, r.z and r.r, stop test.
+// alpha and beta are computed on the device from the reduced scalars, so the
+// host only polls the stop flag once per chunk of iterations; each chunk is
+// captured once as a hipGraph and replayed.
+#include <cstring>
+#include <vector>
+
+#include "sn_common.hpp"
+#include "sn_pattern.hpp"
+
+namespace sn {
+
+struct CgState {
+    double rho_cur, rho_prev, pq, rr, tol, bnorm;
+    int it, maxiter, done, pad;  // done: 0 running, 1 converged, 2 iteration cap, 3 breakdown
+    unsigned int counter;
+    unsigned int pad2;
+};
+
+constexpr int kCgBlock = 256;
+
+__device__ __forceinline__ void store_partial(double *partials, double v) {
+    __hip_atomic_store(&partials[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns true on the last block to arrive (after acquire), false elsewhere.
+__device__ __forceinline__ bool arrive_last(unsigned int *counter) {
+    __shared__ bool s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return s_last;
+}
+
+// Sum of partials[0..nb) in a fixed order by the whole block.
+__device__ __forceinline__ double final_sum(const double *partials, int nb, double *lds) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x)
+        s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return block_sum(s, lds);
+}
+
+// ---------------------------------------------------------------- init
+// r = b - A_c x0 (x0 masked; f32 mode starts from 0), dinv, z, p_old = 0,
+// partial sums of b.b, r.r, r.z.
+template <typename V, int LPR>
+__global__ __launch_bounds__(kCgBlock) void cg_init_kernel(
+    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const int *__restrict__ diag, const V *__restrict__ vals, const unsigned char *__restrict__ mask,
+    const double *__restrict__ b, const double *__restrict__ x0, V *__restrict__ r,
+    V *__restrict__ z, V *__restrict__ dinv, V *__restrict__ pold, float *__restrict__ xs,
+    double *__restrict__ partials, CgState *__restrict__ st, double rtol, double atol,
+    int maxiter) {
+    __shared__ double lds[kCgBlock / kWave];
+    const int lane = threadIdx.x % LPR;
+    const int rows_per_block = kCgBlock / LPR;
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
+         row += (long long)gridDim.x * rows_per_block) {
+        const int i = (int)row;
+        const bool free_row = mask ? mask[i] != 0 : true;
+        double acc = 0.0;
+        if (free_row && x0) {
+            for (int k = indptr[i] + lane; k < indptr[i + 1]; k += LPR) {
+                const int j = indices[k];
+                const bool fj = mask ? mask[j] != 0 : true;
+                if (fj) acc += (double)vals[k] * x0[j];
+            }
+        }
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+        if (lane == 0) {
+            V ri = 0, di = 0;
+            if (free_row) {
+                const double bi = b[i];
+                bb += bi * bi;
+                ri = (V)(bi - acc);
+                const int d = diag[i];
+                di = (d >= 0) ? (V)(1.0 / (double)vals[d]) : (V)0;
+            }
+            const V zi = di * ri;
+            r[i] = ri;
+            z[i] = zi;
+            dinv[i] = di;
+            pold[i] = 0;
+            if (xs) xs[i] = 0;
+            rr += (double)ri * (double)ri;
+            rz += (double)ri * (double)zi;
+        }
+    }
+    // three reductions through the partial buffer (3 slots per block)
+    const double sbb = block_sum(bb, lds);
+    const double srr = block_sum(rr, lds);
+    const double srz = block_sum(rz, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[3 * blockIdx.x], sbb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 1], srr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 2], srz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!arrive_last(&st->counter)) return;
+    double tb = 0.0, tr = 0.0, tz = 0.0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+        tb += __hip_atomic_load(&partials[3 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tr += __hip_atomic_load(&partials[3 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tz += __hip_atomic_load(&partials[3 * k + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    tb = block_sum(tb, lds);
+    tr = block_sum(tr, lds);
+    tz = block_sum(tz, lds);
+    if (threadIdx.x == 0) {
+        const double bnorm = sqrt(tb);
+        const double rt = rtol * bnorm;
+        st->bnorm = bnorm;
+        st->tol = atol > rt ? atol : rt;  // _get_atol_rtol
+        st->rr = tr;
+        st->rho_cur = tz;
+        st->rho_prev = 0.0;
+        st->pq = 0.0;
+        st->it = 0;
+        st->maxiter = maxiter;
+        if (bnorm == 0.0) st->done = 1;
+        else if (maxiter <= 0) st->done = 2;
+        else st->done = (sqrt(tr) < st->tol) ? 1 : 0;
+        st->counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------- kernel A
+template <typename V, int LPR>
+__global__ __launch_bounds__(kCgBlock) void cg_spmv_kernel(
+    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const V *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
+    const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
+    double *__restrict__ partials, CgState *__restrict__ st) {
+    __shared__ double lds[kCgBlock / kWave];
+    if (st->done) return;
+    const V beta = (V)(st->it > 0 ? st->rho_cur / st->rho_prev : 0.0);
+    const int lane = threadIdx.x % LPR;
+    const int rows_per_block = kCgBlock / LPR;
+    double pq = 0.0;
+    for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
+         row += (long long)gridDim.x * rows_per_block) {
+        const int i = (int)row;
+        V acc = 0;
+        const int k1 = indptr[i + 1];
+        for (int k = indptr[i] + lane; k < k1; k += LPR) {
+            const int j = indices[k];
+            acc += vals[k] * (z[j] + beta * pold[j]);
+        }
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+        if (lane == 0) {
+            const bool free_row = mask ? mask[i] != 0 : true;
+            const V pi = free_row ? z[i] + beta * pold[i] : (V)0;
+            const V qi = free_row ? acc : (V)0;
+            pnew[i] = pi;
+            q[i] = qi;
+            pq += (double)pi * (double)qi;
+        }
+    }
+    const double s = block_sum(pq, lds);
+    if (threadIdx.x == 0) store_partial(partials, s);
+    if (!arrive_last(&st->counter)) return;
+    const double tot = final_sum(partials, gridDim.x, lds);
+    if (threadIdx.x == 0) {
+        st->pq = tot;
+        if (!(tot == tot) || tot == 0.0 || isinf(tot)) st->done = 3;
+        st->counter = 0;
+    }
+}
+
+// ---------------------------------------------------------------- kernel B
+template <typename V, typename X>
+__global__ __launch_bounds__(kCgBlock) void cg_update_kernel(
+    int n, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
+    const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r, V *__restrict__ z,
+    X *__restrict__ x, double *__restrict__ partials, CgState *__restrict__ st) {
+    __shared__ double lds[kCgBlock / kWave];
+    if (st->done) return;
+    const double alpha_d = st->rho_cur / st->pq;
+    const V alpha = (V)alpha_d;
+    double rz = 0.0, rr = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const bool free_row = mask ? mask[i] != 0 : true;
+        if (!free_row) continue;
+        x[i] += (X)(alpha * pnew[i]);
+        const V ri = r[i] - alpha * q[i];
+        const V zi = dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+    }
+    const double s1 = block_sum(rz, lds);
+    const double s2 = block_sum(rr, lds);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[2 * blockIdx.x], s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[2 * blockIdx.x + 1], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!arrive_last(&st->counter)) return;
+    double t1 = 0.0, t2 = 0.0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+        t1 += __hip_atomic_load(&partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t2 += __hip_atomic_load(&partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    t1 = block_sum(t1, lds);
+    t2 = block_sum(t2, lds);
+    if (threadIdx.x == 0) {
+        st->rho_prev = st->rho_cur;
+        st->rho_cur = t1;
+        st->rr = t2;
+        st->it += 1;
+        if (st->it >= st->maxiter) st->done = 2;
+        else if (sqrt(t2) < st->tol) st->done = 1;
+        if (!(t1 == t1) || isinf(t1)) st->done = 3;
+        st->counter = 0;
+    }
+}
+
+// x64 += xs (f32 correction) on free rows; x64 = 0 on free rows when b = 0
+__global__ void cg_finalize_kernel(int n, const unsigned char *__restrict__ mask,
+                                   const float *__restrict__ xs, double *__restrict__ x,
+                                   const CgState *__restrict__ st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mask && !mask[i]) return;
+    if (xs) x[i] += (double)xs[i];
+}
+
+__global__ void zero_free_kernel(int n, const unsigned char *__restrict__ mask,
+                                 double *__restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mask && !mask[i]) return;
+    x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- helpers
+template <int LPR>
+__global__ __launch_bounds__(kCgBlock) void spmv_f64_kernel(
+    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const double *__restrict__ vals, const unsigned char *__restrict__ mask,
+    const double *__restrict__ x, double *__restrict__ y) {
+    const int lane = threadIdx.x % LPR;
+    const int rows_per_block = kCgBlock / LPR;
+    for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
+         row += (long long)gridDim.x * rows_per_block) {
+        const int i = (int)row;
+        const bool fi = mask ? mask[i] != 0 : true;
+        double acc = 0.0;
+        if (fi) {
+            for (int k = indptr[i] + lane; k < indptr[i + 1]; k += LPR) {
+                const int j = indices[k];
+                if (!mask || mask[j]) acc += vals[k] * x[j];
+            }
+        }
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+        if (lane == 0) y[i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(kCgBlock) void dot_partials_kernel(
+    int n, const double *__restrict__ a, const double *__restrict__ b,
+    const unsigned char *__restrict__ mask, int op, double *__restrict__ partials) {
+    __shared__ double lds[kCgBlock / kWave];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (mask && !mask[i]) continue;
+        if (op == 0) s += a[i] * b[i];
+        else s = fmax(s, fabs(a[i]));
+    }
+    if (op == 0) {
+        s = block_sum(s, lds);
+    } else {
+        s = wave_max(s);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x / 64] = s;
+        __syncthreads();
+        s = 0.0;
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) s = fmax(s, lds[w]);
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+}  // namespace sn
+
+using namespace sn;
+
+struct sn_cg {
+    const sn_pattern *pat = nullptr;
+    int precision = 64;
+    int lpr = 8;
+    int chunk = 32;
+    int grid_rows = 0;   // blocks of the row-parallel kernels
+    int grid_vec = 0;    // blocks of the vector kernels
+    void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
+    float *xs = nullptr;
+    double *partials = nullptr;
+    CgState *st = nullptr;
+    CgState *st_host = nullptr;  // pinned
+    // cached graph of one chunk (2*chunk kernels), keyed by its pointers
+    hipGraphExec_t graph = nullptr;
+    const void *g_vals = nullptr;
+    const unsigned char *g_mask = nullptr;
+    void *g_x = nullptr;
+    hipStream_t g_stream = nullptr;
+};
+
+namespace sn {
+
+template <typename V, int LPR>
+static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
+                             int parity, hipStream_t s) {
+    const sn_pattern *P = C->pat;
+    V *pold = (V *)(parity ? C->p1 : C->p0);
+    V *pnew = (V *)(parity ? C->p0 : C->p1);
+    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,
+                       P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew, (V *)C->q,
+                       C->partials, C->st);
+    if (C->precision == 64)
+        hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kCgBlock), 0, s,
+                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (double *)xdst, C->partials, C->st);
+    else
+        hipLaunchKernelGGL((cg_update_kernel<V, float>), dim3(C->grid_vec), dim3(kCgBlock), 0, s,
+                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (float *)xdst, C->partials, C->st);
+}
+
+template <typename V>
+static void launch_iteration_lpr(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
+                                 int parity, hipStream_t s) {
+    switch (C->lpr) {
+        case 4: launch_iteration<V, 4>(C, vals, mask, xdst, parity, s); break;
+        case 16: launch_iteration<V, 16>(C, vals, mask, xdst, parity, s); break;
+        default: launch_iteration<V, 8>(C, vals, mask, xdst, parity, s); break;
+    }
+}
+
+template <typename V, int LPR>
+static void launch_init(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
+                        const double *x0, double rtol, double atol, int maxiter, hipStream_t s) {
+    const sn_pattern *P = C->pat;
+    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,
+                       P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r, (V *)C->z,
+                       (V *)C->dinv, (V *)C->p0, C->precision == 32 ? C->xs : (float *)nullptr,
+                       C->partials, C->st, rtol, atol, maxiter);
+}
+
+template <typename V>
+static void launch_init_lpr(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
+                            const double *x0, double rtol, double atol, int maxiter,
+                            hipStream_t s) {
+    switch (C->lpr) {
+        case 4: launch_init<V, 4>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
+        case 16: launch_init<V, 16>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
+        default: launch_init<V, 8>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
+    }
+}
+
+static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
+                         hipStream_t s) {
+    // an even number of iterations keeps the p buffers in the same parity
+    for (int k = 0; k < C->chunk; ++k) {
+        if (C->precision == 64)
+            launch_iteration_lpr<double>(C, (const double *)vals, mask, xdst, k & 1, s);
+        else
+            launch_iteration_lpr<float>(C, (const float *)vals, mask, xdst, k & 1, s);
+    }
+}
+
+}  // namespace sn
+
+extern "C" {
+
+int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
+                 void *stream) {
+    SN_REQUIRE(out && P, "sn_cg_create: null argument");
+    SN_REQUIRE(precision == 64 || precision == 32, "sn_cg_create: precision must be 64 or 32");
+    if (lpr == 0) lpr = 8;
+    SN_REQUIRE(lpr == 4 || lpr == 8 || lpr == 16, "sn_cg_create: lanes_per_row in {4,8,16}");
+    if (chunk <= 0) chunk = 32;
+    if (chunk & 1) chunk += 1;
+    sn_cg *C = new sn_cg();
+    C->pat = P;
+    C->precision = precision;
+    C->lpr = lpr;
+    C->chunk = chunk;
+    const int cus = num_cus();
+    const long long rows_per_block = kCgBlock / lpr;
+    C->grid_rows = grid_for(P->n, (int)rows_per_block, 8 * cus);
+    C->grid_vec = grid_for(P->n, kCgBlock, 4 * cus);
+    const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
+    SN_HIP(hipMalloc(&C->r, vb));
+    SN_HIP(hipMalloc(&C->z, vb));
+    SN_HIP(hipMalloc(&C->p0, vb));
+    SN_HIP(hipMalloc(&C->p1, vb));
+    SN_HIP(hipMalloc(&C->q, vb));
+    SN_HIP(hipMalloc(&C->dinv, vb));
+    if (precision == 32) SN_HIP(hipMalloc(&C->xs, (size_t)P->n * 4));
+    const int maxg = C->grid_rows > C->grid_vec ? C->grid_rows : C->grid_vec;
+    SN_HIP(hipMalloc(&C->partials, (size_t)3 * maxg * sizeof(double)));
+    SN_HIP(hipMalloc(&C->st, sizeof(CgState)));
+    SN_HIP(hipHostMalloc((void **)&C->st_host, sizeof(CgState), hipHostMallocDefault));
+    std::memset(C->st_host, 0, sizeof(CgState));
+    SN_HIP(hipMemcpyAsync(C->st, C->st_host, sizeof(CgState), hipMemcpyHostToDevice,
+                          as_stream(stream)));
+    SN_HIP(hipStreamSynchronize(as_stream(stream)));
+    *out = C;
+    return SN_OK;
+}
+
+void sn_cg_destroy(sn_cg *C) {
+    if (!C) return;
+    if (C->graph) (void)hipGraphExecDestroy(C->graph);
+    (void)hipFree(C->r);
+    (void)hipFree(C->z);
+    (void)hipFree(C->p0);
+    (void)hipFree(C->p1);
+    (void)hipFree(C->q);
+    (void)hipFree(C->dinv);
+    if (C->xs) (void)hipFree(C->xs);
+    (void)hipFree(C->partials);
+    (void)hipFree(C->st);
+    (void)hipHostFree(C->st_host);
+    delete C;
+}
+
+int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const double *b, double *x,
+                double rtol, double atol, int maxiter, int *iters_out, double *rnorm_out,
+                void *stream) {
+    SN_REQUIRE(C && vals && b && x, "sn_cg_solve: null argument");
+    const sn_pattern *P = C->pat;
+    hipStream_t s = as_stream(stream);
+    if (maxiter < 0) {
+        long long nfree = P->n;
+        if (mask) {
+            std::vector<unsigned char> h(P->n);
+            SN_HIP(hipMemcpyAsync(h.data(), mask, P->n, hipMemcpyDeviceToHost, s));
+            SN_HIP(hipStreamSynchronize(s));
+            nfree = 0;
+            for (int i = 0; i < P->n; ++i) nfree += h[i] != 0;
+        }
+        maxiter = (int)(nfree * 10 > 2000000000LL ? 2000000000LL : nfree * 10);
+    }
+    void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
+    const double *x0 = (C->precision == 64) ? x : nullptr;
+    if (C->precision == 64)
+        launch_init_lpr<double>(C, (const double *)vals, mask, b, x0, rtol, atol, maxiter, s);
+    else
+        launch_init_lpr<float>(C, (const float *)vals, mask, b, x0, rtol, atol, maxiter, s);
+    SN_CHECK_LAUNCH();
+
+    // (re)capture the chunk graph when the pointers or stream changed
+    if (!C->graph || C->g_vals != vals || C->g_mask != mask || C->g_x != xdst || C->g_stream != s) {
+        if (C->graph) {
+            SN_HIP(hipGraphExecDestroy(C->graph));
+            C->graph = nullptr;
+        }
+        hipStream_t cap = s;
+        bool own = false;
+        if (cap == nullptr) {
+            SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            own = true;
+        }
+        hipGraph_t g;
+        SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        launch_chunk(C, vals, mask, xdst, cap);
+        SN_HIP(hipStreamEndCapture(cap, &g));
+        SN_HIP(hipGraphInstantiate(&C->graph, g, nullptr, nullptr, 0));
+        SN_HIP(hipGraphDestroy(g));
+        if (own) SN_HIP(hipStreamDestroy(cap));
+        C->g_vals = vals;
+        C->g_mask = mask;
+        C->g_x = xdst;
+        C->g_stream = s;
+    }
+    SN_HIP(hipMemcpyAsync(C->st_host, C->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SN_HIP(hipStreamSynchronize(s));
+    while (C->st_host->done == 0) {
+        SN_HIP(hipGraphLaunch(C->graph, s));
+        SN_HIP(hipMemcpyAsync(C->st_host, C->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SN_HIP(hipStreamSynchronize(s));
+    }
+    const CgState h = *C->st_host;
+    if (h.bnorm == 0.0) {
+        // scipy returns b itself (all zeros) when ||b|| == 0
+        if (C->precision == 64)
+            hipLaunchKernelGGL(zero_free_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
+                               mask, x);
+    } else if (C->precision == 32) {
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
+                           mask, C->xs, x, C->st);
+    }
+    SN_CHECK_LAUNCH();
+    if (iters_out) *iters_out = h.it;
+    if (rnorm_out) *rnorm_out = sqrt(h.rr);
+    if (h.done == 2) {
+        set_error("sn_cg_solve: iteration cap %d reached (||r|| = %.3e, tol = %.3e)", h.maxiter,
+                  sqrt(h.rr), h.tol);
+        return SN_ERR_NOT_CONVERGED;
+    }
+    if (h.done == 3) {
+        set_error("sn_cg_solve: breakdown at iteration %d (p.Ap = %.3e, rho = %.3e)", h.it, h.pq,
+                  h.rho_cur);
+        return SN_ERR_BREAKDOWN;
+    }
+    return SN_OK;
+}
+
+int sn_spmv_f64(const sn_pattern *P, const double *vals, const unsigned char *mask,
+                const double *x, double *y, void *stream) {
+    SN_REQUIRE(P && vals && x && y, "sn_spmv_f64: null argument");
+    hipLaunchKernelGGL(spmv_f64_kernel<8>, dim3(grid_for(P->n, kCgBlock / 8, 8 * num_cus())),
+                       dim3(kCgBlock), 0, as_stream(stream), P->n, P->indptr, P->indices, vals,
+                       mask, x, y);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+static int reduce_host(int n, const double *a, const double *b, const unsigned char *mask, int op,
+                       double *out, void *stream) {
+    SN_REQUIRE(a && out && (op != 0 || b), "reduction: null argument");
+    hipStream_t s = as_stream(stream);
+    const int G = grid_for(n, kCgBlock, 1024);
+    double *part = nullptr;
+    SN_HIP(hipMallocAsync(&part, G * sizeof(double), s));
+    hipLaunchKernelGGL(dot_partials_kernel, dim3(G), dim3(kCgBlock), 0, s, n, a, b, mask, op, part);
+    SN_CHECK_LAUNCH();
+    std::vector<double> h(G);
+    SN_HIP(hipMemcpyAsync(h.data(), part, G * sizeof(double), hipMemcpyDeviceToHost, s));
+    SN_HIP(hipFreeAsync(part, s));
+    SN_HIP(hipStreamSynchronize(s));
+    double r = 0.0;
+    for (int k = 0; k < G; ++k) r = (op == 0) ? r + h[k] : (h[k] > r ? h[k] : r);
+    *out = r;
+    return SN_OK;
+}
+
+int sn_dot_f64(int n, const double *a, const double *b, const unsigned char *mask, double *out,
+               void *stream) {
+    return reduce_host(n, a, b, mask, 0, out, stream);
+}
+
+int sn_absmax_f64(int n, const double *a, const unsigned char *mask, double *out, void *stream) {
+    return reduce_host(n, a, nullptr, mask, 1, out, stream);
+}
+
+}  // extern "C"

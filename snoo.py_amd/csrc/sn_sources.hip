@@ -1,0 +1,272 @@
+// Source fields and field evaluation: Biot-Savart, brick field, quadrature
+// points and curl A at quadrature points.
+#include "sn_common.hpp"
+
+namespace sn {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int num_cus() {
+    static int cached = 0;
+    if (cached == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cached = prop.multiProcessorCount;
+        if (cached <= 0) cached = 256;
+    }
+    return cached;
+}
+
+// ---------------------------------------------------------------------------
+// Biot-Savart (fem_c/ctools.c:1761-1923).
+// One thread per target; segments staged through LDS in tiles.  Each target
+// sums its segments in order, the reference's single-thread order, and the
+// sum is scaled by current*1e-7 once at the end (ctools.c:1908-1912).
+// ---------------------------------------------------------------------------
+constexpr int kBsBlock = 256;
+constexpr int kBsTile = 256;
+
+__global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
+    const double *__restrict__ src, int num_src, const double *__restrict__ tar,
+    int num_tar, double scale, double rad, double *__restrict__ B, int accumulate) {
+    __shared__ double seg[kBsTile * 6];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    if (i < num_tar) {
+        tx = tar[3 * i];
+        ty = tar[3 * i + 1];
+        tz = tar[3 * i + 2];
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const double r2 = rad * rad;
+    for (int base = 0; base < num_src; base += kBsTile) {
+        const int cnt = min(kBsTile, num_src - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt * 6; k += blockDim.x) seg[k] = src[6 * base + k];
+        __syncthreads();
+        if (i < num_tar) {
+            for (int j = 0; j < cnt; ++j) {
+                const double *s = &seg[6 * j];
+                const double nx = s[3] - s[0], ny = s[4] - s[1], nz = s[5] - s[2];
+                const double nn = nx * nx + ny * ny + nz * nz;
+                const double d1x = s[0] - tx, d1y = s[1] - ty, d1z = s[2] - tz;
+                const double d2x = s[3] - tx, d2y = s[4] - ty, d2z = s[5] - tz;
+                const double l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
+                const double l2 = sqrt(d2x * d2x + d2y * d2y + d2z * d2z);
+                const double tp = -1.0 * (d1x * nx + d1y * ny + d1z * nz) / nn;
+                const double px = tx - (s[0] + tp * nx);
+                const double py = ty - (s[1] + tp * ny);
+                const double pz = tz - (s[2] + tp * nz);
+                const double dd = px * px + py * py + pz * pz;
+                if ((dd < r2 && tp <= 1.0 && tp >= 0.0) || l1 < rad || l2 < rad) {
+                    // inside the filament core: mu0 I r/(2 pi R^2) along n x p
+                    const double bx = ny * pz - nz * py;
+                    const double by = nz * px - nx * pz;
+                    const double bz = nx * py - ny * px;
+                    const double nb = sqrt(bx * bx + by * by + bz * bz);
+                    const double f = 2.0 * sqrt(dd) / rad / rad / nb;
+                    ax += bx * f;
+                    ay += by * f;
+                    az += bz * f;
+                } else {
+                    // finite straight segment (Russenschuck 5.51)
+                    const double d12 = d1x * d2x + d1y * d2y + d1z * d2z;
+                    const double f = (l1 + l2) / (l1 * l2 + d12) / l1 / l2;
+                    ax += (d1y * d2z - d1z * d2y) * f;
+                    ay += (d1z * d2x - d1x * d2z) * f;
+                    az += (d1x * d2y - d1y * d2x) * f;
+                }
+            }
+        }
+    }
+    if (i < num_tar) {
+        double b0 = accumulate ? B[3 * i] : 0.0;
+        double b1 = accumulate ? B[3 * i + 1] : 0.0;
+        double b2 = accumulate ? B[3 * i + 2] : 0.0;
+        B[3 * i] = b0 + ax * scale;
+        B[3 * i + 1] = b1 + ay * scale;
+        B[3 * i + 2] = b2 + az * scale;
+    }
+}
+
+// Brick field (snoopy/brick_field.py:400-433): first brick whose six faces all
+// have (p - r).n <= 0 sets the field.
+struct BrickTable {
+    double face[3 * 6 * 6];
+    double field[9];
+};
+
+__global__ void brick_field_kernel(BrickTable tab, const double *__restrict__ tar,
+                                   int num_tar, double *__restrict__ B, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_tar) return;
+    const double px = tar[3 * i], py = tar[3 * i + 1], pz = tar[3 * i + 2];
+    double b[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) {
+        bool inside = true;
+        for (int f = 0; f < 6; ++f) {
+            const double *F = &tab.face[(d * 6 + f) * 6];
+            const double proj = (px - F[0]) * F[3] + (py - F[1]) * F[4] + (pz - F[2]) * F[5];
+            if (proj > 0.0) inside = false;
+        }
+        if (inside) {
+            b[0] = tab.field[3 * d];
+            b[1] = tab.field[3 * d + 1];
+            b[2] = tab.field[3 * d + 2];
+            break;
+        }
+    }
+    for (int c = 0; c < 3; ++c) B[3 * i + c] = (accumulate ? B[3 * i + c] : 0.0) + b[c];
+}
+
+// ---------------------------------------------------------------------------
+// Quadrature points and curl A at quadrature points.
+// ---------------------------------------------------------------------------
+__global__ void quad_points_kernel(int num_el, const double *__restrict__ nodes,
+                                   const int *__restrict__ cells, const double *__restrict__ phi,
+                                   int Q, double *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)num_el * Q) return;
+    const int e = (int)(t / Q), q = (int)(t % Q);
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        const int n = cells[4 * e + a];
+        const double f = phi[4 * q + a];
+        p0 += f * nodes[3 * n];
+        p1 += f * nodes[3 * n + 1];
+        p2 += f * nodes[3 * n + 2];
+    }
+    out[3 * t] = p0;
+    out[3 * t + 1] = p1;
+    out[3 * t + 2] = p2;
+}
+
+// curl A is constant on an affine tetrahedron: B = sum_k g_k x_k / det with
+// g_k = J c_k (fem_c/ctools.c:922-958); the reference evaluates it per q.
+__global__ __launch_bounds__(256) void b_hcurl_kernel(
+    int num_el, const double *__restrict__ nodes, const int *__restrict__ cells,
+    const int *__restrict__ gids, const int *__restrict__ orient, const double *__restrict__ curls,
+    const double *__restrict__ phi, int Q, const double *__restrict__ x,
+    double *__restrict__ pts, double *__restrict__ B) {
+    // q = 0 slice of every orientation, curls[((o*Q)*6 + j)*3 + c], staged in LDS
+    __shared__ double curl0[64 * 18];
+    for (int k = threadIdx.x; k < 64 * 18; k += blockDim.x)
+        curl0[k] = curls[(size_t)(k / 18) * Q * 18 + (k % 18)];
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= num_el) return;
+    double X[4][3];
+    for (int a = 0; a < 4; ++a) {
+        const int n = cells[4 * e + a];
+        X[a][0] = nodes[3 * n];
+        X[a][1] = nodes[3 * n + 1];
+        X[a][2] = nodes[3 * n + 2];
+    }
+    double J[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) J[k][l] = X[l + 1][k] - X[0][k];
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    const int o = orient[e];
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        const double *c = &curl0[(o * 6 + k) * 3];
+        const double xv = x[gids[6 * e + k]];
+        b0 += (J[0][0] * c[0] + J[0][1] * c[1] + J[0][2] * c[2]) * xv / det;
+        b1 += (J[1][0] * c[0] + J[1][1] * c[1] + J[1][2] * c[2]) * xv / det;
+        b2 += (J[2][0] * c[0] + J[2][1] * c[1] + J[2][2] * c[2]) * xv / det;
+    }
+    for (int q = 0; q < Q; ++q) {
+        const long long t = (long long)e * Q + q;
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const double f = phi[4 * q + a];
+            p0 += f * X[a][0];
+            p1 += f * X[a][1];
+            p2 += f * X[a][2];
+        }
+        pts[3 * t] = p0;
+        pts[3 * t + 1] = p1;
+        pts[3 * t + 2] = p2;
+        B[3 * t] = b0;
+        B[3 * t + 1] = b1;
+        B[3 * t + 2] = b2;
+    }
+}
+
+}  // namespace sn
+
+using namespace sn;
+
+extern "C" {
+
+const char *sn_last_error(void) { return g_err.c_str(); }
+
+int sn_version(void) { return 1; }
+
+int sn_biot_savart_f64(const double *src, int num_src, const double *tar, int num_tar,
+                       double current, double radius, double *B, int accumulate,
+                       void *stream) {
+    SN_REQUIRE(num_src >= 0 && num_tar >= 0, "sn_biot_savart_f64: negative size");
+    if (num_tar == 0) return SN_OK;
+    SN_REQUIRE(tar && B && (num_src == 0 || src), "sn_biot_savart_f64: null pointer");
+    hipStream_t s = as_stream(stream);
+    const int grid = (num_tar + kBsBlock - 1) / kBsBlock;
+    hipLaunchKernelGGL(biot_savart_kernel, dim3(grid), dim3(kBsBlock), 0, s, src, num_src,
+                       tar, num_tar, current * 1e-7, radius, B, accumulate);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_brick_field_f64(const double *faces_host, const double *fields_host, const double *tar,
+                       int num_tar, double *B, int accumulate, void *stream) {
+    SN_REQUIRE(num_tar >= 0, "sn_brick_field_f64: negative size");
+    if (num_tar == 0) return SN_OK;
+    SN_REQUIRE(faces_host && fields_host && tar && B, "sn_brick_field_f64: null pointer");
+    BrickTable tab;
+    for (int k = 0; k < 108; ++k) tab.face[k] = faces_host[k];
+    for (int k = 0; k < 9; ++k) tab.field[k] = fields_host[k];
+    hipLaunchKernelGGL(brick_field_kernel, dim3((num_tar + 255) / 256), dim3(256), 0,
+                       as_stream(stream), tab, tar, num_tar, B, accumulate);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_quad_points_f64(int num_el, const double *nodes, const int *cells,
+                       const double *phi, int Q, double *out, void *stream) {
+    SN_REQUIRE(num_el >= 0 && Q >= 1, "sn_quad_points_f64: bad size (Q=%d)", Q);
+    if (num_el == 0) return SN_OK;
+    SN_REQUIRE(nodes && cells && phi && out, "sn_quad_points_f64: null pointer");
+    const long long n = (long long)num_el * Q;
+    hipLaunchKernelGGL(quad_points_kernel, dim3(grid_for(n, 256, 1 << 30)), dim3(256), 0,
+                       as_stream(stream), num_el, nodes, cells, phi, Q, out);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_B_Hcurl_f64(int num_el, const double *nodes, const int *cells, const int *gids,
+                   const int *orient, const double *curls, const double *phi, int Q,
+                   const double *x, double *pts, double *B, void *stream) {
+    SN_REQUIRE(num_el >= 0 && Q >= 1, "sn_B_Hcurl_f64: bad size (Q=%d)", Q);
+    if (num_el == 0) return SN_OK;
+    SN_REQUIRE(nodes && cells && gids && orient && curls && phi && x && pts && B,
+               "sn_B_Hcurl_f64: null pointer");
+    hipLaunchKernelGGL(b_hcurl_kernel, dim3((num_el + 255) / 256), dim3(256), 0,
+                       as_stream(stream), num_el, nodes, cells, gids, orient, curls, phi, Q, x,
+                       pts, B);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+}  // extern "C"

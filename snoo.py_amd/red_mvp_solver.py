@@ -1,0 +1,304 @@
+"""Reduced-vector-potential Newton solver (snoopy/red_mvp_solver.py:14-224).
+
+``RedMVPSolver`` keeps the reference constructor, attributes and ``solve``
+semantics: source fields from the coils at the iron quadrature points; Newton
+with b = Jx - Kx + rhs; Dirichlet DOFs removed; Jacobi-PCG with scipy's rule
+(atol = tolerance, rtol = 1e-5, x0 = x_n, maxiter = 10 n by default); the
+iterate replaced by the CG solution; stop when max|h| < tolerance max|x|.
+Every array stays on the device between steps; the host sees one scalar test
+per Newton step and one stop flag per chunk of CG iterations.
+
+``solve_newton`` is the configuration solver: inexact Newton on the FP64
+residual R(x) = K(x)x - rhs(x) (matrix-free, per element), stopped on
+||R(x)|| <= rtol ||R(0)||, with the linear solves in FP64 or (precision
+"mixed") in FP32 with FP64 dot products.  Each mixed step is one step of
+iterative refinement: the FP64 residual is recomputed at the new iterate.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .curl_curl_assembler import CurlCurlAssembler
+from .device import as_dev_f64, device
+from .materials import NU0, ConstantReluctance
+from .mesh_tools import get_edge_boundary_dofs
+from .sources import source_field_device
+
+
+class RedMVPSolver:
+    """Reference-compatible reduced vector potential solver."""
+
+    def __init__(self, gmsh_model, coil_list, iron_tags, air_tags, reluctances,
+                 max_newton_iterations=25, element_order=1, quad_order=6, verbose=True):
+        self.iron_tags = list(iron_tags)
+        self.air_tags = list(air_tags)
+        self.coil_list = list(coil_list)
+        self.domain_tags = self.iron_tags + self.air_tags
+        material_properties = list(reluctances) + [ConstantReluctance(NU0)] * len(self.air_tags)
+        self.max_newton_iterations = max_newton_iterations
+        self.element_order = element_order
+        self.quad_order = quad_order
+        self.verbose = verbose
+        self.model = gmsh_model
+        self.dirichlet_dofs = get_edge_boundary_dofs(gmsh_model, ['Dirichlet Boundary'],
+                                                     element_order)
+        self.curl_curl_factory = CurlCurlAssembler(gmsh_model.mesh, self.domain_tags,
+                                                   material_properties, element_order)
+        self.num_dofs_mvp = self.curl_curl_factory.num_dofs
+        self._src = None
+        self._mask_dev = None
+        self.stats = {}
+
+    def get_global_ids(self):
+        return self.curl_curl_factory.global_ids
+
+    def make_boundary_mask(self):
+        """Boolean mask, False at the Dirichlet DOFs (red_mvp_solver.py:105-119)."""
+        mask = np.ones((self.num_dofs_mvp,), dtype=np.bool_)
+        mask[np.asarray(self.dirichlet_dofs, dtype=np.int64) - 1] = False
+        return mask
+
+    # ------------------------------------------------------------------ setup
+    def source_fields_device(self):
+        """B_s at the quadrature points of each iron domain; None for air
+        (red_mvp_solver.py:149-164)."""
+        if self._src is None:
+            t0 = time.perf_counter()
+            n_iron = len(self.iron_tags)
+            pts = self.curl_curl_factory.get_quadrature_points_device(self.quad_order,
+                                                                      list(range(n_iron)))
+            src = [source_field_device(self.coil_list, p).reshape(-1) for p in pts]
+            src += [None] * len(self.air_tags)
+            import torch
+            torch.cuda.synchronize()
+            self.stats['t_source'] = time.perf_counter() - t0
+            self.stats['num_targets'] = int(sum(p.shape[0] for p in pts))
+            self._src = src
+        return self._src
+
+    def mask_device(self):
+        if self._mask_dev is None:
+            import torch
+            self._mask_dev = torch.from_numpy(self.make_boundary_mask().astype(np.uint8)).to(device())
+        return self._mask_dev
+
+    def _log(self, *a):
+        if self.verbose:
+            print(*a)
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, tolerance=1e-4, x_0=np.zeros((0,)), maxiter=-1, apply_gauge=False):
+        """Newton iterations of the reference (red_mvp_solver.py:121-224)."""
+        import torch
+        lib = nat.load()
+        asm = self.curl_curl_factory
+        n = self.num_dofs_mvp
+        if len(x_0) == 0:
+            x_0 = np.zeros((n,))
+        src = self.source_fields_device()
+        mask = self.mask_device()
+        if apply_gauge:
+            from .mesh_tools import get_cotree_dofs
+            tree, _ = get_cotree_dofs(asm.tet_mesh)
+            mask = mask.clone()
+            mask[torch.from_numpy(np.asarray(tree, dtype=np.int64)).to(device())] = 0
+        d = asm._device_state()
+        N = d['num_el']
+        locJ = torch.empty(21 * N, dtype=torch.float64, device=device())
+        loc6 = torch.empty(6 * N, dtype=torch.float64, device=device())
+        J = torch.empty(d['nnz'], dtype=torch.float64, device=device())
+        b = torch.empty(n, dtype=torch.float64, device=device())
+        x_n = as_dev_f64(x_0).clone()
+        cg = _CG(asm, 64)
+        self._log('***********************')
+        self._log('start Newton iterations')
+        it_hist = []
+        x_np1 = x_n
+        for i in range(self.max_newton_iterations):
+            asm.newton_local_device(x_n, self.quad_order, src, 0, False, locJ, loc6)
+            nat.check(lib.sn_csr_gather_one(d['pattern'], 0, nat.ptr(locJ), nat.ptr(J),
+                                            nat.stream_handle()))
+            nat.check(lib.sn_dof_gather_f64(d['pattern'], nat.ptr(loc6), nat.ptr(b),
+                                            nat.stream_handle()))
+            if _absmax(b) < 1e-14:
+                x_np1 = 0.0 * x_n
+                self._log('  zero excitation!')
+                break
+            x_np1 = x_n.clone()
+            iters, _, rc = cg.solve(J, mask, b, x_np1, rtol=1e-5, atol=tolerance, maxiter=maxiter)
+            it_hist.append(iters)
+            x_np1[mask == 0] = 0.0
+            h = x_np1 - x_n
+            hmax, xmax = _absmax(h), _absmax(x_np1)
+            self._log(f"  step {i}")
+            self._log(f"    max relative increment {hmax / xmax if xmax > 0 else float('inf'):.3e}")
+            if hmax < tolerance * xmax:
+                self._log('  tolerance reached!')
+                break
+            x_n = x_np1
+        self.stats['cg_iterations'] = it_hist
+        return x_np1.cpu().numpy()
+
+    def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
+                     eta_min=1e-6, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True):
+        """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
+
+        Returns the device solution (float64) and fills ``self.stats`` with the
+        residual history, per-step CG iteration counts and timings."""
+        drv = self.newton_driver(precision, lanes_per_row, chunk)
+        return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
+                         eta_min=eta_min, max_cg=max_cg, line_search=line_search,
+                         stats=self.stats)
+
+    def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32):
+        key = ('drv', precision, lanes_per_row, chunk)
+        if key not in self.__dict__.setdefault('_drivers', {}):
+            self._drivers[key] = NewtonDriver(self.curl_curl_factory, self.source_fields_device(),
+                                              self.mask_device(), self.quad_order, precision,
+                                              lanes_per_row, chunk)
+        return self._drivers[key]
+
+
+def _absmax(t):
+    out = ctypes.c_double()
+    nat.check(nat.load().sn_absmax_f64(int(t.numel()), nat.ptr(t), None, ctypes.byref(out),
+                                       nat.stream_handle()))
+    return out.value
+
+
+class _CG:
+    """Owning wrapper of an sn_cg workspace."""
+
+    def __init__(self, asm, precision, lanes_per_row=8, chunk=32):
+        self.h = ctypes.c_void_p()
+        nat.check(nat.load().sn_cg_create(ctypes.byref(self.h), asm.pattern, int(precision),
+                                          int(lanes_per_row), int(chunk), nat.stream_handle()))
+
+    def solve(self, vals, mask, b, x, rtol, atol, maxiter=-1, allow_cap=True):
+        it = ctypes.c_int()
+        rn = ctypes.c_double()
+        rc = nat.load().sn_cg_solve(self.h, nat.ptr(vals), nat.ptr(mask), nat.ptr(b), nat.ptr(x),
+                                    float(rtol), float(atol), int(maxiter), ctypes.byref(it),
+                                    ctypes.byref(rn), nat.stream_handle())
+        if rc == nat.SN_ERR_NOT_CONVERGED and allow_cap:
+            # scipy returns its last iterate with info = maxiter; so does this
+            return it.value, rn.value, rc
+        nat.check(rc)
+        return it.value, rn.value, rc
+
+    def __del__(self):
+        try:
+            if self.h and nat._lib is not None:
+                nat._lib.sn_cg_destroy(self.h)
+        except Exception:
+            pass
+
+
+class NewtonDriver:
+    """Device-resident inexact Newton for one mesh, precision and source field.
+
+    precision "fp64": FP64 Jacobian, FP64 PCG solving J dx = -R.
+    precision "mixed": FP32 Jacobian (element arithmetic in FP64, values
+    rounded once), FP32 PCG with FP64 dot products, x += dx in FP64; R(x) is
+    always FP64 (element-local K_e x_e - rhs_e, gathered in fixed order).
+    """
+
+    def __init__(self, asm, src_dev, mask_dev, quad_order, precision='fp64', lanes_per_row=8,
+                 chunk=32):
+        import torch
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError('precision must be "fp64" or "mixed"')
+        self.asm, self.src, self.mask, self.quad_order = asm, src_dev, mask_dev, quad_order
+        self.precision = precision
+        self.f32 = precision == 'mixed'
+        d = asm._device_state()
+        self.d = d
+        N, n, nnz = d['num_el'], asm.num_dofs, d['nnz']
+        jt = torch.float32 if self.f32 else torch.float64
+        dev = device()
+        self.locJ = torch.empty(21 * N, dtype=jt, device=dev)
+        self.loc6 = torch.empty(6 * N, dtype=torch.float64, device=dev)
+        self.J = torch.empty(nnz, dtype=jt, device=dev)
+        self.R = torch.empty(n, dtype=torch.float64, device=dev)
+        self.b = torch.empty(n, dtype=torch.float64, device=dev)
+        self.dx = torch.empty(n, dtype=torch.float64, device=dev)
+        self.cg = _CG(asm, 32 if self.f32 else 64, lanes_per_row, chunk)
+        self._r0 = None
+
+    def residual(self, x_dev):
+        """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2."""
+        lib = nat.load()
+        self.asm.newton_local_device(x_dev, self.quad_order, self.src, 1, self.f32, self.locJ,
+                                     self.loc6)
+        nat.check(lib.sn_csr_gather_one(self.d['pattern'], int(self.f32), nat.ptr(self.locJ),
+                                        nat.ptr(self.J), nat.stream_handle()))
+        nat.check(lib.sn_dof_gather_f64(self.d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
+                                        nat.stream_handle()))
+        out = ctypes.c_double()
+        nat.check(lib.sn_dot_f64(int(self.R.numel()), nat.ptr(self.R), nat.ptr(self.R),
+                                 nat.ptr(self.mask), ctypes.byref(out), nat.stream_handle()))
+        return float(np.sqrt(out.value))
+
+    def r0(self):
+        """||R(0)|| (the norm of the source term), cached."""
+        if self._r0 is None:
+            import torch
+            z = torch.zeros(self.asm.num_dofs, dtype=torch.float64, device=device())
+            self._r0 = self.residual(z)
+        return self._r0
+
+    def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=1e-6, max_cg=-1,
+              line_search=True, stats=None):
+        import torch
+        stats = stats if stats is not None else {}
+        n = self.asm.num_dofs
+        r0 = self.r0()
+        x = torch.zeros(n, dtype=torch.float64, device=device()) if x0 is None else \
+            as_dev_f64(x0).clone()
+        x[self.mask == 0] = 0.0
+        t0 = time.perf_counter()
+        rn = self.residual(x)
+        hist, cg_its, etas = [rn / r0], [], []
+        eta = eta_max
+        rn_prev = None
+        k = 0
+        converged = rn <= rtol * r0
+        while not converged and k < max_newton:
+            # forcing term (Eisenstat-Walker choice 2 with safeguards), never
+            # asking for more than the remaining distance to the target
+            if rn_prev is not None:
+                eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
+            eta = max(eta, min(eta_max, 0.5 * rtol * r0 / rn))
+            etas.append(eta)
+            torch.neg(self.R, out=self.b)
+            if self.f32:
+                x_try = x.clone()
+                its, _, _ = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
+                                          maxiter=max_cg)
+            else:
+                self.dx.zero_()
+                its, _, _ = self.cg.solve(self.J, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
+                                          maxiter=max_cg)
+                x_try = x + self.dx
+            cg_its.append(its)
+            rn_try = self.residual(x_try)
+            step = 1.0
+            if line_search:
+                dx = x_try - x
+                tries = 0
+                while not (rn_try < rn) and tries < 6:
+                    step *= 0.5
+                    x_try = x + step * dx
+                    rn_try = self.residual(x_try)
+                    tries += 1
+            rn_prev, rn, x = rn, rn_try, x_try
+            hist.append(rn / r0)
+            k += 1
+            converged = rn <= rtol * r0
+        torch.cuda.synchronize()
+        stats.update({'residual_history': hist, 'cg_iterations': cg_its, 'etas': etas,
+                      'newton_steps': k, 'converged': converged, 'r0': r0,
+                      't_newton': time.perf_counter() - t0, 'precision': self.precision})
+        return x

@@ -1,0 +1,286 @@
+"""GPU parity: every HIP kernel through the C ABI against the oracle.
+
+Tolerances: integer/index work (CSR pattern, DOF numbering) bit-exact;
+float64 kernels within 1e-12 relative to the largest entry (only the
+summation order differs from the reference loop); float32 Jacobian within
+1e-6 relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@pytest.fixture(scope='module')
+def dipole():
+    from snoopy_amd.magnets import Dipole
+    print('case: dipole h=0.12', flush=True)
+    return Dipole(h=0.12)
+
+
+@pytest.fixture(scope='module')
+def solver(dipole):
+    return dipole.solver(quad_order=2, verbose=False)
+
+
+@pytest.fixture(scope='module')
+def oprob(solver):
+    src = [s.cpu().numpy().reshape(-1, 3) if s is not None else None
+           for s in solver.source_fields_device()]
+    return orc.OracleProblem(solver, source_fields=src)
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+# ------------------------------------------------------------------ sources
+
+def test_biot_savart_matches_reference_goldens():
+    from snoopy_amd.sources import biot_savart_host
+    g = np.load(os.path.join(GOLDEN, 'biot_savart.npz'))
+    B = biot_savart_host(g['segs'], g['tar'], float(g['current']), float(g['rad']))
+    assert _rel(B, g['B']) < 1e-12
+
+
+def test_biot_savart_matches_oracle_on_coil_targets(dipole):
+    torch = _torch()
+    from snoopy_amd.sources import biot_savart_device, source_field_device
+    from snoopy_amd.device import as_dev_f64
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-0.6, 0.6, size=(3000, 3))
+    Bg = source_field_device(dipole.coils, as_dev_f64(pts)).cpu().numpy()
+    Bo = orc.coil_field(dipole.coils, pts, threads=4)
+    assert _rel(Bg, Bo) < 1e-12
+    # accumulate flag adds, 0 overwrites; empty target sets are a no-op
+    c = dipole.coils[0]
+    segs = as_dev_f64(c.get_segments())
+    t = as_dev_f64(pts)
+    out = torch.ones_like(t)
+    biot_savart_device(segs, t, c.current, c.rad, out, accumulate=True)
+    ref = orc.biot_savart(c.get_segments(), pts, c.current, c.rad) + 1.0
+    assert _rel(out.cpu().numpy(), ref) < 1e-12
+    empty = torch.empty((0, 3), dtype=torch.float64, device=t.device)
+    biot_savart_device(segs, empty, c.current, c.rad, empty)
+
+
+def test_brick_field_matches_oracle():
+    from snoopy_amd.brick_field import BrickField
+    bf = BrickField(0.0, 0.5, 0.52, 1.02, 0.0, 0.6, 0.62, 1.1, 1.19, 1.69, 1.2, 1.7, 4.52,
+                    Z_pos=-2.26, B_goal=1.7, type=1)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform([-0.2, -0.2, -2.5], [1.3, 1.9, 2.5], size=(4000, 3))
+    Bg = bf.compute_B(pts)
+    Bo = orc.brick_field(bf, pts)
+    np.testing.assert_array_equal(Bg, Bo)
+    assert np.abs(Bg).max() > 0
+
+
+# ------------------------------------------------------------------ mesh data
+
+def test_csr_pattern_bit_exact(solver):
+    asm = solver.curl_curl_factory
+    ip, ix = asm.csr_pattern_host()
+    g = asm.tet_mesh.glob_ids
+    r = np.repeat(g, 6, axis=1).ravel()
+    c = np.tile(g, (1, 6)).ravel()
+    ref = orc.coo_to_csr(asm.num_dofs, r, c, np.ones(len(r)))
+    np.testing.assert_array_equal(ip, ref.indptr)
+    np.testing.assert_array_equal(ix, ref.indices)
+
+
+def test_quadrature_points_and_B_at_points(solver):
+    asm = solver.curl_curl_factory
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(asm.num_dofs) * 1e-2
+    for order in (2, 3):
+        pts, B = asm.compute_B(x, quad_order=order)
+        parts_p, parts_B = [], []
+        for d in range(asm.num_materials):
+            inp = orc.domain_inputs(asm, d, order)
+            p, b = orc.B_hcurl(inp['nodes'], inp['cells'], inp['gids'], inp['orient'],
+                               inp['curls'], inp['phi'], inp['d_phi'], x)
+            parts_p.append(p)
+            parts_B.append(b)
+        assert _rel(pts, np.concatenate(parts_p)) < 1e-14
+        assert _rel(B, np.concatenate(parts_B)) < 1e-12
+        qp = asm.get_quadrature_points(order, [0])
+        inp = orc.domain_inputs(asm, 0, order)
+        assert _rel(qp[0], orc.quad_points(inp['nodes'], inp['cells'], inp['phi'])) < 1e-14
+
+
+# ------------------------------------------------------------------ assembly
+
+@pytest.mark.parametrize('material', ['bsq', 'pchip'])
+@pytest.mark.parametrize('order', [2, 3])
+def test_assembly_matches_oracle(dipole, material, order):
+    from snoopy_amd.materials import Reluctance
+    if material == 'pchip':
+        d = json.load(open(os.path.join(GOLDEN, 'aisi1010_bh.json')))
+        rel = Reluctance(B_data=d['B(T)'], H_data=d['H(A/m)'])
+    else:
+        rel = dipole.reluctance
+    s = dipole.solver(quad_order=order, reluctance=rel, verbose=False)
+    asm = s.curl_curl_factory
+    src = [t.cpu().numpy().reshape(-1, 3) if t is not None else np.zeros((0, 3))
+           for t in s.source_fields_device()]
+    rng = np.random.default_rng(order)
+    x = rng.standard_normal(asm.num_dofs) * 5e-3
+    print('case: assembly', material, 'order', order, 'dofs', asm.num_dofs, flush=True)
+    K, J, rhs = asm.compute_stiffness_and_jacobi_matrix_c(x, order, src)
+    Ko, Jo, rhso = orc.assemble_like_reference(asm, x, order,
+                                               [t if len(t) else None for t in src])
+    assert abs(K - Ko).max() <= 1e-12 * abs(Ko).max()
+    assert abs(J - Jo).max() <= 1e-12 * abs(Jo).max()
+    assert _rel(rhs, rhso) < 1e-12
+    assert K.shape == Ko.shape == (asm.num_dofs, asm.num_dofs)
+
+
+def test_assembly_is_deterministic(solver):
+    asm = solver.curl_curl_factory
+    x = np.random.default_rng(9).standard_normal(asm.num_dofs) * 1e-2
+    src = solver.source_fields_device()
+    a = asm.assemble_device(x, 2, src)
+    b = asm.assemble_device(x, 2, src)
+    for u, v in zip(a, b):
+        assert _torch().equal(u, v)
+
+
+@pytest.mark.parametrize('mode', [0, 1])
+def test_newton_element_kernel_modes(solver, oprob, mode):
+    torch = _torch()
+    from snoopy_amd import _native as nat
+    asm = solver.curl_curl_factory
+    d = asm._device_state()
+    N = d['num_el']
+    x = np.random.default_rng(21).standard_normal(asm.num_dofs) * 5e-3
+    xd = torch.from_numpy(x).cuda()
+    K, dK, rhs = oprob.assemble(x)
+    want = (dK @ x + rhs) if mode == 0 else (K @ x - rhs)
+    Jo = (dK + K).tocsr()
+    for f32 in (False, True):
+        locJ = torch.empty(21 * N, dtype=torch.float32 if f32 else torch.float64, device='cuda')
+        loc6 = torch.empty(6 * N, dtype=torch.float64, device='cuda')
+        asm.newton_local_device(xd, 2, solver.source_fields_device(), mode, f32, locJ, loc6)
+        vec = torch.empty(asm.num_dofs, dtype=torch.float64, device='cuda')
+        nat.check(nat.load().sn_dof_gather_f64(d['pattern'], nat.ptr(loc6), nat.ptr(vec),
+                                               nat.stream_handle()))
+        Jv = torch.empty(d['nnz'], dtype=locJ.dtype, device='cuda')
+        nat.check(nat.load().sn_csr_gather_one(d['pattern'], int(f32), nat.ptr(locJ), nat.ptr(Jv),
+                                               nat.stream_handle()))
+        ip, ix = asm.csr_pattern_host()
+        from scipy.sparse import csr_array
+        Jg = csr_array((Jv.double().cpu().numpy(), ix, ip), shape=Jo.shape)
+        tol = 1e-6 if f32 else 1e-12
+        assert abs(Jg - Jo).max() <= tol * abs(Jo).max()
+        assert _rel(vec.cpu().numpy(), want) < 1e-11
+
+
+# ------------------------------------------------------------------ CG
+
+def test_cg_matches_oracle_pcg(solver, oprob):
+    torch = _torch()
+    from snoopy_amd.red_mvp_solver import _CG
+    asm = solver.curl_curl_factory
+    K, dK, J, rhs = asm.assemble_device(np.zeros(asm.num_dofs), 2, solver.source_fields_device())
+    mask = solver.mask_device()
+    x = torch.zeros(asm.num_dofs, dtype=torch.float64, device='cuda')
+    cg = _CG(asm, 64)
+    it, rn, rc = cg.solve(J, mask, rhs, x, rtol=1e-5, atol=0.0)
+    ip, ix = asm.csr_pattern_host()
+    from scipy.sparse import csr_array
+    Jh = csr_array((J.cpu().numpy(), ix, ip), shape=(asm.num_dofs,) * 2)
+    xo, ito, info = orc.pcg(Jh, rhs.cpu().numpy(), np.zeros(asm.num_dofs), oprob.mask, rtol=1e-5)
+    assert info == 0 and rc == 0
+    assert abs(it - ito) <= 2, (it, ito)
+    xg = x.cpu().numpy()
+    assert _rel(xg, xo) < 1e-4   # both stop at ||r|| < 1e-5 ||b||
+    # identical iteration-capped runs: scipy returns the iterate after maxiter steps
+    x.zero_()
+    it, rn, rc = cg.solve(J, mask, rhs, x, rtol=1e-12, atol=0.0, maxiter=25)
+    xo, ito, info = orc.pcg(Jh, rhs.cpu().numpy(), np.zeros(asm.num_dofs), oprob.mask,
+                            rtol=1e-12, maxiter=25)
+    assert it == ito == 25 and info == 25
+    assert _rel(x.cpu().numpy(), xo) < 1e-9
+    # b = 0: scipy returns b itself (zeros on the free DOFs)
+    x.fill_(1.0)
+    it, rn, rc = cg.solve(J, mask, torch.zeros_like(rhs), x, rtol=1e-5, atol=0.0)
+    assert it == 0 and float(x[mask == 1].abs().max()) == 0.0
+
+
+def test_cg_mixed_precision_refinement(solver):
+    torch = _torch()
+    from snoopy_amd.red_mvp_solver import _CG
+    asm = solver.curl_curl_factory
+    K, dK, J, rhs = asm.assemble_device(np.zeros(asm.num_dofs), 2, solver.source_fields_device())
+    mask = solver.mask_device()
+    x64 = torch.zeros(asm.num_dofs, dtype=torch.float64, device='cuda')
+    _CG(asm, 64).solve(J, mask, rhs, x64, rtol=1e-10, atol=0.0)
+    cg32 = _CG(asm, 32)
+    J32 = J.float()
+    x = torch.zeros_like(x64)
+    from snoopy_amd import _native as nat
+    r = torch.empty_like(x)
+    for k in range(4):   # iterative refinement: r = b - J x in FP64, solve in FP32
+        nat.check(nat.load().sn_spmv_f64(asm.pattern, nat.ptr(J), nat.ptr(mask), nat.ptr(x),
+                                         nat.ptr(r), nat.stream_handle()))
+        r = rhs - r
+        r[mask == 0] = 0
+        cg32.solve(J32, mask, r, x, rtol=1e-3, atol=0.0)
+    err = float((x - x64).abs().max() / x64.abs().max())
+    assert err < 1e-8, err
+
+
+# ------------------------------------------------------------------ solvers
+
+def test_reference_solve_matches_oracle(dipole, oprob):
+    s = dipole.solver(quad_order=2, verbose=False)
+    x = s.solve(tolerance=1e-6)
+    xo, hist = orc.reference_newton(oprob, tolerance=1e-6)
+    assert len(s.stats['cg_iterations']) == len(hist)
+    asm = s.curl_curl_factory
+    Bg = asm.element_B(x)
+    Bo = asm.element_B(xo)
+    assert _rel(Bg, Bo) < 1e-5
+
+
+def test_newton_fp64_and_mixed_reach_tolerance_and_agree(solver):
+    x64 = solver.solve_newton('fp64', rtol=1e-10)
+    st = dict(solver.stats)
+    assert st['converged'] and st['residual_history'][-1] <= 1e-10
+    xm = solver.solve_newton('mixed', rtol=1e-10)
+    stm = dict(solver.stats)
+    assert stm['converged'] and stm['residual_history'][-1] <= 1e-10
+    asm = solver.curl_curl_factory
+    B64 = asm.element_B(x64.cpu().numpy())
+    Bm = asm.element_B(xm.cpu().numpy())
+    assert _rel(Bm, B64) < 1e-7
+
+
+def test_newton_matches_oracle_newton(solver, oprob):
+    x = solver.solve_newton('fp64', rtol=1e-9)
+    xo, info = orc.residual_newton(oprob, rtol=1e-9)
+    asm = solver.curl_curl_factory
+    assert _rel(asm.element_B(x.cpu().numpy()), asm.element_B(xo)) < 1e-7
+
+
+def test_zero_excitation_returns_zero(dipole):
+    from snoopy_amd.red_mvp_solver import RedMVPSolver
+    from snoopy_amd.racetrack_coil import RacetrackCoil
+    c = dipole.coils[0]
+    zero = RacetrackCoil(c.kp, c.y, c.rad, 0.0)
+    s = RedMVPSolver(dipole.model, [zero], [1], [2], [dipole.reluctance], quad_order=2,
+                     verbose=False)
+    x = s.solve()
+    assert np.all(x == 0.0)
