@@ -1,0 +1,243 @@
+"""Benchmark of the nonlinear reduced-vector-potential solve on one MI355X.
+
+Workload (the metric's configuration): the window-frame dipole (yoke
+1.0 x 0.6 x 0.5 m, aperture 0.3 x 0.2 m, two racetrack coils at 1.5e6 A/m^2,
+0.5 m air pad) meshed with ~1M Kuhn tetrahedra (h = 0.02 m), lowest-order
+Nedelec elements, 4-point quadrature, iron nu = cubic B-spline in B^2 fitted
+to 60 samples of B(H) = mu0 H + 1.9 tanh(mu0 4000 H / 1.9).
+
+One step = one complete solve: source field at the iron quadrature points
+(Biot-Savart) plus Newton from the start vector to ||R|| <= 1e-8 ||R(0)||.
+Mesh, CSR pattern and element-to-nonzero map are set up before the timed
+region (inputs resident in HBM).  The chosen form (mixed precision, plus the
+learned warm start when a trained model is present) is timed for --steps
+solves; the plain FP64 GPU solve is timed once beside it, and the relative B
+error at 10k aperture probes is reported against an FP64 solve at 1e-10.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s at 1M tets, mixed+learned warm start vs plain FP64, B err<1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--tets', type=float, default=1.0e6)
+    ap.add_argument('--rtol', type=float, default=1e-8)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-cg-iters', type=int, default=30)
+    ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--lanes', type=int, default=8)
+    return ap.parse_args()
+
+
+def probe_B(dp, solver, x_dev, probes):
+    """B = B_s + curl A at the probe points (element-constant curl A)."""
+    from snoopy_amd.mesh import locate_points
+    from snoopy_amd.sources import source_field_device
+    from snoopy_amd.device import as_dev_f64
+    el = locate_points(dp.mesh, probes)
+    Be = solver.curl_curl_factory.element_B(x_dev.cpu().numpy())
+    Bs = source_field_device(dp.coils, as_dev_f64(probes)).cpu().numpy()
+    return Bs + Be[el]
+
+
+def b_error(B, Bref):
+    d = np.linalg.norm(B - Bref, axis=1)
+    return float(d.max() / np.linalg.norm(Bref, axis=1).max()), \
+        float(np.linalg.norm(B - Bref) / np.linalg.norm(Bref))
+
+
+def cpu_baseline(dp, solver, gpu_stats, cg_iters, threads):
+    """Oracle (C restatement of the reference path) on the host cores: Biot-Savart
+    (OpenMP, `threads`), one assembly + COO->CSR (1 thread) and `cg_iters`
+    Jacobi-PCG iterations (1 thread) on the full 1M-tet system; the solve time
+    is extrapolated with the FP64 GPU solve's Newton and CG iteration counts
+    (the same algorithm, so the same counts)."""
+    from oracle import oracle as orc
+    t0 = time.perf_counter()
+    prob = orc.OracleProblem(solver, threads=threads)
+    t_bs = prob.t_source
+    n = prob.n
+    t1 = time.perf_counter()
+    K, dK, rhs = prob.assemble(np.zeros(n))
+    J = (dK + K).tocsr()
+    t_asm = time.perf_counter() - t1
+    b = rhs
+    t2 = time.perf_counter()
+    _, it, _ = orc.pcg(J, b, np.zeros(n), prob.mask, rtol=1e-30, atol=0.0, maxiter=cg_iters)
+    t_cg = (time.perf_counter() - t2) / max(it, 1)
+    n_asm = len(gpu_stats['residual_history']) + 1      # residual evaluations incl. R(0)
+    n_cg = int(sum(gpu_stats['cg_iterations']))
+    t_solve = t_bs + n_asm * t_asm + n_cg * t_cg
+    sample_s = time.perf_counter() - t0
+    return {'value': 1.0 / t_solve, 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
+            'sample': ('oracle C restatement on the host: Biot-Savart at %d iron quadrature points '
+                       '(%d OpenMP threads, %.2f s), one assembly + COO->CSR of %d tets (1 thread, '
+                       '%.2f s), %d Jacobi-PCG iterations (1 thread, %.1f ms each); solve time '
+                       'extrapolated with the FP64 GPU solve counts (%d assemblies, %d CG '
+                       'iterations) = %.1f s; sample wall time %.1f s; cpu_count=%s, %s'
+                       % (prob.src[0].shape[0], threads, t_bs, dp.num_tets, t_asm, it, 1e3 * t_cg,
+                          n_asm, n_cg, t_solve, sample_s, os.cpu_count(), platform.processor()
+                          or platform.machine()))}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only (this build has no multi-GPU path); use --gpus 1',
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from snoopy_amd import _native as nat
+    from snoopy_amd.magnets import Dipole, mesh_size_for
+    from snoopy_amd.red_mvp_solver import _CG
+
+    h, _ = mesh_size_for(args.tets)
+    t0 = time.perf_counter()
+    dp = Dipole(h=h)
+    solver = dp.solver(quad_order=2, verbose=False)
+    asm = solver.curl_curl_factory
+    asm._device_state()
+    log('mesh: h=%.4f tets=%d edges=%d nnz=%d iron=%.3f setup %.1fs'
+        % (h, dp.num_tets, asm.num_dofs, asm.nnz, dp.iron_fraction(), time.perf_counter() - t0))
+
+    warm = None
+    try:
+        from snoopy_amd import warm_start
+        warm = warm_start.load_default(dp)
+    except Exception as e:  # no trained model in the tree
+        log('warm start unavailable:', e)
+
+    def one_solve(precision, rtol, use_warm):
+        solver._src = None                      # Biot-Savart is part of every solve
+        x0 = warm.predict(dp) if (use_warm and warm is not None) else None
+        return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes)
+
+    # plain FP64 GPU solve (the drop-in method at the configuration's tolerance)
+    one_solve('fp64', args.rtol, False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    x64 = one_solve('fp64', args.rtol, False)
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t
+    st64 = dict(solver.stats)
+    log('fp64: %.3f s, newton %d, cg %s' % (t_fp64, st64['newton_steps'], st64['cg_iterations']))
+
+    # FP64 reference for the B error (tighter tolerance)
+    xref = solver.solve_newton('fp64', rtol=1e-10, lanes_per_row=args.lanes)
+    probes = dp.probes(10000, seed=2)
+    Bref = probe_B(dp, solver, xref, probes)
+
+    # chosen form: mixed precision (+ warm start)
+    for _ in range(args.warmup):
+        one_solve('mixed', args.rtol, True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        xm = one_solve('mixed', args.rtol, True)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t) / args.steps
+    stm = dict(solver.stats)
+    log('mixed: %.3f s/solve, newton %d, cg %s' % (t_mixed, stm['newton_steps'],
+                                                    stm['cg_iterations']))
+    Bm = probe_B(dp, solver, xm, probes)
+    B64 = probe_B(dp, solver, x64, probes)
+    err_m, err_m_l2 = b_error(Bm, Bref)
+    err_64, _ = b_error(B64, Bref)
+
+    # roofline of the dominant kernel: the CG SpMV kernel of the mixed solve
+    drv = solver.newton_driver('mixed', args.lanes, 32)
+    ms = ctypes.c_double()
+    nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
+                                         ctypes.byref(ms), nat.stream_handle()))
+    n, nnz = asm.num_dofs, asm.nnz
+    s = 4  # float32 values and vectors
+    spmv_bytes = nnz * (s + 4) + 4 * (n + 1) + 4 * s * n + n
+    spmv_gbs = spmv_bytes / (ms.value * 1e-3) / 1e9
+    # element kernel (mixed, residual mode) timed with torch events on its stream
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    reps = 10
+    for _ in range(reps):
+        asm.newton_local_device(xm, 2, solver.source_fields_device(), 1, True, drv.locJ, drv.loc6)
+    ev1.record()
+    torch.cuda.synchronize()
+    asm_ms = ev0.elapsed_time(ev1) / reps
+    N = dp.num_tets
+    n_iron = int(np.sum(dp.mesh.tags == 1))
+    asm_bytes = 16 * N + 24 * N + 4 * N + 24 * asm.tet_mesh.num_nodes + 8 * n + \
+        4 * 3 * 8 * n_iron + 21 * 4 * N + 6 * 8 * N
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(dp, solver, st64, args.cpu_cg_iters, args.cpu_threads)
+        except Exception as e:
+            log('cpu baseline failed:', repr(e))
+
+    value = 1.0 / t_mixed
+    out = {
+        'metric': METRIC,
+        'value': value,
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': 1e3 * t_mixed,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32+f64',
+        'data': 'synthetic',
+        'config': {'workload': 'window-frame dipole, %d tets, %d edge DOFs, nnz %d, Q=4, '
+                               'B-spline-in-B^2 iron, Newton to 1e-8' % (N, n, nnz),
+                   'tets': N, 'edges': n, 'nnz': nnz, 'iron_fraction': dp.iron_fraction(),
+                   'precision': 'mixed (FP32 Jacobian/SpMV/CG, FP64 residual and dots)',
+                   'warm_start': warm is not None, 'parallelism': 'single GPU'},
+        'fp64_solves_per_s': 1.0 / t_fp64,
+        'speedup_vs_fp64': t_fp64 / t_mixed,
+        'B_rel_err_max': err_m,
+        'B_rel_err_l2': err_m_l2,
+        'B_rel_err_fp64': err_64,
+        'newton_steps': stm['newton_steps'],
+        'cg_iterations': stm['cg_iterations'],
+        'newton_steps_fp64': st64['newton_steps'],
+        'cg_iterations_fp64': st64['cg_iterations'],
+        'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'cg_spmv_kernel<float,%d>' % args.lanes,
+                     'avg_ms': ms.value, 'algorithmic_bytes': spmv_bytes},
+        'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
+                            'GB/s': asm_bytes / (asm_ms * 1e-3) / 1e9},
+        'cpu_baseline': cpu,
+    }
+    if cpu:
+        out['speedup_vs_cpu'] = value / cpu['value']
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

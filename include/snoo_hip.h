@@ -194,6 +194,13 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
                 const double *b_dev, double *x_dev, double rtol, double atol,
                 int maxiter, int *iters_host, double *rnorm_host, void *stream);
 
+/* Average duration (ms) of the CG SpMV kernel (kernel A: p = z + beta p,
+ * q = A p, p.q) over `reps` back-to-back launches on `stream`, timed with HIP
+ * events; run after a solve on the same workspace (it overwrites the CG
+ * scratch vectors). */
+int sn_cg_time_spmv(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev, int reps,
+                    double *avg_ms_host, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,7 @@ SIGNATURES = {
     'sn_absmax_f64': (_c_int, [_c_int, _vp, _vp, _P_dbl, _vp]),
     'sn_cg_create': (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int, _c_int, _vp]),
     'sn_cg_destroy': (None, [_vp]),
+    'sn_cg_time_spmv': (_c_int, [_vp, _vp, _vp, _c_int, _P_dbl, _vp]),
     'sn_cg_solve': (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_int, _P_int, _P_dbl,
                              _vp]),
 }

@@ -253,6 +253,16 @@ __global__ void zero_free_kernel(int n, const unsigned char *__restrict__ mask,
     x[i] = 0.0;
 }
 
+template <typename V>
+__global__ void fill_free_kernel(int n, const unsigned char *__restrict__ mask, V *__restrict__ z,
+                                 V *__restrict__ p, V v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool f = mask ? mask[i] != 0 : true;
+    z[i] = f ? v : (V)0;
+    p[i] = (V)0;
+}
+
 // ---------------------------------------------------------------- helpers
 template <int LPR>
 __global__ __launch_bounds__(kCgBlock) void spmv_f64_kernel(
@@ -525,6 +535,55 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
                   h.rho_cur);
         return SN_ERR_BREAKDOWN;
     }
+    return SN_OK;
+}
+
+int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int reps,
+                    double *avg_ms, void *stream) {
+    SN_REQUIRE(C && vals && avg_ms && reps > 0, "sn_cg_time_spmv: bad argument");
+    const sn_pattern *P = C->pat;
+    hipStream_t s = as_stream(stream);
+    // a running state with beta = 1 and z = 1 on the free rows, p_old = 0
+    CgState h;
+    std::memset(&h, 0, sizeof(h));
+    h.it = 1;
+    h.rho_cur = 1.0;
+    h.rho_prev = 1.0;
+    h.maxiter = 0x7fffffff;
+    SN_HIP(hipMemcpyAsync(C->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    if (C->precision == 64)
+        hipLaunchKernelGGL(fill_free_kernel<double>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
+                           P->n, mask, (double *)C->z, (double *)C->p0, 1.0);
+    else
+        hipLaunchKernelGGL(fill_free_kernel<float>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
+                           P->n, mask, (float *)C->z, (float *)C->p0, 1.0f);
+    SN_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    SN_HIP(hipEventCreate(&e0));
+    SN_HIP(hipEventCreate(&e1));
+    SN_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+        const int lpr = C->lpr;
+#define SN_T(V, L)                                                                              \
+    hipLaunchKernelGGL((cg_spmv_kernel<V, L>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,  \
+                       P->indptr, P->indices, (const V *)vals, mask, (const V *)C->z,           \
+                       (const V *)C->p0, (V *)C->p1, (V *)C->q, C->partials, C->st)
+        if (C->precision == 64) {
+            if (lpr == 4) SN_T(double, 4); else if (lpr == 16) SN_T(double, 16); else SN_T(double, 8);
+        } else {
+            if (lpr == 4) SN_T(float, 4); else if (lpr == 16) SN_T(float, 16); else SN_T(float, 8);
+        }
+#undef SN_T
+    }
+    SN_HIP(hipEventRecord(e1, s));
+    SN_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    SN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    SN_HIP(hipEventDestroy(e0));
+    SN_HIP(hipEventDestroy(e1));
+    SN_HIP(hipMemcpy(&h, C->st, sizeof(h), hipMemcpyDeviceToHost));
+    SN_REQUIRE(h.done == 0, "sn_cg_time_spmv: state left running expected (done=%d)", h.done);
+    *avg_ms = (double)ms / reps;
     return SN_OK;
 }
 
