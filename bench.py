@@ -48,6 +48,8 @@ def parse():
     ap.add_argument('--cpu-cg-iters', type=int, default=30)
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--lanes', type=int, default=8)
+    ap.add_argument('--quick', action='store_true',
+                    help='timed solves only (no B-error reference, roofline or CPU leg)')
     return ap.parse_args()
 
 
@@ -134,7 +136,7 @@ def main():
     def one_solve(precision, rtol, use_warm):
         solver._src = None                      # Biot-Savart is part of every solve
         x0 = warm.predict(dp) if (use_warm and warm is not None) else None
-        return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes)
+        return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes, log=log)
 
     # plain FP64 GPU solve (the drop-in method at the configuration's tolerance)
     one_solve('fp64', args.rtol, False)
@@ -145,11 +147,6 @@ def main():
     t_fp64 = time.perf_counter() - t
     st64 = dict(solver.stats)
     log('fp64: %.3f s, newton %d, cg %s' % (t_fp64, st64['newton_steps'], st64['cg_iterations']))
-
-    # FP64 reference for the B error (tighter tolerance)
-    xref = solver.solve_newton('fp64', rtol=1e-10, lanes_per_row=args.lanes)
-    probes = dp.probes(10000, seed=2)
-    Bref = probe_B(dp, solver, xref, probes)
 
     # chosen form: mixed precision (+ warm start)
     for _ in range(args.warmup):
@@ -163,6 +160,17 @@ def main():
     stm = dict(solver.stats)
     log('mixed: %.3f s/solve, newton %d, cg %s' % (t_mixed, stm['newton_steps'],
                                                     stm['cg_iterations']))
+    if args.quick:
+        print(json.dumps({'quick': True, 'fp64_s': t_fp64, 'mixed_s': t_mixed,
+                          'fp64_stats': {k: v for k, v in st64.items()},
+                          'mixed_stats': {k: v for k, v in stm.items()}}), flush=True)
+        return
+    # FP64 reference for the B error: Newton down to the FP64 residual floor
+    log('fp64 reference solve (rtol 1e-11 or the residual floor)')
+    xref = solver.solve_newton('fp64', rtol=1e-11, lanes_per_row=args.lanes, log=log)
+    stref = dict(solver.stats)
+    probes = dp.probes(10000, seed=2)
+    Bref = probe_B(dp, solver, xref, probes)
     Bm = probe_B(dp, solver, xm, probes)
     B64 = probe_B(dp, solver, x64, probes)
     err_m, err_m_l2 = b_error(Bm, Bref)
@@ -191,8 +199,10 @@ def main():
     asm_bytes = 16 * N + 24 * N + 4 * N + 24 * asm.tet_mesh.num_nodes + 8 * n + \
         4 * 3 * 8 * n_iron + 21 * 4 * N + 6 * 8 * N
 
+    log('roofline: spmv %.4f ms (%.0f GB/s), element kernel %.3f ms' % (ms.value, spmv_gbs, asm_ms))
     cpu = None
     if not args.no_cpu:
+        log('cpu baseline leg')
         try:
             cpu = cpu_baseline(dp, solver, st64, args.cpu_cg_iters, args.cpu_threads)
         except Exception as e:
@@ -222,6 +232,8 @@ def main():
         'B_rel_err_max': err_m,
         'B_rel_err_l2': err_m_l2,
         'B_rel_err_fp64': err_64,
+        'B_reference_residual': stref['residual_history'][-1],
+        'final_residual': stm['residual_history'][-1],
         'newton_steps': stm['newton_steps'],
         'cg_iterations': stm['cg_iterations'],
         'newton_steps_fp64': st64['newton_steps'],

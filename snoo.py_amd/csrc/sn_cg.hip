@@ -5,14 +5,20 @@
 // snoopy/red_mvp_solver.py:195-205 with Dirichlet rows/columns removed by a
 // mask instead of a fancy-indexed copy of the matrix.
 //
-// Two kernels per iteration, each ending in a deterministic grid reduction
-// (per-block partials summed in block order by the last block to finish):
-//   A: p_new = z + beta p_old (formed on the fly for every column read),
-//      q = A p_new, p.q
-//   B: x += alpha p_new, r -= alpha q, z = dinv r, r.z and r.r, stop test.
-// alpha and beta are computed on the device from the reduced scalars, so the
-// host only polls the stop flag once per chunk of iterations; each chunk is
-// captured once as a hipGraph and replayed.
+// Two kernels per iteration:
+//   A (row-parallel): reduce B's partials -> rho, ||r||^2; stop test;
+//      p_new = z + beta p_old formed on the fly for every column read;
+//      q = A p_new; per-block partial of p.q
+//   B (vector): reduce A's partials -> p.q; alpha = rho / p.q;
+//      x += alpha p_new, r -= alpha q, z = dinv r; per-block partials of r.z, r.r
+// Every block of a kernel re-reduces the previous kernel's per-block partials
+// in the same fixed order, so all blocks hold bit-identical scalars without
+// any cross-block synchronisation inside a kernel (no atomics, no device-scope
+// fences: on a multi-XCD part those would write back L2 per block).  Kernel
+// boundaries order the partial writes before their reads.  The host polls the
+// stop flag once per chunk of iterations; each chunk is captured once as a
+// hipGraph and replayed.
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -22,55 +28,36 @@
 namespace sn {
 
 struct CgState {
-    double rho_cur, rho_prev, pq, rr, tol, bnorm;
+    double rho[2];   // rho of even / odd iterations (written by kernel A, block 0)
+    double pq, rr, tol, bnorm;
     int it, maxiter, done, pad;  // done: 0 running, 1 converged, 2 iteration cap, 3 breakdown
-    unsigned int counter;
-    unsigned int pad2;
 };
 
-constexpr int kCgBlock = 256;
+constexpr int kRowBlock = 256;   // kernel A (SpMV) block
+constexpr int kVecBlock = 1024;  // kernel B and init block
 
-__device__ __forceinline__ void store_partial(double *partials, double v) {
-    __hip_atomic_store(&partials[blockIdx.x], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Returns true on the last block to arrive (after acquire), false elsewhere.
-__device__ __forceinline__ bool arrive_last(unsigned int *counter) {
-    __shared__ bool s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return s_last;
-}
-
-// Sum of partials[0..nb) in a fixed order by the whole block.
-__device__ __forceinline__ double final_sum(const double *partials, int nb, double *lds) {
+// Fixed-order block sum of partials[k*stride + off], k in [0, nb).
+template <int BLOCK>
+__device__ __forceinline__ double reduce_partials(const double *__restrict__ partials, int nb,
+                                                  int stride, int off, double *lds) {
     double s = 0.0;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x)
-        s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = threadIdx.x; k < nb; k += BLOCK) s += partials[k * stride + off];
     return block_sum(s, lds);
 }
 
 // ---------------------------------------------------------------- init
-// r = b - A_c x0 (x0 masked; f32 mode starts from 0), dinv, z, p_old = 0,
-// partial sums of b.b, r.r, r.z.
+// r = b - A_c x0 (x0 masked; the f32 solve starts from 0), dinv, z, p_old = 0,
+// per-block partials: (r.z, r.r) in the layout kernel B uses, b.b separately.
 template <typename V, int LPR>
-__global__ __launch_bounds__(kCgBlock) void cg_init_kernel(
+__global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
     const int *__restrict__ diag, const V *__restrict__ vals, const unsigned char *__restrict__ mask,
     const double *__restrict__ b, const double *__restrict__ x0, V *__restrict__ r,
     V *__restrict__ z, V *__restrict__ dinv, V *__restrict__ pold, float *__restrict__ xs,
-    double *__restrict__ partials, CgState *__restrict__ st, double rtol, double atol,
-    int maxiter) {
-    __shared__ double lds[kCgBlock / kWave];
+    double *__restrict__ partB, double *__restrict__ partbb) {
+    __shared__ double lds[kVecBlock / kWave];
     const int lane = threadIdx.x % LPR;
-    const int rows_per_block = kCgBlock / LPR;
+    const int rows_per_block = kVecBlock / LPR;
     double bb = 0.0, rr = 0.0, rz = 0.0;
     for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
          row += (long long)gridDim.x * rows_per_block) {
@@ -105,55 +92,62 @@ __global__ __launch_bounds__(kCgBlock) void cg_init_kernel(
             rz += (double)ri * (double)zi;
         }
     }
-    // three reductions through the partial buffer (3 slots per block)
     const double sbb = block_sum(bb, lds);
-    const double srr = block_sum(rr, lds);
     const double srz = block_sum(rz, lds);
+    const double srr = block_sum(rr, lds);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[3 * blockIdx.x], sbb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 1], srr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 2], srz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        partB[2 * blockIdx.x] = srz;
+        partB[2 * blockIdx.x + 1] = srr;
+        partbb[blockIdx.x] = sbb;
     }
-    if (!arrive_last(&st->counter)) return;
-    double tb = 0.0, tr = 0.0, tz = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-        tb += __hip_atomic_load(&partials[3 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tr += __hip_atomic_load(&partials[3 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tz += __hip_atomic_load(&partials[3 * k + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    tb = block_sum(tb, lds);
-    tr = block_sum(tr, lds);
-    tz = block_sum(tz, lds);
+}
+
+// one block: tol, ||b||, state for iteration 0
+__global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
+    int nb, const double *__restrict__ partbb, CgState *__restrict__ st, double rtol,
+    double atol, int maxiter) {
+    __shared__ double lds[kVecBlock / kWave];
+    const double bb = reduce_partials<kVecBlock>(partbb, nb, 1, 0, lds);
     if (threadIdx.x == 0) {
-        const double bnorm = sqrt(tb);
+        const double bnorm = sqrt(bb);
         const double rt = rtol * bnorm;
         st->bnorm = bnorm;
         st->tol = atol > rt ? atol : rt;  // _get_atol_rtol
-        st->rr = tr;
-        st->rho_cur = tz;
-        st->rho_prev = 0.0;
+        st->rho[0] = st->rho[1] = 0.0;
         st->pq = 0.0;
+        st->rr = 0.0;
         st->it = 0;
         st->maxiter = maxiter;
-        if (bnorm == 0.0) st->done = 1;
-        else if (maxiter <= 0) st->done = 2;
-        else st->done = (sqrt(tr) < st->tol) ? 1 : 0;
-        st->counter = 0;
+        st->done = (bnorm == 0.0) ? 1 : 0;
     }
 }
 
 // ---------------------------------------------------------------- kernel A
+// parity = iteration & 1: reads rho[parity ^ 1] (previous), writes rho[parity].
 template <typename V, int LPR>
-__global__ __launch_bounds__(kCgBlock) void cg_spmv_kernel(
+__global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
     const V *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
     const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
-    double *__restrict__ partials, CgState *__restrict__ st) {
-    __shared__ double lds[kCgBlock / kWave];
+    const double *__restrict__ partB, int nbB, double *__restrict__ partA,
+    CgState *__restrict__ st, int parity) {
+    __shared__ double lds[kRowBlock / kWave];
     if (st->done) return;
-    const V beta = (V)(st->it > 0 ? st->rho_cur / st->rho_prev : 0.0);
+    const double rho = reduce_partials<kRowBlock>(partB, nbB, 2, 0, lds);
+    const double rr = reduce_partials<kRowBlock>(partB, nbB, 2, 1, lds);
+    const int it = st->it;
+    // scipy: the iteration cap ends the loop before the residual test
+    const int stop = (it >= st->maxiter) ? 2 : ((sqrt(rr) < st->tol) ? 1 : 0);
+    if (stop) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->rr = rr;
+            st->done = stop;
+        }
+        return;
+    }
+    const V beta = (V)(it > 0 ? rho / st->rho[parity ^ 1] : 0.0);
     const int lane = threadIdx.x % LPR;
-    const int rows_per_block = kCgBlock / LPR;
+    const int rows_per_block = kRowBlock / LPR;
     double pq = 0.0;
     for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
          row += (long long)gridDim.x * rows_per_block) {
@@ -176,25 +170,33 @@ __global__ __launch_bounds__(kCgBlock) void cg_spmv_kernel(
         }
     }
     const double s = block_sum(pq, lds);
-    if (threadIdx.x == 0) store_partial(partials, s);
-    if (!arrive_last(&st->counter)) return;
-    const double tot = final_sum(partials, gridDim.x, lds);
     if (threadIdx.x == 0) {
-        st->pq = tot;
-        if (!(tot == tot) || tot == 0.0 || isinf(tot)) st->done = 3;
-        st->counter = 0;
+        partA[blockIdx.x] = s;
+        if (blockIdx.x == 0) {
+            st->rho[parity] = rho;
+            st->rr = rr;
+        }
     }
 }
 
 // ---------------------------------------------------------------- kernel B
 template <typename V, typename X>
-__global__ __launch_bounds__(kCgBlock) void cg_update_kernel(
+__global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
     int n, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
     const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r, V *__restrict__ z,
-    X *__restrict__ x, double *__restrict__ partials, CgState *__restrict__ st) {
-    __shared__ double lds[kCgBlock / kWave];
+    X *__restrict__ x, const double *__restrict__ partA, int nbA, double *__restrict__ partB,
+    CgState *__restrict__ st, int parity) {
+    __shared__ double lds[kVecBlock / kWave];
     if (st->done) return;
-    const double alpha_d = st->rho_cur / st->pq;
+    const double pq = reduce_partials<kVecBlock>(partA, nbA, 1, 0, lds);
+    if (!(pq == pq) || pq == 0.0 || isinf(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->pq = pq;
+            st->done = 3;
+        }
+        return;
+    }
+    const double alpha_d = st->rho[parity] / pq;
     const V alpha = (V)alpha_d;
     double rz = 0.0, rr = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -212,37 +214,22 @@ __global__ __launch_bounds__(kCgBlock) void cg_update_kernel(
     const double s1 = block_sum(rz, lds);
     const double s2 = block_sum(rr, lds);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[2 * blockIdx.x], s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[2 * blockIdx.x + 1], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (!arrive_last(&st->counter)) return;
-    double t1 = 0.0, t2 = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-        t1 += __hip_atomic_load(&partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t2 += __hip_atomic_load(&partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    t1 = block_sum(t1, lds);
-    t2 = block_sum(t2, lds);
-    if (threadIdx.x == 0) {
-        st->rho_prev = st->rho_cur;
-        st->rho_cur = t1;
-        st->rr = t2;
-        st->it += 1;
-        if (st->it >= st->maxiter) st->done = 2;
-        else if (sqrt(t2) < st->tol) st->done = 1;
-        if (!(t1 == t1) || isinf(t1)) st->done = 3;
-        st->counter = 0;
+        partB[2 * blockIdx.x] = s1;
+        partB[2 * blockIdx.x + 1] = s2;
+        if (blockIdx.x == 0) {
+            st->pq = pq;
+            st->it = st->it + 1;
+        }
     }
 }
 
-// x64 += xs (f32 correction) on free rows; x64 = 0 on free rows when b = 0
+// x64 += xs (f32 correction) on free rows
 __global__ void cg_finalize_kernel(int n, const unsigned char *__restrict__ mask,
-                                   const float *__restrict__ xs, double *__restrict__ x,
-                                   const CgState *__restrict__ st) {
+                                   const float *__restrict__ xs, double *__restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (mask && !mask[i]) return;
-    if (xs) x[i] += (double)xs[i];
+    x[i] += (double)xs[i];
 }
 
 __global__ void zero_free_kernel(int n, const unsigned char *__restrict__ mask,
@@ -263,14 +250,22 @@ __global__ void fill_free_kernel(int n, const unsigned char *__restrict__ mask, 
     p[i] = (V)0;
 }
 
+// partials that reduce to (rho = 1, ||r||^2 = 1) for the SpMV timing helper
+__global__ void fill_partials_kernel(int nb, double *__restrict__ partB) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    partB[2 * k] = (k == 0) ? 1.0 : 0.0;
+    partB[2 * k + 1] = (k == 0) ? 1.0 : 0.0;
+}
+
 // ---------------------------------------------------------------- helpers
 template <int LPR>
-__global__ __launch_bounds__(kCgBlock) void spmv_f64_kernel(
+__global__ __launch_bounds__(kRowBlock) void spmv_f64_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
     const double *__restrict__ vals, const unsigned char *__restrict__ mask,
     const double *__restrict__ x, double *__restrict__ y) {
     const int lane = threadIdx.x % LPR;
-    const int rows_per_block = kCgBlock / LPR;
+    const int rows_per_block = kRowBlock / LPR;
     for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
          row += (long long)gridDim.x * rows_per_block) {
         const int i = (int)row;
@@ -288,10 +283,10 @@ __global__ __launch_bounds__(kCgBlock) void spmv_f64_kernel(
     }
 }
 
-__global__ __launch_bounds__(kCgBlock) void dot_partials_kernel(
+__global__ __launch_bounds__(kRowBlock) void dot_partials_kernel(
     int n, const double *__restrict__ a, const double *__restrict__ b,
     const unsigned char *__restrict__ mask, int op, double *__restrict__ partials) {
-    __shared__ double lds[kCgBlock / kWave];
+    __shared__ double lds[kRowBlock / kWave];
     double s = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -321,11 +316,11 @@ struct sn_cg {
     int precision = 64;
     int lpr = 8;
     int chunk = 32;
-    int grid_rows = 0;   // blocks of the row-parallel kernels
-    int grid_vec = 0;    // blocks of the vector kernels
+    int grid_rows = 0;   // blocks of kernel A
+    int grid_vec = 0;    // blocks of kernel B and init
     void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
     float *xs = nullptr;
-    double *partials = nullptr;
+    double *partA = nullptr, *partB = nullptr, *partbb = nullptr;
     CgState *st = nullptr;
     CgState *st_host = nullptr;  // pinned
     // cached graph of one chunk (2*chunk kernels), keyed by its pointers
@@ -339,42 +334,46 @@ struct sn_cg {
 namespace sn {
 
 template <typename V, int LPR>
-static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
-                             int parity, hipStream_t s) {
+static void launch_spmv(sn_cg *C, const V *vals, const unsigned char *mask, int parity,
+                        hipStream_t s) {
     const sn_pattern *P = C->pat;
     V *pold = (V *)(parity ? C->p1 : C->p0);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
-    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,
+    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s, P->n,
                        P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew, (V *)C->q,
-                       C->partials, C->st);
-    if (C->precision == 64)
-        hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kCgBlock), 0, s,
-                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (double *)xdst, C->partials, C->st);
-    else
-        hipLaunchKernelGGL((cg_update_kernel<V, float>), dim3(C->grid_vec), dim3(kCgBlock), 0, s,
-                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (float *)xdst, C->partials, C->st);
+                       C->partB, C->grid_vec, C->partA, C->st, parity);
 }
 
 template <typename V>
-static void launch_iteration_lpr(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
-                                 int parity, hipStream_t s) {
+static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
+                             int parity, hipStream_t s) {
+    const sn_pattern *P = C->pat;
     switch (C->lpr) {
-        case 4: launch_iteration<V, 4>(C, vals, mask, xdst, parity, s); break;
-        case 16: launch_iteration<V, 16>(C, vals, mask, xdst, parity, s); break;
-        default: launch_iteration<V, 8>(C, vals, mask, xdst, parity, s); break;
+        case 4: launch_spmv<V, 4>(C, vals, mask, parity, s); break;
+        case 16: launch_spmv<V, 16>(C, vals, mask, parity, s); break;
+        default: launch_spmv<V, 8>(C, vals, mask, parity, s); break;
     }
+    V *pnew = (V *)(parity ? C->p0 : C->p1);
+    if (C->precision == 64)
+        hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
+                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (double *)xdst, C->partA, C->grid_rows, C->partB, C->st,
+                           parity);
+    else
+        hipLaunchKernelGGL((cg_update_kernel<V, float>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
+                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (float *)xdst, C->partA, C->grid_rows, C->partB, C->st,
+                           parity);
 }
 
 template <typename V, int LPR>
 static void launch_init(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
-                        const double *x0, double rtol, double atol, int maxiter, hipStream_t s) {
+                        const double *x0, hipStream_t s) {
     const sn_pattern *P = C->pat;
-    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,
+    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s, P->n,
                        P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r, (V *)C->z,
                        (V *)C->dinv, (V *)C->p0, C->precision == 32 ? C->xs : (float *)nullptr,
-                       C->partials, C->st, rtol, atol, maxiter);
+                       C->partB, C->partbb);
 }
 
 template <typename V>
@@ -382,20 +381,22 @@ static void launch_init_lpr(sn_cg *C, const V *vals, const unsigned char *mask, 
                             const double *x0, double rtol, double atol, int maxiter,
                             hipStream_t s) {
     switch (C->lpr) {
-        case 4: launch_init<V, 4>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
-        case 16: launch_init<V, 16>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
-        default: launch_init<V, 8>(C, vals, mask, b, x0, rtol, atol, maxiter, s); break;
+        case 4: launch_init<V, 4>(C, vals, mask, b, x0, s); break;
+        case 16: launch_init<V, 16>(C, vals, mask, b, x0, s); break;
+        default: launch_init<V, 8>(C, vals, mask, b, x0, s); break;
     }
+    hipLaunchKernelGGL(cg_init_state_kernel, dim3(1), dim3(kVecBlock), 0, s, C->grid_vec,
+                       C->partbb, C->st, rtol, atol, maxiter);
 }
 
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
                          hipStream_t s) {
-    // an even number of iterations keeps the p buffers in the same parity
+    // an even number of iterations keeps the p buffers and rho slots in step
     for (int k = 0; k < C->chunk; ++k) {
         if (C->precision == 64)
-            launch_iteration_lpr<double>(C, (const double *)vals, mask, xdst, k & 1, s);
+            launch_iteration<double>(C, (const double *)vals, mask, xdst, k & 1, s);
         else
-            launch_iteration_lpr<float>(C, (const float *)vals, mask, xdst, k & 1, s);
+            launch_iteration<float>(C, (const float *)vals, mask, xdst, k & 1, s);
     }
 }
 
@@ -417,9 +418,8 @@ int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int c
     C->lpr = lpr;
     C->chunk = chunk;
     const int cus = num_cus();
-    const long long rows_per_block = kCgBlock / lpr;
-    C->grid_rows = grid_for(P->n, (int)rows_per_block, 8 * cus);
-    C->grid_vec = grid_for(P->n, kCgBlock, 4 * cus);
+    C->grid_rows = grid_for(P->n, kRowBlock / lpr, 8 * cus);
+    C->grid_vec = grid_for(P->n, kVecBlock, cus);
     const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
     SN_HIP(hipMalloc(&C->r, vb));
     SN_HIP(hipMalloc(&C->z, vb));
@@ -428,8 +428,9 @@ int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int c
     SN_HIP(hipMalloc(&C->q, vb));
     SN_HIP(hipMalloc(&C->dinv, vb));
     if (precision == 32) SN_HIP(hipMalloc(&C->xs, (size_t)P->n * 4));
-    const int maxg = C->grid_rows > C->grid_vec ? C->grid_rows : C->grid_vec;
-    SN_HIP(hipMalloc(&C->partials, (size_t)3 * maxg * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partA, (size_t)C->grid_rows * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * sizeof(double)));
     SN_HIP(hipMalloc(&C->st, sizeof(CgState)));
     SN_HIP(hipHostMalloc((void **)&C->st_host, sizeof(CgState), hipHostMallocDefault));
     std::memset(C->st_host, 0, sizeof(CgState));
@@ -450,7 +451,9 @@ void sn_cg_destroy(sn_cg *C) {
     (void)hipFree(C->q);
     (void)hipFree(C->dinv);
     if (C->xs) (void)hipFree(C->xs);
-    (void)hipFree(C->partials);
+    (void)hipFree(C->partA);
+    (void)hipFree(C->partB);
+    (void)hipFree(C->partbb);
     (void)hipFree(C->st);
     (void)hipHostFree(C->st_host);
     delete C;
@@ -471,7 +474,7 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
             nfree = 0;
             for (int i = 0; i < P->n; ++i) nfree += h[i] != 0;
         }
-        maxiter = (int)(nfree * 10 > 2000000000LL ? 2000000000LL : nfree * 10);
+        maxiter = (int)(nfree * 10 > INT_MAX ? INT_MAX : nfree * 10);
     }
     void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
     const double *x0 = (C->precision == 64) ? x : nullptr;
@@ -520,7 +523,7 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
                                mask, x);
     } else if (C->precision == 32) {
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
-                           mask, C->xs, x, C->st);
+                           mask, C->xs, x);
     }
     SN_CHECK_LAUNCH();
     if (iters_out) *iters_out = h.it;
@@ -531,8 +534,7 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
         return SN_ERR_NOT_CONVERGED;
     }
     if (h.done == 3) {
-        set_error("sn_cg_solve: breakdown at iteration %d (p.Ap = %.3e, rho = %.3e)", h.it, h.pq,
-                  h.rho_cur);
+        set_error("sn_cg_solve: breakdown at iteration %d (p.Ap = %.3e)", h.it, h.pq);
         return SN_ERR_BREAKDOWN;
     }
     return SN_OK;
@@ -543,37 +545,44 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     SN_REQUIRE(C && vals && avg_ms && reps > 0, "sn_cg_time_spmv: bad argument");
     const sn_pattern *P = C->pat;
     hipStream_t s = as_stream(stream);
-    // a running state with beta = 1 and z = 1 on the free rows, p_old = 0
+    // a running state: rho = 1 now and before (beta = 1), ||r||^2 = 1 above a
+    // zero tolerance, z = 1 on the free rows and p_old = 0
     CgState h;
     std::memset(&h, 0, sizeof(h));
     h.it = 1;
-    h.rho_cur = 1.0;
-    h.rho_prev = 1.0;
-    h.maxiter = 0x7fffffff;
+    h.rho[0] = h.rho[1] = 1.0;
+    h.tol = 0.0;
+    h.maxiter = INT_MAX;
     SN_HIP(hipMemcpyAsync(C->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fill_partials_kernel, dim3(grid_for(C->grid_vec, 256)), dim3(256), 0, s,
+                       C->grid_vec, C->partB);
     if (C->precision == 64)
         hipLaunchKernelGGL(fill_free_kernel<double>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
-                           P->n, mask, (double *)C->z, (double *)C->p0, 1.0);
+                           P->n, mask, (double *)C->z, (double *)C->p1, 1.0);
     else
         hipLaunchKernelGGL(fill_free_kernel<float>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
-                           P->n, mask, (float *)C->z, (float *)C->p0, 1.0f);
+                           P->n, mask, (float *)C->z, (float *)C->p1, 1.0f);
     SN_CHECK_LAUNCH();
     hipEvent_t e0, e1;
     SN_HIP(hipEventCreate(&e0));
     SN_HIP(hipEventCreate(&e1));
     SN_HIP(hipEventRecord(e0, s));
+    // parity 1 reads p1 (zero) and rho[0], writes p0 and rho[1]: the state stays
+    // the same for every repetition
     for (int r = 0; r < reps; ++r) {
-        const int lpr = C->lpr;
-#define SN_T(V, L)                                                                              \
-    hipLaunchKernelGGL((cg_spmv_kernel<V, L>), dim3(C->grid_rows), dim3(kCgBlock), 0, s, P->n,  \
-                       P->indptr, P->indices, (const V *)vals, mask, (const V *)C->z,           \
-                       (const V *)C->p0, (V *)C->p1, (V *)C->q, C->partials, C->st)
         if (C->precision == 64) {
-            if (lpr == 4) SN_T(double, 4); else if (lpr == 16) SN_T(double, 16); else SN_T(double, 8);
+            switch (C->lpr) {
+                case 4: launch_spmv<double, 4>(C, (const double *)vals, mask, 1, s); break;
+                case 16: launch_spmv<double, 16>(C, (const double *)vals, mask, 1, s); break;
+                default: launch_spmv<double, 8>(C, (const double *)vals, mask, 1, s); break;
+            }
         } else {
-            if (lpr == 4) SN_T(float, 4); else if (lpr == 16) SN_T(float, 16); else SN_T(float, 8);
+            switch (C->lpr) {
+                case 4: launch_spmv<float, 4>(C, (const float *)vals, mask, 1, s); break;
+                case 16: launch_spmv<float, 16>(C, (const float *)vals, mask, 1, s); break;
+                default: launch_spmv<float, 8>(C, (const float *)vals, mask, 1, s); break;
+            }
         }
-#undef SN_T
     }
     SN_HIP(hipEventRecord(e1, s));
     SN_HIP(hipEventSynchronize(e1));
@@ -582,7 +591,7 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     SN_HIP(hipEventDestroy(e0));
     SN_HIP(hipEventDestroy(e1));
     SN_HIP(hipMemcpy(&h, C->st, sizeof(h), hipMemcpyDeviceToHost));
-    SN_REQUIRE(h.done == 0, "sn_cg_time_spmv: state left running expected (done=%d)", h.done);
+    SN_REQUIRE(h.done == 0, "sn_cg_time_spmv: the timed launches stopped early (done=%d)", h.done);
     *avg_ms = (double)ms / reps;
     return SN_OK;
 }
@@ -590,8 +599,8 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
 int sn_spmv_f64(const sn_pattern *P, const double *vals, const unsigned char *mask,
                 const double *x, double *y, void *stream) {
     SN_REQUIRE(P && vals && x && y, "sn_spmv_f64: null argument");
-    hipLaunchKernelGGL(spmv_f64_kernel<8>, dim3(grid_for(P->n, kCgBlock / 8, 8 * num_cus())),
-                       dim3(kCgBlock), 0, as_stream(stream), P->n, P->indptr, P->indices, vals,
+    hipLaunchKernelGGL(spmv_f64_kernel<8>, dim3(grid_for(P->n, kRowBlock / 8, 8 * num_cus())),
+                       dim3(kRowBlock), 0, as_stream(stream), P->n, P->indptr, P->indices, vals,
                        mask, x, y);
     SN_CHECK_LAUNCH();
     return SN_OK;
@@ -601,10 +610,11 @@ static int reduce_host(int n, const double *a, const double *b, const unsigned c
                        double *out, void *stream) {
     SN_REQUIRE(a && out && (op != 0 || b), "reduction: null argument");
     hipStream_t s = as_stream(stream);
-    const int G = grid_for(n, kCgBlock, 1024);
+    const int G = grid_for(n, kRowBlock, 1024);
     double *part = nullptr;
     SN_HIP(hipMallocAsync(&part, G * sizeof(double), s));
-    hipLaunchKernelGGL(dot_partials_kernel, dim3(G), dim3(kCgBlock), 0, s, n, a, b, mask, op, part);
+    hipLaunchKernelGGL(dot_partials_kernel, dim3(G), dim3(kRowBlock), 0, s, n, a, b, mask, op,
+                       part);
     SN_CHECK_LAUNCH();
     std::vector<double> h(G);
     SN_HIP(hipMemcpyAsync(h.data(), part, G * sizeof(double), hipMemcpyDeviceToHost, s));

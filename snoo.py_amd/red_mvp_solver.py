@@ -142,7 +142,8 @@ class RedMVPSolver:
         return x_np1.cpu().numpy()
 
     def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
-                     eta_min=1e-6, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True):
+                     eta_min=None, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True,
+                     log=None):
         """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
 
         Returns the device solution (float64) and fills ``self.stats`` with the
@@ -150,7 +151,7 @@ class RedMVPSolver:
         drv = self.newton_driver(precision, lanes_per_row, chunk)
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
-                         stats=self.stats)
+                         stats=self.stats, log=log)
 
     def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32):
         key = ('drv', precision, lanes_per_row, chunk)
@@ -249,10 +250,19 @@ class NewtonDriver:
             self._r0 = self.residual(z)
         return self._r0
 
-    def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=1e-6, max_cg=-1,
-              line_search=True, stats=None):
+    def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=None, max_cg=-1,
+              line_search=True, stats=None, log=None, stall_steps=3):
+        """Newton steps until ||R|| <= rtol ||R(0)||, ``max_newton`` steps, or
+        ``stall_steps`` consecutive steps that reduce ||R|| by less than 10%
+        (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
+        linear solve at min(10 n, 20000) iterations."""
         import torch
         stats = stats if stats is not None else {}
+        if eta_min is None:
+            # float32 CG cannot be asked for much more than 3 digits per solve
+            eta_min = 1e-3 if self.f32 else 1e-6
+        if max_cg < 0:
+            max_cg = min(10 * self.asm.num_dofs, 20000)
         n = self.asm.num_dofs
         r0 = self.r0()
         x = torch.zeros(n, dtype=torch.float64, device=device()) if x0 is None else \
@@ -264,8 +274,9 @@ class NewtonDriver:
         eta = eta_max
         rn_prev = None
         k = 0
+        stalls = 0
         converged = rn <= rtol * r0
-        while not converged and k < max_newton:
+        while not converged and k < max_newton and stalls < stall_steps:
             # forcing term (Eisenstat-Walker choice 2 with safeguards), never
             # asking for more than the remaining distance to the target
             if rn_prev is not None:
@@ -293,10 +304,14 @@ class NewtonDriver:
                     x_try = x + step * dx
                     rn_try = self.residual(x_try)
                     tries += 1
+            stalls = stalls + 1 if rn_try > 0.9 * rn else 0
             rn_prev, rn, x = rn, rn_try, x_try
             hist.append(rn / r0)
             k += 1
             converged = rn <= rtol * r0
+            if log is not None:
+                log('  newton %d: |R|/|R0| = %.3e, cg %d, eta %.1e, step %.3g, %.3f s'
+                    % (k, rn / r0, its, eta, step, time.perf_counter() - t0))
         torch.cuda.synchronize()
         stats.update({'residual_history': hist, 'cg_iterations': cg_its, 'etas': etas,
                       'newton_steps': k, 'converged': converged, 'r0': r0,

@@ -241,8 +241,15 @@ def test_cg_mixed_precision_refinement(solver):
         r = rhs - r
         r[mask == 0] = 0
         cg32.solve(J32, mask, r, x, rtol=1e-3, atol=0.0)
-    err = float((x - x64).abs().max() / x64.abs().max())
-    assert err < 1e-8, err
+    # the curl-curl system is singular (gradients in the air), so compare what
+    # the solution determines: the FP64 residual and B = curl A
+    nat.check(nat.load().sn_spmv_f64(asm.pattern, nat.ptr(J), nat.ptr(mask), nat.ptr(x),
+                                     nat.ptr(r), nat.stream_handle()))
+    res = float(((rhs - r)[mask == 1]).norm() / rhs[mask == 1].norm())
+    assert res < 1e-10, res
+    B64 = asm.element_B(x64.cpu().numpy())
+    Bm = asm.element_B(x.cpu().numpy())
+    assert _rel(Bm, B64) < 1e-8
 
 
 # ------------------------------------------------------------------ solvers
