@@ -45,9 +45,11 @@ def parse():
     ap.add_argument('--tets', type=float, default=1.0e6)
     ap.add_argument('--rtol', type=float, default=1e-8)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
-    ap.add_argument('--cpu-cg-iters', type=int, default=30)
+    ap.add_argument('--cpu-cg-iters', type=int, default=600)
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--lanes', type=int, default=8)
+    ap.add_argument('--copy-calib', action='store_true',
+                    help='only run a plain 256 MiB device copy 5 times (HBM counter calibration)')
     ap.add_argument('--quick', action='store_true',
                     help='timed solves only (no B-error reference, roofline or CPU leg)')
     return ap.parse_args()
@@ -111,6 +113,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calib:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_calib_bytes_read': src.numel() * 4,
+                          'copy_calib_bytes_written': dst.numel() * 4}))
+        return
     import __graft_entry__
     __graft_entry__.load_package()
     from snoopy_amd import _native as nat
@@ -200,6 +211,12 @@ def main():
         4 * 3 * 8 * n_iron + 21 * 4 * N + 6 * 8 * N
 
     log('roofline: spmv %.4f ms (%.0f GB/s), element kernel %.3f ms' % (ms.value, spmv_gbs, asm_ms))
+    traffic = None
+    tfile = os.path.join(ROOT, 'profiles', 'cg_spmv_f32_dipole1M_traffic.json')
+    if os.path.exists(tfile):
+        tr = json.load(open(tfile))
+        if tr.get('tets') == N and tr.get('lanes_per_row') == args.lanes:
+            traffic = tr['hbm_bytes_per_launch']
     cpu = None
     if not args.no_cpu:
         log('cpu baseline leg')
@@ -239,7 +256,7 @@ def main():
         'newton_steps_fp64': st64['newton_steps'],
         'cg_iterations_fp64': st64['cg_iterations'],
         'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': None,
+                     'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': traffic,
                      'kernel': 'cg_spmv_kernel<float,%d>' % args.lanes,
                      'avg_ms': ms.value, 'algorithmic_bytes': spmv_bytes},
         'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
