@@ -165,6 +165,9 @@ int sn_dot_f64(int n, const double *a_dev, const double *b_dev,
                const unsigned char *mask_dev, double *out_host, void *stream);
 int sn_absmax_f64(int n, const double *a_dev, const unsigned char *mask_dev,
                   double *out_host, void *stream);
+/* out_host[v] = sum over rows [row0[v], row0[v+1]) with mask != 0 of a*b. */
+int sn_seg_dot_f64(int nseg, const int *row0_dev, const double *a_dev, const double *b_dev,
+                   const unsigned char *mask_dev, double *out_host, void *stream);
 
 /* ------------------------------------------------------------------- PCG */
 
@@ -177,6 +180,23 @@ typedef struct sn_cg sn_cg;
 int sn_cg_create(sn_cg **out, const sn_pattern *pat, int precision,
                  int lanes_per_row, int chunk, void *stream);
 void sn_cg_destroy(sn_cg *cg);
+
+/* Workspace for a batch of independent systems stored as one block-diagonal
+ * matrix: segment v owns rows [seg_row0_host[v], seg_row0_host[v+1]) and no
+ * entry couples two segments.  Each segment runs its own CG (own alpha, beta,
+ * tolerance and stop flag) inside the same kernels and the same captured
+ * graph; blocks are shared out over the segments in proportion to their
+ * rows. */
+int sn_cg_create_batched(sn_cg **out, const sn_pattern *pat, int precision, int lanes_per_row,
+                         int chunk, int nseg, const int *seg_row0_host, void *stream);
+/* sn_cg_solve for every segment at once: rtol_host [nseg]; per-segment
+ * iteration counts, final residual norms and status (SN_OK,
+ * SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN) to the host arrays (may be NULL).
+ * Returns the worst status. */
+int sn_cg_solve_batched(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
+                        const double *b_dev, double *x_dev, const double *rtol_host, double atol,
+                        int maxiter, int *iters_host, double *rnorm_host, int *status_host,
+                        void *stream);
 
 /* Jacobi-PCG on the system restricted to mask != 0, scipy 1.15.3 cg
  * semantics as driven by snoopy/red_mvp_solver.py:195-205:
@@ -200,6 +220,39 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
  * scratch vectors). */
 int sn_cg_time_spmv(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev, int reps,
                     double *avg_ms_host, void *stream);
+
+/* ------------------------------------------------- learned warm start
+ * No counterpart in the reference (it starts Newton from A = 0 or a given
+ * x_0, snoopy/red_mvp_solver.py:146-147,174); these kernels train and run the
+ * network that predicts x_0. */
+
+/* C[M,N] = op(A)[M,K] op(B)[K,N] on the BF16 matrix cores with FP32
+ * accumulation.  ta/tb = 1 read A/B transposed (A stored [K][lda], B stored
+ * [N][ldb]).  Epilogue: + bias[N] (may be NULL); epi 1: Zout = pre-activation
+ * (may be NULL), C = GELU; epi 2: C *= GELU'(Zin).  Outputs Cf (float32)
+ * and/or Cb (bfloat16), leading dimension ldc. */
+int sn_gemm_bf16(int ta, int tb, int epi, int M, int N, int K, const void *A_dev, int lda,
+                 const void *B_dev, int ldb, const float *bias_dev, const float *Zin_dev,
+                 float *Zout_dev, float *Cf_dev, void *Cb_dev, int ldc, void *stream);
+/* dY = 2 (Y - T) scale (float32 and bfloat16 copies); mean squared error to
+ * loss_host (may be NULL, then no host synchronisation). */
+int sn_mse_grad(int n, const float *Y_dev, const float *T_dev, float scale, float *dYf_dev,
+                void *dYb_dev, double *loss_host, void *stream);
+/* db[n] = sum_m dZ[m][n] */
+int sn_colsum_f32(int M, int N, const float *dZ_dev, float *db_dev, void *stream);
+/* one Adam step (bias-corrected, step >= 1, decoupled weight decay wd) on
+ * float32 parameters; pb_dev (bfloat16 copy, may be NULL) is refreshed. */
+int sn_adam_f32(int n, float *p_dev, const float *g_dev, float *m1_dev, float *m2_dev,
+                void *pb_dev, float lr, float b1, float b2, float eps, int step, float wd,
+                void *stream);
+int sn_f32_to_bf16(int n, const float *x_dev, void *y_dev, void *stream);
+/* out[i] = mean[i] + sum_j U[j*n + i] c[j]  (mean may be NULL) */
+int sn_pod_expand(int n, int r, const float *U_dev, const float *c_dev, const double *mean_dev,
+                  double *out_dev, void *stream);
+/* x_f[i] = sum_k w[6i+k] x_c[idx[6i+k]] (idx < 0 skipped): edge DOFs of one
+ * mesh interpolated from another (Whitney interpolation weights). */
+int sn_edge_transfer(int nf, const int *idx_dev, const double *w_dev, const double *xc_dev,
+                     double *xf_dev, void *stream);
 
 #ifdef __cplusplus
 }

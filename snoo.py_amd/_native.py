@@ -39,12 +39,26 @@ SIGNATURES = {
     'sn_csr_gather_one': (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
     'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),
+    'sn_seg_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_absmax_f64': (_c_int, [_c_int, _vp, _vp, _P_dbl, _vp]),
     'sn_cg_create': (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int, _c_int, _vp]),
     'sn_cg_destroy': (None, [_vp]),
+    'sn_cg_create_batched': (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int, _c_int, _c_int,
+                                      _vp, _vp]),
+    'sn_cg_solve_batched': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp, _vp, _vp,
+                                     _vp]),
     'sn_cg_time_spmv': (_c_int, [_vp, _vp, _vp, _c_int, _P_dbl, _vp]),
     'sn_cg_solve': (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_dbl, _c_dbl, _c_int, _P_int, _P_dbl,
                              _vp]),
+    'sn_gemm_bf16': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _vp,
+                              _c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    'sn_mse_grad': (_c_int, [_c_int, _vp, _vp, ctypes.c_float, _vp, _vp, _P_dbl, _vp]),
+    'sn_colsum_f32': (_c_int, [_c_int, _c_int, _vp, _vp, _vp]),
+    'sn_adam_f32': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, ctypes.c_float, _c_int, ctypes.c_float, _vp]),
+    'sn_f32_to_bf16': (_c_int, [_c_int, _vp, _vp, _vp]),
+    'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_edge_transfer': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
 }
 
 SN_OK, SN_ERR_ARG, SN_ERR_HIP, SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN = 0, 1, 2, 3, 4

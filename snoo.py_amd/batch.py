@@ -1,0 +1,208 @@
+"""Many magnet variants solved together on one GPU.
+
+The variants' meshes (any sizes) are joined into one disjoint mesh: node,
+element and edge ids are offset per variant, so the assembled Jacobian is
+block diagonal with one block per variant and each variant's DOFs form one
+contiguous row segment.  One element-kernel launch per (variant, domain), one
+deterministic scatter for the whole batch, and one segment-aware CG
+(``sn_cg_create_batched``) whose captured chunk graph advances every
+variant's own CG (own alpha, beta, tolerance, stop flag) per replay.  Newton
+runs in lock step: every variant gets its own forcing term and stop test; a
+converged variant is frozen (its right-hand side is zero, so its CG segment
+stops at once).  Residuals are FP64, the Jacobian/CG FP32 ("mixed") or FP64.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .curl_curl_assembler import CurlCurlAssembler
+from .device import as_dev_i32, device
+from .materials import NU0, ConstantReluctance
+from .mesh import IRON_TAG, TetMesh, TetModel
+from .sources import source_field_device
+
+
+def join_meshes(meshes):
+    """Disjoint union of TetMesh objects (tags of mesh v: 2v+1 iron, 2v+2 air)."""
+    nodes, cells, tags, edges, gids, orient, bnd = [], [], [], [], [], [], []
+    node_off = edge_off = 0
+    edge_row0 = [0]
+    for v, m in enumerate(meshes):
+        nodes.append(m.nodes)
+        cells.append(m.cells + node_off)
+        tags.append(np.where(m.tags == IRON_TAG, 2 * v + 1, 2 * v + 2).astype(np.int32))
+        edges.append(m.edges + node_off)
+        gids.append(m.glob_ids + edge_off)
+        orient.append(m.orient)
+        bnd.append(m.boundary_edges + edge_off)
+        node_off += m.num_nodes
+        edge_off += m.num_edges
+        edge_row0.append(edge_off)
+    j = TetMesh.__new__(TetMesh)
+    j.nodes = np.ascontiguousarray(np.concatenate(nodes))
+    j.cells = np.ascontiguousarray(np.concatenate(cells), dtype=np.int32)
+    j.tags = np.concatenate(tags)
+    j.edges = np.ascontiguousarray(np.concatenate(edges), dtype=np.int32)
+    j.glob_ids = np.ascontiguousarray(np.concatenate(gids), dtype=np.int32)
+    j.orient = np.concatenate(orient).astype(np.int32)
+    j.boundary_edges = np.concatenate(bnd).astype(np.int32)
+    j.grid = None
+    return j, np.array(edge_row0, dtype=np.int64)
+
+
+class BatchedSolver:
+    """Newton for a list of Dipole objects, all on the device at once."""
+
+    def __init__(self, dipoles, quad_order=2, precision='mixed', lanes_per_row=8, chunk=32):
+        import torch
+        self.dipoles = list(dipoles)
+        self.nvar = len(self.dipoles)
+        self.quad_order = quad_order
+        self.f32 = precision == 'mixed'
+        self.precision = precision
+        t0 = time.perf_counter()
+        self.mesh, self.row0 = join_meshes([d.mesh for d in self.dipoles])
+        self.model = TetModel(self.mesh)
+        tags, mats = [], []
+        for v, d in enumerate(self.dipoles):
+            tags += [2 * v + 1, 2 * v + 2]
+            mats += [d.reluctance, ConstantReluctance(NU0)]
+        self.asm = CurlCurlAssembler(self.model.mesh, tags, mats)
+        d = self.asm._device_state()
+        self.n = self.asm.num_dofs
+        mask = np.ones(self.n, dtype=np.uint8)
+        mask[self.mesh.boundary_edges] = 0
+        self.mask = torch.from_numpy(mask).to(device())
+        self.row0_dev = as_dev_i32(self.row0)
+        N, nnz = d['num_el'], d['nnz']
+        jt = torch.float32 if self.f32 else torch.float64
+        self.locJ = torch.empty(21 * N, dtype=jt, device=device())
+        self.loc6 = torch.empty(6 * N, dtype=torch.float64, device=device())
+        self.J = torch.empty(nnz, dtype=jt, device=device())
+        self.R = torch.empty(self.n, dtype=torch.float64, device=device())
+        self.b = torch.empty(self.n, dtype=torch.float64, device=device())
+        self.cg = ctypes.c_void_p()
+        row0 = (ctypes.c_int * (self.nvar + 1))(*[int(r) for r in self.row0])
+        nat.check(nat.load().sn_cg_create_batched(ctypes.byref(self.cg), d['pattern'],
+                                                  32 if self.f32 else 64, lanes_per_row, chunk,
+                                                  self.nvar, row0, nat.stream_handle()))
+        self.t_setup = time.perf_counter() - t0
+        self.src = None
+
+    def __del__(self):
+        try:
+            if self.cg and nat._lib is not None:
+                nat._lib.sn_cg_destroy(self.cg)
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ pieces
+    def source_fields(self):
+        """Biot-Savart of each variant's coils at its iron quadrature points."""
+        pts = self.asm.get_quadrature_points_device(self.quad_order,
+                                                    [2 * v for v in range(self.nvar)])
+        src = []
+        for v, d in enumerate(self.dipoles):
+            src += [source_field_device(d.coils, pts[v]).reshape(-1), None]
+        self.src = src
+        return src
+
+    def residual(self, x):
+        """R(x) for all variants (FP64) and the Jacobian; per-variant ||R||."""
+        lib = nat.load()
+        d = self.asm._device_state()
+        self.asm.newton_local_device(x, self.quad_order, self.src, 1, self.f32, self.locJ,
+                                     self.loc6)
+        nat.check(lib.sn_csr_gather_one(d['pattern'], int(self.f32), nat.ptr(self.locJ),
+                                        nat.ptr(self.J), nat.stream_handle()))
+        nat.check(lib.sn_dof_gather_f64(d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
+                                        nat.stream_handle()))
+        out = (ctypes.c_double * self.nvar)()
+        nat.check(lib.sn_seg_dot_f64(self.nvar, nat.ptr(self.row0_dev), nat.ptr(self.R),
+                                     nat.ptr(self.R), nat.ptr(self.mask), out,
+                                     nat.stream_handle()))
+        return np.sqrt(np.array(out[:]))
+
+    def _segment_mask(self, flags):
+        """Per-DOF float64 factor: 1 on the rows of variants with flags[v]."""
+        import torch
+        f = torch.from_numpy(np.repeat(np.asarray(flags, dtype=np.float64),
+                                       np.diff(self.row0))).to(device())
+        return f
+
+    # ------------------------------------------------------------ solve
+    def solve(self, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=None, max_cg=-1, x0=None,
+              log=None):
+        """Newton for every variant to ||R_v|| <= rtol ||R_v(0)||."""
+        import torch
+        lib = nat.load()
+        t0 = time.perf_counter()
+        self.source_fields()
+        if eta_min is None:
+            eta_min = 1e-3 if self.f32 else 1e-6
+        if max_cg < 0:
+            max_cg = 5000 if self.f32 else 20000
+        z = torch.zeros(self.n, dtype=torch.float64, device=device())
+        r0 = self.residual(z)
+        x = z.clone() if x0 is None else x0.clone()
+        x[self.mask == 0] = 0.0
+        rn = self.residual(x)
+        hist = [rn / r0]
+        eta = np.full(self.nvar, eta_max)
+        rn_prev = None
+        conv = rn <= rtol * r0
+        cg_its = []
+        k = 0
+        while not conv.all() and k < max_newton:
+            if rn_prev is not None:
+                eta = np.minimum(eta_max, np.maximum(0.9 * (rn / rn_prev) ** 2, eta_min))
+            eta = np.maximum(eta, np.minimum(eta_max, 0.5 * rtol * r0 / np.maximum(rn, 1e-300)))
+            active = self._segment_mask(~conv)
+            torch.mul(self.R, active, out=self.b)
+            self.b.neg_()
+            rt = (ctypes.c_double * self.nvar)(*[float(e) for e in eta])
+            its = (ctypes.c_int * self.nvar)()
+            if self.f32:
+                x_try = x.clone()
+                rc = lib.sn_cg_solve_batched(self.cg, nat.ptr(self.J), nat.ptr(self.mask),
+                                             nat.ptr(self.b), nat.ptr(x_try), rt, 0.0, max_cg,
+                                             its, None, None, nat.stream_handle())
+            else:
+                dx = torch.zeros_like(x)
+                rc = lib.sn_cg_solve_batched(self.cg, nat.ptr(self.J), nat.ptr(self.mask),
+                                             nat.ptr(self.b), nat.ptr(dx), rt, 0.0, max_cg,
+                                             its, None, None, nat.stream_handle())
+                x_try = x + dx
+            if rc not in (nat.SN_OK, nat.SN_ERR_NOT_CONVERGED):
+                nat.check(rc)
+            cg_its.append(list(its[:]))
+            rn_try = self.residual(x_try)
+            # per-variant backtracking on the residual norm
+            dxs = x_try - x
+            step = np.ones(self.nvar)
+            for _ in range(6):
+                bad = (rn_try >= rn) & ~conv
+                if not bad.any():
+                    break
+                step[bad] *= 0.5
+                x_try = x + self._segment_mask(step) * dxs
+                rn_try = self.residual(x_try)
+            rn_prev, rn, x = rn, np.where(conv, rn, rn_try), x_try
+            k += 1
+            hist.append(rn / r0)
+            conv = rn <= rtol * r0
+            if log is not None:
+                log('  batch newton %d: worst |R|/|R0| %.3e, converged %d/%d, cg max %d, %.3f s'
+                    % (k, float(np.max(rn / r0)), int(conv.sum()), self.nvar,
+                       max(cg_its[-1]), time.perf_counter() - t0))
+        torch.cuda.synchronize()
+        self.stats = {'residual_history': np.array(hist), 'cg_iterations': cg_its,
+                      'newton_steps': k, 'converged': conv, 't_solve': time.perf_counter() - t0}
+        return x
+
+    def split(self, x):
+        """Per-variant host vectors of a joined solution."""
+        h = x.cpu().numpy()
+        return [h[self.row0[v]:self.row0[v + 1]] for v in range(self.nvar)]

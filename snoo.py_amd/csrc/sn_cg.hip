@@ -28,20 +28,30 @@
 namespace sn {
 
 struct CgState {
-    double rho[2];   // rho of even / odd iterations (written by kernel A, block 0)
+    double rho[2];   // rho of even / odd iterations (written by the segment's first A block)
     double pq, rr, tol, bnorm;
     int it, maxiter, done, pad;  // done: 0 running, 1 converged, 2 iteration cap, 3 breakdown
+};
+
+// Row segments of a batched solve (one segment per independent system of a
+// block-diagonal matrix) and the blocks of each grid that serve them.
+struct SegMap {
+    const int *row0;   // [nseg+1] first row of each segment
+    const int *blkA0;  // [nseg+1] first block of each segment in kernel A's grid
+    const int *blkB0;  // [nseg+1] first block of each segment in kernel B's / init's grid
+    const int *segA;   // [grid A] segment of each block
+    const int *segB;   // [grid B] segment of each block
 };
 
 constexpr int kRowBlock = 256;   // kernel A (SpMV) block
 constexpr int kVecBlock = 1024;  // kernel B and init block
 
-// Fixed-order block sum of partials[k*stride + off], k in [0, nb).
+// Fixed-order block sum of partials[k*stride + off], k in [b0, b1).
 template <int BLOCK>
-__device__ __forceinline__ double reduce_partials(const double *__restrict__ partials, int nb,
-                                                  int stride, int off, double *lds) {
+__device__ __forceinline__ double reduce_range(const double *__restrict__ partials, int b0, int b1,
+                                               int stride, int off, double *lds) {
     double s = 0.0;
-    for (int k = threadIdx.x; k < nb; k += BLOCK) s += partials[k * stride + off];
+    for (int k = b0 + threadIdx.x; k < b1; k += BLOCK) s += partials[k * stride + off];
     return block_sum(s, lds);
 }
 
@@ -50,17 +60,20 @@ __device__ __forceinline__ double reduce_partials(const double *__restrict__ par
 // per-block partials: (r.z, r.r) in the layout kernel B uses, b.b separately.
 template <typename V, int LPR>
 __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
-    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
     const int *__restrict__ diag, const V *__restrict__ vals, const unsigned char *__restrict__ mask,
     const double *__restrict__ b, const double *__restrict__ x0, V *__restrict__ r,
     V *__restrict__ z, V *__restrict__ dinv, V *__restrict__ pold, float *__restrict__ xs,
     double *__restrict__ partB, double *__restrict__ partbb) {
     __shared__ double lds[kVecBlock / kWave];
+    const int seg = sm.segB[blockIdx.x];
+    const int lb = blockIdx.x - sm.blkB0[seg], nbl = sm.blkB0[seg + 1] - sm.blkB0[seg];
+    const int r0 = sm.row0[seg], r1 = sm.row0[seg + 1];
     const int lane = threadIdx.x % LPR;
     const int rows_per_block = kVecBlock / LPR;
     double bb = 0.0, rr = 0.0, rz = 0.0;
-    for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
-         row += (long long)gridDim.x * rows_per_block) {
+    for (long long row = r0 + (long long)lb * rows_per_block + threadIdx.x / LPR; row < r1;
+         row += (long long)nbl * rows_per_block) {
         const int i = (int)row;
         const bool free_row = mask ? mask[i] != 0 : true;
         double acc = 0.0;
@@ -102,23 +115,25 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
     }
 }
 
-// one block: tol, ||b||, state for iteration 0
+// one block per segment: tol, ||b||, state for iteration 0
 __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
-    int nb, const double *__restrict__ partbb, CgState *__restrict__ st, double rtol,
-    double atol, int maxiter) {
+    SegMap sm, const double *__restrict__ partbb, CgState *__restrict__ st,
+    const double *__restrict__ rtol, double atol, int maxiter) {
     __shared__ double lds[kVecBlock / kWave];
-    const double bb = reduce_partials<kVecBlock>(partbb, nb, 1, 0, lds);
+    const int seg = blockIdx.x;
+    const double bb = reduce_range<kVecBlock>(partbb, sm.blkB0[seg], sm.blkB0[seg + 1], 1, 0, lds);
     if (threadIdx.x == 0) {
+        CgState *S = st + seg;
         const double bnorm = sqrt(bb);
-        const double rt = rtol * bnorm;
-        st->bnorm = bnorm;
-        st->tol = atol > rt ? atol : rt;  // _get_atol_rtol
-        st->rho[0] = st->rho[1] = 0.0;
-        st->pq = 0.0;
-        st->rr = 0.0;
-        st->it = 0;
-        st->maxiter = maxiter;
-        st->done = (bnorm == 0.0) ? 1 : 0;
+        const double rt = rtol[seg] * bnorm;
+        S->bnorm = bnorm;
+        S->tol = atol > rt ? atol : rt;  // _get_atol_rtol
+        S->rho[0] = S->rho[1] = 0.0;
+        S->pq = 0.0;
+        S->rr = 0.0;
+        S->it = 0;
+        S->maxiter = maxiter;
+        S->done = (bnorm == 0.0) ? 1 : 0;
     }
 }
 
@@ -126,31 +141,36 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
 // parity = iteration & 1: reads rho[parity ^ 1] (previous), writes rho[parity].
 template <typename V, int LPR>
 __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
-    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
     const V *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
     const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
-    const double *__restrict__ partB, int nbB, double *__restrict__ partA,
-    CgState *__restrict__ st, int parity) {
+    const double *__restrict__ partB, double *__restrict__ partA, CgState *__restrict__ st,
+    int parity) {
     __shared__ double lds[kRowBlock / kWave];
-    if (st->done) return;
-    const double rho = reduce_partials<kRowBlock>(partB, nbB, 2, 0, lds);
-    const double rr = reduce_partials<kRowBlock>(partB, nbB, 2, 1, lds);
-    const int it = st->it;
+    const int seg = sm.segA[blockIdx.x];
+    CgState *S = st + seg;
+    if (S->done) return;
+    const int lb = blockIdx.x - sm.blkA0[seg], nbl = sm.blkA0[seg + 1] - sm.blkA0[seg];
+    const int b0 = sm.blkB0[seg], b1 = sm.blkB0[seg + 1];
+    const double rho = reduce_range<kRowBlock>(partB, b0, b1, 2, 0, lds);
+    const double rr = reduce_range<kRowBlock>(partB, b0, b1, 2, 1, lds);
+    const int it = S->it;
     // scipy: the iteration cap ends the loop before the residual test
-    const int stop = (it >= st->maxiter) ? 2 : ((sqrt(rr) < st->tol) ? 1 : 0);
+    const int stop = (it >= S->maxiter) ? 2 : ((sqrt(rr) < S->tol) ? 1 : 0);
     if (stop) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->rr = rr;
-            st->done = stop;
+        if (lb == 0 && threadIdx.x == 0) {
+            S->rr = rr;
+            S->done = stop;
         }
         return;
     }
-    const V beta = (V)(it > 0 ? rho / st->rho[parity ^ 1] : 0.0);
+    const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    const int r0 = sm.row0[seg], r1 = sm.row0[seg + 1];
     const int lane = threadIdx.x % LPR;
     const int rows_per_block = kRowBlock / LPR;
     double pq = 0.0;
-    for (long long row = (long long)blockIdx.x * rows_per_block + threadIdx.x / LPR; row < n;
-         row += (long long)gridDim.x * rows_per_block) {
+    for (long long row = r0 + (long long)lb * rows_per_block + threadIdx.x / LPR; row < r1;
+         row += (long long)nbl * rows_per_block) {
         const int i = (int)row;
         V acc = 0;
         const int k1 = indptr[i + 1];
@@ -172,9 +192,9 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     const double s = block_sum(pq, lds);
     if (threadIdx.x == 0) {
         partA[blockIdx.x] = s;
-        if (blockIdx.x == 0) {
-            st->rho[parity] = rho;
-            st->rr = rr;
+        if (lb == 0) {
+            S->rho[parity] = rho;
+            S->rr = rr;
         }
     }
 }
@@ -182,25 +202,29 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
 // ---------------------------------------------------------------- kernel B
 template <typename V, typename X>
 __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
-    int n, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
+    SegMap sm, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
     const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r, V *__restrict__ z,
-    X *__restrict__ x, const double *__restrict__ partA, int nbA, double *__restrict__ partB,
+    X *__restrict__ x, const double *__restrict__ partA, double *__restrict__ partB,
     CgState *__restrict__ st, int parity) {
     __shared__ double lds[kVecBlock / kWave];
-    if (st->done) return;
-    const double pq = reduce_partials<kVecBlock>(partA, nbA, 1, 0, lds);
+    const int seg = sm.segB[blockIdx.x];
+    CgState *S = st + seg;
+    if (S->done) return;
+    const int lb = blockIdx.x - sm.blkB0[seg], nbl = sm.blkB0[seg + 1] - sm.blkB0[seg];
+    const double pq = reduce_range<kVecBlock>(partA, sm.blkA0[seg], sm.blkA0[seg + 1], 1, 0, lds);
     if (!(pq == pq) || pq == 0.0 || isinf(pq)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            st->pq = pq;
-            st->done = 3;
+        if (lb == 0 && threadIdx.x == 0) {
+            S->pq = pq;
+            S->done = 3;
         }
         return;
     }
-    const double alpha_d = st->rho[parity] / pq;
+    const double alpha_d = S->rho[parity] / pq;
     const V alpha = (V)alpha_d;
+    const int r0 = sm.row0[seg], r1 = sm.row0[seg + 1];
     double rz = 0.0, rr = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
+    for (long long i = r0 + (long long)lb * blockDim.x + threadIdx.x; i < r1;
+         i += (long long)nbl * blockDim.x) {
         const bool free_row = mask ? mask[i] != 0 : true;
         if (!free_row) continue;
         x[i] += (X)(alpha * pnew[i]);
@@ -216,9 +240,9 @@ __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
     if (threadIdx.x == 0) {
         partB[2 * blockIdx.x] = s1;
         partB[2 * blockIdx.x + 1] = s2;
-        if (blockIdx.x == 0) {
-            st->pq = pq;
-            st->it = st->it + 1;
+        if (lb == 0) {
+            S->pq = pq;
+            S->it = S->it + 1;
         }
     }
 }
@@ -250,12 +274,29 @@ __global__ void fill_free_kernel(int n, const unsigned char *__restrict__ mask, 
     p[i] = (V)0;
 }
 
-// partials that reduce to (rho = 1, ||r||^2 = 1) for the SpMV timing helper
-__global__ void fill_partials_kernel(int nb, double *__restrict__ partB) {
+// partials that reduce to (rho = 1, ||r||^2 = 1) in every segment, for the
+// SpMV timing helper
+__global__ void fill_partials_kernel(SegMap sm, int nb, double *__restrict__ partB) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
-    partB[2 * k] = (k == 0) ? 1.0 : 0.0;
-    partB[2 * k + 1] = (k == 0) ? 1.0 : 0.0;
+    const bool first = sm.blkB0[sm.segB[k]] == k;
+    partB[2 * k] = first ? 1.0 : 0.0;
+    partB[2 * k + 1] = first ? 1.0 : 0.0;
+}
+
+// one block per segment: out[v] = sum over rows of segment v (mask != 0) of a*b
+__global__ __launch_bounds__(kVecBlock) void seg_dot_kernel(const int *__restrict__ row0,
+                                                            const double *__restrict__ a,
+                                                            const double *__restrict__ b,
+                                                            const unsigned char *__restrict__ mask,
+                                                            double *__restrict__ out) {
+    __shared__ double lds[kVecBlock / kWave];
+    const int v = blockIdx.x;
+    double s = 0.0;
+    for (int i = row0[v] + threadIdx.x; i < row0[v + 1]; i += kVecBlock)
+        if (!mask || mask[i]) s += a[i] * b[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) out[v] = s;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -316,19 +357,25 @@ struct sn_cg {
     int precision = 64;
     int lpr = 8;
     int chunk = 32;
+    int nseg = 1;
     int grid_rows = 0;   // blocks of kernel A
     int grid_vec = 0;    // blocks of kernel B and init
+    std::vector<int> row0_h;            // [nseg+1]
+    int *row0 = nullptr, *blkA0 = nullptr, *blkB0 = nullptr, *segA = nullptr, *segB = nullptr;
+    double *rtol_dev = nullptr;         // [nseg]
     void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
     float *xs = nullptr;
     double *partA = nullptr, *partB = nullptr, *partbb = nullptr;
-    CgState *st = nullptr;
-    CgState *st_host = nullptr;  // pinned
+    CgState *st = nullptr;       // [nseg]
+    CgState *st_host = nullptr;  // [nseg], pinned
     // cached graph of one chunk (2*chunk kernels), keyed by its pointers
     hipGraphExec_t graph = nullptr;
     const void *g_vals = nullptr;
     const unsigned char *g_mask = nullptr;
     void *g_x = nullptr;
     hipStream_t g_stream = nullptr;
+
+    SegMap map() const { return SegMap{row0, blkA0, blkB0, segA, segB}; }
 };
 
 namespace sn {
@@ -339,54 +386,56 @@ static void launch_spmv(sn_cg *C, const V *vals, const unsigned char *mask, int 
     const sn_pattern *P = C->pat;
     V *pold = (V *)(parity ? C->p1 : C->p0);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
-    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s, P->n,
-                       P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew, (V *)C->q,
-                       C->partB, C->grid_vec, C->partA, C->st, parity);
+    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s,
+                       C->map(), P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew,
+                       (V *)C->q, C->partB, C->partA, C->st, parity);
 }
 
 template <typename V>
-static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
-                             int parity, hipStream_t s) {
-    const sn_pattern *P = C->pat;
+static void launch_spmv_lpr(sn_cg *C, const V *vals, const unsigned char *mask, int parity,
+                            hipStream_t s) {
     switch (C->lpr) {
         case 4: launch_spmv<V, 4>(C, vals, mask, parity, s); break;
         case 16: launch_spmv<V, 16>(C, vals, mask, parity, s); break;
         default: launch_spmv<V, 8>(C, vals, mask, parity, s); break;
     }
+}
+
+template <typename V>
+static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
+                             int parity, hipStream_t s) {
+    launch_spmv_lpr<V>(C, vals, mask, parity, s);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
     if (C->precision == 64)
         hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
-                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (double *)xdst, C->partA, C->grid_rows, C->partB, C->st,
-                           parity);
+                           C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (double *)xdst, C->partA, C->partB, C->st, parity);
     else
         hipLaunchKernelGGL((cg_update_kernel<V, float>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
-                           P->n, mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (float *)xdst, C->partA, C->grid_rows, C->partB, C->st,
-                           parity);
+                           C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
+                           (V *)C->z, (float *)xdst, C->partA, C->partB, C->st, parity);
 }
 
 template <typename V, int LPR>
 static void launch_init(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
                         const double *x0, hipStream_t s) {
     const sn_pattern *P = C->pat;
-    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s, P->n,
-                       P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r, (V *)C->z,
-                       (V *)C->dinv, (V *)C->p0, C->precision == 32 ? C->xs : (float *)nullptr,
-                       C->partB, C->partbb);
+    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
+                       C->map(), P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r,
+                       (V *)C->z, (V *)C->dinv, (V *)C->p0,
+                       C->precision == 32 ? C->xs : (float *)nullptr, C->partB, C->partbb);
 }
 
 template <typename V>
 static void launch_init_lpr(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
-                            const double *x0, double rtol, double atol, int maxiter,
-                            hipStream_t s) {
+                            const double *x0, double atol, int maxiter, hipStream_t s) {
     switch (C->lpr) {
         case 4: launch_init<V, 4>(C, vals, mask, b, x0, s); break;
         case 16: launch_init<V, 16>(C, vals, mask, b, x0, s); break;
         default: launch_init<V, 8>(C, vals, mask, b, x0, s); break;
     }
-    hipLaunchKernelGGL(cg_init_state_kernel, dim3(1), dim3(kVecBlock), 0, s, C->grid_vec,
-                       C->partbb, C->st, rtol, atol, maxiter);
+    hipLaunchKernelGGL(cg_init_state_kernel, dim3(C->nseg), dim3(kVecBlock), 0, s, C->map(),
+                       C->partbb, C->st, C->rtol_dev, atol, maxiter);
 }
 
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
@@ -400,71 +449,34 @@ static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, 
     }
 }
 
-}  // namespace sn
+// Blocks of a grid of about `target` blocks shared out over the segments in
+// proportion to their rows (at least one, at most one per `rows_per_block`).
+static void share_blocks(const std::vector<int> &row0, int target, int rows_per_block,
+                         std::vector<int> &blk0, std::vector<int> &seg_of) {
+    const int nseg = (int)row0.size() - 1;
+    const long long n = row0[nseg] - row0[0];
+    blk0.assign(nseg + 1, 0);
+    seg_of.clear();
+    for (int v = 0; v < nseg; ++v) {
+        const long long nv = row0[v + 1] - row0[v];
+        long long b = (nv * target + n - 1) / (n > 0 ? n : 1);
+        const long long cap = (nv + rows_per_block - 1) / rows_per_block;
+        if (b > cap) b = cap;
+        if (b < 1) b = 1;
+        blk0[v + 1] = blk0[v] + (int)b;
+        for (long long k = 0; k < b; ++k) seg_of.push_back(v);
+    }
+}
 
-extern "C" {
-
-int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
-                 void *stream) {
-    SN_REQUIRE(out && P, "sn_cg_create: null argument");
-    SN_REQUIRE(precision == 64 || precision == 32, "sn_cg_create: precision must be 64 or 32");
-    if (lpr == 0) lpr = 8;
-    SN_REQUIRE(lpr == 4 || lpr == 8 || lpr == 16, "sn_cg_create: lanes_per_row in {4,8,16}");
-    if (chunk <= 0) chunk = 32;
-    if (chunk & 1) chunk += 1;
-    sn_cg *C = new sn_cg();
-    C->pat = P;
-    C->precision = precision;
-    C->lpr = lpr;
-    C->chunk = chunk;
-    const int cus = num_cus();
-    C->grid_rows = grid_for(P->n, kRowBlock / lpr, 8 * cus);
-    C->grid_vec = grid_for(P->n, kVecBlock, cus);
-    const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
-    SN_HIP(hipMalloc(&C->r, vb));
-    SN_HIP(hipMalloc(&C->z, vb));
-    SN_HIP(hipMalloc(&C->p0, vb));
-    SN_HIP(hipMalloc(&C->p1, vb));
-    SN_HIP(hipMalloc(&C->q, vb));
-    SN_HIP(hipMalloc(&C->dinv, vb));
-    if (precision == 32) SN_HIP(hipMalloc(&C->xs, (size_t)P->n * 4));
-    SN_HIP(hipMalloc(&C->partA, (size_t)C->grid_rows * sizeof(double)));
-    SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * sizeof(double)));
-    SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * sizeof(double)));
-    SN_HIP(hipMalloc(&C->st, sizeof(CgState)));
-    SN_HIP(hipHostMalloc((void **)&C->st_host, sizeof(CgState), hipHostMallocDefault));
-    std::memset(C->st_host, 0, sizeof(CgState));
-    SN_HIP(hipMemcpyAsync(C->st, C->st_host, sizeof(CgState), hipMemcpyHostToDevice,
-                          as_stream(stream)));
-    SN_HIP(hipStreamSynchronize(as_stream(stream)));
-    *out = C;
+static int upload_ints(int **dst, const std::vector<int> &h) {
+    SN_HIP(hipMalloc(dst, h.size() * sizeof(int)));
+    SN_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
     return SN_OK;
 }
 
-void sn_cg_destroy(sn_cg *C) {
-    if (!C) return;
-    if (C->graph) (void)hipGraphExecDestroy(C->graph);
-    (void)hipFree(C->r);
-    (void)hipFree(C->z);
-    (void)hipFree(C->p0);
-    (void)hipFree(C->p1);
-    (void)hipFree(C->q);
-    (void)hipFree(C->dinv);
-    if (C->xs) (void)hipFree(C->xs);
-    (void)hipFree(C->partA);
-    (void)hipFree(C->partB);
-    (void)hipFree(C->partbb);
-    (void)hipFree(C->st);
-    (void)hipHostFree(C->st_host);
-    delete C;
-}
-
-int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const double *b, double *x,
-                double rtol, double atol, int maxiter, int *iters_out, double *rnorm_out,
-                void *stream) {
-    SN_REQUIRE(C && vals && b && x, "sn_cg_solve: null argument");
+static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const double *b,
+                  double *x, const double *rtol_host, double atol, int maxiter, hipStream_t s) {
     const sn_pattern *P = C->pat;
-    hipStream_t s = as_stream(stream);
     if (maxiter < 0) {
         long long nfree = P->n;
         if (mask) {
@@ -476,12 +488,14 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
         }
         maxiter = (int)(nfree * 10 > INT_MAX ? INT_MAX : nfree * 10);
     }
+    SN_HIP(hipMemcpyAsync(C->rtol_dev, rtol_host, C->nseg * sizeof(double),
+                          hipMemcpyHostToDevice, s));
     void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
     const double *x0 = (C->precision == 64) ? x : nullptr;
     if (C->precision == 64)
-        launch_init_lpr<double>(C, (const double *)vals, mask, b, x0, rtol, atol, maxiter, s);
+        launch_init_lpr<double>(C, (const double *)vals, mask, b, x0, atol, maxiter, s);
     else
-        launch_init_lpr<float>(C, (const float *)vals, mask, b, x0, rtol, atol, maxiter, s);
+        launch_init_lpr<float>(C, (const float *)vals, mask, b, x0, atol, maxiter, s);
     SN_CHECK_LAUNCH();
 
     // (re)capture the chunk graph when the pointers or stream changed
@@ -508,24 +522,123 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
         C->g_x = xdst;
         C->g_stream = s;
     }
-    SN_HIP(hipMemcpyAsync(C->st_host, C->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    const size_t sb = C->nseg * sizeof(CgState);
+    SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
     SN_HIP(hipStreamSynchronize(s));
-    while (C->st_host->done == 0) {
+    for (;;) {
+        bool running = false;
+        for (int v = 0; v < C->nseg; ++v) running |= (C->st_host[v].done == 0);
+        if (!running) break;
         SN_HIP(hipGraphLaunch(C->graph, s));
-        SN_HIP(hipMemcpyAsync(C->st_host, C->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
         SN_HIP(hipStreamSynchronize(s));
     }
-    const CgState h = *C->st_host;
-    if (h.bnorm == 0.0) {
+    if (C->precision == 64) {
         // scipy returns b itself (all zeros) when ||b|| == 0
-        if (C->precision == 64)
-            hipLaunchKernelGGL(zero_free_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
-                               mask, x);
-    } else if (C->precision == 32) {
+        for (int v = 0; v < C->nseg; ++v) {
+            if (C->st_host[v].bnorm != 0.0) continue;
+            const int r0 = C->row0_h[v], nv = C->row0_h[v + 1] - r0;
+            if (nv > 0)
+                hipLaunchKernelGGL(zero_free_kernel, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv,
+                                   mask ? mask + r0 : nullptr, x + r0);
+        }
+    } else {
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
                            mask, C->xs, x);
     }
     SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+}  // namespace sn
+
+extern "C" {
+
+int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
+                         int nseg, const int *seg_row0_host, void *stream) {
+    SN_REQUIRE(out && P, "sn_cg_create: null argument");
+    SN_REQUIRE(precision == 64 || precision == 32, "sn_cg_create: precision must be 64 or 32");
+    if (lpr == 0) lpr = 8;
+    SN_REQUIRE(lpr == 4 || lpr == 8 || lpr == 16, "sn_cg_create: lanes_per_row in {4,8,16}");
+    SN_REQUIRE(nseg >= 1, "sn_cg_create: nseg must be >= 1");
+    std::vector<int> row0(nseg + 1);
+    if (seg_row0_host) {
+        for (int v = 0; v <= nseg; ++v) row0[v] = seg_row0_host[v];
+    } else {
+        SN_REQUIRE(nseg == 1, "sn_cg_create: segment rows missing");
+        row0[0] = 0;
+        row0[1] = P->n;
+    }
+    SN_REQUIRE(row0[0] == 0 && row0[nseg] == P->n, "sn_cg_create: segments must cover [0, n)");
+    for (int v = 0; v < nseg; ++v)
+        SN_REQUIRE(row0[v + 1] >= row0[v], "sn_cg_create: segment rows must be ascending");
+    if (chunk <= 0) chunk = 32;
+    if (chunk & 1) chunk += 1;
+    sn_cg *C = new sn_cg();
+    C->pat = P;
+    C->precision = precision;
+    C->lpr = lpr;
+    C->chunk = chunk;
+    C->nseg = nseg;
+    C->row0_h = row0;
+    const int cus = num_cus();
+    std::vector<int> blkA0, segA, blkB0, segB;
+    share_blocks(row0, 8 * cus, kRowBlock / lpr, blkA0, segA);
+    share_blocks(row0, cus, kVecBlock, blkB0, segB);
+    C->grid_rows = blkA0[nseg];
+    C->grid_vec = blkB0[nseg];
+    int rc;
+    if ((rc = upload_ints(&C->row0, row0))) return rc;
+    if ((rc = upload_ints(&C->blkA0, blkA0))) return rc;
+    if ((rc = upload_ints(&C->blkB0, blkB0))) return rc;
+    if ((rc = upload_ints(&C->segA, segA))) return rc;
+    if ((rc = upload_ints(&C->segB, segB))) return rc;
+    const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
+    SN_HIP(hipMalloc(&C->r, vb));
+    SN_HIP(hipMalloc(&C->z, vb));
+    SN_HIP(hipMalloc(&C->p0, vb));
+    SN_HIP(hipMalloc(&C->p1, vb));
+    SN_HIP(hipMalloc(&C->q, vb));
+    SN_HIP(hipMalloc(&C->dinv, vb));
+    if (precision == 32) SN_HIP(hipMalloc(&C->xs, (size_t)P->n * 4));
+    SN_HIP(hipMalloc(&C->partA, (size_t)C->grid_rows * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * sizeof(double)));
+    SN_HIP(hipMalloc(&C->rtol_dev, (size_t)nseg * sizeof(double)));
+    SN_HIP(hipMalloc(&C->st, nseg * sizeof(CgState)));
+    SN_HIP(hipHostMalloc((void **)&C->st_host, nseg * sizeof(CgState), hipHostMallocDefault));
+    std::memset(C->st_host, 0, nseg * sizeof(CgState));
+    SN_HIP(hipMemcpyAsync(C->st, C->st_host, nseg * sizeof(CgState), hipMemcpyHostToDevice,
+                          as_stream(stream)));
+    SN_HIP(hipStreamSynchronize(as_stream(stream)));
+    *out = C;
+    return SN_OK;
+}
+
+int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
+                 void *stream) {
+    return sn_cg_create_batched(out, P, precision, lpr, chunk, 1, nullptr, stream);
+}
+
+void sn_cg_destroy(sn_cg *C) {
+    if (!C) return;
+    if (C->graph) (void)hipGraphExecDestroy(C->graph);
+    void *bufs[] = {C->r, C->z, C->p0, C->p1, C->q, C->dinv, C->xs, C->partA, C->partB,
+                    C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0, C->blkB0, C->segA, C->segB};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    (void)hipHostFree(C->st_host);
+    delete C;
+}
+
+int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const double *b, double *x,
+                double rtol, double atol, int maxiter, int *iters_out, double *rnorm_out,
+                void *stream) {
+    SN_REQUIRE(C && vals && b && x, "sn_cg_solve: null argument");
+    SN_REQUIRE(C->nseg == 1, "sn_cg_solve: batched workspace, use sn_cg_solve_batched");
+    int rc = cg_run(C, vals, mask, b, x, &rtol, atol, maxiter, as_stream(stream));
+    if (rc) return rc;
+    const CgState h = C->st_host[0];
     if (iters_out) *iters_out = h.it;
     if (rnorm_out) *rnorm_out = sqrt(h.rr);
     if (h.done == 2) {
@@ -540,22 +653,43 @@ int sn_cg_solve(sn_cg *C, const void *vals, const unsigned char *mask, const dou
     return SN_OK;
 }
 
+int sn_cg_solve_batched(sn_cg *C, const void *vals, const unsigned char *mask, const double *b,
+                        double *x, const double *rtol_host, double atol, int maxiter,
+                        int *iters_host, double *rnorm_host, int *status_host, void *stream) {
+    SN_REQUIRE(C && vals && b && x && rtol_host, "sn_cg_solve_batched: null argument");
+    int rc = cg_run(C, vals, mask, b, x, rtol_host, atol, maxiter, as_stream(stream));
+    if (rc) return rc;
+    int worst = SN_OK;
+    for (int v = 0; v < C->nseg; ++v) {
+        const CgState &h = C->st_host[v];
+        if (iters_host) iters_host[v] = h.it;
+        if (rnorm_host) rnorm_host[v] = sqrt(h.rr);
+        const int st = h.done == 2 ? SN_ERR_NOT_CONVERGED : (h.done == 3 ? SN_ERR_BREAKDOWN : SN_OK);
+        if (status_host) status_host[v] = st;
+        if (st == SN_ERR_BREAKDOWN || (st == SN_ERR_NOT_CONVERGED && worst == SN_OK)) worst = st;
+    }
+    if (worst != SN_OK) set_error("sn_cg_solve_batched: a segment did not converge");
+    return worst;
+}
+
 int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int reps,
                     double *avg_ms, void *stream) {
     SN_REQUIRE(C && vals && avg_ms && reps > 0, "sn_cg_time_spmv: bad argument");
     const sn_pattern *P = C->pat;
     hipStream_t s = as_stream(stream);
-    // a running state: rho = 1 now and before (beta = 1), ||r||^2 = 1 above a
-    // zero tolerance, z = 1 on the free rows and p_old = 0
-    CgState h;
-    std::memset(&h, 0, sizeof(h));
-    h.it = 1;
-    h.rho[0] = h.rho[1] = 1.0;
-    h.tol = 0.0;
-    h.maxiter = INT_MAX;
-    SN_HIP(hipMemcpyAsync(C->st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    // a running state in every segment: rho = 1 now and before (beta = 1),
+    // ||r||^2 = 1 above a zero tolerance, z = 1 on the free rows, p_old = 0
+    std::vector<CgState> h(C->nseg);
+    for (auto &e : h) {
+        std::memset(&e, 0, sizeof(e));
+        e.it = 1;
+        e.rho[0] = e.rho[1] = 1.0;
+        e.tol = 0.0;
+        e.maxiter = INT_MAX;
+    }
+    SN_HIP(hipMemcpyAsync(C->st, h.data(), C->nseg * sizeof(CgState), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(fill_partials_kernel, dim3(grid_for(C->grid_vec, 256)), dim3(256), 0, s,
-                       C->grid_vec, C->partB);
+                       C->map(), C->grid_vec, C->partB);
     if (C->precision == 64)
         hipLaunchKernelGGL(fill_free_kernel<double>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
                            P->n, mask, (double *)C->z, (double *)C->p1, 1.0);
@@ -570,19 +704,8 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     // parity 1 reads p1 (zero) and rho[0], writes p0 and rho[1]: the state stays
     // the same for every repetition
     for (int r = 0; r < reps; ++r) {
-        if (C->precision == 64) {
-            switch (C->lpr) {
-                case 4: launch_spmv<double, 4>(C, (const double *)vals, mask, 1, s); break;
-                case 16: launch_spmv<double, 16>(C, (const double *)vals, mask, 1, s); break;
-                default: launch_spmv<double, 8>(C, (const double *)vals, mask, 1, s); break;
-            }
-        } else {
-            switch (C->lpr) {
-                case 4: launch_spmv<float, 4>(C, (const float *)vals, mask, 1, s); break;
-                case 16: launch_spmv<float, 16>(C, (const float *)vals, mask, 1, s); break;
-                default: launch_spmv<float, 8>(C, (const float *)vals, mask, 1, s); break;
-            }
-        }
+        if (C->precision == 64) launch_spmv_lpr<double>(C, (const double *)vals, mask, 1, s);
+        else launch_spmv_lpr<float>(C, (const float *)vals, mask, 1, s);
     }
     SN_HIP(hipEventRecord(e1, s));
     SN_HIP(hipEventSynchronize(e1));
@@ -590,8 +713,10 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     SN_HIP(hipEventElapsedTime(&ms, e0, e1));
     SN_HIP(hipEventDestroy(e0));
     SN_HIP(hipEventDestroy(e1));
-    SN_HIP(hipMemcpy(&h, C->st, sizeof(h), hipMemcpyDeviceToHost));
-    SN_REQUIRE(h.done == 0, "sn_cg_time_spmv: the timed launches stopped early (done=%d)", h.done);
+    SN_HIP(hipMemcpy(h.data(), C->st, C->nseg * sizeof(CgState), hipMemcpyDeviceToHost));
+    for (const auto &e : h)
+        SN_REQUIRE(e.done == 0, "sn_cg_time_spmv: the timed launches stopped early (done=%d)",
+                   e.done);
     *avg_ms = (double)ms / reps;
     return SN_OK;
 }
@@ -623,6 +748,20 @@ static int reduce_host(int n, const double *a, const double *b, const unsigned c
     double r = 0.0;
     for (int k = 0; k < G; ++k) r = (op == 0) ? r + h[k] : (h[k] > r ? h[k] : r);
     *out = r;
+    return SN_OK;
+}
+
+int sn_seg_dot_f64(int nseg, const int *row0_dev, const double *a, const double *b,
+                   const unsigned char *mask, double *out_host, void *stream) {
+    SN_REQUIRE(nseg >= 1 && row0_dev && a && b && out_host, "sn_seg_dot_f64: bad argument");
+    hipStream_t s = as_stream(stream);
+    double *dev = nullptr;
+    SN_HIP(hipMallocAsync(&dev, nseg * sizeof(double), s));
+    hipLaunchKernelGGL(seg_dot_kernel, dim3(nseg), dim3(kVecBlock), 0, s, row0_dev, a, b, mask, dev);
+    SN_CHECK_LAUNCH();
+    SN_HIP(hipMemcpyAsync(out_host, dev, nseg * sizeof(double), hipMemcpyDeviceToHost, s));
+    SN_HIP(hipFreeAsync(dev, s));
+    SN_HIP(hipStreamSynchronize(s));
     return SN_OK;
 }
 

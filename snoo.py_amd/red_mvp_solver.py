@@ -149,6 +149,7 @@ class RedMVPSolver:
         Returns the device solution (float64) and fills ``self.stats`` with the
         residual history, per-step CG iteration counts and timings."""
         drv = self.newton_driver(precision, lanes_per_row, chunk)
+        drv.retarget(self.source_fields_device())
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
                          stats=self.stats, log=log)
@@ -228,6 +229,15 @@ class NewtonDriver:
         self.cg = _CG(asm, 32 if self.f32 else 64, lanes_per_row, chunk)
         self._r0 = None
 
+    def retarget(self, src_dev):
+        """Use new source fields / materials / node coordinates (drops the
+        cached ||R(0)|| when any of them changed)."""
+        key = (id(src_dev), tuple(id(m) for m in self.asm.material_list))
+        if key != getattr(self, '_key', None):
+            self.src = src_dev
+            self._r0 = None
+            self._key = key
+
     def residual(self, x_dev):
         """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2."""
         lib = nat.load()
@@ -262,7 +272,7 @@ class NewtonDriver:
             # float32 CG cannot be asked for much more than 3 digits per solve
             eta_min = 1e-3 if self.f32 else 1e-6
         if max_cg < 0:
-            max_cg = min(10 * self.asm.num_dofs, 20000)
+            max_cg = min(10 * self.asm.num_dofs, 5000 if self.f32 else 20000)
         n = self.asm.num_dofs
         r0 = self.r0()
         x = torch.zeros(n, dtype=torch.float64, device=device()) if x0 is None else \
@@ -286,8 +296,10 @@ class NewtonDriver:
             torch.neg(self.R, out=self.b)
             if self.f32:
                 x_try = x.clone()
-                its, _, _ = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
-                                          maxiter=max_cg)
+                # a capped float32 solve still returns its last iterate; the next
+                # Newton step restarts from the float64 residual there
+                its, _, rc = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
+                                           maxiter=max_cg)
             else:
                 self.dx.zero_()
                 its, _, _ = self.cg.solve(self.J, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
