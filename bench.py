@@ -47,7 +47,11 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-cg-iters', type=int, default=600)
     ap.add_argument('--cpu-threads', type=int, default=16)
-    ap.add_argument('--lanes', type=int, default=8)
+    ap.add_argument('--lanes', type=int, default=0, help='CG lanes per row (0 = autotune)')
+    ap.add_argument('--workload', default='dipole1m', choices=['dipole1m', 'batch64', 'chain6'],
+                    help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
+                         'batched vs one at a time; chain6: 6-yoke chain')
+    ap.add_argument('--variants', type=int, default=64)
     ap.add_argument('--copy-calib', action='store_true',
                     help='only run a plain 256 MiB device copy 5 times (HBM counter calibration)')
     ap.add_argument('--quick', action='store_true',
@@ -124,6 +128,10 @@ def main():
         return
     import __graft_entry__
     __graft_entry__.load_package()
+    if args.workload == 'batch64':
+        return bench_batch(args)
+    if args.workload == 'chain6':
+        return bench_chain(args)
     from snoopy_amd import _native as nat
     from snoopy_amd.magnets import Dipole, mesh_size_for
     from snoopy_amd.red_mvp_solver import _CG
@@ -137,6 +145,9 @@ def main():
     log('mesh: h=%.4f tets=%d edges=%d nnz=%d iron=%.3f setup %.1fs'
         % (h, dp.num_tets, asm.num_dofs, asm.nnz, dp.iron_fraction(), time.perf_counter() - t0))
 
+    if args.lanes == 0:
+        args.lanes = solver.tuned_lanes('mixed')
+        log('autotuned lanes per row: %s' % solver.stats['autotune_lanes'])
     warm = None
     try:
         from snoopy_amd import warm_start
@@ -266,6 +277,104 @@ def main():
     if cpu:
         out['speedup_vs_cpu'] = value / cpu['value']
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """Seeded dipole variants (batched-workload ranges), meshes of 20k-400k
+    tets, Newton-solved together (one joined block-diagonal system, one
+    segment-aware CG graph) and one at a time; mixed precision, 1e-8."""
+    import torch
+    from snoopy_amd.batch import BatchedSolver
+    from snoopy_amd.magnets import Dipole, mesh_size_for, sample_variant
+    rng = np.random.default_rng(12345)
+    dps = []
+    t0 = time.perf_counter()
+    for seed in range(args.variants):
+        v = sample_variant(seed)
+        target = float(np.exp(rng.uniform(np.log(2e4), np.log(4e5))))
+        h, _ = mesh_size_for(target, **v)
+        dps.append(Dipole(h=h, **v))
+    tets = [d.num_tets for d in dps]
+    log('batch: %d variants, %d..%d tets, total %d (mesh %.1f s)'
+        % (len(dps), min(tets), max(tets), sum(tets), time.perf_counter() - t0))
+    bs = BatchedSolver(dps, precision='mixed', lanes_per_row=8)
+    log('joined: %d edges, nnz %d, setup %.1f s' % (bs.n, bs.asm.nnz, bs.t_setup))
+    bs.solve(rtol=args.rtol, log=log)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        bs.solve(rtol=args.rtol)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t) / args.steps
+    conv = int(bs.stats['converged'].sum())
+    log('batched: %.3f s for %d variants (%d converged), newton %d'
+        % (t_batch, len(dps), conv, bs.stats['newton_steps']))
+    # the same variants one at a time
+    t_seq = 0.0
+    for d in dps:
+        s = d.solver(quad_order=2, verbose=False)
+        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t
+    log('one at a time: %.3f s' % t_seq)
+    print(json.dumps({
+        'metric': 'variant solves/s, %d seeded dipoles of 20k-400k tets, Newton to %g'
+                  % (len(dps), args.rtol),
+        'value': len(dps) / t_batch, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps,
+        'higher_is_better': True, 'dtype': 'f32+f64', 'data': 'synthetic',
+        'config': {'workload': 'batch of %d seeded dipole variants' % len(dps),
+                   'tets_total': sum(tets), 'tets_min': min(tets), 'tets_max': max(tets),
+                   'edges_total': bs.n, 'nnz_total': bs.asm.nnz},
+        'batched_s': t_batch, 'one_at_a_time_s': t_seq, 'speedup': t_seq / t_batch,
+        'converged': conv, 'newton_steps': bs.stats['newton_steps']}), flush=True)
+
+
+def bench_chain(args):
+    """Six yokes with alternating polarity along z, 0.1 m apart, seeded
+    dimensions; mixed vs FP64 GPU solve."""
+    import torch
+    from snoopy_amd.magnets import Dipole, chain_yokes, chain_mesh_size_for
+    yokes = chain_yokes(6, seed=7)
+    h, est = chain_mesh_size_for(args.tets if args.tets != 1.0e6 else 6.0e6, yokes)
+    t0 = time.perf_counter()
+    dp = Dipole(h=h, yokes=yokes)
+    solver = dp.solver(quad_order=2, verbose=False)
+    asm = solver.curl_curl_factory
+    asm._device_state()
+    log('chain: %d tets, %d edges, nnz %d, setup %.1f s'
+        % (dp.num_tets, asm.num_dofs, asm.nnz, time.perf_counter() - t0))
+    out = {}
+    for prec in ('fp64', 'mixed'):
+        solver._src = None
+        solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8, log=log)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        solver._src = None
+        x = solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8)
+        torch.cuda.synchronize()
+        out[prec] = (time.perf_counter() - t, dict(solver.stats), x)
+    axis = dp.beam_axis(5000)
+    from snoopy_amd.mesh import locate_points
+    from snoopy_amd.sources import source_field_device
+    from snoopy_amd.device import as_dev_f64
+    el = locate_points(dp.mesh, axis)
+    Bs = source_field_device(dp.coils, as_dev_f64(axis)).cpu().numpy()
+    B = {p: Bs + asm.element_B(out[p][2].cpu().numpy())[el] for p in out}
+    err = float(np.linalg.norm(B['mixed'] - B['fp64'], axis=1).max() /
+                np.linalg.norm(B['fp64'], axis=1).max())
+    print(json.dumps({
+        'metric': 'solves/s, 6-yoke chain, %d tets, Newton to %g' % (dp.num_tets, args.rtol),
+        'value': 1.0 / out['mixed'][0], 'unit': 'solves/s', 'n_gpus': 1,
+        'higher_is_better': True, 'dtype': 'f32+f64', 'data': 'synthetic',
+        'config': {'workload': '6 alternating yokes along z', 'tets': dp.num_tets,
+                   'edges': asm.num_dofs, 'nnz': asm.nnz},
+        'fp64_solves_per_s': 1.0 / out['fp64'][0], 'mixed_s': out['mixed'][0],
+        'fp64_s': out['fp64'][0], 'B_axis_rel_err_mixed_vs_fp64': err,
+        'cg_iterations_mixed': out['mixed'][1]['cg_iterations'],
+        'cg_iterations_fp64': out['fp64'][1]['cg_iterations']}), flush=True)
 
 
 if __name__ == '__main__':

@@ -175,7 +175,8 @@ typedef struct sn_cg sn_cg;
 
 /* Workspace for Jacobi-PCG on one pattern.  precision 64: float64 matrix and
  * vectors; precision 32: float32 matrix and vectors with float64 dot-product
- * accumulation.  lanes_per_row in {0 (auto), 4, 8, 16}; chunk = iterations
+ * accumulation; precision 3264: float32 matrix values (the bulk of the bytes)
+ * with float64 vectors, behaving as precision 64 otherwise.  lanes_per_row in {0 (auto), 4, 8, 16}; chunk = iterations
  * per captured hipGraph (0 = default). */
 int sn_cg_create(sn_cg **out, const sn_pattern *pat, int precision,
                  int lanes_per_row, int chunk, void *stream);

@@ -84,17 +84,28 @@ class BatchedSolver:
         self.R = torch.empty(self.n, dtype=torch.float64, device=device())
         self.b = torch.empty(self.n, dtype=torch.float64, device=device())
         self.cg = ctypes.c_void_p()
+        self.cg_safe = ctypes.c_void_p()
         row0 = (ctypes.c_int * (self.nvar + 1))(*[int(r) for r in self.row0])
         nat.check(nat.load().sn_cg_create_batched(ctypes.byref(self.cg), d['pattern'],
                                                   32 if self.f32 else 64, lanes_per_row, chunk,
                                                   self.nvar, row0, nat.stream_handle()))
+        if self.f32:
+            # float64 Jacobian and CG for segments whose float32 CG stalls
+            nat.check(nat.load().sn_cg_create_batched(ctypes.byref(self.cg_safe), d['pattern'],
+                                                      64, lanes_per_row, chunk, self.nvar, row0,
+                                                      nat.stream_handle()))
+            self.locJ64 = torch.empty(21 * N, dtype=torch.float64, device=device())
+            self.loc6b = torch.empty(6 * N, dtype=torch.float64, device=device())
+            self.J64 = torch.empty(nnz, dtype=torch.float64, device=device())
         self.t_setup = time.perf_counter() - t0
         self.src = None
 
     def __del__(self):
         try:
-            if self.cg and nat._lib is not None:
-                nat._lib.sn_cg_destroy(self.cg)
+            if nat._lib is not None:
+                for h in (self.cg, self.cg_safe):
+                    if h:
+                        nat._lib.sn_cg_destroy(h)
         except Exception:
             pass
 
@@ -109,12 +120,17 @@ class BatchedSolver:
         self.src = src
         return src
 
-    def residual(self, x):
-        """R(x) for all variants (FP64) and the Jacobian; per-variant ||R||."""
+    def residual(self, x, active=None):
+        """R(x) (FP64) and the Jacobian; per-variant ||R||.  Only the variants
+        flagged in ``active`` are re-assembled (the others keep their last
+        local values, so their entries of J and R are unchanged)."""
         lib = nat.load()
         d = self.asm._device_state()
+        doms = None
+        if active is not None:
+            doms = [2 * v + k for v in range(self.nvar) if active[v] for k in (0, 1)]
         self.asm.newton_local_device(x, self.quad_order, self.src, 1, self.f32, self.locJ,
-                                     self.loc6)
+                                     self.loc6, domains=doms)
         nat.check(lib.sn_csr_gather_one(d['pattern'], int(self.f32), nat.ptr(self.locJ),
                                         nat.ptr(self.J), nat.stream_handle()))
         nat.check(lib.sn_dof_gather_f64(d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
@@ -143,7 +159,7 @@ class BatchedSolver:
         if eta_min is None:
             eta_min = 1e-3 if self.f32 else 1e-6
         if max_cg < 0:
-            max_cg = 5000 if self.f32 else 20000
+            max_cg = 50000
         z = torch.zeros(self.n, dtype=torch.float64, device=device())
         r0 = self.residual(z)
         x = z.clone() if x0 is None else x0.clone()
@@ -164,11 +180,32 @@ class BatchedSolver:
             self.b.neg_()
             rt = (ctypes.c_double * self.nvar)(*[float(e) for e in eta])
             its = (ctypes.c_int * self.nvar)()
+            status = (ctypes.c_int * self.nvar)()
             if self.f32:
                 x_try = x.clone()
                 rc = lib.sn_cg_solve_batched(self.cg, nat.ptr(self.J), nat.ptr(self.mask),
-                                             nat.ptr(self.b), nat.ptr(x_try), rt, 0.0, max_cg,
-                                             its, None, None, nat.stream_handle())
+                                             nat.ptr(self.b), nat.ptr(x_try), rt, 0.0,
+                                             min(max_cg, 4000), its, None, status,
+                                             nat.stream_handle())
+                stalled = np.array(status[:]) == nat.SN_ERR_NOT_CONVERGED
+                if stalled.any():
+                    # redo the stalled segments in float64 (Jacobian and CG)
+                    self.asm.newton_local_device(x, self.quad_order, self.src, 1, False,
+                                                 self.locJ64, self.loc6b)
+                    nat.check(lib.sn_csr_gather_one(self.asm._device_state()['pattern'], 0,
+                                                    nat.ptr(self.locJ64), nat.ptr(self.J64),
+                                                    nat.stream_handle()))
+                    b2 = self.b * self._segment_mask(stalled)
+                    dx = torch.zeros_like(x)
+                    its2 = (ctypes.c_int * self.nvar)()
+                    rc = lib.sn_cg_solve_batched(self.cg_safe, nat.ptr(self.J64),
+                                                 nat.ptr(self.mask), nat.ptr(b2), nat.ptr(dx), rt,
+                                                 0.0, max_cg, its2, None, None,
+                                                 nat.stream_handle())
+                    keep = self._segment_mask(~stalled)
+                    x_try = keep * x_try + (1.0 - keep) * (x + dx)
+                    for v in np.nonzero(stalled)[0]:
+                        its[v] += its2[v]
             else:
                 dx = torch.zeros_like(x)
                 rc = lib.sn_cg_solve_batched(self.cg, nat.ptr(self.J), nat.ptr(self.mask),
@@ -178,7 +215,7 @@ class BatchedSolver:
             if rc not in (nat.SN_OK, nat.SN_ERR_NOT_CONVERGED):
                 nat.check(rc)
             cg_its.append(list(its[:]))
-            rn_try = self.residual(x_try)
+            rn_try = self.residual(x_try, active=~conv)
             # per-variant backtracking on the residual norm
             dxs = x_try - x
             step = np.ones(self.nvar)
@@ -188,7 +225,7 @@ class BatchedSolver:
                     break
                 step[bad] *= 0.5
                 x_try = x + self._segment_mask(step) * dxs
-                rn_try = self.residual(x_try)
+                rn_try = self.residual(x_try, active=bad)
             rn_prev, rn, x = rn, np.where(conv, rn, rn_try), x_try
             k += 1
             hist.append(rn / r0)

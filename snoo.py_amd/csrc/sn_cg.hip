@@ -58,10 +58,10 @@ __device__ __forceinline__ double reduce_range(const double *__restrict__ partia
 // ---------------------------------------------------------------- init
 // r = b - A_c x0 (x0 masked; the f32 solve starts from 0), dinv, z, p_old = 0,
 // per-block partials: (r.z, r.r) in the layout kernel B uses, b.b separately.
-template <typename V, int LPR>
+template <typename V, typename M, int LPR>
 __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
     SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
-    const int *__restrict__ diag, const V *__restrict__ vals, const unsigned char *__restrict__ mask,
+    const int *__restrict__ diag, const M *__restrict__ vals, const unsigned char *__restrict__ mask,
     const double *__restrict__ b, const double *__restrict__ x0, V *__restrict__ r,
     V *__restrict__ z, V *__restrict__ dinv, V *__restrict__ pold, float *__restrict__ xs,
     double *__restrict__ partB, double *__restrict__ partbb) {
@@ -139,10 +139,10 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
 
 // ---------------------------------------------------------------- kernel A
 // parity = iteration & 1: reads rho[parity ^ 1] (previous), writes rho[parity].
-template <typename V, int LPR>
+template <typename V, typename M, int LPR>
 __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
-    const V *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
+    const M *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
     const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
     const double *__restrict__ partB, double *__restrict__ partA, CgState *__restrict__ st,
     int parity) {
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
         const int k1 = indptr[i + 1];
         for (int k = indptr[i] + lane; k < k1; k += LPR) {
             const int j = indices[k];
-            acc += vals[k] * (z[j] + beta * pold[j]);
+            acc += (V)vals[k] * (z[j] + beta * pold[j]);
         }
 #pragma unroll
         for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
@@ -354,7 +354,8 @@ using namespace sn;
 
 struct sn_cg {
     const sn_pattern *pat = nullptr;
-    int precision = 64;
+    int precision = 64;  // vector precision
+    bool mat32 = false;  // float32 matrix values (with float64 vectors when precision = 64)
     int lpr = 8;
     int chunk = 32;
     int nseg = 1;
@@ -380,31 +381,31 @@ struct sn_cg {
 
 namespace sn {
 
-template <typename V, int LPR>
-static void launch_spmv(sn_cg *C, const V *vals, const unsigned char *mask, int parity,
+template <typename V, typename M, int LPR>
+static void launch_spmv(sn_cg *C, const M *vals, const unsigned char *mask, int parity,
                         hipStream_t s) {
     const sn_pattern *P = C->pat;
     V *pold = (V *)(parity ? C->p1 : C->p0);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
-    hipLaunchKernelGGL((cg_spmv_kernel<V, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s,
+    hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s,
                        C->map(), P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew,
                        (V *)C->q, C->partB, C->partA, C->st, parity);
 }
 
-template <typename V>
-static void launch_spmv_lpr(sn_cg *C, const V *vals, const unsigned char *mask, int parity,
+template <typename V, typename M>
+static void launch_spmv_lpr(sn_cg *C, const M *vals, const unsigned char *mask, int parity,
                             hipStream_t s) {
     switch (C->lpr) {
-        case 4: launch_spmv<V, 4>(C, vals, mask, parity, s); break;
-        case 16: launch_spmv<V, 16>(C, vals, mask, parity, s); break;
-        default: launch_spmv<V, 8>(C, vals, mask, parity, s); break;
+        case 4: launch_spmv<V, M, 4>(C, vals, mask, parity, s); break;
+        case 16: launch_spmv<V, M, 16>(C, vals, mask, parity, s); break;
+        default: launch_spmv<V, M, 8>(C, vals, mask, parity, s); break;
     }
 }
 
-template <typename V>
-static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask, void *xdst,
+template <typename V, typename M>
+static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask, void *xdst,
                              int parity, hipStream_t s) {
-    launch_spmv_lpr<V>(C, vals, mask, parity, s);
+    launch_spmv_lpr<V, M>(C, vals, mask, parity, s);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
     if (C->precision == 64)
         hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
@@ -416,23 +417,23 @@ static void launch_iteration(sn_cg *C, const V *vals, const unsigned char *mask,
                            (V *)C->z, (float *)xdst, C->partA, C->partB, C->st, parity);
 }
 
-template <typename V, int LPR>
-static void launch_init(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
+template <typename V, typename M, int LPR>
+static void launch_init(sn_cg *C, const M *vals, const unsigned char *mask, const double *b,
                         const double *x0, hipStream_t s) {
     const sn_pattern *P = C->pat;
-    hipLaunchKernelGGL((cg_init_kernel<V, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
+    hipLaunchKernelGGL((cg_init_kernel<V, M, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
                        C->map(), P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r,
                        (V *)C->z, (V *)C->dinv, (V *)C->p0,
                        C->precision == 32 ? C->xs : (float *)nullptr, C->partB, C->partbb);
 }
 
-template <typename V>
-static void launch_init_lpr(sn_cg *C, const V *vals, const unsigned char *mask, const double *b,
+template <typename V, typename M>
+static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, const double *b,
                             const double *x0, double atol, int maxiter, hipStream_t s) {
     switch (C->lpr) {
-        case 4: launch_init<V, 4>(C, vals, mask, b, x0, s); break;
-        case 16: launch_init<V, 16>(C, vals, mask, b, x0, s); break;
-        default: launch_init<V, 8>(C, vals, mask, b, x0, s); break;
+        case 4: launch_init<V, M, 4>(C, vals, mask, b, x0, s); break;
+        case 16: launch_init<V, M, 16>(C, vals, mask, b, x0, s); break;
+        default: launch_init<V, M, 8>(C, vals, mask, b, x0, s); break;
     }
     hipLaunchKernelGGL(cg_init_state_kernel, dim3(C->nseg), dim3(kVecBlock), 0, s, C->map(),
                        C->partbb, C->st, C->rtol_dev, atol, maxiter);
@@ -442,10 +443,12 @@ static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, 
                          hipStream_t s) {
     // an even number of iterations keeps the p buffers and rho slots in step
     for (int k = 0; k < C->chunk; ++k) {
-        if (C->precision == 64)
-            launch_iteration<double>(C, (const double *)vals, mask, xdst, k & 1, s);
+        if (C->precision == 64 && !C->mat32)
+            launch_iteration<double, double>(C, (const double *)vals, mask, xdst, k & 1, s);
+        else if (C->precision == 64)
+            launch_iteration<double, float>(C, (const float *)vals, mask, xdst, k & 1, s);
         else
-            launch_iteration<float>(C, (const float *)vals, mask, xdst, k & 1, s);
+            launch_iteration<float, float>(C, (const float *)vals, mask, xdst, k & 1, s);
     }
 }
 
@@ -492,10 +495,12 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
                           hipMemcpyHostToDevice, s));
     void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
     const double *x0 = (C->precision == 64) ? x : nullptr;
-    if (C->precision == 64)
-        launch_init_lpr<double>(C, (const double *)vals, mask, b, x0, atol, maxiter, s);
+    if (C->precision == 64 && !C->mat32)
+        launch_init_lpr<double, double>(C, (const double *)vals, mask, b, x0, atol, maxiter, s);
+    else if (C->precision == 64)
+        launch_init_lpr<double, float>(C, (const float *)vals, mask, b, x0, atol, maxiter, s);
     else
-        launch_init_lpr<float>(C, (const float *)vals, mask, b, x0, atol, maxiter, s);
+        launch_init_lpr<float, float>(C, (const float *)vals, mask, b, x0, atol, maxiter, s);
     SN_CHECK_LAUNCH();
 
     // (re)capture the chunk graph when the pointers or stream changed
@@ -557,10 +562,13 @@ extern "C" {
 int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
                          int nseg, const int *seg_row0_host, void *stream) {
     SN_REQUIRE(out && P, "sn_cg_create: null argument");
-    SN_REQUIRE(precision == 64 || precision == 32, "sn_cg_create: precision must be 64 or 32");
+    SN_REQUIRE(precision == 64 || precision == 32 || precision == 3264,
+               "sn_cg_create: precision must be 64, 32 or 3264");
     if (lpr == 0) lpr = 8;
     SN_REQUIRE(lpr == 4 || lpr == 8 || lpr == 16, "sn_cg_create: lanes_per_row in {4,8,16}");
     SN_REQUIRE(nseg >= 1, "sn_cg_create: nseg must be >= 1");
+    const bool mat32 = precision != 64;
+    if (precision == 3264) precision = 64;
     std::vector<int> row0(nseg + 1);
     if (seg_row0_host) {
         for (int v = 0; v <= nseg; ++v) row0[v] = seg_row0_host[v];
@@ -577,6 +585,7 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
     sn_cg *C = new sn_cg();
     C->pat = P;
     C->precision = precision;
+    C->mat32 = mat32;
     C->lpr = lpr;
     C->chunk = chunk;
     C->nseg = nseg;
@@ -704,8 +713,12 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     // parity 1 reads p1 (zero) and rho[0], writes p0 and rho[1]: the state stays
     // the same for every repetition
     for (int r = 0; r < reps; ++r) {
-        if (C->precision == 64) launch_spmv_lpr<double>(C, (const double *)vals, mask, 1, s);
-        else launch_spmv_lpr<float>(C, (const float *)vals, mask, 1, s);
+        if (C->precision == 64 && !C->mat32)
+            launch_spmv_lpr<double, double>(C, (const double *)vals, mask, 1, s);
+        else if (C->precision == 64)
+            launch_spmv_lpr<double, float>(C, (const float *)vals, mask, 1, s);
+        else
+            launch_spmv_lpr<float, float>(C, (const float *)vals, mask, 1, s);
     }
     SN_HIP(hipEventRecord(e1, s));
     SN_HIP(hipEventSynchronize(e1));

@@ -224,13 +224,15 @@ class CurlCurlAssembler:
                                         nat.stream_handle()))
         return K, dK, J, rhs
 
-    def newton_local_device(self, x_dev, quad_order, source_fields_dev, mode, f32, locJ, loc6):
-        """Fused element kernel of the Newton solvers over all domains."""
+    def newton_local_device(self, x_dev, quad_order, source_fields_dev, mode, f32, locJ, loc6,
+                            domains=None):
+        """Fused element kernel of the Newton solvers over all domains (or the
+        listed ones; the local values of the others are left as they are)."""
         d = self._device_state()
         T = self.tables(quad_order)
         N = d['num_el']
         lib = nat.load()
-        for n in range(self.num_materials):
+        for n in (range(self.num_materials) if domains is None else domains):
             b, e = int(d['offsets'][n]), int(d['offsets'][n + 1])
             m = self.model(n)
             bs = source_fields_dev[n] if source_fields_dev is not None else None

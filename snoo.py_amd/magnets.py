@@ -150,3 +150,38 @@ def sample_variant(seed, with_warm_start_inputs=False):
     v['ax'] = min(v['ax'], v['W'] - 0.2)
     v['ay'] = min(v['ay'], v['H'] - 0.2)
     return v
+
+
+def chain_yokes(n=6, seed=7, gap=0.1, L=0.5):
+    """n yokes along z with alternating polarity, ``gap`` apart, dimensions
+    drawn from the batched-workload ranges (W, H, ax, ay)."""
+    out = []
+    z = 0.0
+    for k in range(n):
+        v = sample_variant(seed * 1000 + k)
+        out.append(dict(W=v['W'], H=v['H'], L=L, ax=v['ax'], ay=v['ay'], z0=z,
+                        sign=1.0 if k % 2 == 0 else -1.0))
+        z += L + gap
+    zc = 0.5 * (out[0]['z0'] + out[-1]['z0'])
+    for y in out:
+        y['z0'] -= zc
+    return out
+
+
+def chain_mesh_size_for(target_tets, yokes, pad=0.5, ratio=1.2):
+    """Grid spacing h whose chain mesh has about ``target_tets`` tetrahedra."""
+    xb = sorted({v for yk in yokes for v in (-yk['W'] / 2, -yk['ax'] / 2, yk['ax'] / 2, yk['W'] / 2)})
+    yb = sorted({v for yk in yokes for v in (-yk['H'] / 2, -yk['ay'] / 2, yk['ay'] / 2, yk['H'] / 2)})
+    zb = sorted({v for yk in yokes for v in (yk['z0'] - yk['L'] / 2, yk['z0'] + yk['L'] / 2)})
+
+    def count(h):
+        n = [len(axis_lines(b, h, pad, ratio)) - 1 for b in (xb, yb, zb)]
+        return 6 * n[0] * n[1] * n[2]
+    lo, hi = 1e-3, 1.0
+    for _ in range(50):
+        mid = math.sqrt(lo * hi)
+        if count(mid) > target_tets:
+            lo = mid
+        else:
+            hi = mid
+    return hi, count(hi)

@@ -155,12 +155,31 @@ class RedMVPSolver:
                          stats=self.stats, log=log)
 
     def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32):
+        if lanes_per_row == 0:
+            lanes_per_row = self.tuned_lanes(precision)
         key = ('drv', precision, lanes_per_row, chunk)
         if key not in self.__dict__.setdefault('_drivers', {}):
             self._drivers[key] = NewtonDriver(self.curl_curl_factory, self.source_fields_device(),
                                               self.mask_device(), self.quad_order, precision,
                                               lanes_per_row, chunk)
         return self._drivers[key]
+
+
+def _tuned_lanes(solver, precision):
+    import torch
+    from .autotune import tune_lanes
+    asm = solver.curl_curl_factory
+    d = asm._device_state()
+    drv = solver.newton_driver(precision, 8, 32)
+    drv.retarget(solver.source_fields_device())
+    drv.residual(torch.zeros(asm.num_dofs, dtype=torch.float64, device=device()))
+    best, times = tune_lanes(d['pattern'], asm.num_dofs, d['nnz'], drv.J, drv.mask,
+                             32 if precision == 'mixed' else 64)
+    solver.stats['autotune_lanes'] = {'best': best, 'ms': times}
+    return best
+
+
+RedMVPSolver.tuned_lanes = _tuned_lanes
 
 
 def _absmax(t):
@@ -227,6 +246,12 @@ class NewtonDriver:
         self.b = torch.empty(n, dtype=torch.float64, device=dev)
         self.dx = torch.empty(n, dtype=torch.float64, device=dev)
         self.cg = _CG(asm, 32 if self.f32 else 64, lanes_per_row, chunk)
+        # float64 Jacobian and CG for the steps where the float32 CG stalls:
+        # float32 rounding of J breaks its exact gradient null space, which
+        # can stall CG on large meshes at tight forcing terms
+        self.cg_safe = _CG(asm, 64, lanes_per_row, chunk) if self.f32 else None
+        self._J64 = None
+        self.eta_fp64 = 0.0   # steps with eta <= this go straight to float64
         self._r0 = None
 
     def retarget(self, src_dev):
@@ -252,6 +277,25 @@ class NewtonDriver:
                                  nat.ptr(self.mask), ctypes.byref(out), nat.stream_handle()))
         return float(np.sqrt(out.value))
 
+    def _fp64_step(self, x, eta, max_cg):
+        """One Newton step of the mixed solver in float64: the float64
+        Jacobian at x (element kernel in float64 output mode) and float64 CG."""
+        import torch
+        lib = nat.load()
+        if self._J64 is None:
+            N = self.d['num_el']
+            self._locJ64 = torch.empty(21 * N, dtype=torch.float64, device=device())
+            self._J64 = torch.empty(self.d['nnz'], dtype=torch.float64, device=device())
+            self._loc6b = torch.empty_like(self.loc6)
+        self.asm.newton_local_device(x, self.quad_order, self.src, 1, False, self._locJ64,
+                                     self._loc6b)
+        nat.check(lib.sn_csr_gather_one(self.d['pattern'], 0, nat.ptr(self._locJ64),
+                                        nat.ptr(self._J64), nat.stream_handle()))
+        self.dx.zero_()
+        its, _, _ = self.cg_safe.solve(self._J64, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
+                                       maxiter=max_cg)
+        return its, x + self.dx
+
     def r0(self):
         """||R(0)|| (the norm of the source term), cached."""
         if self._r0 is None:
@@ -265,14 +309,17 @@ class NewtonDriver:
         """Newton steps until ||R|| <= rtol ||R(0)||, ``max_newton`` steps, or
         ``stall_steps`` consecutive steps that reduce ||R|| by less than 10%
         (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
-        linear solve at min(10 n, 20000) iterations."""
+        linear solve at min(10 n, 50000) iterations; a float32 solve is cut at
+        4x the previous step's count (at least 1500) and, if it stalls there,
+        the step is redone in float64 (Jacobian and CG), as are later steps
+        with a forcing term at least as small."""
         import torch
         stats = stats if stats is not None else {}
         if eta_min is None:
             # float32 CG cannot be asked for much more than 3 digits per solve
             eta_min = 1e-3 if self.f32 else 1e-6
         if max_cg < 0:
-            max_cg = min(10 * self.asm.num_dofs, 5000 if self.f32 else 20000)
+            max_cg = min(10 * self.asm.num_dofs, 50000)
         n = self.asm.num_dofs
         r0 = self.r0()
         x = torch.zeros(n, dtype=torch.float64, device=device()) if x0 is None else \
@@ -280,7 +327,7 @@ class NewtonDriver:
         x[self.mask == 0] = 0.0
         t0 = time.perf_counter()
         rn = self.residual(x)
-        hist, cg_its, etas = [rn / r0], [], []
+        hist, cg_its, etas, modes = [rn / r0], [], [], []
         eta = eta_max
         rn_prev = None
         k = 0
@@ -294,17 +341,26 @@ class NewtonDriver:
             eta = max(eta, min(eta_max, 0.5 * rtol * r0 / rn))
             etas.append(eta)
             torch.neg(self.R, out=self.b)
-            if self.f32:
+            mode = 'fp64'
+            if self.f32 and eta > self.eta_fp64:
                 x_try = x.clone()
-                # a capped float32 solve still returns its last iterate; the next
-                # Newton step restarts from the float64 residual there
+                cap32 = max(1500, 4 * max(cg_its[-1] if cg_its else 0, 1))
                 its, _, rc = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
-                                           maxiter=max_cg)
+                                           maxiter=min(cap32, max_cg))
+                mode = 'fp32'
+                if rc == nat.SN_ERR_NOT_CONVERGED:
+                    more, x_try = self._fp64_step(x, eta, max_cg)
+                    its += more
+                    self.eta_fp64 = max(self.eta_fp64, eta)
+                    mode = 'fp32 stalled -> fp64'
+            elif self.f32:
+                its, x_try = self._fp64_step(x, eta, max_cg)
             else:
                 self.dx.zero_()
                 its, _, _ = self.cg.solve(self.J, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
                                           maxiter=max_cg)
                 x_try = x + self.dx
+            modes.append(mode)
             cg_its.append(its)
             rn_try = self.residual(x_try)
             step = 1.0
@@ -326,6 +382,7 @@ class NewtonDriver:
                     % (k, rn / r0, its, eta, step, time.perf_counter() - t0))
         torch.cuda.synchronize()
         stats.update({'residual_history': hist, 'cg_iterations': cg_its, 'etas': etas,
+                      'cg_modes': modes,
                       'newton_steps': k, 'converged': converged, 'r0': r0,
                       't_newton': time.perf_counter() - t0, 'precision': self.precision})
         return x
