@@ -48,6 +48,8 @@ def parse():
     ap.add_argument('--cpu-cg-iters', type=int, default=600)
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--lanes', type=int, default=0, help='CG lanes per row (0 = autotune)')
+    ap.add_argument('--precond', default='jacobi', choices=['jacobi', 'ams'],
+                    help='CG preconditioner of the timed solves')
     ap.add_argument('--workload', default='dipole1m', choices=['dipole1m', 'batch64', 'chain6'],
                     help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
                          'batched vs one at a time; chain6: 6-yoke chain')
@@ -158,7 +160,8 @@ def main():
     def one_solve(precision, rtol, use_warm):
         solver._src = None                      # Biot-Savart is part of every solve
         x0 = warm.predict(dp) if (use_warm and warm is not None) else None
-        return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes, log=log)
+        return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes, log=log,
+                                   precond=args.precond)
 
     # plain FP64 GPU solve (the drop-in method at the configuration's tolerance)
     one_solve('fp64', args.rtol, False)

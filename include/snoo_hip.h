@@ -137,7 +137,9 @@ int sn_dof_gather_f64(const sn_pattern *pat, const double *loc6_dev,
  *   mode 0: b_e = dK_e x_e + rhs_e  (b = Jx - Kx + rhs, red_mvp_solver.py:186)
  *   mode 1: R_e = K_e x_e - rhs_e   (Newton residual R(x) = K(x)x - rhs(x))
  * Element arithmetic is float64 in both precisions.  Bs_dev == NULL marks a
- * source-free (air) domain. */
+ * source-free (air) domain.  nu_el_dev (may be NULL) receives the element's
+ * mean reluctivity sum_q nu_q w_q / sum_q w_q (float32, indexed like the
+ * elements), the coefficient of the auxiliary-space preconditioner. */
 int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin,
                       int el_count, const double *nodes_dev, const int *cells_dev,
                       const int *glob_ids_dev, const int *orient_dev,
@@ -145,7 +147,7 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin,
                       const double *w_dev, int Q, int type_spec, int n_int,
                       const double *coeffs_dev, const double *knots_dev,
                       const double *x_dev, const double *Bs_dev,
-                      void *locJ_dev, double *loc6_dev, void *stream);
+                      void *locJ_dev, double *loc6_dev, float *nu_el_dev, void *stream);
 
 /* CSR values from one local pair buffer, float64 or float32. */
 int sn_csr_gather_one(const sn_pattern *pat, int f32, const void *loc_dev,
@@ -221,6 +223,61 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
  * scratch vectors). */
 int sn_cg_time_spmv(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev, int reps,
                     double *avg_ms_host, void *stream);
+
+/* ------------------------------------------ auxiliary-space preconditioner
+ * No counterpart in the reference, whose CG uses the Jacobi diagonal only
+ * (snoopy/red_mvp_solver.py:198-199).  Hiptmair-Xu preconditioner of the
+ * singular curl-curl Jacobian:  M r = D^-1 r + weight * Pi B Pi^T r, with
+ * (Pi y)_e = (x_hi - x_lo)/2 . (y_lo + y_hi) the Whitney interpolation of a
+ * P1 vector field and B one V(1,1)-cycle (damped Jacobi, weight omega) of
+ * unsmoothed-aggregation AMG on the P1 Laplacian with the element
+ * reluctivity, applied to the three components at once (float32). */
+
+typedef struct sn_ams sn_ams;
+
+/* Edge side of the preconditioner (host arrays, copied).
+ * edge_nodes_host [2*n_edges]: level-0 ids of the (lo, hi) nodes of each
+ *   edge; ids >= n_free_nodes mark Dirichlet nodes (their value is 0).
+ * edge_tan_host [4*n_edges]: (x_hi - x_lo)/2 and one padding float.
+ * inc_ptr_host [n_free_nodes+1], inc_edge_host: free edges at each free node.
+ * n_el: elements of the Laplacian (the element order of sn_ams_update). */
+int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el,
+                  const int *edge_nodes_host, const float *edge_tan_host,
+                  const int *inc_ptr_host, const int *inc_edge_host, float omega,
+                  float weight, void *stream);
+/* Append a level (host arrays, copied), finest first.  CSR pattern
+ * (indptr [n+1], indices [nnz], with a diagonal in every row); gather plan:
+ * vals[k] = sum of src[gather_list[gather_ptr[k] .. gather_ptr[k+1])] in that
+ * order, where src is the element pair values (entry slot*n_el + e, slot =
+ * index of (a <= b) in the row-major upper triangle of the 4x4 element
+ * matrix, in level-0 ids) for level 0 and the previous level's values
+ * otherwise.  agg_ptr [n+1] (NULL for level 0): row k of this level is the
+ * aggregate of rows [agg_ptr[k], agg_ptr[k+1]) of the previous level. */
+int sn_ams_add_level(sn_ams *ams, int n, int nnz, const int *indptr_host,
+                     const int *indices_host, const int *gather_ptr_host,
+                     const int *gather_list_host, const int *agg_ptr_host, void *stream);
+/* Numeric refresh for new element reluctivities: element Laplacians,
+ * every level's values and diagonal, dense inverse of the coarsest level
+ * (at most 1024 rows).  cells_dev [4*n_el] uses the ids of nodes_dev. */
+int sn_ams_update(sn_ams *ams, const double *nodes_dev, const int *cells_dev,
+                  const float *nu_el_dev, void *stream);
+/* z = M r (float64 vectors; entries with mask == 0 set to 0). */
+int sn_ams_apply(sn_ams *ams, const unsigned char *mask_dev, const double *r_dev,
+                 const double *dinv_dev, double *z_dev, void *stream);
+int sn_ams_num_levels(const sn_ams *ams);
+/* Host set-up helper: greedy aggregation of the strength graph of a CSR
+ * matrix (|a_ij| >= theta sqrt|a_ii a_jj|); agg_host [n] receives aggregate
+ * ids 0..n_agg-1 (deterministic, index order). */
+int sn_amg_aggregate(int n, const int *indptr_host, const int *indices_host,
+                     const double *vals_host, double theta, int *agg_host, int *n_agg);
+/* rows and nonzeros of one level; copy of its current values (float32 [nnz]) */
+int sn_ams_level_info(const sn_ams *ams, int level, int *n, int *nnz);
+int sn_ams_level_values(const sn_ams *ams, int level, float *vals_dev, void *stream);
+void sn_ams_destroy(sn_ams *ams);
+/* Use `ams` (NULL: the Jacobi diagonal) as the preconditioner of every later
+ * sn_cg_solve on this workspace; the handle must stay alive and be updated
+ * (sn_ams_update) before the solves.  Single-system workspaces only. */
+int sn_cg_set_ams(sn_cg *cg, sn_ams *ams);
 
 /* ------------------------------------------------- learned warm start
  * No counterpart in the reference (it starts Newton from A = 0 or a given

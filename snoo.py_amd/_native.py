@@ -35,7 +35,7 @@ SIGNATURES = {
     'sn_dof_gather_f64': (_c_int, [_vp, _vp, _vp, _vp]),
     'sn_element_newton': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                    _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
-                                   _vp, _vp, _vp]),
+                                   _vp, _vp, _vp, _vp]),
     'sn_csr_gather_one': (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
     'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),
@@ -59,6 +59,18 @@ SIGNATURES = {
     'sn_f32_to_bf16': (_c_int, [_c_int, _vp, _vp, _vp]),
     'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_edge_transfer': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_ams_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                               ctypes.c_float, ctypes.c_float, _vp]),
+    'sn_ams_add_level': (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_ams_update': (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'sn_ams_apply': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_ams_num_levels': (_c_int, [_vp]),
+    'sn_amg_aggregate': (_c_int, [_c_int, _vp, _vp, _vp, ctypes.c_double, _vp,
+                                  ctypes.POINTER(_c_int)]),
+    'sn_ams_level_info': (_c_int, [_vp, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    'sn_ams_level_values': (_c_int, [_vp, _c_int, _vp, _vp]),
+    'sn_ams_destroy': (None, [_vp]),
+    'sn_cg_set_ams': (_c_int, [_vp, _vp]),
 }
 
 SN_OK, SN_ERR_ARG, SN_ERR_HIP, SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN = 0, 1, 2, 3, 4
@@ -114,6 +126,14 @@ def ptr(t):
     if t is None:
         return None
     return ctypes.c_void_p(t.data_ptr())
+
+
+def ptr_np(a):
+    """Host address of a C-contiguous numpy array, None for None."""
+    if a is None:
+        return None
+    assert a.flags['C_CONTIGUOUS']
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def stream_handle(stream=None):

@@ -1,0 +1,563 @@
+// Auxiliary-space preconditioner for the curl-curl Jacobian (no counterpart
+// in the reference, which preconditions its CG with the Jacobi diagonal
+// only, snoopy/red_mvp_solver.py:198-199).
+//
+// The Jacobian J of the reduced-vector-potential problem is singular: every
+// discrete gradient is in its kernel (curl grad = 0), so point smoothers and
+// nodal AMG alone cannot reach its smooth error.  The Hiptmair-Xu
+// preconditioner splits the edge space into high-frequency edge functions
+// (handled by the Jacobi diagonal) and interpolated vector nodal fields
+// (handled by a multigrid cycle of the nodal vector Laplacian):
+//
+//   M r = D^-1 r + w * Pi B_L Pi^T r
+//
+//   Pi      edge DOF of the Whitney interpolant of a P1 vector field:
+//           (Pi y)_e = (x_hi - x_lo)/2 . (y_lo + y_hi)
+//   B_L     one V(1,1)-cycle of unsmoothed-aggregation AMG applied to the
+//           three components of y at once; L is the P1 Laplacian with the
+//           element reluctivity, sum_e nu_e vol_e grad(l_a).grad(l_b), and
+//           Dirichlet nodes (nodes of Dirichlet edges) removed.  Since
+//           grad-fields lie in the kernel of J, no gradient-space correction
+//           is needed.
+//
+// Set-up: the aggregation (host, once per mesh) orders every level so that
+// the rows of each aggregate are contiguous; restriction is then a sum over
+// a contiguous range and prolongation a broadcast.  All matrices are formed
+// on the device by fixed-order gathers (element values -> level 0, level l ->
+// level l+1), so a refresh per Newton step costs a few bandwidth-bound
+// passes; the coarsest matrix (<= a few hundred rows) is inverted by one
+// block with Gauss-Jordan elimination (SPD, no pivoting).
+//
+// Apply (all float32, 3 right-hand sides packed in float4):
+//   edge_restrict    b0 = Pi^T r                    (thread per free node)
+//   down (per level) x = w D^-1 b, b' = P^T (b - A x) (wave per aggregate:
+//                    the aggregate's rows and their nonzeros are contiguous)
+//   coarse           y = C^-1 b                     (wave per row, dense)
+//   up (per level)   x' = x + P e, y = x' + w D^-1 (b - A x')
+//   edge_prolong     z = D_J^-1 r + w Pi y0, partial sums of r.z
+// Pre- and post-smoothing are the same damped Jacobi step, so the cycle is a
+// symmetric operator and M is symmetric positive definite on the range of J.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "sn_ams.hpp"
+#include "sn_common.hpp"
+
+namespace sn {
+
+constexpr int kAmsBlock = 256;
+constexpr int kAmsEdgeBlock = 1024;
+constexpr int kAmsMaxCoarse = 1024;
+
+__device__ __forceinline__ float4 f4_zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+__device__ __forceinline__ void f4_fma(float4 &a, float s, const float4 &b) {
+    a.x += s * b.x;
+    a.y += s * b.y;
+    a.z += s * b.z;
+}
+__device__ __forceinline__ float4 f4_add(const float4 &a, const float4 &b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, 0.f);
+}
+template <int W>
+__device__ __forceinline__ void f4_reduce(float4 &a) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) {
+        a.x += __shfl_xor(a.x, off, kWave);
+        a.y += __shfl_xor(a.y, off, kWave);
+        a.z += __shfl_xor(a.z, off, kWave);
+    }
+}
+
+// ------------------------------------------------------------ numeric set-up
+// P1 stiffness of each element with its reluctivity: 10 pair values
+// (a <= b, row-major upper triangle) nu_e vol_e grad(l_a).grad(l_b).
+__global__ __launch_bounds__(kAmsBlock) void ams_lap_local_kernel(
+    int n_el, const double *__restrict__ nodes, const int *__restrict__ cells,
+    const float *__restrict__ nu_el, float *__restrict__ loc) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    double X[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int nd = cells[4 * e + a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[a][k] = nodes[3 * nd + k];
+    }
+    double E[3][3];  // E[c] = X[c+1] - X[0]
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) E[c][k] = X[c + 1][k] - X[0][k];
+    // rows of E^-1 (E with columns e1, e2, e3): (e2 x e3, e3 x e1, e1 x e2) / det
+    double g[4][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double *u = E[(c + 1) % 3], *v = E[(c + 2) % 3];
+        g[c + 1][0] = u[1] * v[2] - u[2] * v[1];
+        g[c + 1][1] = u[2] * v[0] - u[0] * v[2];
+        g[c + 1][2] = u[0] * v[1] - u[1] * v[0];
+    }
+    const double det = E[0][0] * g[1][0] + E[0][1] * g[1][1] + E[0][2] * g[1][2];
+    const double inv = 1.0 / det;
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g[a][k] *= inv;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[0][k] = -(g[1][k] + g[2][k] + g[3][k]);
+    const double s = (double)nu_el[e] * fabs(det) / 6.0;
+    int p = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b) {
+            const double v = s * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
+            loc[(long long)p * n_el + e] = (float)v;
+            ++p;
+        }
+}
+
+__global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
+                                  const int *__restrict__ glist, const float *__restrict__ in,
+                                  float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    double s = 0.0;
+    for (int k = gptr[i]; k < gptr[i + 1]; ++k) s += (double)in[glist[k]];
+    out[i] = (float)s;
+}
+
+__global__ void ams_dinv_kernel(int n, const int *__restrict__ diagpos,
+                                const float *__restrict__ vals, float *__restrict__ dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dinv[i] = 1.0f / vals[diagpos[i]];
+}
+
+// One block: dense inverse of the coarsest matrix by Gauss-Jordan elimination
+// in place (symmetric positive definite, so no pivoting is needed).
+__global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
+    int nc, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const float *__restrict__ vals, double *__restrict__ a, float *__restrict__ cinv) {
+    __shared__ double col[kAmsMaxCoarse];
+    __shared__ double piv;
+    const int t = threadIdx.x, T = blockDim.x;
+    for (int k = t; k < nc * nc; k += T) a[k] = 0.0;
+    __syncthreads();
+    for (int i = t; i < nc; i += T)
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k) a[(size_t)i * nc + indices[k]] = vals[k];
+    __syncthreads();
+    for (int k = 0; k < nc; ++k) {
+        if (t == 0) piv = 1.0 / a[(size_t)k * nc + k];
+        for (int i = t; i < nc; i += T) col[i] = a[(size_t)i * nc + k];
+        __syncthreads();
+        const double pk = piv;
+        for (int j = t; j < nc; j += T) a[(size_t)k * nc + j] = (j == k) ? pk : a[(size_t)k * nc + j] * pk;
+        __syncthreads();
+        for (int idx = t; idx < nc * nc; idx += T) {
+            const int i = idx / nc, j = idx % nc;
+            if (i == k) continue;
+            const double f = col[i];
+            if (j == k) a[idx] = -f * pk;
+            else a[idx] -= f * a[(size_t)k * nc + j];
+        }
+        __syncthreads();
+    }
+    for (int k = t; k < nc * nc; k += T) cinv[k] = (float)a[k];
+}
+
+// ------------------------------------------------------------------- apply
+template <typename V>
+__global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
+    int n_free, const int *__restrict__ inc_ptr, const int *__restrict__ inc_edge,
+    const float4 *__restrict__ tan, const V *__restrict__ r, float4 *__restrict__ b0,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_free) return;
+    float4 acc = f4_zero();
+    for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+        const int e = inc_edge[k];
+        f4_fma(acc, (float)r[e], tan[e]);
+    }
+    b0[a] = acc;
+}
+
+// wave per aggregate: x = w D^-1 b on its rows; b_c = sum_rows b - sum_nnz a_ij x_j
+__global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
+    int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ indptr,
+    const int *__restrict__ indices, const float *__restrict__ vals,
+    const float *__restrict__ dinv, const float4 *__restrict__ b, float4 *__restrict__ x,
+    float4 *__restrict__ bc, float omega, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wid >= n_agg) return;  // uniform over the wave
+    const int r0 = agg_ptr[wid], r1 = agg_ptr[wid + 1];
+    float4 s = f4_zero();
+    for (int i = r0 + lane; i < r1; i += kWave) {
+        const float4 bi = b[i];
+        const float d = omega * dinv[i];
+        x[i] = make_float4(d * bi.x, d * bi.y, d * bi.z, 0.f);
+        s = f4_add(s, bi);
+    }
+    const int k1 = indptr[r1];
+    for (int k = indptr[r0] + lane; k < k1; k += kWave) {
+        const int j = indices[k];
+        f4_fma(s, -vals[k] * omega * dinv[j], b[j]);
+    }
+    f4_reduce<kWave>(s);
+    if (lane == 0) bc[wid] = s;
+}
+
+// wave per row of the coarsest level: y = C^-1 b
+__global__ __launch_bounds__(kAmsBlock) void ams_coarse_kernel(
+    int nc, const float *__restrict__ cinv, const float4 *__restrict__ b, float4 *__restrict__ y,
+    const int *__restrict__ done) {
+    if (done && *done) return;
+    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wid >= nc) return;
+    float4 acc = f4_zero();
+    for (int j = lane; j < nc; j += kWave) f4_fma(acc, cinv[(size_t)wid * nc + j], b[j]);
+    f4_reduce<kWave>(acc);
+    if (lane == 0) y[wid] = acc;
+}
+
+// LPR lanes per row: x' = x + e[coarse_of], y = x' + w D^-1 (b - A x')
+template <int LPR>
+__global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
+    int n, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const float *__restrict__ vals, const float *__restrict__ dinv,
+    const int *__restrict__ coarse_of, const float4 *__restrict__ b,
+    const float4 *__restrict__ x, const float4 *__restrict__ ec, float4 *__restrict__ y,
+    float omega, const int *__restrict__ done) {
+    if (done && *done) return;
+    const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    const int lane = threadIdx.x % LPR;
+    float4 acc = f4_zero();
+    if (row < n) {
+        for (int k = indptr[row] + lane; k < indptr[row + 1]; k += LPR) {
+            const int j = indices[k];
+            f4_fma(acc, vals[k], f4_add(x[j], ec[coarse_of[j]]));
+        }
+    }
+    f4_reduce<LPR>(acc);
+    if (row < n && lane == 0) {
+        const float4 xi = f4_add(x[row], ec[coarse_of[row]]);
+        const float4 bi = b[row];
+        const float d = omega * dinv[row];
+        y[row] = make_float4(xi.x + d * (bi.x - acc.x), xi.y + d * (bi.y - acc.y),
+                             xi.z + d * (bi.z - acc.z), 0.f);
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
+    int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
+    const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
+    const float4 *__restrict__ tan, const float4 *__restrict__ y0, float weight,
+    V *__restrict__ z, double *__restrict__ partials, int stride, const int *__restrict__ done) {
+    __shared__ double lds[kAmsEdgeBlock / kWave];
+    if (done && *done) return;
+    double rz = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_edges;
+         i += (long long)gridDim.x * blockDim.x) {
+        const bool f = mask ? mask[i] != 0 : true;
+        V zi = 0;
+        if (f) {
+            const int2 en = edge_nodes[i];
+            float4 s = f4_zero();
+            if (en.x < n_free) s = f4_add(s, y0[en.x]);
+            if (en.y < n_free) s = f4_add(s, y0[en.y]);
+            const float4 t = tan[i];
+            const float aux = t.x * s.x + t.y * s.y + t.z * s.z;
+            const V ri = r[i];
+            zi = dinv[i] * ri + (V)(weight * aux);
+            rz += (double)ri * (double)zi;
+        }
+        z[i] = zi;
+    }
+    const double sum = block_sum(rz, lds);
+    if (threadIdx.x == 0 && partials) partials[(long long)stride * blockIdx.x] = sum;
+}
+
+template <typename V>
+static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
+                           double *partials, int stride, int grid, const int *done,
+                           hipStream_t s) {
+    const int L = (int)A->lv.size();
+    AmsLevel &l0 = A->lv[0];
+    hipLaunchKernelGGL((ams_edge_restrict_kernel<V>), dim3(grid_for(A->n_free, kAmsBlock)),
+                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->edge_tan, r,
+                       l0.b, done);
+    for (int l = 0; l + 1 < L; ++l) {
+        AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
+        const int waves_per_block = kAmsBlock / kWave;
+        hipLaunchKernelGGL(ams_down_kernel, dim3(grid_for(c.n, waves_per_block)),
+                           dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, f.indptr, f.indices, f.vals,
+                           f.dinv, f.b, f.x, c.b, A->omega, done);
+    }
+    AmsLevel &lc = A->lv[L - 1];
+    hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, kAmsBlock / kWave)),
+                       dim3(kAmsBlock), 0, s, lc.n, A->cinv, lc.b, lc.y, done);
+    for (int l = L - 2; l >= 0; --l) {
+        AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
+        hipLaunchKernelGGL((ams_up_kernel<8>), dim3(grid_for((long long)f.n * 8, kAmsBlock)),
+                           dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv,
+                           f.coarse_of, f.b, f.x, c.y, f.y, A->omega, done);
+    }
+    hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
+                       A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->edge_tan, l0.y,
+                       A->weight, z, partials, stride, done);
+}
+
+int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
+                     const unsigned char *mask, void *z, double *partials, int stride, int grid,
+                     const int *done, hipStream_t s) {
+    SN_REQUIRE(A && A->ready, "AMS preconditioner used before sn_ams_update");
+    if (v64)
+        launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
+                               partials, stride, grid, done, s);
+    else
+        launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
+                              partials, stride, grid, done, s);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+template <typename T>
+static int upload(T **dst, const T *src, size_t count, hipStream_t s) {
+    SN_HIP(hipMalloc(dst, (count > 0 ? count : 1) * sizeof(T)));
+    if (count > 0) SN_HIP(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return SN_OK;
+}
+
+}  // namespace sn
+
+using namespace sn;
+
+extern "C" {
+
+int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const int *edge_nodes,
+                  const float *edge_tan, const int *inc_ptr, const int *inc_edge, float omega,
+                  float weight, void *stream) {
+    SN_REQUIRE(out && edge_nodes && edge_tan && inc_ptr && inc_edge, "sn_ams_create: null argument");
+    SN_REQUIRE(n_edges > 0 && n_free_nodes > 0 && n_el > 0, "sn_ams_create: empty mesh");
+    SN_REQUIRE((long long)n_el * 10 < (1LL << 31), "sn_ams_create: too many elements");
+    hipStream_t s = as_stream(stream);
+    sn_ams *A = new sn_ams();
+    A->n_edges = n_edges;
+    A->n_free = n_free_nodes;
+    A->n_el = n_el;
+    A->omega = omega;
+    A->weight = weight;
+    for (int e = 0; e < 2 * n_edges; ++e)
+        SN_REQUIRE(edge_nodes[e] >= 0, "sn_ams_create: negative node id at edge %d", e / 2);
+    for (int a = 0; a < n_free_nodes; ++a)
+        SN_REQUIRE(inc_ptr[a] <= inc_ptr[a + 1], "sn_ams_create: incidence pointers not ascending");
+    const int ninc = inc_ptr[n_free_nodes];
+    for (int k = 0; k < ninc; ++k)
+        SN_REQUIRE(inc_edge[k] >= 0 && inc_edge[k] < n_edges, "sn_ams_create: edge id out of range");
+    int rc;
+    if ((rc = upload(&A->edge_nodes, reinterpret_cast<const int2 *>(edge_nodes), n_edges, s))) return rc;
+    if ((rc = upload(&A->edge_tan, reinterpret_cast<const float4 *>(edge_tan), n_edges, s))) return rc;
+    if ((rc = upload(&A->inc_ptr, inc_ptr, (size_t)n_free_nodes + 1, s))) return rc;
+    if ((rc = upload(&A->inc_edge, inc_edge, (size_t)ninc, s))) return rc;
+    SN_HIP(hipMalloc(&A->loc, (size_t)10 * n_el * sizeof(float)));
+    SN_HIP(hipStreamSynchronize(s));
+    *out = A;
+    return SN_OK;
+}
+
+int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *indices,
+                     const int *gather_ptr, const int *gather_list, const int *agg_ptr,
+                     void *stream) {
+    SN_REQUIRE(A && indptr && indices && gather_ptr && gather_list, "sn_ams_add_level: null argument");
+    SN_REQUIRE(n > 0 && nnz >= n, "sn_ams_add_level: bad sizes (n=%d, nnz=%d)", n, nnz);
+    const int L = (int)A->lv.size();
+    SN_REQUIRE(L > 0 || n == A->n_free, "sn_ams_add_level: level 0 must have one row per free node");
+    SN_REQUIRE(L == 0 || agg_ptr, "sn_ams_add_level: coarse level without aggregates");
+    SN_REQUIRE(indptr[0] == 0 && indptr[n] == nnz, "sn_ams_add_level: indptr does not span nnz");
+    const long long src_n = (L == 0) ? 10LL * A->n_el : (long long)A->lv[L - 1].nnz;
+    std::vector<int> diag(n, -1);
+    for (int i = 0; i < n; ++i) {
+        SN_REQUIRE(indptr[i] <= indptr[i + 1], "sn_ams_add_level: indptr not ascending");
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
+            SN_REQUIRE(indices[k] >= 0 && indices[k] < n, "sn_ams_add_level: column out of range");
+            if (indices[k] == i) diag[i] = k;
+        }
+        SN_REQUIRE(diag[i] >= 0, "sn_ams_add_level: row %d has no diagonal", i);
+    }
+    SN_REQUIRE(gather_ptr[0] == 0, "sn_ams_add_level: gather plan must start at 0");
+    for (int k = 0; k < nnz; ++k)
+        SN_REQUIRE(gather_ptr[k] <= gather_ptr[k + 1], "sn_ams_add_level: gather plan not ascending");
+    const int ng = gather_ptr[nnz];
+    for (int k = 0; k < ng; ++k)
+        SN_REQUIRE(gather_list[k] >= 0 && gather_list[k] < src_n,
+                   "sn_ams_add_level: gather entry %d out of range", k);
+    std::vector<int> coarse_of;
+    if (L > 0) {
+        const int nf = A->lv[L - 1].n;
+        SN_REQUIRE(agg_ptr[0] == 0 && agg_ptr[n] == nf,
+                   "sn_ams_add_level: aggregates must cover the previous level's rows");
+        coarse_of.assign(nf, -1);
+        for (int I = 0; I < n; ++I) {
+            SN_REQUIRE(agg_ptr[I] < agg_ptr[I + 1], "sn_ams_add_level: empty aggregate %d", I);
+            for (int i = agg_ptr[I]; i < agg_ptr[I + 1]; ++i) coarse_of[i] = I;
+        }
+    }
+    hipStream_t s = as_stream(stream);
+    AmsLevel lv;
+    lv.n = n;
+    lv.nnz = nnz;
+    int rc;
+    if ((rc = upload(&lv.indptr, indptr, (size_t)n + 1, s))) return rc;
+    if ((rc = upload(&lv.indices, indices, (size_t)nnz, s))) return rc;
+    if ((rc = upload(&lv.diagpos, diag.data(), (size_t)n, s))) return rc;
+    if ((rc = upload(&lv.gptr, gather_ptr, (size_t)nnz + 1, s))) return rc;
+    if ((rc = upload(&lv.glist, gather_list, (size_t)ng, s))) return rc;
+    if (L > 0) {
+        if ((rc = upload(&lv.agg_ptr, agg_ptr, (size_t)n + 1, s))) return rc;
+        if ((rc = upload(&A->lv[L - 1].coarse_of, coarse_of.data(), coarse_of.size(), s))) return rc;
+    }
+    SN_HIP(hipMalloc(&lv.vals, (size_t)nnz * sizeof(float)));
+    SN_HIP(hipMalloc(&lv.dinv, (size_t)n * sizeof(float)));
+    SN_HIP(hipMalloc(&lv.b, (size_t)n * sizeof(float4)));
+    SN_HIP(hipMalloc(&lv.x, (size_t)n * sizeof(float4)));
+    SN_HIP(hipMalloc(&lv.y, (size_t)n * sizeof(float4)));
+    SN_HIP(hipStreamSynchronize(s));
+    A->lv.push_back(lv);
+    A->ready = false;
+    return SN_OK;
+}
+
+int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float *nu_el,
+                  void *stream) {
+    SN_REQUIRE(A && nodes && cells && nu_el, "sn_ams_update: null argument");
+    SN_REQUIRE(!A->lv.empty(), "sn_ams_update: no levels");
+    const int L = (int)A->lv.size();
+    const int nc = A->lv[L - 1].n;
+    SN_REQUIRE(nc <= kAmsMaxCoarse, "sn_ams_update: coarsest level has %d rows (max %d)", nc,
+               kAmsMaxCoarse);
+    hipStream_t s = as_stream(stream);
+    if (!A->cinv) {
+        SN_HIP(hipMalloc(&A->dense, (size_t)nc * nc * sizeof(double)));
+        SN_HIP(hipMalloc(&A->cinv, (size_t)nc * nc * sizeof(float)));
+    }
+    hipLaunchKernelGGL(ams_lap_local_kernel, dim3(grid_for(A->n_el, kAmsBlock)), dim3(kAmsBlock),
+                       0, s, A->n_el, nodes, cells, nu_el, A->loc);
+    for (int l = 0; l < L; ++l) {
+        AmsLevel &v = A->lv[l];
+        const float *src = (l == 0) ? A->loc : A->lv[l - 1].vals;
+        hipLaunchKernelGGL(ams_gather_kernel, dim3(grid_for(v.nnz, 256)), dim3(256), 0, s, v.nnz,
+                           v.gptr, v.glist, src, v.vals);
+        hipLaunchKernelGGL(ams_dinv_kernel, dim3(grid_for(v.n, 256)), dim3(256), 0, s, v.n,
+                           v.diagpos, v.vals, v.dinv);
+    }
+    AmsLevel &c = A->lv[L - 1];
+    hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(1024), 0, s, nc, c.indptr,
+                       c.indices, c.vals, A->dense, A->cinv);
+    SN_CHECK_LAUNCH();
+    A->ready = true;
+    return SN_OK;
+}
+
+int sn_ams_apply(sn_ams *A, const unsigned char *mask, const double *r, const double *dinv,
+                 double *z, void *stream) {
+    SN_REQUIRE(A && r && dinv && z, "sn_ams_apply: null argument");
+    return ams_launch_apply(A, true, r, dinv, mask, z, nullptr, 0,
+                            grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr,
+                            as_stream(stream));
+}
+
+int sn_ams_level_info(const sn_ams *A, int level, int *n, int *nnz) {
+    SN_REQUIRE(A && level >= 0 && level < (int)A->lv.size(), "sn_ams_level_info: bad level");
+    if (n) *n = A->lv[level].n;
+    if (nnz) *nnz = A->lv[level].nnz;
+    return SN_OK;
+}
+
+int sn_ams_level_values(const sn_ams *A, int level, float *vals, void *stream) {
+    SN_REQUIRE(A && vals && level >= 0 && level < (int)A->lv.size(), "sn_ams_level_values: bad argument");
+    SN_HIP(hipMemcpyAsync(vals, A->lv[level].vals, (size_t)A->lv[level].nnz * sizeof(float),
+                          hipMemcpyDeviceToDevice, as_stream(stream)));
+    return SN_OK;
+}
+
+int sn_ams_num_levels(const sn_ams *A) { return A ? (int)A->lv.size() : 0; }
+
+// Greedy aggregation on the strength graph (host): j is a strong neighbour
+// of i when |a_ij| >= theta sqrt(|a_ii a_jj|).  Pass 1 makes an aggregate of
+// every node whose strong neighbourhood is still unassigned; pass 2 attaches
+// each remaining node to the aggregate of its strongest assigned neighbour
+// (assignments of pass 1 only); pass 3 groups what is left with its
+// unassigned strong neighbours.  Nodes are visited in index order, so the
+// result is deterministic.
+int sn_amg_aggregate(int n, const int *indptr, const int *indices, const double *vals,
+                     double theta, int *agg, int *n_agg) {
+    SN_REQUIRE(n > 0 && indptr && indices && vals && agg && n_agg, "sn_amg_aggregate: bad argument");
+    std::vector<double> d(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
+            SN_REQUIRE(indices[k] >= 0 && indices[k] < n, "sn_amg_aggregate: column out of range");
+            if (indices[k] == i) d[i] = std::fabs(vals[k]);
+        }
+    auto strong = [&](int i, int k) {
+        const int j = indices[k];
+        return j != i && std::fabs(vals[k]) >= theta * std::sqrt(d[i] * d[j]);
+    };
+    for (int i = 0; i < n; ++i) agg[i] = -1;
+    int na = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free_nb = true;
+        for (int k = indptr[i]; k < indptr[i + 1] && free_nb; ++k)
+            if (strong(i, k) && agg[indices[k]] >= 0) free_nb = false;
+        if (!free_nb) continue;
+        agg[i] = na;
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+            if (strong(i, k)) agg[indices[k]] = na;
+        ++na;
+    }
+    std::vector<int> pass1(agg, agg + n);
+    for (int i = 0; i < n; ++i) {
+        if (pass1[i] >= 0) continue;
+        double best = -1.0;
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
+            const int j = indices[k];
+            if (!strong(i, k) || pass1[j] < 0) continue;
+            const double w = std::fabs(vals[k]);
+            if (w > best) {
+                best = w;
+                agg[i] = pass1[j];
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        agg[i] = na;
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k)
+            if (strong(i, k) && agg[indices[k]] < 0) agg[indices[k]] = na;
+        ++na;
+    }
+    *n_agg = na;
+    return SN_OK;
+}
+
+void sn_ams_destroy(sn_ams *A) {
+    if (!A) return;
+    for (AmsLevel &v : A->lv) {
+        void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.coarse_of,
+                        v.vals, v.dinv, v.b, v.x, v.y};
+        for (void *p : bufs)
+            if (p) (void)hipFree(p);
+    }
+    void *bufs[] = {A->edge_nodes, A->edge_tan, A->inc_ptr, A->inc_edge, A->loc, A->dense, A->cinv};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    delete A;
+}
+
+}  // extern "C"

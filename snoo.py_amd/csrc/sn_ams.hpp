@@ -1,0 +1,49 @@
+// Auxiliary-space (Hiptmair-Xu) preconditioner state, shared by sn_ams.hip
+// (set-up, numeric refresh, kernels) and sn_cg.hip (which launches the apply
+// inside its captured CG chunk).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+struct AmsLevel {
+    int n = 0;              // unknowns (level 0: free mesh nodes)
+    int nnz = 0;
+    int *indptr = nullptr;  // [n+1]
+    int *indices = nullptr; // [nnz]
+    int *diagpos = nullptr; // [n]
+    int *gptr = nullptr;    // [nnz+1] gather plan of vals
+    int *glist = nullptr;   // level 0: slot*n_el + e of the element values; else nnz of level-1
+    int *agg_ptr = nullptr; // [n+1] rows of level-1 forming each row here (null at level 0)
+    int *coarse_of = nullptr; // [n] row of level+1 containing each row (null at the coarsest)
+    float *vals = nullptr;  // [nnz]
+    float *dinv = nullptr;  // [n]
+    float4 *b = nullptr, *x = nullptr, *y = nullptr;  // right side, pre-smoothed, result
+};
+
+struct sn_ams {
+    int n_edges = 0, n_free = 0, n_el = 0;
+    int2 *edge_nodes = nullptr;  // [n_edges] level-0 ids of (lo, hi); >= n_free: fixed node
+    float4 *edge_tan = nullptr;  // [n_edges] (x_hi - x_lo) / 2
+    int *inc_ptr = nullptr;      // [n_free+1] free edges at each free node
+    int *inc_edge = nullptr;
+    float *loc = nullptr;        // [10*n_el] element Laplacian pair values
+    std::vector<AmsLevel> lv;
+    double *dense = nullptr;     // [nc*nc] scratch of the coarsest inversion
+    float *cinv = nullptr;       // [nc*nc] inverse of the coarsest matrix
+    float omega = 0.6f;          // damped-Jacobi weight of the nodal smoother
+    float weight = 1.0f;         // weight of the auxiliary correction
+    bool ready = false;          // numeric values present
+};
+
+namespace sn {
+
+// z = dinv .* r + weight * Pi V(Pi^T r) on the free edges (0 elsewhere), with
+// per-block partial sums of r.z written to partials[stride*b], b < grid
+// (blocks of 1024 threads striding over the edges).  `done` (may be null)
+// points to a device flag that turns every kernel into a no-op when set.
+int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
+                     const unsigned char *mask, void *z, double *partials, int stride, int grid,
+                     const int *done, hipStream_t s);
+
+}  // namespace sn

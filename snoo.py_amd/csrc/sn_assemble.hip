@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
     const double *__restrict__ curls, const double *__restrict__ w, int Q, int type_spec,
     int n_int, const double *__restrict__ coeffs, const double *__restrict__ knots,
     const double *__restrict__ x, const double *__restrict__ Bs, void *__restrict__ out_a,
-    double *__restrict__ out_b, double *__restrict__ out6) {
+    double *__restrict__ out_b, double *__restrict__ out6, float *__restrict__ nu_el) {
     __shared__ double s_curl[64 * 18];
     __shared__ double s_c[4 * kMaxLdsIntervals];
     __shared__ double s_x[kMaxLdsIntervals + 1];
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
 #pragma unroll
         for (int k = 0; k < 3; ++k) Bc[k] += G[j][k] * xv[j] * inv_det;
 
-    double sum_nu_w = 0.0;
+    double sum_nu_w = 0.0, sum_w = 0.0;
     double dK[21];
     double rhs[6];
 #pragma unroll
@@ -149,6 +149,7 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
         nu_eval(m, bm, nu, dnu);
         const double wq = w[q];
         sum_nu_w += nu * wq;
+        sum_w += wq;
         if (bm >= 1e-14) {
             const double a = dnu * wq * inv_det / bm;
             double P[6];
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
         }
     }
 
+    if (nu_el) nu_el[e] = (float)(sum_nu_w / sum_w);  // mean reluctivity (nodal preconditioner)
     const double kscale = sum_nu_w * inv_det;
     double Kx[6], dKx[6];
 #pragma unroll
@@ -287,7 +289,7 @@ int sn_element_local_f64(int num_el_total, int el_begin, int el_count, const dou
                        dim3((el_count + kElBlock - 1) / kElBlock), dim3(kElBlock), 0,
                        as_stream(stream), num_el_total, el_begin, el_count, nodes, cells, gids,
                        orient, curls, w, Q, type_spec, n_int, coeffs, knots, x, Bs,
-                       (void *)locK, locdK, loc6);
+                       (void *)locK, locdK, loc6, (float *)nullptr);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -297,7 +299,7 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
                       const double *curls, const double *d_phi, const double *w, int Q,
                       int type_spec, int n_int, const double *coeffs, const double *knots,
                       const double *x, const double *Bs, void *locJ, double *loc6,
-                      void *stream) {
+                      float *nu_el, void *stream) {
     (void)d_phi;
     int rc = check_elem_args(num_el_total, el_begin, el_count, Q, type_spec, n_int, nodes, cells,
                              gids, orient, curls, w, coeffs, knots, x);
@@ -310,7 +312,7 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
 #define SN_LAUNCH_EL(M, T)                                                                     \
     hipLaunchKernelGGL((element_kernel<M, T>), grid, block, 0, s, num_el_total, el_begin,     \
                        el_count, nodes, cells, gids, orient, curls, w, Q, type_spec, n_int,    \
-                       coeffs, knots, x, Bs, locJ, (double *)nullptr, loc6)
+                       coeffs, knots, x, Bs, locJ, (double *)nullptr, loc6, nu_el)
     if (mode == 0) {
         if (out_f32) SN_LAUNCH_EL(MODE_NEWTON_B, float);
         else SN_LAUNCH_EL(MODE_NEWTON_B, double);

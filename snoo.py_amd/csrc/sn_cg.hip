@@ -22,6 +22,9 @@
 #include <cstring>
 #include <vector>
 
+#include <cstddef>
+
+#include "sn_ams.hpp"
 #include "sn_common.hpp"
 #include "sn_pattern.hpp"
 
@@ -200,7 +203,9 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
 }
 
 // ---------------------------------------------------------------- kernel B
-template <typename V, typename X>
+// JAC: z = dinv r here (Jacobi); otherwise z and the r.z partials come from
+// the auxiliary-space apply that follows, and only r.r is written.
+template <typename V, typename X, bool JAC>
 __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
     SegMap sm, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
     const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r, V *__restrict__ z,
@@ -229,16 +234,18 @@ __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
         if (!free_row) continue;
         x[i] += (X)(alpha * pnew[i]);
         const V ri = r[i] - alpha * q[i];
-        const V zi = dinv[i] * ri;
         r[i] = ri;
-        z[i] = zi;
-        rz += (double)ri * (double)zi;
+        if (JAC) {
+            const V zi = dinv[i] * ri;
+            z[i] = zi;
+            rz += (double)ri * (double)zi;
+        }
         rr += (double)ri * (double)ri;
     }
-    const double s1 = block_sum(rz, lds);
+    const double s1 = JAC ? block_sum(rz, lds) : 0.0;
     const double s2 = block_sum(rr, lds);
     if (threadIdx.x == 0) {
-        partB[2 * blockIdx.x] = s1;
+        if (JAC) partB[2 * blockIdx.x] = s1;
         partB[2 * blockIdx.x + 1] = s2;
         if (lb == 0) {
             S->pq = pq;
@@ -367,6 +374,7 @@ struct sn_cg {
     void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
     float *xs = nullptr;
     double *partA = nullptr, *partB = nullptr, *partbb = nullptr;
+    sn_ams *ams = nullptr;       // auxiliary-space preconditioner (null: Jacobi)
     CgState *st = nullptr;       // [nseg]
     CgState *st_host = nullptr;  // [nseg], pinned
     // cached graph of one chunk (2*chunk kernels), keyed by its pointers
@@ -377,6 +385,10 @@ struct sn_cg {
     hipStream_t g_stream = nullptr;
 
     SegMap map() const { return SegMap{row0, blkA0, blkB0, segA, segB}; }
+    const int *done_flag() const {
+        return reinterpret_cast<const int *>(reinterpret_cast<const char *>(st) +
+                                             offsetof(CgState, done));
+    }
 };
 
 namespace sn {
@@ -407,14 +419,20 @@ static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask,
                              int parity, hipStream_t s) {
     launch_spmv_lpr<V, M>(C, vals, mask, parity, s);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
-    if (C->precision == 64)
-        hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
-                           C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (double *)xdst, C->partA, C->partB, C->st, parity);
-    else
-        hipLaunchKernelGGL((cg_update_kernel<V, float>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
-                           C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,
-                           (V *)C->z, (float *)xdst, C->partA, C->partB, C->st, parity);
+#define SN_LAUNCH_UPD(X, JAC)                                                                 \
+    hipLaunchKernelGGL((cg_update_kernel<V, X, JAC>), dim3(C->grid_vec), dim3(kVecBlock), 0, s, \
+                       C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,  \
+                       (V *)C->z, (X *)xdst, C->partA, C->partB, C->st, parity)
+    if (C->ams) {
+        if (C->precision == 64) SN_LAUNCH_UPD(double, false);
+        else SN_LAUNCH_UPD(float, false);
+        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB,
+                               2, C->grid_vec, C->done_flag(), s);
+    } else {
+        if (C->precision == 64) SN_LAUNCH_UPD(double, true);
+        else SN_LAUNCH_UPD(float, true);
+    }
+#undef SN_LAUNCH_UPD
 }
 
 template <typename V, typename M, int LPR>
@@ -437,6 +455,9 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     }
     hipLaunchKernelGGL(cg_init_state_kernel, dim3(C->nseg), dim3(kVecBlock), 0, s, C->map(),
                        C->partbb, C->st, C->rtol_dev, atol, maxiter);
+    if (C->ams)
+        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB, 2,
+                               C->grid_vec, nullptr, s);
 }
 
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
@@ -491,6 +512,7 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
         }
         maxiter = (int)(nfree * 10 > INT_MAX ? INT_MAX : nfree * 10);
     }
+    SN_REQUIRE(!C->ams || C->ams->ready, "CG: auxiliary-space preconditioner has no values yet");
     SN_HIP(hipMemcpyAsync(C->rtol_dev, rtol_host, C->nseg * sizeof(double),
                           hipMemcpyHostToDevice, s));
     void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
@@ -627,6 +649,19 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
 int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
                  void *stream) {
     return sn_cg_create_batched(out, P, precision, lpr, chunk, 1, nullptr, stream);
+}
+
+int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
+    SN_REQUIRE(C, "sn_cg_set_ams: null workspace");
+    SN_REQUIRE(!A || C->nseg == 1, "sn_cg_set_ams: batched workspaces use the Jacobi diagonal");
+    SN_REQUIRE(!A || A->n_edges == C->pat->n, "sn_cg_set_ams: %d edges, matrix has %d rows",
+               A ? A->n_edges : 0, C->pat->n);
+    if (C->graph) {
+        SN_HIP(hipGraphExecDestroy(C->graph));
+        C->graph = nullptr;
+    }
+    C->ams = A;
+    return SN_OK;
 }
 
 void sn_cg_destroy(sn_cg *C) {

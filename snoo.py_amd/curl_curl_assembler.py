@@ -225,9 +225,11 @@ class CurlCurlAssembler:
         return K, dK, J, rhs
 
     def newton_local_device(self, x_dev, quad_order, source_fields_dev, mode, f32, locJ, loc6,
-                            domains=None):
+                            domains=None, nu_el=None):
         """Fused element kernel of the Newton solvers over all domains (or the
-        listed ones; the local values of the others are left as they are)."""
+        listed ones; the local values of the others are left as they are).
+        ``nu_el`` (float32 device tensor, one entry per element in device
+        order, optional) receives each element's mean reluctivity."""
         d = self._device_state()
         T = self.tables(quad_order)
         N = d['num_el']
@@ -240,7 +242,8 @@ class CurlCurlAssembler:
                 int(bool(f32)), int(mode), N, b, e - b, nat.ptr(d['nodes']), nat.ptr(d['cells']),
                 nat.ptr(d['gids']), nat.ptr(d['orient']), nat.ptr(T.curls), nat.ptr(T.dphi),
                 nat.ptr(T.w), T.Q, m.type_spec, m.n_int, nat.ptr(m.c), nat.ptr(m.x),
-                nat.ptr(x_dev), nat.ptr(bs), nat.ptr(locJ), nat.ptr(loc6), nat.stream_handle()))
+                nat.ptr(x_dev), nat.ptr(bs), nat.ptr(locJ), nat.ptr(loc6),
+                nat.ptr(nu_el) if nu_el is not None else None, nat.stream_handle()))
 
     def compute_stiffness_and_jacobi_matrix_c(self, x, quad_order=2, source_fields=[]):
         """(K, J, rhs) as the reference returns them (curl_curl_assembler.py:815-932).

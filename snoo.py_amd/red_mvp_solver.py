@@ -143,25 +143,25 @@ class RedMVPSolver:
 
     def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
                      eta_min=None, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True,
-                     log=None):
+                     log=None, precond='jacobi'):
         """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
 
         Returns the device solution (float64) and fills ``self.stats`` with the
         residual history, per-step CG iteration counts and timings."""
-        drv = self.newton_driver(precision, lanes_per_row, chunk)
+        drv = self.newton_driver(precision, lanes_per_row, chunk, precond)
         drv.retarget(self.source_fields_device())
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
                          stats=self.stats, log=log)
 
-    def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32):
+    def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32, precond='jacobi'):
         if lanes_per_row == 0:
             lanes_per_row = self.tuned_lanes(precision)
-        key = ('drv', precision, lanes_per_row, chunk)
+        key = ('drv', precision, lanes_per_row, chunk, precond)
         if key not in self.__dict__.setdefault('_drivers', {}):
             self._drivers[key] = NewtonDriver(self.curl_curl_factory, self.source_fields_device(),
                                               self.mask_device(), self.quad_order, precision,
-                                              lanes_per_row, chunk)
+                                              lanes_per_row, chunk, precond)
         return self._drivers[key]
 
 
@@ -224,13 +224,19 @@ class NewtonDriver:
     precision "mixed": FP32 Jacobian (element arithmetic in FP64, values
     rounded once), FP32 PCG with FP64 dot products, x += dx in FP64; R(x) is
     always FP64 (element-local K_e x_e - rhs_e, gathered in fixed order).
+    precond "jacobi": the reference's diagonal preconditioner; "ams": the
+    auxiliary-space preconditioner (ams.py), refreshed with the element
+    reluctivities of every Jacobian.
     """
 
     def __init__(self, asm, src_dev, mask_dev, quad_order, precision='fp64', lanes_per_row=8,
-                 chunk=32):
+                 chunk=32, precond='jacobi'):
         import torch
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be "fp64" or "mixed"')
+        if precond not in ('jacobi', 'ams'):
+            raise ValueError('precond must be "jacobi" or "ams"')
+        self.precond = precond
         self.asm, self.src, self.mask, self.quad_order = asm, src_dev, mask_dev, quad_order
         self.precision = precision
         self.f32 = precision == 'mixed'
@@ -253,6 +259,23 @@ class NewtonDriver:
         self._J64 = None
         self.eta_fp64 = 0.0   # steps with eta <= this go straight to float64
         self._r0 = None
+        self.nu_el = torch.empty(N, dtype=torch.float32, device=dev) if precond == 'ams' else None
+        self.ams = None
+
+    def _refresh_preconditioner(self):
+        """Auxiliary-space preconditioner for the Jacobian of the last
+        residual() call: built on first use, values refreshed after."""
+        if self.precond != 'ams':
+            return
+        if self.ams is None:
+            from .ams import AmsPreconditioner
+            free = self.mask.cpu().numpy().astype(bool)
+            self.ams = AmsPreconditioner(self.asm, free, self.nu_el)
+            for cg in (self.cg, self.cg_safe):
+                if cg is not None:
+                    nat.check(nat.load().sn_cg_set_ams(cg.h, self.ams.handle))
+        else:
+            self.ams.update(self.nu_el)
 
     def retarget(self, src_dev):
         """Use new source fields / materials / node coordinates (drops the
@@ -267,7 +290,7 @@ class NewtonDriver:
         """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2."""
         lib = nat.load()
         self.asm.newton_local_device(x_dev, self.quad_order, self.src, 1, self.f32, self.locJ,
-                                     self.loc6)
+                                     self.loc6, nu_el=self.nu_el)
         nat.check(lib.sn_csr_gather_one(self.d['pattern'], int(self.f32), nat.ptr(self.locJ),
                                         nat.ptr(self.J), nat.stream_handle()))
         nat.check(lib.sn_dof_gather_f64(self.d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
@@ -341,6 +364,7 @@ class NewtonDriver:
             eta = max(eta, min(eta_max, 0.5 * rtol * r0 / rn))
             etas.append(eta)
             torch.neg(self.R, out=self.b)
+            self._refresh_preconditioner()
             mode = 'fp64'
             if self.f32 and eta > self.eta_fp64:
                 x_try = x.clone()
@@ -384,5 +408,9 @@ class NewtonDriver:
         stats.update({'residual_history': hist, 'cg_iterations': cg_its, 'etas': etas,
                       'cg_modes': modes,
                       'newton_steps': k, 'converged': converged, 'r0': r0,
-                      't_newton': time.perf_counter() - t0, 'precision': self.precision})
+                      't_newton': time.perf_counter() - t0, 'precision': self.precision,
+                      'precond': self.precond})
+        if self.ams is not None:
+            stats['ams_levels'] = self.ams.plan.sizes()
+            stats['ams_setup_s'] = self.ams.plan.t_setup
         return x

@@ -1,0 +1,141 @@
+"""CPU: the auxiliary-space preconditioner oracle pinned by closed forms, and
+the product's host-side plan (aggregation, numbering, gather plans) checked
+against it."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import ams_oracle as ao
+
+
+@pytest.fixture(scope='module')
+def small():
+    import importlib
+    mg = importlib.import_module('snoopy_amd.magnets')
+    d = mg.Dipole(h=0.1)
+    solver = d.solver()
+    asm = solver.curl_curl_factory
+    order = np.concatenate(asm._sel)
+    tm = d.mesh
+    cells = tm.cells[order]
+    rng = np.random.default_rng(3)
+    nu = np.where(tm.tags[order] == 1, 200.0, 7.9e5) * rng.uniform(0.5, 2.0, len(order))
+    free = solver.make_boundary_mask().astype(bool)
+    return tm, cells, nu, free
+
+
+def test_interpolation_reproduces_linear_fields(small):
+    tm, cells, nu, free = small
+    M = np.array([[0.3, -1.2, 0.5], [2.0, 0.1, -0.7], [0.4, 0.9, 1.1]])
+    c = np.array([0.2, -0.4, 0.9])
+    A = lambda p: p @ M.T + c
+    Pi = ao.interpolation_full(tm.nodes, tm.edges)
+    dofs = Pi @ A(tm.nodes).reshape(-1)
+    np.testing.assert_allclose(dofs, tm.edge_dofs_of_field(A), rtol=1e-12, atol=1e-14)
+
+
+def test_laplacian_closed_forms(small):
+    tm, cells, nu, free = small
+    L = ao.laplacian_full(tm.nodes, cells, nu)
+    scale = np.abs(L.diagonal()).max()
+    np.testing.assert_allclose(L @ np.ones(tm.num_nodes), 0.0, atol=1e-10 * scale)
+    a = np.array([0.7, -1.3, 0.4])
+    phi = tm.nodes @ a
+    Lphi = L @ phi
+    bnd = np.zeros(tm.num_nodes, dtype=bool)
+    bnd[tm.edges[tm.boundary_edges].ravel()] = True
+    # with a constant coefficient, linear functions are discretely harmonic
+    # away from the boundary ...
+    L1 = ao.laplacian_full(tm.nodes, cells, np.ones(len(cells)))
+    assert np.abs((L1 @ phi)[~bnd]).max() < 1e-9 * np.abs(L1.diagonal()).max()
+    # ... and their energy is sum_e nu_e vol_e |a|^2
+    X = tm.nodes[cells]
+    vol = np.abs(np.einsum('mi,mi->m', X[:, 1] - X[:, 0],
+                           np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6
+    np.testing.assert_allclose(phi @ Lphi, np.sum(nu * vol) * (a @ a), rtol=1e-10)
+
+
+def _two_level(seed=0, n=60, nc=9):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n))
+    A = sp.csr_array(B @ B.T + n * np.eye(n))
+    sizes = rng.multinomial(n - nc, np.ones(nc) / nc) + 1
+    agg_ptr = np.concatenate([[0], np.cumsum(sizes)])
+    return A, agg_ptr
+
+
+def test_vcycle_equals_closed_two_level_operator():
+    A, agg_ptr = _two_level()
+    o = ao.OracleAms.__new__(ao.OracleAms)
+    P = ao.prolongation(agg_ptr)
+    o.A = [A, (P.T @ A @ P).tocsr()]
+    o.P = [P]
+    o.omega = 0.6
+    n = A.shape[0]
+    V = np.stack([o.vcycle(np.eye(n)[:, k]) for k in range(n)], axis=1)
+    Ad, Pd = A.toarray(), P.toarray()
+    S = np.diag(0.6 / np.diag(Ad))
+    I = np.eye(n)
+    C = Pd.T @ Ad @ Pd
+    ref = S + (I - S @ Ad) @ S + (I - S @ Ad) @ Pd @ np.linalg.solve(C, Pd.T) @ (I - Ad @ S)
+    np.testing.assert_allclose(V, ref, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(V, V.T, rtol=1e-12, atol=1e-14)
+
+
+def test_product_plan_against_oracle(small):
+    """The product's plan: aggregates contiguous on every level; its gather
+    plans applied to the element values give the oracle's Galerkin matrices;
+    its edge data and incidence give the oracle's Pi and Pi^T."""
+    import importlib
+    ams = importlib.import_module('snoopy_amd.ams')
+    tm, cells, nu, free = small
+    plan = ams.AmsPlan(tm.nodes, tm.edges, cells, free, nu)
+    assert len(plan.levels) >= 3 and plan.levels[-1]['n'] <= 128
+    l0 = np.full(tm.num_nodes, plan.n_free)
+    l0[plan.free_ids] = plan.perm0
+    o = ao.OracleAms(tm.nodes, tm.edges, cells, free, nu, l0,
+                     [lv['agg_ptr'] for lv in plan.levels[1:]])
+    loc = ams.element_laplacians(tm.nodes, cells, nu).T.reshape(-1)   # slot*n_el + e
+    src = loc
+    for l, lv in enumerate(plan.levels):
+        vals = np.add.reduceat(src[lv['glist']], lv['gptr'][:-1]) if lv['nnz'] else src[:0]
+        empty = np.diff(lv['gptr']) == 0
+        vals[empty] = 0.0
+        M = sp.csr_array((vals, lv['indices'], lv['indptr']), shape=(lv['n'], lv['n']))
+        ref = o.A[l]
+        scale = np.abs(ref.data).max()
+        assert np.abs((M - ref).toarray() if lv['n'] < 3000 else (M - ref).data).max() < 1e-10 * scale
+        src = vals
+    # Pi^T through the incidence lists and Pi through the edge end points
+    rng = np.random.default_rng(1)
+    r = np.where(free, rng.standard_normal(tm.num_edges), 0.0)
+    b = np.zeros((plan.n_free, 3))
+    for a in range(plan.n_free):
+        es = plan.inc_edge[plan.inc_ptr[a]:plan.inc_ptr[a + 1]]
+        b[a] = (plan.edge_tan[es, :3] * r[es, None]).sum(axis=0)
+    ref = o.Pi.T @ r   # (tangents are stored in float32)
+    np.testing.assert_allclose(b.reshape(-1), ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+    y = rng.standard_normal((plan.n_free + 1, 3))
+    y[plan.n_free] = 0.0
+    en = np.minimum(plan.edge_nodes, plan.n_free)
+    z = np.einsum('ek,ek->e', plan.edge_tan[:, :3], y[en[:, 0]] + y[en[:, 1]])
+    ref = (o.Pi @ y[:-1].reshape(-1))[free]
+    np.testing.assert_allclose(z[free], ref, rtol=0, atol=1e-6 * np.abs(ref).max())
+
+
+def test_oracle_preconditioner_symmetric_positive(small):
+    import importlib
+    ams = importlib.import_module('snoopy_amd.ams')
+    tm, cells, nu, free = small
+    plan = ams.AmsPlan(tm.nodes, tm.edges, cells, free, nu)
+    l0 = np.full(tm.num_nodes, plan.n_free)
+    l0[plan.free_ids] = plan.perm0
+    o = ao.OracleAms(tm.nodes, tm.edges, cells, free, nu, l0,
+                     [lv['agg_ptr'] for lv in plan.levels[1:]])
+    rng = np.random.default_rng(2)
+    dinv = rng.uniform(0.5, 2.0, tm.num_edges)
+    u = np.where(free, rng.standard_normal(tm.num_edges), 0.0)
+    v = np.where(free, rng.standard_normal(tm.num_edges), 0.0)
+    Mu, Mv = o.apply(u, dinv), o.apply(v, dinv)
+    assert abs(v @ Mu - u @ Mv) < 1e-9 * abs(u @ Mu)
+    assert u @ Mu > 0 and v @ Mv > 0
