@@ -168,52 +168,61 @@ __global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
 }
 
 // ------------------------------------------------------------------- apply
-template <typename V>
+// LPR lanes per free node: b0 = Pi^T r (tangents stored per incidence, so the
+// incidence stream is read in order) and the pre-smoothed x0 = w D^-1 b0.
+template <typename V, int LPR>
 __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     int n_free, const int *__restrict__ inc_ptr, const int *__restrict__ inc_edge,
-    const float4 *__restrict__ tan, const V *__restrict__ r, float4 *__restrict__ b0,
-    const int *__restrict__ done) {
+    const float4 *__restrict__ inc_tan, const V *__restrict__ r, const float *__restrict__ dinv0,
+    float omega, float4 *__restrict__ b0, float4 *__restrict__ x0, const int *__restrict__ done) {
     if (done && *done) return;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_free) return;
+    const int a = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    const int lane = threadIdx.x % LPR;
     float4 acc = f4_zero();
-    for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
-        const int e = inc_edge[k];
-        f4_fma(acc, (float)r[e], tan[e]);
+    if (a < n_free)
+        for (int k = inc_ptr[a] + lane; k < inc_ptr[a + 1]; k += LPR)
+            f4_fma(acc, (float)r[inc_edge[k]], inc_tan[k]);
+    f4_reduce<LPR>(acc);
+    if (a < n_free && lane == 0) {
+        b0[a] = acc;
+        const float d = omega * dinv0[a];
+        x0[a] = make_float4(d * acc.x, d * acc.y, d * acc.z, 0.f);
     }
-    b0[a] = acc;
 }
 
-// wave per aggregate: x = w D^-1 b on its rows; b_c = sum_rows b - sum_nnz a_ij x_j
+// wave per aggregate of the next level: b_c = sum_rows b - sum_nnz a_ij x_j
+// (x = w D^-1 b already stored for this level), and x_c = w D_c^-1 b_c unless
+// the next level is the coarsest
 __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
     int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ indptr,
-    const int *__restrict__ indices, const float *__restrict__ vals,
-    const float *__restrict__ dinv, const float4 *__restrict__ b, float4 *__restrict__ x,
-    float4 *__restrict__ bc, float omega, const int *__restrict__ done) {
+    const int *__restrict__ indices, const float *__restrict__ vals, const float4 *__restrict__ b,
+    const float4 *__restrict__ x, const float *__restrict__ dinv_c, float omega,
+    float4 *__restrict__ bc, float4 *__restrict__ xc, const int *__restrict__ done) {
     if (done && *done) return;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (wid >= n_agg) return;  // uniform over the wave
     const int r0 = agg_ptr[wid], r1 = agg_ptr[wid + 1];
     float4 s = f4_zero();
-    for (int i = r0 + lane; i < r1; i += kWave) {
-        const float4 bi = b[i];
-        const float d = omega * dinv[i];
-        x[i] = make_float4(d * bi.x, d * bi.y, d * bi.z, 0.f);
-        s = f4_add(s, bi);
-    }
+    for (int i = r0 + lane; i < r1; i += kWave) s = f4_add(s, b[i]);
     const int k1 = indptr[r1];
-    for (int k = indptr[r0] + lane; k < k1; k += kWave) {
-        const int j = indices[k];
-        f4_fma(s, -vals[k] * omega * dinv[j], b[j]);
-    }
+    for (int k = indptr[r0] + lane; k < k1; k += kWave) f4_fma(s, -vals[k], x[indices[k]]);
     f4_reduce<kWave>(s);
-    if (lane == 0) bc[wid] = s;
+    if (lane == 0) {
+        bc[wid] = s;
+        if (xc) {
+            const float d = omega * dinv_c[wid];
+            xc[wid] = make_float4(d * s.x, d * s.y, d * s.z, 0.f);
+        }
+    }
 }
 
-// wave per row of the coarsest level: y = C^-1 b
+// wave per row I of the coarsest level: e_I = (C^-1 b)_I, then prolongation
+// into the finer level's pre-smoothed iterate, x_f[i] += e_I for the rows i
+// of aggregate I (or, with a single level, y = e)
 __global__ __launch_bounds__(kAmsBlock) void ams_coarse_kernel(
-    int nc, const float *__restrict__ cinv, const float4 *__restrict__ b, float4 *__restrict__ y,
+    int nc, const float *__restrict__ cinv, const float4 *__restrict__ b,
+    const int *__restrict__ agg_ptr, float4 *__restrict__ x_f, float4 *__restrict__ y,
     const int *__restrict__ done) {
     if (done && *done) return;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -222,34 +231,41 @@ __global__ __launch_bounds__(kAmsBlock) void ams_coarse_kernel(
     float4 acc = f4_zero();
     for (int j = lane; j < nc; j += kWave) f4_fma(acc, cinv[(size_t)wid * nc + j], b[j]);
     f4_reduce<kWave>(acc);
-    if (lane == 0) y[wid] = acc;
+    if (x_f) {
+        for (int i = agg_ptr[wid] + lane; i < agg_ptr[wid + 1]; i += kWave)
+            x_f[i] = f4_add(x_f[i], acc);
+    } else if (lane == 0) {
+        y[wid] = acc;
+    }
 }
 
-// LPR lanes per row: x' = x + e[coarse_of], y = x' + w D^-1 (b - A x')
+// LPR lanes per row of level l, whose x already holds x + P e_(l+1):
+// y = x + w D^-1 (b - A x); then y is prolongated into the finer level
+// (x_f[i] += y for the rows i of this row's aggregate), or stored at level 0
 template <int LPR>
 __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
-    const float *__restrict__ vals, const float *__restrict__ dinv,
-    const int *__restrict__ coarse_of, const float4 *__restrict__ b,
-    const float4 *__restrict__ x, const float4 *__restrict__ ec, float4 *__restrict__ y,
-    float omega, const int *__restrict__ done) {
+    const float *__restrict__ vals, const float *__restrict__ dinv, const float4 *__restrict__ b,
+    const float4 *__restrict__ x, float omega, const int *__restrict__ agg_ptr,
+    float4 *__restrict__ x_f, float4 *__restrict__ y, const int *__restrict__ done) {
     if (done && *done) return;
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
     float4 acc = f4_zero();
-    if (row < n) {
-        for (int k = indptr[row] + lane; k < indptr[row + 1]; k += LPR) {
-            const int j = indices[k];
-            f4_fma(acc, vals[k], f4_add(x[j], ec[coarse_of[j]]));
-        }
-    }
+    if (row < n)
+        for (int k = indptr[row] + lane; k < indptr[row + 1]; k += LPR)
+            f4_fma(acc, vals[k], x[indices[k]]);
     f4_reduce<LPR>(acc);
-    if (row < n && lane == 0) {
-        const float4 xi = f4_add(x[row], ec[coarse_of[row]]);
-        const float4 bi = b[row];
-        const float d = omega * dinv[row];
-        y[row] = make_float4(xi.x + d * (bi.x - acc.x), xi.y + d * (bi.y - acc.y),
-                             xi.z + d * (bi.z - acc.z), 0.f);
+    if (row >= n) return;
+    const float4 xi = x[row], bi = b[row];
+    const float d = omega * dinv[row];
+    const float4 yi = make_float4(xi.x + d * (bi.x - acc.x), xi.y + d * (bi.y - acc.y),
+                                  xi.z + d * (bi.z - acc.z), 0.f);
+    if (x_f) {
+        for (int i = agg_ptr[row] + lane; i < agg_ptr[row + 1]; i += LPR)
+            x_f[i] = f4_add(x_f[i], yi);
+    } else if (lane == 0) {
+        y[row] = yi;
     }
 }
 
@@ -289,24 +305,29 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
                            hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
-    hipLaunchKernelGGL((ams_edge_restrict_kernel<V>), dim3(grid_for(A->n_free, kAmsBlock)),
-                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->edge_tan, r,
-                       l0.b, done);
+    constexpr int kIncLanes = 8;
+    hipLaunchKernelGGL((ams_edge_restrict_kernel<V, kIncLanes>),
+                       dim3(grid_for((long long)A->n_free * kIncLanes, kAmsBlock)),
+                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->inc_tan, r,
+                       l0.dinv, A->omega, l0.b, l0.x, done);
+    const int waves_per_block = kAmsBlock / kWave;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
-        const int waves_per_block = kAmsBlock / kWave;
         hipLaunchKernelGGL(ams_down_kernel, dim3(grid_for(c.n, waves_per_block)),
                            dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, f.indptr, f.indices, f.vals,
-                           f.dinv, f.b, f.x, c.b, A->omega, done);
+                           f.b, f.x, c.dinv, A->omega, c.b,
+                           (l + 2 < L) ? c.x : (float4 *)nullptr, done);
     }
     AmsLevel &lc = A->lv[L - 1];
-    hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, kAmsBlock / kWave)),
-                       dim3(kAmsBlock), 0, s, lc.n, A->cinv, lc.b, lc.y, done);
+    hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)), dim3(kAmsBlock),
+                       0, s, lc.n, A->cinv, lc.b, lc.agg_ptr,
+                       L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, l0.y, done);
     for (int l = L - 2; l >= 0; --l) {
-        AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
+        AmsLevel &f = A->lv[l];
         hipLaunchKernelGGL((ams_up_kernel<8>), dim3(grid_for((long long)f.n * 8, kAmsBlock)),
-                           dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv,
-                           f.coarse_of, f.b, f.x, c.y, f.y, A->omega, done);
+                           dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv, f.b,
+                           f.x, A->omega, f.agg_ptr, l > 0 ? A->lv[l - 1].x : (float4 *)nullptr,
+                           l0.y, done);
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->edge_tan, l0.y,
@@ -365,6 +386,11 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const i
     if ((rc = upload(&A->edge_tan, reinterpret_cast<const float4 *>(edge_tan), n_edges, s))) return rc;
     if ((rc = upload(&A->inc_ptr, inc_ptr, (size_t)n_free_nodes + 1, s))) return rc;
     if ((rc = upload(&A->inc_edge, inc_edge, (size_t)ninc, s))) return rc;
+    std::vector<float> itan((size_t)4 * (ninc > 0 ? ninc : 1), 0.0f);
+    for (int k = 0; k < ninc; ++k)
+        for (int c = 0; c < 4; ++c) itan[4 * (size_t)k + c] = edge_tan[4 * (size_t)inc_edge[k] + c];
+    if ((rc = upload(&A->inc_tan, reinterpret_cast<const float4 *>(itan.data()), (size_t)ninc, s)))
+        return rc;
     SN_HIP(hipMalloc(&A->loc, (size_t)10 * n_el * sizeof(float)));
     SN_HIP(hipStreamSynchronize(s));
     *out = A;
@@ -397,16 +423,12 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     for (int k = 0; k < ng; ++k)
         SN_REQUIRE(gather_list[k] >= 0 && gather_list[k] < src_n,
                    "sn_ams_add_level: gather entry %d out of range", k);
-    std::vector<int> coarse_of;
     if (L > 0) {
         const int nf = A->lv[L - 1].n;
         SN_REQUIRE(agg_ptr[0] == 0 && agg_ptr[n] == nf,
                    "sn_ams_add_level: aggregates must cover the previous level's rows");
-        coarse_of.assign(nf, -1);
-        for (int I = 0; I < n; ++I) {
+        for (int I = 0; I < n; ++I)
             SN_REQUIRE(agg_ptr[I] < agg_ptr[I + 1], "sn_ams_add_level: empty aggregate %d", I);
-            for (int i = agg_ptr[I]; i < agg_ptr[I + 1]; ++i) coarse_of[i] = I;
-        }
     }
     hipStream_t s = as_stream(stream);
     AmsLevel lv;
@@ -418,15 +440,13 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     if ((rc = upload(&lv.diagpos, diag.data(), (size_t)n, s))) return rc;
     if ((rc = upload(&lv.gptr, gather_ptr, (size_t)nnz + 1, s))) return rc;
     if ((rc = upload(&lv.glist, gather_list, (size_t)ng, s))) return rc;
-    if (L > 0) {
+    if (L > 0)
         if ((rc = upload(&lv.agg_ptr, agg_ptr, (size_t)n + 1, s))) return rc;
-        if ((rc = upload(&A->lv[L - 1].coarse_of, coarse_of.data(), coarse_of.size(), s))) return rc;
-    }
     SN_HIP(hipMalloc(&lv.vals, (size_t)nnz * sizeof(float)));
     SN_HIP(hipMalloc(&lv.dinv, (size_t)n * sizeof(float)));
     SN_HIP(hipMalloc(&lv.b, (size_t)n * sizeof(float4)));
     SN_HIP(hipMalloc(&lv.x, (size_t)n * sizeof(float4)));
-    SN_HIP(hipMalloc(&lv.y, (size_t)n * sizeof(float4)));
+    if (L == 0) SN_HIP(hipMalloc(&lv.y, (size_t)n * sizeof(float4)));
     SN_HIP(hipStreamSynchronize(s));
     A->lv.push_back(lv);
     A->ready = false;
@@ -549,12 +569,13 @@ int sn_amg_aggregate(int n, const int *indptr, const int *indices, const double 
 void sn_ams_destroy(sn_ams *A) {
     if (!A) return;
     for (AmsLevel &v : A->lv) {
-        void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.coarse_of,
-                        v.vals, v.dinv, v.b, v.x, v.y};
+        void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.vals,
+                        v.dinv, v.b, v.x, v.y};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }
-    void *bufs[] = {A->edge_nodes, A->edge_tan, A->inc_ptr, A->inc_edge, A->loc, A->dense, A->cinv};
+    void *bufs[] = {A->edge_nodes, A->edge_tan, A->inc_ptr, A->inc_edge, A->inc_tan, A->loc,
+                    A->dense, A->cinv};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     delete A;

@@ -15,10 +15,11 @@ struct AmsLevel {
     int *gptr = nullptr;    // [nnz+1] gather plan of vals
     int *glist = nullptr;   // level 0: slot*n_el + e of the element values; else nnz of level-1
     int *agg_ptr = nullptr; // [n+1] rows of level-1 forming each row here (null at level 0)
-    int *coarse_of = nullptr; // [n] row of level+1 containing each row (null at the coarsest)
     float *vals = nullptr;  // [nnz]
     float *dinv = nullptr;  // [n]
-    float4 *b = nullptr, *x = nullptr, *y = nullptr;  // right side, pre-smoothed, result
+    float4 *b = nullptr;    // right-hand side of the cycle at this level
+    float4 *x = nullptr;    // pre-smoothed iterate, then + prolongated coarse correction
+    float4 *y = nullptr;    // level 0 only: the cycle's result
 };
 
 struct sn_ams {
@@ -27,6 +28,7 @@ struct sn_ams {
     float4 *edge_tan = nullptr;  // [n_edges] (x_hi - x_lo) / 2
     int *inc_ptr = nullptr;      // [n_free+1] free edges at each free node
     int *inc_edge = nullptr;
+    float4 *inc_tan = nullptr;   // tangent of each incidence's edge, in incidence order
     float *loc = nullptr;        // [10*n_el] element Laplacian pair values
     std::vector<AmsLevel> lv;
     double *dense = nullptr;     // [nc*nc] scratch of the coarsest inversion

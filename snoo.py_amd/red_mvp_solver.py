@@ -261,6 +261,11 @@ class NewtonDriver:
         self._r0 = None
         self.nu_el = torch.empty(N, dtype=torch.float32, device=dev) if precond == 'ams' else None
         self.ams = None
+        if precond == 'ams' and self.f32:
+            # with the auxiliary-space preconditioner the float32 CG reaches
+            # a few 1e-3 before the rounding of J stalls it; tighter forcing
+            # terms go to float64 from the start
+            self.eta_fp64 = 4.9e-3
 
     def _refresh_preconditioner(self):
         """Auxiliary-space preconditioner for the Jacobian of the last

@@ -61,12 +61,20 @@ def test_apply_matches_oracle(case):
     free = drv.mask.cpu().numpy().astype(bool)
     r = np.where(free, rng.standard_normal(n), 0.0)
     dinv = rng.uniform(0.5, 2.0, n)
-    z_ref = o.apply(r, dinv)
-    z = drv.ams.apply(drv.mask, torch.from_numpy(r).cuda(), torch.from_numpy(dinv).cuda())
+    rd = torch.from_numpy(r).cuda()
+    # the auxiliary part alone (zero diagonal part), which runs in float32
+    z_ref = o.apply(r, np.zeros(n))
+    z = drv.ams.apply(drv.mask, rd, torch.zeros(n, dtype=torch.float64, device='cuda'))
     z = z.cpu().numpy()
     err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
-    print('apply max rel diff %.2e' % err, flush=True)
+    print('auxiliary part max rel diff %.2e' % err, flush=True)
     assert err < 1e-4
+    # the whole preconditioner
+    z_ref = o.apply(r, dinv)
+    z = drv.ams.apply(drv.mask, rd, torch.from_numpy(dinv).cuda()).cpu().numpy()
+    err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+    print('apply max rel diff %.2e' % err, flush=True)
+    assert err < 1e-6
     assert np.all(z[~free] == 0.0)
 
 
