@@ -48,8 +48,11 @@ def parse():
     ap.add_argument('--cpu-cg-iters', type=int, default=600)
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--lanes', type=int, default=0, help='CG lanes per row (0 = autotune)')
-    ap.add_argument('--precond', default='jacobi', choices=['jacobi', 'ams'],
+    ap.add_argument('--precond', default='ams', choices=['jacobi', 'ams'],
                     help='CG preconditioner of the timed solves')
+    ap.add_argument('--ams-weight', type=float, default=None)
+    ap.add_argument('--ams-omega', type=float, default=None)
+    ap.add_argument('--ams-theta', type=float, default=None)
     ap.add_argument('--workload', default='dipole1m', choices=['dipole1m', 'batch64', 'chain6'],
                     help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
                          'batched vs one at a time; chain6: 6-yoke chain')
@@ -157,21 +160,37 @@ def main():
     except Exception as e:  # no trained model in the tree
         log('warm start unavailable:', e)
 
-    def one_solve(precision, rtol, use_warm):
+    ams_opts = {k: v for k, v in (('weight', args.ams_weight), ('omega', args.ams_omega),
+                                   ('theta', args.ams_theta)) if v is not None}
+
+    def one_solve(precision, rtol, use_warm, precond=None):
+        precond = precond or args.precond
         solver._src = None                      # Biot-Savart is part of every solve
+        if precond == 'ams':
+            solver.newton_driver(precision, args.lanes, 32, 'ams').ams_options = ams_opts
         x0 = warm.predict(dp) if (use_warm and warm is not None) else None
         return solver.solve_newton(precision, rtol=rtol, x0=x0, lanes_per_row=args.lanes, log=log,
-                                   precond=args.precond)
+                                   precond=precond)
 
-    # plain FP64 GPU solve (the drop-in method at the configuration's tolerance)
-    one_solve('fp64', args.rtol, False)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    x64 = one_solve('fp64', args.rtol, False)
-    torch.cuda.synchronize()
-    t_fp64 = time.perf_counter() - t
-    st64 = dict(solver.stats)
-    log('fp64: %.3f s, newton %d, cg %s' % (t_fp64, st64['newton_steps'], st64['cg_iterations']))
+    def timed_once(precision, precond):
+        one_solve(precision, args.rtol, False, precond)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = one_solve(precision, args.rtol, False, precond)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        st = dict(solver.stats)
+        log('%s/%s: %.3f s, newton %d, cg %s' % (precision, precond, dt, st['newton_steps'],
+                                                 st['cg_iterations']))
+        return dt, st, x
+
+    # FP64 GPU solves: with the reference's Jacobi preconditioner (the
+    # drop-in method) and with the chosen preconditioner
+    t_jac, st_jac, _ = timed_once('fp64', 'jacobi')
+    if args.precond == 'jacobi':
+        t_fp64, st64, x64 = t_jac, st_jac, _
+    else:
+        t_fp64, st64, x64 = timed_once('fp64', args.precond)
 
     # chosen form: mixed precision (+ warm start)
     for _ in range(args.warmup):
@@ -187,6 +206,7 @@ def main():
                                                     stm['cg_iterations']))
     if args.quick:
         print(json.dumps({'quick': True, 'fp64_s': t_fp64, 'mixed_s': t_mixed,
+                          'fp64_jacobi_s': t_jac,
                           'fp64_stats': {k: v for k, v in st64.items()},
                           'mixed_stats': {k: v for k, v in stm.items()}}), flush=True)
         return
@@ -202,7 +222,7 @@ def main():
     err_64, _ = b_error(B64, Bref)
 
     # roofline of the dominant kernel: the CG SpMV kernel of the mixed solve
-    drv = solver.newton_driver('mixed', args.lanes, 32)
+    drv = solver.newton_driver('mixed', args.lanes, 32, args.precond)
     ms = ctypes.c_double()
     nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
                                          ctypes.byref(ms), nat.stream_handle()))
@@ -235,7 +255,7 @@ def main():
     if not args.no_cpu:
         log('cpu baseline leg')
         try:
-            cpu = cpu_baseline(dp, solver, st64, args.cpu_cg_iters, args.cpu_threads)
+            cpu = cpu_baseline(dp, solver, st_jac, args.cpu_cg_iters, args.cpu_threads)
         except Exception as e:
             log('cpu baseline failed:', repr(e))
 
@@ -257,9 +277,15 @@ def main():
                                'B-spline-in-B^2 iron, Newton to 1e-8' % (N, n, nnz),
                    'tets': N, 'edges': n, 'nnz': nnz, 'iron_fraction': dp.iron_fraction(),
                    'precision': 'mixed (FP32 Jacobian/SpMV/CG, FP64 residual and dots)',
+                   'preconditioner': args.precond,
                    'warm_start': warm is not None, 'parallelism': 'single GPU'},
         'fp64_solves_per_s': 1.0 / t_fp64,
+        'fp64_jacobi_solves_per_s': 1.0 / t_jac,
         'speedup_vs_fp64': t_fp64 / t_mixed,
+        'speedup_vs_fp64_jacobi': t_jac / t_mixed,
+        'cg_iterations_fp64_jacobi': st_jac['cg_iterations'],
+        'ams_levels': stm.get('ams_levels'),
+        'ams_setup_s': stm.get('ams_setup_s'),
         'B_rel_err_max': err_m,
         'B_rel_err_l2': err_m_l2,
         'B_rel_err_fp64': err_64,
@@ -300,7 +326,7 @@ def bench_batch(args):
     tets = [d.num_tets for d in dps]
     log('batch: %d variants, %d..%d tets, total %d (mesh %.1f s)'
         % (len(dps), min(tets), max(tets), sum(tets), time.perf_counter() - t0))
-    bs = BatchedSolver(dps, precision='mixed', lanes_per_row=8)
+    bs = BatchedSolver(dps, precision='mixed', lanes_per_row=8, precond=args.precond)
     log('joined: %d edges, nnz %d, setup %.1f s' % (bs.n, bs.asm.nnz, bs.t_setup))
     bs.solve(rtol=args.rtol, log=log)
     torch.cuda.synchronize()
@@ -316,10 +342,10 @@ def bench_batch(args):
     t_seq = 0.0
     for d in dps:
         s = d.solver(quad_order=2, verbose=False)
-        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8)
+        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8, precond=args.precond)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8)
+        s.solve_newton('mixed', rtol=args.rtol, lanes_per_row=8, precond=args.precond)
         torch.cuda.synchronize()
         t_seq += time.perf_counter() - t
     log('one at a time: %.3f s' % t_seq)
@@ -330,7 +356,7 @@ def bench_batch(args):
         'higher_is_better': True, 'dtype': 'f32+f64', 'data': 'synthetic',
         'config': {'workload': 'batch of %d seeded dipole variants' % len(dps),
                    'tets_total': sum(tets), 'tets_min': min(tets), 'tets_max': max(tets),
-                   'edges_total': bs.n, 'nnz_total': bs.asm.nnz},
+                   'edges_total': bs.n, 'nnz_total': bs.asm.nnz, 'precond': args.precond},
         'batched_s': t_batch, 'one_at_a_time_s': t_seq, 'speedup': t_seq / t_batch,
         'converged': conv, 'newton_steps': bs.stats['newton_steps']}), flush=True)
 
@@ -350,15 +376,20 @@ def bench_chain(args):
     log('chain: %d tets, %d edges, nnz %d, setup %.1f s'
         % (dp.num_tets, asm.num_dofs, asm.nnz, time.perf_counter() - t0))
     out = {}
-    for prec in ('fp64', 'mixed'):
+    forms = [('fp64', 'jacobi'), ('mixed', args.precond)]
+    if args.precond != 'jacobi':
+        forms.insert(1, ('fp64', args.precond))
+    for prec, pc in forms:
         solver._src = None
-        solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8, log=log)
+        solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8, log=log, precond=pc)
         torch.cuda.synchronize()
         t = time.perf_counter()
         solver._src = None
-        x = solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8)
+        x = solver.solve_newton(prec, rtol=args.rtol, lanes_per_row=8, precond=pc)
         torch.cuda.synchronize()
-        out[prec] = (time.perf_counter() - t, dict(solver.stats), x)
+        out[(prec, pc)] = (time.perf_counter() - t, dict(solver.stats), x)
+        log('%s/%s: %.3f s, cg %s' % (prec, pc, out[(prec, pc)][0],
+                                      out[(prec, pc)][1]['cg_iterations']))
     axis = dp.beam_axis(5000)
     from snoopy_amd.mesh import locate_points
     from snoopy_amd.sources import source_field_device
@@ -366,18 +397,18 @@ def bench_chain(args):
     el = locate_points(dp.mesh, axis)
     Bs = source_field_device(dp.coils, as_dev_f64(axis)).cpu().numpy()
     B = {p: Bs + asm.element_B(out[p][2].cpu().numpy())[el] for p in out}
-    err = float(np.linalg.norm(B['mixed'] - B['fp64'], axis=1).max() /
-                np.linalg.norm(B['fp64'], axis=1).max())
+    ref = B[('fp64', 'jacobi')]
+    chosen = ('mixed', args.precond)
+    err = float(np.linalg.norm(B[chosen] - ref, axis=1).max() / np.linalg.norm(ref, axis=1).max())
     print(json.dumps({
         'metric': 'solves/s, 6-yoke chain, %d tets, Newton to %g' % (dp.num_tets, args.rtol),
-        'value': 1.0 / out['mixed'][0], 'unit': 'solves/s', 'n_gpus': 1,
+        'value': 1.0 / out[chosen][0], 'unit': 'solves/s', 'n_gpus': 1,
         'higher_is_better': True, 'dtype': 'f32+f64', 'data': 'synthetic',
         'config': {'workload': '6 alternating yokes along z', 'tets': dp.num_tets,
-                   'edges': asm.num_dofs, 'nnz': asm.nnz},
-        'fp64_solves_per_s': 1.0 / out['fp64'][0], 'mixed_s': out['mixed'][0],
-        'fp64_s': out['fp64'][0], 'B_axis_rel_err_mixed_vs_fp64': err,
-        'cg_iterations_mixed': out['mixed'][1]['cg_iterations'],
-        'cg_iterations_fp64': out['fp64'][1]['cg_iterations']}), flush=True)
+                   'edges': asm.num_dofs, 'nnz': asm.nnz, 'precond': args.precond},
+        'seconds': {'%s/%s' % k: v[0] for k, v in out.items()},
+        'cg_iterations': {'%s/%s' % k: v[1]['cg_iterations'] for k, v in out.items()},
+        'B_axis_rel_err_vs_fp64_jacobi': err}), flush=True)
 
 
 if __name__ == '__main__':

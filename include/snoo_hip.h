@@ -275,8 +275,10 @@ int sn_ams_level_info(const sn_ams *ams, int level, int *n, int *nnz);
 int sn_ams_level_values(const sn_ams *ams, int level, float *vals_dev, void *stream);
 void sn_ams_destroy(sn_ams *ams);
 /* Use `ams` (NULL: the Jacobi diagonal) as the preconditioner of every later
- * sn_cg_solve on this workspace; the handle must stay alive and be updated
- * (sn_ams_update) before the solves.  Single-system workspaces only. */
+ * sn_cg_solve / sn_cg_solve_batched on this workspace; the handle must stay
+ * alive and be updated (sn_ams_update) before the solves.  For a batched
+ * workspace the preconditioner is built on the joined (block-diagonal) mesh,
+ * so its cycle never couples two segments. */
 int sn_cg_set_ams(sn_cg *cg, sn_ams *ams);
 
 /* ------------------------------------------------- learned warm start

@@ -123,6 +123,10 @@ class AmsPlan:
         while levels[-1].shape[0] > coarse_max and len(levels) < max_levels:
             Al = levels[-1]
             agg, na = aggregate(Al, theta)
+            if na > 0.5 * Al.shape[0]:
+                # too few strong connections (anisotropic or weakly coupled
+                # rows): aggregate over every connection instead
+                agg, na = aggregate(Al, 0.0)
             if na >= Al.shape[0]:
                 break
             fr = np.repeat(np.arange(Al.shape[0]), np.diff(Al.indptr))
@@ -203,7 +207,7 @@ class AmsPlan:
 class AmsPreconditioner:
     """Device handle of the preconditioner for one assembler and mask."""
 
-    def __init__(self, asm, free_edge, nu_el_dev, omega=0.6, weight=1.0, theta=0.08,
+    def __init__(self, asm, free_edge, nu_el_dev, omega=0.7, weight=2.0, theta=0.08,
                  coarse_max=128):
         d = asm._device_state()
         tm = asm.tet_mesh
@@ -225,6 +229,7 @@ class AmsPreconditioner:
                                            nat.ptr_np(lv['agg_ptr']) if l > 0 else None,
                                            nat.stream_handle()))
         self._asm = asm
+        self.omega, self.weight = float(omega), float(weight)
         self.update(nu_el_dev)
 
     def update(self, nu_el_dev):

@@ -55,7 +55,8 @@ def join_meshes(meshes):
 class BatchedSolver:
     """Newton for a list of Dipole objects, all on the device at once."""
 
-    def __init__(self, dipoles, quad_order=2, precision='mixed', lanes_per_row=8, chunk=32):
+    def __init__(self, dipoles, quad_order=2, precision='mixed', lanes_per_row=8, chunk=32,
+                 precond='jacobi'):
         import torch
         self.dipoles = list(dipoles)
         self.nvar = len(self.dipoles)
@@ -97,8 +98,27 @@ class BatchedSolver:
             self.locJ64 = torch.empty(21 * N, dtype=torch.float64, device=device())
             self.loc6b = torch.empty(6 * N, dtype=torch.float64, device=device())
             self.J64 = torch.empty(nnz, dtype=torch.float64, device=device())
+        self.precond = precond
+        self.nu_el = torch.empty(N, dtype=torch.float32, device=device()) \
+            if precond == 'ams' else None
+        self.ams = None
         self.t_setup = time.perf_counter() - t0
         self.src = None
+
+    def _refresh_preconditioner(self):
+        """Auxiliary-space preconditioner of the joined mesh (its cycle is
+        block diagonal like the Jacobian) for the last assembled Jacobian."""
+        if self.precond != 'ams':
+            return
+        if self.ams is None:
+            from .ams import AmsPreconditioner
+            self.ams = AmsPreconditioner(self.asm, self.mask.cpu().numpy().astype(bool),
+                                         self.nu_el)
+            for h in (self.cg, self.cg_safe):
+                if h:
+                    nat.check(nat.load().sn_cg_set_ams(h, self.ams.handle))
+        else:
+            self.ams.update(self.nu_el)
 
     def __del__(self):
         try:
@@ -130,7 +150,7 @@ class BatchedSolver:
         if active is not None:
             doms = [2 * v + k for v in range(self.nvar) if active[v] for k in (0, 1)]
         self.asm.newton_local_device(x, self.quad_order, self.src, 1, self.f32, self.locJ,
-                                     self.loc6, domains=doms)
+                                     self.loc6, domains=doms, nu_el=self.nu_el)
         nat.check(lib.sn_csr_gather_one(d['pattern'], int(self.f32), nat.ptr(self.locJ),
                                         nat.ptr(self.J), nat.stream_handle()))
         nat.check(lib.sn_dof_gather_f64(d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
@@ -178,16 +198,24 @@ class BatchedSolver:
             active = self._segment_mask(~conv)
             torch.mul(self.R, active, out=self.b)
             self.b.neg_()
+            self._refresh_preconditioner()
             rt = (ctypes.c_double * self.nvar)(*[float(e) for e in eta])
             its = (ctypes.c_int * self.nvar)()
             status = (ctypes.c_int * self.nvar)()
             if self.f32:
+                # float32 CG; with the auxiliary-space preconditioner, forcing
+                # terms below 5e-3 go to float64 directly (the rounding of J
+                # stalls the float32 CG there)
+                force64 = (eta < 4.9e-3) & ~conv if self.precond == 'ams' else \
+                    np.zeros(self.nvar, dtype=bool)
                 x_try = x.clone()
+                b32 = self.b * self._segment_mask(~force64) if force64.any() else self.b
+                cap = 600 if self.precond == 'ams' else 4000
                 rc = lib.sn_cg_solve_batched(self.cg, nat.ptr(self.J), nat.ptr(self.mask),
-                                             nat.ptr(self.b), nat.ptr(x_try), rt, 0.0,
-                                             min(max_cg, 4000), its, None, status,
+                                             nat.ptr(b32), nat.ptr(x_try), rt, 0.0,
+                                             min(max_cg, cap), its, None, status,
                                              nat.stream_handle())
-                stalled = np.array(status[:]) == nat.SN_ERR_NOT_CONVERGED
+                stalled = (np.array(status[:]) == nat.SN_ERR_NOT_CONVERGED) | force64
                 if stalled.any():
                     # redo the stalled segments in float64 (Jacobian and CG)
                     self.asm.newton_local_device(x, self.quad_order, self.src, 1, False,
@@ -236,7 +264,8 @@ class BatchedSolver:
                        max(cg_its[-1]), time.perf_counter() - t0))
         torch.cuda.synchronize()
         self.stats = {'residual_history': np.array(hist), 'cg_iterations': cg_its,
-                      'newton_steps': k, 'converged': conv, 't_solve': time.perf_counter() - t0}
+                      'newton_steps': k, 'converged': conv, 't_solve': time.perf_counter() - t0,
+                      'precond': self.precond}
         return x
 
     def split(self, x):

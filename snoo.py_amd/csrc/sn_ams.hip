@@ -269,17 +269,31 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     }
 }
 
+// Blocks of a batched solve serve one row segment each (seg_of_blk, blk0,
+// row0 as in the CG's vector kernel), so every partial sum of r.z belongs to
+// one segment; without segments the grid strides over all edges.
 template <typename V>
 __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
     const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
     const float4 *__restrict__ tan, const float4 *__restrict__ y0, float weight,
-    V *__restrict__ z, double *__restrict__ partials, int stride, const int *__restrict__ done) {
+    V *__restrict__ z, double *__restrict__ partials, int stride, const int *__restrict__ row0,
+    const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
+    const int *__restrict__ done) {
     __shared__ double lds[kAmsEdgeBlock / kWave];
     if (done && *done) return;
+    long long e0 = 0, e1 = n_edges;
+    int lb = blockIdx.x, nbl = gridDim.x;
+    if (seg_of_blk) {
+        const int seg = seg_of_blk[blockIdx.x];
+        e0 = row0[seg];
+        e1 = row0[seg + 1];
+        lb = blockIdx.x - blk0[seg];
+        nbl = blk0[seg + 1] - blk0[seg];
+    }
     double rz = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_edges;
-         i += (long long)gridDim.x * blockDim.x) {
+    for (long long i = e0 + (long long)lb * blockDim.x + threadIdx.x; i < e1;
+         i += (long long)nbl * blockDim.x) {
         const bool f = mask ? mask[i] != 0 : true;
         V zi = 0;
         if (f) {
@@ -301,8 +315,8 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
 
 template <typename V>
 static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
-                           double *partials, int stride, int grid, const int *done,
-                           hipStream_t s) {
+                           double *partials, int stride, int grid, const AmsSegments *seg,
+                           const int *done, hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
     constexpr int kIncLanes = 8;
@@ -331,19 +345,20 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->edge_tan, l0.y,
-                       A->weight, z, partials, stride, done);
+                       A->weight, z, partials, stride, seg ? seg->row0 : nullptr,
+                       seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done);
 }
 
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
                      const unsigned char *mask, void *z, double *partials, int stride, int grid,
-                     const int *done, hipStream_t s) {
+                     const AmsSegments *seg, const int *done, hipStream_t s) {
     SN_REQUIRE(A && A->ready, "AMS preconditioner used before sn_ams_update");
     if (v64)
         launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
-                               partials, stride, grid, done, s);
+                               partials, stride, grid, seg, done, s);
     else
         launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
-                              partials, stride, grid, done, s);
+                              partials, stride, grid, seg, done, s);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -488,7 +503,7 @@ int sn_ams_apply(sn_ams *A, const unsigned char *mask, const double *r, const do
                  double *z, void *stream) {
     SN_REQUIRE(A && r && dinv && z, "sn_ams_apply: null argument");
     return ams_launch_apply(A, true, r, dinv, mask, z, nullptr, 0,
-                            grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr,
+                            grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr, nullptr,
                             as_stream(stream));
 }
 

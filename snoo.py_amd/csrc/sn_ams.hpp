@@ -40,12 +40,20 @@ struct sn_ams {
 
 namespace sn {
 
+// Row segments of a batched CG and the blocks serving each (device arrays).
+struct AmsSegments {
+    const int *row0;        // [nseg+1]
+    const int *blk0;        // [nseg+1]
+    const int *seg_of_blk;  // [grid]
+};
+
 // z = dinv .* r + weight * Pi V(Pi^T r) on the free edges (0 elsewhere), with
 // per-block partial sums of r.z written to partials[stride*b], b < grid
-// (blocks of 1024 threads striding over the edges).  `done` (may be null)
-// points to a device flag that turns every kernel into a no-op when set.
+// (blocks of 1024 threads striding over the edges, or over the rows of
+// their segment when `seg` is given).  `done` (may be null) points to a
+// device flag that turns every kernel into a no-op when set.
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
                      const unsigned char *mask, void *z, double *partials, int stride, int grid,
-                     const int *done, hipStream_t s);
+                     const AmsSegments *seg, const int *done, hipStream_t s);
 
 }  // namespace sn

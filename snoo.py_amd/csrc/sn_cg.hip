@@ -426,8 +426,9 @@ static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask,
     if (C->ams) {
         if (C->precision == 64) SN_LAUNCH_UPD(double, false);
         else SN_LAUNCH_UPD(float, false);
+        const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB,
-                               2, C->grid_vec, C->done_flag(), s);
+                               2, C->grid_vec, &seg, C->nseg == 1 ? C->done_flag() : nullptr, s);
     } else {
         if (C->precision == 64) SN_LAUNCH_UPD(double, true);
         else SN_LAUNCH_UPD(float, true);
@@ -455,9 +456,11 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     }
     hipLaunchKernelGGL(cg_init_state_kernel, dim3(C->nseg), dim3(kVecBlock), 0, s, C->map(),
                        C->partbb, C->st, C->rtol_dev, atol, maxiter);
-    if (C->ams)
+    if (C->ams) {
+        const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB, 2,
-                               C->grid_vec, nullptr, s);
+                               C->grid_vec, &seg, nullptr, s);
+    }
 }
 
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
@@ -653,7 +656,6 @@ int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int c
 
 int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
     SN_REQUIRE(C, "sn_cg_set_ams: null workspace");
-    SN_REQUIRE(!A || C->nseg == 1, "sn_cg_set_ams: batched workspaces use the Jacobi diagonal");
     SN_REQUIRE(!A || A->n_edges == C->pat->n, "sn_cg_set_ams: %d edges, matrix has %d rows",
                A ? A->n_edges : 0, C->pat->n);
     if (C->graph) {

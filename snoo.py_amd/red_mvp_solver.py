@@ -261,6 +261,7 @@ class NewtonDriver:
         self._r0 = None
         self.nu_el = torch.empty(N, dtype=torch.float32, device=dev) if precond == 'ams' else None
         self.ams = None
+        self.ams_options = {}   # omega, weight, theta, coarse_max of AmsPreconditioner
         if precond == 'ams' and self.f32:
             # with the auxiliary-space preconditioner the float32 CG reaches
             # a few 1e-3 before the rounding of J stalls it; tighter forcing
@@ -275,7 +276,7 @@ class NewtonDriver:
         if self.ams is None:
             from .ams import AmsPreconditioner
             free = self.mask.cpu().numpy().astype(bool)
-            self.ams = AmsPreconditioner(self.asm, free, self.nu_el)
+            self.ams = AmsPreconditioner(self.asm, free, self.nu_el, **self.ams_options)
             for cg in (self.cg, self.cg_safe):
                 if cg is not None:
                     nat.check(nat.load().sn_cg_set_ams(cg.h, self.ams.handle))

@@ -36,7 +36,8 @@ def _oracle(d, solver, drv):
     l0[plan.free_ids] = plan.perm0
     nu = drv.nu_el.cpu().numpy().astype(np.float64)
     return ao.OracleAms(tm.nodes, tm.edges, tm.cells[order], drv.mask.cpu().numpy().astype(bool),
-                        nu, l0, [lv['agg_ptr'] for lv in plan.levels[1:]])
+                        nu, l0, [lv['agg_ptr'] for lv in plan.levels[1:]],
+                        omega=drv.ams.omega, weight=drv.ams.weight)
 
 
 def test_level_matrices_match_oracle(case):

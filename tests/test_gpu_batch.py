@@ -51,16 +51,18 @@ def test_segmented_cg_matches_single_solves(variants):
         assert np.abs(Bb - Bs).max() <= 1e-4 * np.abs(Bs).max()
 
 
-@pytest.mark.parametrize('precision', ['mixed', 'fp64'])
-def test_batched_newton_matches_single_newton(variants, precision):
+@pytest.mark.parametrize('precision,precond', [('mixed', 'jacobi'), ('fp64', 'jacobi'),
+                                               ('mixed', 'ams'), ('fp64', 'ams')])
+def test_batched_newton_matches_single_newton(variants, precision, precond):
     from snoopy_amd.batch import BatchedSolver
-    bs = BatchedSolver(variants, precision=precision)
+    bs = BatchedSolver(variants, precision=precision, precond=precond)
     x = bs.solve(rtol=1e-8)
     assert bs.stats['converged'].all()
+    print(precision, precond, 'batched cg', bs.stats['cg_iterations'], flush=True)
     parts = bs.split(x)
     for v, d in enumerate(variants):
         s = d.solver(quad_order=2, verbose=False)
-        xv = s.solve_newton(precision, rtol=1e-8)
+        xv = s.solve_newton(precision, rtol=1e-8, precond=precond)
         asm = s.curl_curl_factory
         Bb = asm.element_B(parts[v])
         Bs = asm.element_B(xv.cpu().numpy())
