@@ -221,7 +221,10 @@ def main():
     err_m, err_m_l2 = b_error(Bm, Bref)
     err_64, _ = b_error(B64, Bref)
 
-    # roofline of the dominant kernel: the CG SpMV kernel of the mixed solve
+    # roofline of the dominant kernel: the CG SpMV kernel of the mixed solve,
+    # 200 back-to-back launches on the solver's stream between two HIP events
+    # (inside a solve the same kernel runs with a colder cache; the kernel
+    # trace summary in profiles/ gives both figures)
     drv = solver.newton_driver('mixed', args.lanes, 32, args.precond)
     ms = ctypes.c_double()
     nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
@@ -247,6 +250,7 @@ def main():
     log('roofline: spmv %.4f ms (%.0f GB/s), element kernel %.3f ms' % (ms.value, spmv_gbs, asm_ms))
     traffic = None
     tfile = os.path.join(ROOT, 'profiles', 'cg_spmv_f32_dipole1M_traffic.json')
+    # (PMC counts of the same kernel at this mesh and lane count; see profiles/)
     if os.path.exists(tfile):
         tr = json.load(open(tfile))
         if tr.get('tets') == N and tr.get('lanes_per_row') == args.lanes:
@@ -298,7 +302,7 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': traffic,
                      'kernel': 'cg_spmv_kernel<float,%d>' % args.lanes,
-                     'avg_ms': ms.value, 'algorithmic_bytes': spmv_bytes},
+                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': spmv_bytes},
         'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
                             'GB/s': asm_bytes / (asm_ms * 1e-3) / 1e9},
         'cpu_baseline': cpu,

@@ -339,7 +339,8 @@ class NewtonDriver:
         ``stall_steps`` consecutive steps that reduce ||R|| by less than 10%
         (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
         linear solve at min(10 n, 50000) iterations; a float32 solve is cut at
-        4x the previous step's count (at least 1500) and, if it stalls there,
+        4x the previous step's count (at least 1500, 300 with the auxiliary-
+        space preconditioner) and, if it stalls there,
         the step is redone in float64 (Jacobian and CG), as are later steps
         with a forcing term at least as small."""
         import torch
@@ -373,8 +374,9 @@ class NewtonDriver:
             self._refresh_preconditioner()
             mode = 'fp64'
             if self.f32 and eta > self.eta_fp64:
+                floor = 300 if self.precond == 'ams' else 1500
+                cap32 = max(floor, 4 * max(cg_its[-1] if cg_its else 0, 1))
                 x_try = x.clone()
-                cap32 = max(1500, 4 * max(cg_its[-1] if cg_its else 0, 1))
                 its, _, rc = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
                                            maxiter=min(cap32, max_cg))
                 mode = 'fp32'
