@@ -217,6 +217,13 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
                 const double *b_dev, double *x_dev, double rtol, double atol,
                 int maxiter, int *iters_host, double *rnorm_host, void *stream);
 
+/* Row schedule of the CG SpMV kernel (a tuning knob, results identical up
+ * to the summation order of p.q): bit 0 = each workgroup takes one
+ * contiguous span of rows (else workgroups stride over the rows), bit 1 =
+ * workgroups renumbered so each of the 8 XCDs takes one contiguous eighth of
+ * them.  Default 0. */
+int sn_cg_set_schedule(sn_cg *cg, int schedule);
+
 /* Average duration (ms) of the CG SpMV kernel (kernel A: p = z + beta p,
  * q = A p, p.q) over `reps` back-to-back launches on `stream`, timed with HIP
  * events; run after a solve on the same workspace (it overwrites the CG

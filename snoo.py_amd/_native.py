@@ -71,6 +71,7 @@ SIGNATURES = {
     'sn_ams_level_values': (_c_int, [_vp, _c_int, _vp, _vp]),
     'sn_ams_destroy': (None, [_vp]),
     'sn_cg_set_ams': (_c_int, [_vp, _vp]),
+    'sn_cg_set_schedule': (_c_int, [_vp, _c_int]),
 }
 
 SN_OK, SN_ERR_ARG, SN_ERR_HIP, SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN = 0, 1, 2, 3, 4

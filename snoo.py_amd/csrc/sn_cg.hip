@@ -148,12 +148,17 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     const M *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
     const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
     const double *__restrict__ partB, double *__restrict__ partA, CgState *__restrict__ st,
-    int parity) {
+    int parity, int sched) {
     __shared__ double lds[kRowBlock / kWave];
     const int seg = sm.segA[blockIdx.x];
     CgState *S = st + seg;
     if (S->done) return;
-    const int lb = blockIdx.x - sm.blkA0[seg], nbl = sm.blkA0[seg + 1] - sm.blkA0[seg];
+    const int nbl = sm.blkA0[seg + 1] - sm.blkA0[seg];
+    int lb = blockIdx.x - sm.blkA0[seg];
+    // schedule bit 1: workgroups are dealt round-robin to the 8 XCDs;
+    // renumber them so that each XCD takes one contiguous eighth of the
+    // blocks (needs a multiple of 8 blocks)
+    if ((sched & 2) && (nbl & 7) == 0) lb = (lb & 7) * (nbl >> 3) + (lb >> 3);
     const int b0 = sm.blkB0[seg], b1 = sm.blkB0[seg + 1];
     const double rho = reduce_range<kRowBlock>(partB, b0, b1, 2, 0, lds);
     const double rr = reduce_range<kRowBlock>(partB, b0, b1, 2, 1, lds);
@@ -171,9 +176,22 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     const int r0 = sm.row0[seg], r1 = sm.row0[seg + 1];
     const int lane = threadIdx.x % LPR;
     const int rows_per_block = kRowBlock / LPR;
+    // schedule bit 0: each block takes one contiguous span of rows;
+    // otherwise blocks stride over the rows in steps of the grid
+    const long long per = (long long)nbl * rows_per_block;
+    long long c0, c1, step;
+    if (sched & 1) {
+        const long long span = (r1 - r0 + per - 1) / per * rows_per_block;
+        c0 = r0 + (long long)lb * span;
+        c1 = c0 + span < r1 ? c0 + span : r1;
+        step = rows_per_block;
+    } else {
+        c0 = r0 + (long long)lb * rows_per_block;
+        c1 = r1;
+        step = per;
+    }
     double pq = 0.0;
-    for (long long row = r0 + (long long)lb * rows_per_block + threadIdx.x / LPR; row < r1;
-         row += (long long)nbl * rows_per_block) {
+    for (long long row = c0 + threadIdx.x / LPR; row < c1; row += step) {
         const int i = (int)row;
         V acc = 0;
         const int k1 = indptr[i + 1];
@@ -364,6 +382,7 @@ struct sn_cg {
     int precision = 64;  // vector precision
     bool mat32 = false;  // float32 matrix values (with float64 vectors when precision = 64)
     int lpr = 8;
+    int sched = 0;   // SpMV row schedule: bit 0 contiguous spans, bit 1 XCD-grouped blocks
     int chunk = 32;
     int nseg = 1;
     int grid_rows = 0;   // blocks of kernel A
@@ -401,7 +420,7 @@ static void launch_spmv(sn_cg *C, const M *vals, const unsigned char *mask, int 
     V *pnew = (V *)(parity ? C->p0 : C->p1);
     hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s,
                        C->map(), P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew,
-                       (V *)C->q, C->partB, C->partA, C->st, parity);
+                       (V *)C->q, C->partB, C->partA, C->st, parity, C->sched);
 }
 
 template <typename V, typename M>
@@ -652,6 +671,16 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
 int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int chunk,
                  void *stream) {
     return sn_cg_create_batched(out, P, precision, lpr, chunk, 1, nullptr, stream);
+}
+
+int sn_cg_set_schedule(sn_cg *C, int sched) {
+    SN_REQUIRE(C && sched >= 0 && sched <= 3, "sn_cg_set_schedule: schedule in [0, 3]");
+    if (C->graph) {
+        SN_HIP(hipGraphExecDestroy(C->graph));
+        C->graph = nullptr;
+    }
+    C->sched = sched;
+    return SN_OK;
 }
 
 int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
