@@ -65,14 +65,12 @@ def parse():
 
 
 def probe_B(dp, solver, x_dev, probes):
-    """B = B_s + curl A at the probe points (element-constant curl A)."""
-    from snoopy_amd.mesh import locate_points
-    from snoopy_amd.sources import source_field_device
-    from snoopy_amd.device import as_dev_f64
-    el = locate_points(dp.mesh, probes)
-    Be = solver.curl_curl_factory.element_B(x_dev.cpu().numpy())
-    Bs = source_field_device(dp.coils, as_dev_f64(probes)).cpu().numpy()
-    return Bs + Be[el]
+    """B at the probe points as the reference evaluates it: coil field plus
+    element-averaged curl A at the nodes of each domain, P1-interpolated
+    (evaluate_fem_solution_node_interpolation)."""
+    from snoopy_amd.evaluators import evaluate_fem_solution_node_interpolation
+    return evaluate_fem_solution_node_interpolation(dp.model, probes, solver, dp.coils,
+                                                    x_dev.cpu().numpy(), [1, 2])
 
 
 def b_error(B, Bref):
@@ -395,12 +393,7 @@ def bench_chain(args):
         log('%s/%s: %.3f s, cg %s' % (prec, pc, out[(prec, pc)][0],
                                       out[(prec, pc)][1]['cg_iterations']))
     axis = dp.beam_axis(5000)
-    from snoopy_amd.mesh import locate_points
-    from snoopy_amd.sources import source_field_device
-    from snoopy_amd.device import as_dev_f64
-    el = locate_points(dp.mesh, axis)
-    Bs = source_field_device(dp.coils, as_dev_f64(axis)).cpu().numpy()
-    B = {p: Bs + asm.element_B(out[p][2].cpu().numpy())[el] for p in out}
+    B = {p: probe_B(dp, solver, out[p][2], axis) for p in out}
     ref = B[('fp64', 'jacobi')]
     chosen = ('mixed', args.precond)
     err = float(np.linalg.norm(B[chosen] - ref, axis=1).max() / np.linalg.norm(ref, axis=1).max())

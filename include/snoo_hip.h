@@ -72,6 +72,22 @@ int sn_B_Hcurl_f64(int num_el, const double *nodes_dev, const int *cells_dev,
                    const double *x_dev, double *points_dev, double *B_dev,
                    void *stream);
 
+/* Element averaging of element vectors at the mesh nodes: out [n_nodes*3] =
+ * mean of Be [*, 3] over the elements node_el[node_ptr[i] .. node_ptr[i+1])
+ * (0 for an empty list).  Replaces the node loop of compute_nodal_fields
+ * (snoopy/curl_curl_assembler.py:2089-2197) for the chosen domains. */
+int sn_nodal_average_f64(int n_nodes, const int *node_ptr_dev, const int *node_el_dev,
+                         const double *Be_dev, double *out_dev, void *stream);
+
+/* P1 interpolation of nodal vector fields at points: point p lies in element
+ * el_dev[p] (cells_dev [*, 4], -1: outside) and uses field field_dev[p] of
+ * nodal_dev [n_fields][n_nodes][3] (-1: outside); out [n_pts*3], 0 outside.
+ * Replaces the point loop of evaluate_interpolation
+ * (snoopy/evaluators.py:297-395) once points are located. */
+int sn_p1_interp_f64(int n_pts, const double *pts_dev, const int *el_dev, const int *field_dev,
+                     int n_nodes, const double *nodes_dev, const int *cells_dev,
+                     const double *nodal_dev, double *out_dev, void *stream);
+
 /* ---------------------------------------------------- pattern and scatter */
 
 typedef struct sn_pattern sn_pattern;

@@ -413,3 +413,40 @@ def brick_field(brick, points):
                 out[m] = vals[d]
                 break
     return out
+
+
+# ------------------------------------------------------------ probe fields
+
+def nodal_fields(num_nodes, cells_list, B_el_list):
+    """compute_nodal_fields (curl_curl_assembler.py:2089-2197): element vectors
+    of the chosen domains summed into their nodes, divided by the number of
+    those elements at each node (nodes with none stay 0).  cells_list /
+    B_el_list: one [n_el, 4] / [n_el, 3] array per chosen domain."""
+    B = np.zeros((num_nodes, 3))
+    mult = np.zeros(num_nodes, dtype=np.int64)
+    for cells, Be in zip(cells_list, B_el_list):
+        for j in range(len(cells)):
+            for a in cells[j]:
+                B[a] += Be[j]
+                mult[a] += 1
+    m = mult != 0
+    B[m] /= mult[m][:, None]
+    return B
+
+
+def p1_interpolate(points, nodes, cells, el, field_of_point, nodal_list):
+    """evaluate_interpolation (evaluators.py:297-411) for located points:
+    local coordinates (u, v, w) of the point in its element, P1 basis
+    (1-u-v-w, u, v, w), the nodal field of the point's domain; points with
+    no element or domain get 0."""
+    out = np.zeros((len(points), 3))
+    for p in range(len(points)):
+        e, f = el[p], field_of_point[p]
+        if e < 0 or f < 0:
+            continue
+        X = nodes[cells[e]]
+        M = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+        u, v, w = np.linalg.solve(M, points[p] - X[0])
+        phi = np.array([1.0 - u - v - w, u, v, w])
+        out[p] = phi @ nodal_list[f][cells[e]]
+    return out

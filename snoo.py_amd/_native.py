@@ -71,6 +71,8 @@ SIGNATURES = {
     'sn_ams_level_values': (_c_int, [_vp, _c_int, _vp, _vp]),
     'sn_ams_destroy': (None, [_vp]),
     'sn_cg_set_ams': (_c_int, [_vp, _vp]),
+    'sn_nodal_average_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_p1_interp_f64': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_cg_set_schedule': (_c_int, [_vp, _c_int]),
 }
 

@@ -205,6 +205,77 @@ __global__ __launch_bounds__(256) void b_hcurl_kernel(
     }
 }
 
+
+// ------------------------------------------------------------ probe fields
+// Nodal fields by element averaging (snoopy/curl_curl_assembler.py:2089-2197):
+// out[i] = mean of the element vectors listed for node i (0 when none); the
+// lists are in ascending element order, so the sum is deterministic.
+__global__ void nodal_average_kernel(int n_nodes, const int *__restrict__ node_ptr,
+                                     const int *__restrict__ node_el,
+                                     const double *__restrict__ Be, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int k0 = node_ptr[i], k1 = node_ptr[i + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int e = node_el[k];
+        s0 += Be[3 * e];
+        s1 += Be[3 * e + 1];
+        s2 += Be[3 * e + 2];
+    }
+    const double c = (k1 > k0) ? (double)(k1 - k0) : 1.0;
+    out[3 * i] = s0 / c;
+    out[3 * i + 1] = s1 / c;
+    out[3 * i + 2] = s2 / c;
+}
+
+// P1 interpolation of one of several nodal vector fields at points with a
+// known containing element (snoopy/evaluators.py:297-395): barycentric
+// coordinates from the element's nodes, field chosen per point; points
+// without an element get 0.
+__global__ void p1_interp_kernel(int n_pts, const double *__restrict__ pts,
+                                 const int *__restrict__ el, const int *__restrict__ field,
+                                 int n_nodes, const double *__restrict__ nodes,
+                                 const int *__restrict__ cells, const double *__restrict__ nodal,
+                                 double *__restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pts) return;
+    const int e = el[p], f = field[p];
+    double r[3] = {0.0, 0.0, 0.0};
+    if (e >= 0 && f >= 0) {
+        int nd[4];
+        double X[4][3];
+        for (int a = 0; a < 4; ++a) {
+            nd[a] = cells[4 * e + a];
+            for (int k = 0; k < 3; ++k) X[a][k] = nodes[3 * nd[a] + k];
+        }
+        // columns e1..e3 of the element map; lambda_1..3 = E^-1 (x - X0)
+        double E[3][3], d[3];
+        for (int k = 0; k < 3; ++k) {
+            for (int c = 0; c < 3; ++c) E[k][c] = X[c + 1][k] - X[0][k];
+            d[k] = pts[3 * p + k] - X[0][k];
+        }
+        const double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
+                           E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                           E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+        double lam[4];
+        // Cramer's rule for E lam = d
+        for (int c = 0; c < 3; ++c) {
+            double M[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 3; ++j) M[k][j] = (j == c) ? d[k] : E[k][j];
+            lam[c + 1] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                          M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                          M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+        }
+        lam[0] = 1.0 - lam[1] - lam[2] - lam[3];
+        const double *F = nodal + (size_t)f * n_nodes * 3;
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 3; ++k) r[k] += lam[a] * F[3 * nd[a] + k];
+    }
+    for (int k = 0; k < 3; ++k) out[3 * p + k] = r[k];
+}
+
 }  // namespace sn
 
 using namespace sn;
@@ -265,6 +336,30 @@ int sn_B_Hcurl_f64(int num_el, const double *nodes, const int *cells, const int 
     hipLaunchKernelGGL(b_hcurl_kernel, dim3((num_el + 255) / 256), dim3(256), 0,
                        as_stream(stream), num_el, nodes, cells, gids, orient, curls, phi, Q, x,
                        pts, B);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_nodal_average_f64(int n_nodes, const int *node_ptr, const int *node_el, const double *Be,
+                         double *out, void *stream) {
+    SN_REQUIRE(n_nodes > 0 && node_ptr && node_el && Be && out,
+               "sn_nodal_average_f64: bad argument");
+    hipLaunchKernelGGL(nodal_average_kernel, dim3(grid_for(n_nodes, 256)), dim3(256), 0,
+                       as_stream(stream), n_nodes, node_ptr, node_el, Be, out);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_p1_interp_f64(int n_pts, const double *pts, const int *el, const int *field, int n_nodes,
+                     const double *nodes, const int *cells, const double *nodal, double *out,
+                     void *stream) {
+    SN_REQUIRE(n_pts >= 0 && n_nodes > 0, "sn_p1_interp_f64: bad sizes");
+    if (n_pts == 0) return SN_OK;
+    SN_REQUIRE(pts && el && field && nodes && cells && nodal && out,
+               "sn_p1_interp_f64: null argument");
+    hipLaunchKernelGGL(p1_interp_kernel, dim3(grid_for(n_pts, 256)), dim3(256), 0,
+                       as_stream(stream), n_pts, pts, el, field, n_nodes, nodes, cells, nodal,
+                       out);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }

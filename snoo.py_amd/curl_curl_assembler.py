@@ -311,17 +311,8 @@ class CurlCurlAssembler:
         return out
 
     def compute_nodal_fields(self, x, mat_ids):
-        """Element-averaged B at the mesh nodes (curl_curl_assembler.py:2089-2197)."""
-        tm = self.tet_mesh
-        Be = self.element_B(x)
-        B_ret = np.zeros((tm.num_nodes, 3))
-        mult = np.zeros(tm.num_nodes, dtype=np.int64)
-        for mid in mat_ids:
-            n = self.volume_tags.index(mid)
-            sel = self._sel[n]
-            for a in range(4):
-                np.add.at(B_ret, tm.cells[sel, a], Be[sel])
-                np.add.at(mult, tm.cells[sel, a], 1)
-        m = mult != 0
-        B_ret[m] /= mult[m][:, None]
-        return B_ret
+        """Element-averaged B at the mesh nodes (curl_curl_assembler.py:2089-2197):
+        the element-constant curl A of the elements of the listed material tags,
+        averaged at each node over those elements (0 at nodes none touches)."""
+        from .evaluators import nodal_fields_device
+        return nodal_fields_device(self, as_dev_f64(x), list(mat_ids)).cpu().numpy()

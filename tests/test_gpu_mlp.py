@@ -25,9 +25,10 @@ def test_gemm_bf16_matches_float64(ta, tb, M, N, K):
     Z = torch.empty_like(C)
     lda = M if ta else K
     ldb = K if tb else N
+    dbias = torch.from_numpy(bias).cuda()
     for epi in (0, 1):
         nat.check(nat.load().sn_gemm_bf16(ta, tb, epi, M, N, K, nat.ptr(dA), lda, nat.ptr(dB), ldb,
-                                          nat.ptr(torch.from_numpy(bias).cuda()), None,
+                                          nat.ptr(dbias), None,
                                           nat.ptr(Z), nat.ptr(C), None, N, nat.stream_handle()))
         got = C.cpu().numpy()
         want = ref if epi == 0 else mo.gelu(ref)

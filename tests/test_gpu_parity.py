@@ -294,3 +294,36 @@ def test_zero_excitation_returns_zero(dipole):
                      verbose=False)
     x = s.solve()
     assert np.all(x == 0.0)
+
+
+def test_probe_fields_match_oracle(dipole, solver, oprob):
+    """evaluate_fem_solution_node_interpolation on the device against the
+    oracle composition: element curl A (oracle B_hcurl), coil field at the
+    nodes (oracle Biot-Savart), per-domain nodal averages and P1
+    interpolation (evaluators.py:297-453, curl_curl_assembler.py:2089-2197)."""
+    from snoopy_amd.evaluators import evaluate_fem_solution_node_interpolation
+    from snoopy_amd.mesh import locate_points
+    x = solver.solve_newton('fp64', rtol=1e-8).cpu().numpy()
+    probes = dipole.probes(2000, seed=5)
+    # some probes in the iron: widen the box beyond the aperture
+    probes[:500] *= np.array([4.0, 4.0, 1.0])
+    got = evaluate_fem_solution_node_interpolation(dipole.model, probes, solver, dipole.coils, x,
+                                                   [1, 2])
+    asm = solver.curl_curl_factory
+    tm = asm.tet_mesh
+    cells_l, Be_l = [], []
+    for dmn in range(asm.num_materials):
+        inp = orc.domain_inputs(asm, dmn, 2)
+        _, Bq = orc.B_hcurl(inp['nodes'], inp['cells'], inp['gids'], inp['orient'],
+                            inp['curls'], inp['phi'], inp['d_phi'], x)
+        cells_l.append(np.asarray(inp['cells']).reshape(-1, 4))
+        Be_l.append(Bq.reshape(len(cells_l[-1]), -1, 3)[:, 0, :])
+    b_coil = orc.coil_field(dipole.coils, tm.nodes)
+    nodal = [orc.nodal_fields(tm.num_nodes, [cells_l[k]], [Be_l[k]]) + b_coil for k in (0, 1)]
+    el = locate_points(tm, probes)
+    field = np.where(el >= 0, np.where(tm.tags[np.maximum(el, 0)] == 1, 0, 1), -1)
+    want = orc.p1_interpolate(probes, tm.nodes, tm.cells, el, field, nodal)
+    print('probe fields: max |B| %.3f, in iron %d' % (np.abs(want).max(), int((field == 0).sum())),
+          flush=True)
+    assert (field == 0).sum() > 0
+    assert _rel(got, want) < 1e-10

@@ -417,3 +417,34 @@ def test_csr_pattern_is_union_of_element_blocks():
     np.testing.assert_array_equal(A.indptr, B.indptr)
     np.testing.assert_array_equal(A.indices, B.indices)
     assert np.all(np.diff(A.indptr) > 0)
+
+
+def test_nodal_fields_and_interpolation_closed_forms():
+    """The probe-field oracle: a uniform element field is reproduced at the
+    nodes and at any point; nodes on a domain interface average only the
+    chosen domain's elements; P1 interpolation is exact for linear fields."""
+    from snoopy_amd.mesh import kuhn_mesh, locate_points
+    xs = np.linspace(-1, 1, 6)
+    tag = lambda x, y, z: np.where(x < 0, 1, 2)
+    m = kuhn_mesh(xs, xs, xs, tag)
+    B0 = np.array([0.3, -1.1, 0.7])
+    nodes, cells = m.nodes, m.cells
+    iron = cells[m.tags == 1]
+    air = cells[m.tags == 2]
+    Bn = orc.nodal_fields(m.num_nodes, [iron, air],
+                          [np.tile(B0, (len(iron), 1)), np.tile(B0, (len(air), 1))])
+    np.testing.assert_allclose(Bn, np.tile(B0, (m.num_nodes, 1)), rtol=0, atol=1e-15)
+    # interface nodes: iron-only average of an iron-only field stays e_x
+    Bi = orc.nodal_fields(m.num_nodes, [iron], [np.tile([1.0, 0, 0], (len(iron), 1))])
+    touched = np.zeros(m.num_nodes, dtype=bool)
+    touched[iron.ravel()] = True
+    np.testing.assert_allclose(Bi[touched], np.tile([1.0, 0, 0], (touched.sum(), 1)))
+    assert np.all(Bi[~touched] == 0.0)
+    # linear nodal field: exact P1 interpolation at random points
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-0.99, 0.99, (200, 3))
+    el = locate_points(m, pts)
+    Mlin = np.array([[1.0, 2.0, -1.0], [0.5, 0.0, 3.0], [-2.0, 1.0, 0.2]])
+    nodal = nodes @ Mlin.T + 0.3
+    got = orc.p1_interpolate(pts, nodes, cells, el, np.zeros(len(pts), dtype=int), [nodal])
+    np.testing.assert_allclose(got, pts @ Mlin.T + 0.3, rtol=1e-12, atol=1e-12)
