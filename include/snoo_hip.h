@@ -125,7 +125,13 @@ int sn_pattern_copy(const sn_pattern *pat, int *indptr_dev, int *indices_dev, in
  * w_dev [Q]; coeffs_dev [4*n_int] (or [1] for SN_NU_CONST), knots_dev
  * [n_int+1].  The element is the affine tetrahedron: the q = 0 slices of the
  * curl and gradient tables are used for every q (both are constant on the
- * lowest-order element); B_s, nu and nu' vary with q. */
+ * lowest-order element); B_s, nu and nu' vary with q.
+ * Diluted steel: dilution_host = {c_num, m_num, c_den, m_den} (NULL: none)
+ * replaces compute_K_dK_Hcurl_red_dil (fem_c/ctools.c:1565-1759) as bound by
+ * fem_c/interface_c.pyx:698-926: with z the physical z of quadrature point q
+ * (phi_dev [Q*4], the P1 basis at the points), s = (c_num + z m_num) /
+ * (c_den + z m_den), nu_q = s nu(s|B_q|), nu'_q = s^2 nu'(s|B_q|).  Like the
+ * reference, only source (Bs_dev != NULL) domains are diluted. */
 int sn_element_local_f64(int num_el_total, int el_begin, int el_count,
                          const double *nodes_dev, const int *cells_dev,
                          const int *glob_ids_dev, const int *orient_dev,
@@ -134,7 +140,7 @@ int sn_element_local_f64(int num_el_total, int el_begin, int el_count,
                          const double *coeffs_dev, const double *knots_dev,
                          const double *x_dev, const double *Bs_dev,
                          double *locK_dev, double *locdK_dev, double *loc6_dev,
-                         void *stream);
+                         const double *phi_dev, const double *dilution_host, void *stream);
 
 /* CSR values of K, dK and J = dK + K from the local pair buffers (the COO->CSR
  * sums of fem_c/interface_c.pyx:367-368 and the K += ..., dK + K of
@@ -155,7 +161,8 @@ int sn_dof_gather_f64(const sn_pattern *pat, const double *loc6_dev,
  * Element arithmetic is float64 in both precisions.  Bs_dev == NULL marks a
  * source-free (air) domain.  nu_el_dev (may be NULL) receives the element's
  * mean reluctivity sum_q nu_q w_q / sum_q w_q (float32, indexed like the
- * elements), the coefficient of the auxiliary-space preconditioner. */
+ * elements), the coefficient of the auxiliary-space preconditioner.
+ * phi_dev / dilution_host: diluted steel, as for sn_element_local_f64. */
 int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin,
                       int el_count, const double *nodes_dev, const int *cells_dev,
                       const int *glob_ids_dev, const int *orient_dev,
@@ -163,7 +170,8 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin,
                       const double *w_dev, int Q, int type_spec, int n_int,
                       const double *coeffs_dev, const double *knots_dev,
                       const double *x_dev, const double *Bs_dev,
-                      void *locJ_dev, double *loc6_dev, float *nu_el_dev, void *stream);
+                      void *locJ_dev, double *loc6_dev, float *nu_el_dev,
+                      const double *phi_dev, const double *dilution_host, void *stream);
 
 /* CSR values from one local pair buffer, float64 or float32. */
 int sn_csr_gather_one(const sn_pattern *pat, int f32, const void *loc_dev,

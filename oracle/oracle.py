@@ -56,6 +56,9 @@ def lib():
         L.or_K_dK_red.restype = None
         L.or_K_dK_red.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp,
                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        L.or_K_dK_red_dil.restype = None
+        L.or_K_dK_red_dil.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ci, _ci, _ci,
+                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.or_K_dK_air.restype = None
         L.or_K_dK_air.argtypes = [_ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp,
                                   _vp, _vp, _vp, _vp, _vp]
@@ -94,6 +97,10 @@ def ref_lib():
         R.compute_K_dK_Hcurl_red.restype = None
         R.compute_K_dK_Hcurl_red.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                              _vp, _vp]
+        R.make_dilution.restype = _vp
+        R.make_dilution.argtypes = [_cd, _cd, _cd, _cd]
+        R.compute_K_dK_Hcurl_red_dil.restype = None
+        R.compute_K_dK_Hcurl_red_dil.argtypes = [_vp] * 14
         R.compute_B_Hcurl.restype = None
         R.compute_B_Hcurl.argtypes = [_vp, _vp, _ci, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
         R.compute_B_line_segs.restype = None
@@ -136,8 +143,11 @@ def nu_eval(type_spec, c, x, B):
 
 # ----------------------------------------------------------------- elements
 
-def element_triplets(nodes, cells, gids, orient, curls, d_phi, w, model, x, Bs=None):
-    """36 triplets per element and rhs, reference order (ctools.c:960-1388).
+def element_triplets(nodes, cells, gids, orient, curls, d_phi, w, model, x, Bs=None,
+                     phi=None, dilution=None):
+    """36 triplets per element and rhs, reference order (ctools.c:960-1388;
+    with ``dilution`` = (c_num, m_num, c_den, m_den) and ``phi`` [Q, 4] the
+    diluted kernel, ctools.c:1565-1759).
 
     model = (type_spec, n_int, c_flat, knots)."""
     ts, n_int, c, kn = model
@@ -154,7 +164,13 @@ def element_triplets(nodes, cells, gids, orient, curls, d_phi, w, model, x, Bs=N
             _f64(x)]
     args = [ne, _p(hold[0]), _p(hold[1]), _p(hold[2]), _p(hold[3]), _p(hold[4]), _p(hold[5]),
             _p(hold[6]), Q, ts, n_int, _p(hold[7]), _p(hold[8]), _p(hold[9])]
-    if Bs is not None:
+    if Bs is not None and dilution is not None:
+        Bsf = _f64(Bs).reshape(-1)
+        ph = _f64(phi).reshape(-1)
+        dl = _f64(dilution)
+        a = args[:6] + [_p(ph)] + args[6:]
+        lib().or_K_dK_red_dil(*a, _p(Bsf), _p(dl), _p(rows), _p(cols), _p(vK), _p(vdK), _p(rhs))
+    elif Bs is not None:
         Bsf = _f64(Bs).reshape(-1)
         lib().or_K_dK_red(*args, _p(Bsf), _p(rows), _p(cols), _p(vK), _p(vdK), _p(rhs))
     else:
@@ -195,9 +211,11 @@ def domain_inputs(asm, n, quad_order):
                 phi=fe.lagrange_basis(q), Q=Q)
 
 
-def assemble_like_reference(asm, x, quad_order, source_fields):
+def assemble_like_reference(asm, x, quad_order, source_fields, dilutions=None):
     """(K, J, rhs) following curl_curl_assembler.py:815-932 with the oracle's
-    element kernels (source_fields[n] None/empty -> air kernel)."""
+    element kernels (source_fields[n] None/empty -> air kernel); with
+    ``dilutions`` (one (c_num, m_num, c_den, m_den) or None per domain) the
+    diluted assembler, curl_curl_assembler_diluted.py:165-304."""
     from scipy.sparse import csr_array
     n = asm.num_dofs
     K = csr_array((n, n))
@@ -210,9 +228,10 @@ def assemble_like_reference(asm, x, quad_order, source_fields):
         bs = source_fields[d] if use_src else None
         if bs is not None and len(bs) == 0:
             bs = None
+        dil = dilutions[d] if dilutions is not None else None
         r, c, vK, vdK, rh = element_triplets(inp['nodes'], inp['cells'], inp['gids'],
                                              inp['orient'], inp['curls'], inp['d_phi'], inp['w'],
-                                             model, x, bs)
+                                             model, x, bs, phi=inp['phi'], dilution=dil)
         K += coo_to_csr(n, r, c, vK)
         dK += coo_to_csr(n, r, c, vdK)
         rhs += rh

@@ -127,13 +127,17 @@ static void or_B_in_element(double *B, const double *Bs, int Q, const int *g,
 /* Element kernels                                                          */
 /* ---------------------------------------------------------------------- */
 
-/* shared body of ctools.c:1195-1388 (with_source=1) and :960-1193 (0) */
+/* shared body of ctools.c:1195-1388 (with_source=1), :960-1193 (0) and the
+ * diluted variant ctools.c:1565-1759 (dil != NULL: s at the quadrature
+ * point's z from evaluate_finite_element, ctools.c:480-504, and
+ * eval_dilution, ctools.c:124-128; nu = s nu(s|B|), nu' = s^2 nu'(s|B|)) */
 static void or_K_dK_body(int with_source, int num_el, const double *nodes,
                          const int *cells, const int *glob_ids,
                          const int *orient, const double *curls,
                          const double *d_phi, const double *w, int Q,
                          int type_spec, int n_int, const double *c,
                          const double *xk, const double *x, const double *Bs,
+                         const double *phi, const double *dil,
                          int *rows, int *cols, double *vK, double *vdK,
                          double *rhs)
 {
@@ -160,7 +164,16 @@ static void or_K_dK_body(int with_source, int num_el, const double *nodes,
         for (int q = 0; q < Q; ++q) {             /* ctools.c:1287-1300 */
             Bmag[q] = sqrt(B[3 * q] * B[3 * q] + B[3 * q + 1] * B[3 * q + 1] +
                            B[3 * q + 2] * B[3 * q + 2]);
-            or_nu_eval(type_spec, n_int, c, xk, Bmag[q], &nu[q], &dnu[q]);
+            if (dil) {                            /* ctools.c:1666-1682 */
+                double z = 0.0;
+                for (int a = 0; a < 4; ++a) z += phi[4 * q + a] * nodes[3 * el[a] + 2];
+                const double s = (dil[0] + z * dil[1]) / (dil[2] + z * dil[3]);
+                or_nu_eval(type_spec, n_int, c, xk, s * Bmag[q], &nu[q], &dnu[q]);
+                nu[q] *= s;
+                dnu[q] *= s * s;
+            } else {
+                or_nu_eval(type_spec, n_int, c, xk, Bmag[q], &nu[q], &dnu[q]);
+            }
         }
         for (int j = 0; j < 6; ++j) {
             for (int k = j; k < 6; ++k) {         /* ctools.c:1303-1361 */
@@ -198,7 +211,19 @@ void or_K_dK_red(int num_el, const double *nodes, const int *cells,
                  int *rows, int *cols, double *vK, double *vdK, double *rhs)
 {
     or_K_dK_body(1, num_el, nodes, cells, glob_ids, orient, curls, d_phi, w, Q,
-                 type_spec, n_int, c, xk, x, Bs, rows, cols, vK, vdK, rhs);
+                 type_spec, n_int, c, xk, x, Bs, NULL, NULL, rows, cols, vK, vdK, rhs);
+}
+
+void or_K_dK_red_dil(int num_el, const double *nodes, const int *cells,
+                     const int *glob_ids, const int *orient,
+                     const double *curls, const double *phi, const double *d_phi,
+                     const double *w, int Q,
+                     int type_spec, int n_int, const double *c, const double *xk,
+                     const double *x, const double *Bs, const double *dil,
+                     int *rows, int *cols, double *vK, double *vdK, double *rhs)
+{
+    or_K_dK_body(1, num_el, nodes, cells, glob_ids, orient, curls, d_phi, w, Q,
+                 type_spec, n_int, c, xk, x, Bs, phi, dil, rows, cols, vK, vdK, rhs);
 }
 
 void or_K_dK_air(int num_el, const double *nodes, const int *cells,
@@ -210,7 +235,7 @@ void or_K_dK_air(int num_el, const double *nodes, const int *cells,
                  int *rows, int *cols, double *vK, double *vdK)
 {
     or_K_dK_body(0, num_el, nodes, cells, glob_ids, orient, curls, d_phi, w, Q,
-                 type_spec, n_int, c, xk, x, NULL, rows, cols, vK, vdK, NULL);
+                 type_spec, n_int, c, xk, x, NULL, NULL, NULL, rows, cols, vK, vdK, NULL);
 }
 
 /* ctools.c:1500-1563 (with evaluate_finite_element, ctools.c:480-504) */

@@ -46,6 +46,15 @@ void or_K_dK_red(int num_el, const double *nodes, const int *cells,
                  int *rows, int *cols, double *vK, double *vdK, double *rhs);
 
 /* Air / source-free element kernel, fem_c/ctools.c:960-1193 (B_s = 0, no rhs). */
+/* diluted steel, ctools.c:1565-1759; dil = {c_num, m_num, c_den, m_den} */
+void or_K_dK_red_dil(int num_el, const double *nodes, const int *cells,
+                     const int *glob_ids, const int *orient,
+                     const double *curls, const double *phi, const double *d_phi,
+                     const double *w, int Q,
+                     int type_spec, int n_int, const double *c, const double *xk,
+                     const double *x, const double *Bs, const double *dil,
+                     int *rows, int *cols, double *vK, double *vdK, double *rhs);
+
 void or_K_dK_air(int num_el, const double *nodes, const int *cells,
                  const int *glob_ids, const int *orient,
                  const double *curls, const double *d_phi,

@@ -7,15 +7,17 @@ RacetrackCoil, BrickField.  Behind them the hot path runs as hand-written HIP
 kernels in libsnoo_hip.so (C ABI: include/snoo_hip.h).
 """
 from .brick_field import BrickField
-from .curl_curl_assembler import CurlCurlAssembler
+from .curl_curl_assembler import CurlCurlAssembler, CurlCurlAssemblerDiluted
 from .finite_element import FiniteElement
-from .materials import (BSplineReluctance, ConstantReluctance, Reluctance, tanh_reluctance)
+from .materials import (BSplineReluctance, ConstantReluctance, Dilution, Reluctance,
+                        tanh_reluctance)
 from .mesh import TetMesh, TetModel, kuhn_mesh
 from .mesh_tools import get_cotree_dofs, get_edge_boundary_dofs
 from .racetrack_coil import RacetrackCoil
-from .red_mvp_solver import NewtonDriver, RedMVPSolver
+from .red_mvp_solver import NewtonDriver, RedMVPSolver, RedMVPSolverDiluted
 
 __all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctance',
            'ConstantReluctance', 'Reluctance', 'tanh_reluctance', 'TetMesh', 'TetModel',
            'kuhn_mesh', 'get_cotree_dofs', 'get_edge_boundary_dofs', 'RacetrackCoil',
-           'NewtonDriver', 'RedMVPSolver']
+           'NewtonDriver', 'RedMVPSolver', 'RedMVPSolverDiluted', 'CurlCurlAssemblerDiluted',
+           'Dilution']

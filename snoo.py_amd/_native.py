@@ -30,12 +30,12 @@ SIGNATURES = {
     'sn_pattern_copy': (_c_int, [_vp, _vp, _vp, _vp, _vp]),
     'sn_element_local_f64': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                       _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                      _vp]),
+                                      _vp, _vp, _vp]),
     'sn_csr_gather_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dof_gather_f64': (_c_int, [_vp, _vp, _vp, _vp]),
     'sn_element_newton': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                    _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
-                                   _vp, _vp, _vp, _vp]),
+                                   _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_csr_gather_one': (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
     'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),

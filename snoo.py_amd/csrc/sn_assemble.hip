@@ -24,6 +24,14 @@ constexpr double kNu0 = 1.0 / 4.0 / 3.141592653589793238462643383279502984 * 1e7
 
 enum { MODE_LOCAL = 0, MODE_NEWTON_B = 1, MODE_NEWTON_R = 2 };
 
+// Diluted steel (fem_c/ctools.c:124-128, 1565-1759): at a quadrature point
+// with physical z, s = (c_num + z m_num) / (c_den + z m_den) scales the
+// reluctivity as nu = s nu(s |B|), dnu/d|B| = s^2 nu'(s |B|).
+struct Dilution {
+    int on;
+    double c_num, m_num, c_den, m_den;
+};
+
 struct NuModel {
     int type;
     int n;
@@ -69,7 +77,8 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
     const double *__restrict__ curls, const double *__restrict__ w, int Q, int type_spec,
     int n_int, const double *__restrict__ coeffs, const double *__restrict__ knots,
     const double *__restrict__ x, const double *__restrict__ Bs, void *__restrict__ out_a,
-    double *__restrict__ out_b, double *__restrict__ out6, float *__restrict__ nu_el) {
+    double *__restrict__ out_b, double *__restrict__ out6, float *__restrict__ nu_el,
+    const double *__restrict__ phi, Dilution dil) {
     __shared__ double s_curl[64 * 18];
     __shared__ double s_c[4 * kMaxLdsIntervals];
     __shared__ double s_x[kMaxLdsIntervals + 1];
@@ -146,7 +155,17 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
         const double b0 = s0 + Bc[0], b1 = s1 + Bc[1], b2 = s2 + Bc[2];
         const double bm = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
         double nu, dnu;
-        nu_eval(m, bm, nu, dnu);
+        if (dil.on) {
+            double z = 0.0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) z += phi[4 * q + a] * nodes[3 * nd[a] + 2];
+            const double sf = (dil.c_num + z * dil.m_num) / (dil.c_den + z * dil.m_den);
+            nu_eval(m, sf * bm, nu, dnu);
+            nu *= sf;
+            dnu *= sf * sf;
+        } else {
+            nu_eval(m, bm, nu, dnu);
+        }
         const double wq = w[q];
         sum_nu_w += nu * wq;
         sum_w += wq;
@@ -273,23 +292,40 @@ using namespace sn;
 
 extern "C" {
 
+static int make_dilution(const double *dil_host, const double *phi, const double *Bs,
+                         Dilution &d) {
+    d.on = 0;
+    d.c_num = d.m_num = d.c_den = d.m_den = 0.0;
+    // as the reference, the dilution applies to source (iron) domains only
+    if (!dil_host || !Bs) return SN_OK;
+    SN_REQUIRE(phi, "dilution needs the P1 basis values phi [Q*4]");
+    d.on = 1;
+    d.c_num = dil_host[0];
+    d.m_num = dil_host[1];
+    d.c_den = dil_host[2];
+    d.m_den = dil_host[3];
+    return SN_OK;
+}
+
 int sn_element_local_f64(int num_el_total, int el_begin, int el_count, const double *nodes,
                          const int *cells, const int *gids, const int *orient,
                          const double *curls, const double *d_phi, const double *w, int Q,
                          int type_spec, int n_int, const double *coeffs, const double *knots,
                          const double *x, const double *Bs, double *locK, double *locdK,
-                         double *loc6, void *stream) {
+                         double *loc6, const double *phi, const double *dil_host, void *stream) {
     (void)d_phi;
     int rc = check_elem_args(num_el_total, el_begin, el_count, Q, type_spec, n_int, nodes, cells,
                              gids, orient, curls, w, coeffs, knots, x);
     if (rc) return rc;
     if (el_count == 0) return SN_OK;
     SN_REQUIRE(locK && locdK && loc6, "sn_element_local_f64: null output");
+    Dilution dil;
+    if ((rc = make_dilution(dil_host, phi, Bs, dil))) return rc;
     hipLaunchKernelGGL((element_kernel<MODE_LOCAL, double>),
                        dim3((el_count + kElBlock - 1) / kElBlock), dim3(kElBlock), 0,
                        as_stream(stream), num_el_total, el_begin, el_count, nodes, cells, gids,
                        orient, curls, w, Q, type_spec, n_int, coeffs, knots, x, Bs,
-                       (void *)locK, locdK, loc6, (float *)nullptr);
+                       (void *)locK, locdK, loc6, (float *)nullptr, phi, dil);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -299,7 +335,7 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
                       const double *curls, const double *d_phi, const double *w, int Q,
                       int type_spec, int n_int, const double *coeffs, const double *knots,
                       const double *x, const double *Bs, void *locJ, double *loc6,
-                      float *nu_el, void *stream) {
+                      float *nu_el, const double *phi, const double *dil_host, void *stream) {
     (void)d_phi;
     int rc = check_elem_args(num_el_total, el_begin, el_count, Q, type_spec, n_int, nodes, cells,
                              gids, orient, curls, w, coeffs, knots, x);
@@ -307,12 +343,14 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
     SN_REQUIRE(mode == 0 || mode == 1, "sn_element_newton: mode must be 0 or 1");
     if (el_count == 0) return SN_OK;
     SN_REQUIRE(locJ && loc6, "sn_element_newton: null output");
+    Dilution dil;
+    if ((rc = make_dilution(dil_host, phi, Bs, dil))) return rc;
     const dim3 grid((el_count + kElBlock - 1) / kElBlock), block(kElBlock);
     hipStream_t s = as_stream(stream);
 #define SN_LAUNCH_EL(M, T)                                                                     \
     hipLaunchKernelGGL((element_kernel<M, T>), grid, block, 0, s, num_el_total, el_begin,     \
                        el_count, nodes, cells, gids, orient, curls, w, Q, type_spec, n_int,    \
-                       coeffs, knots, x, Bs, locJ, (double *)nullptr, loc6, nu_el)
+                       coeffs, knots, x, Bs, locJ, (double *)nullptr, loc6, nu_el, phi, dil)
     if (mode == 0) {
         if (out_f32) SN_LAUNCH_EL(MODE_NEWTON_B, float);
         else SN_LAUNCH_EL(MODE_NEWTON_B, double);

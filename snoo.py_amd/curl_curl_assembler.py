@@ -85,6 +85,14 @@ class CurlCurlAssembler:
         self._dev = None
         self._tables = {}
         self._models = {}
+        self.dilutions = [None] * len(self.material_list)
+
+    def _dilution(self, n):
+        """(c_num, m_num, c_den, m_den) of domain n as a C double[4], or None."""
+        dil = self.dilutions[n] if n < len(self.dilutions) else None
+        if dil is None:
+            return None
+        return (ctypes.c_double * 4)(dil.c_num, dil.m_num, dil.c_den, dil.m_den)
 
     # ------------------------------------------------------------ device state
     def _device_state(self):
@@ -203,7 +211,8 @@ class CurlCurlAssembler:
                 N, b, e - b, nat.ptr(d['nodes']), nat.ptr(d['cells']), nat.ptr(d['gids']),
                 nat.ptr(d['orient']), nat.ptr(T.curls), nat.ptr(T.dphi), nat.ptr(T.w), T.Q,
                 m.type_spec, m.n_int, nat.ptr(m.c), nat.ptr(m.x), nat.ptr(x_dev), nat.ptr(bs),
-                nat.ptr(locK), nat.ptr(locdK), nat.ptr(loc6), nat.stream_handle()))
+                nat.ptr(locK), nat.ptr(locdK), nat.ptr(loc6), nat.ptr(T.phi), self._dilution(n),
+                nat.stream_handle()))
         return locK, locdK, loc6
 
     def assemble_device(self, x, quad_order=2, source_fields=()):
@@ -243,7 +252,8 @@ class CurlCurlAssembler:
                 nat.ptr(d['gids']), nat.ptr(d['orient']), nat.ptr(T.curls), nat.ptr(T.dphi),
                 nat.ptr(T.w), T.Q, m.type_spec, m.n_int, nat.ptr(m.c), nat.ptr(m.x),
                 nat.ptr(x_dev), nat.ptr(bs), nat.ptr(locJ), nat.ptr(loc6),
-                nat.ptr(nu_el) if nu_el is not None else None, nat.stream_handle()))
+                nat.ptr(nu_el) if nu_el is not None else None, nat.ptr(T.phi), self._dilution(n),
+                nat.stream_handle()))
 
     def compute_stiffness_and_jacobi_matrix_c(self, x, quad_order=2, source_fields=[]):
         """(K, J, rhs) as the reference returns them (curl_curl_assembler.py:815-932).
@@ -316,3 +326,15 @@ class CurlCurlAssembler:
         averaged at each node over those elements (0 at nodes none touches)."""
         from .evaluators import nodal_fields_device
         return nodal_fields_device(self, as_dev_f64(x), list(mat_ids)).cpu().numpy()
+
+
+class CurlCurlAssemblerDiluted(CurlCurlAssembler):
+    """Curl-curl assembler with diluted steel (snoopy/curl_curl_assembler_diluted.py:17-300):
+    one dilution object (or None) per domain; domains with a dilution and a
+    source field use the diluted reluctivity s nu(s |B|) with s from the
+    quadrature point's z (fem_c/ctools.c:1565-1759)."""
+
+    def __init__(self, mesh, volume_tags, material_list, dilutions_list, element_order=1):
+        super().__init__(mesh, volume_tags, material_list, element_order)
+        self.dilutions = list(dilutions_list) + [None] * (len(self.material_list) -
+                                                          len(dilutions_list))

@@ -155,3 +155,38 @@ def model_tables(rel):
     c = np.ascontiguousarray(np.asarray(c, dtype=np.float64))
     x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
     return ts, len(x) - 1, c.reshape(-1), x
+
+
+class Dilution:
+    """Dilution of the iron along z (snoopy/materials.py:538-599): the ratio of
+    two linear functions of z built from the entrance (1) and exit (2) x
+    coordinates of middle gap, core, void and yoke; magnet type 1 uses
+    (yoke - void) / (core - middle gap), type 3 the inverse."""
+
+    def __init__(self, x_mgap_1, x_core_1, x_void_1, x_yoke_1,
+                 x_mgap_2, x_core_2, x_void_2, x_yoke_2, z_pos, z_len, mag_type=1):
+        slope = lambda a, b: (b - a) / z_len
+        icpt = lambda a, b: (a * (z_len + z_pos) - b * z_pos) / z_len
+        m_yoke, c_yoke = slope(x_yoke_1, x_yoke_2), icpt(x_yoke_1, x_yoke_2)
+        m_core, c_core = slope(x_core_1, x_core_2), icpt(x_core_1, x_core_2)
+        m_void, c_void = slope(x_void_1, x_void_2), icpt(x_void_1, x_void_2)
+        m_mgap, c_mgap = slope(x_mgap_1, x_mgap_2), icpt(x_mgap_1, x_mgap_2)
+        if mag_type == 1:
+            self.m_num, self.m_den = m_yoke - m_void, m_core - m_mgap
+            self.c_num, self.c_den = c_yoke - c_void, c_core - c_mgap
+        elif mag_type == 3:
+            self.m_num, self.m_den = m_core - m_mgap, m_yoke - m_void
+            self.c_num, self.c_den = c_core - c_mgap, c_yoke - c_void
+        else:
+            raise ValueError('magnet type %r unknown (1 or 3)' % (mag_type,))
+
+    @classmethod
+    def from_coefficients(cls, c_num, m_num, c_den, m_den):
+        d = cls.__new__(cls)
+        d.c_num, d.m_num, d.c_den, d.m_den = float(c_num), float(m_num), float(c_den), float(m_den)
+        return d
+
+    def evaluate(self, z):
+        """s(z) = (c_num + z m_num) / (c_den + z m_den) (fem_c/ctools.c:124-128)."""
+        z = np.asarray(z, dtype=np.float64)
+        return (self.c_num + z * self.m_num) / (self.c_den + z * self.m_den)

@@ -422,3 +422,20 @@ class NewtonDriver:
             stats['ams_levels'] = self.ams.plan.sizes()
             stats['ams_setup_s'] = self.ams.plan.t_setup
         return x
+
+
+class RedMVPSolverDiluted(RedMVPSolver):
+    """Solver with diluted steel (snoopy/red_mvp_solver_diluted.py:13-229):
+    the constructor takes one dilution per iron domain (air is never
+    diluted); ``solve`` and ``solve_newton`` are those of RedMVPSolver on the
+    diluted assembler."""
+
+    def __init__(self, gmsh_model, coil_list, iron_tags, air_tags, reluctances, dilutions,
+                 max_newton_iterations=25, element_order=1, quad_order=6, verbose=True):
+        super().__init__(gmsh_model, coil_list, iron_tags, air_tags, reluctances,
+                         max_newton_iterations, element_order, quad_order, verbose)
+        from .curl_curl_assembler import CurlCurlAssemblerDiluted
+        self.dilutions = list(dilutions) + [None] * len(self.air_tags)
+        self.curl_curl_factory = CurlCurlAssemblerDiluted(
+            gmsh_model.mesh, self.domain_tags, self.curl_curl_factory.material_list,
+            self.dilutions, element_order)

@@ -136,6 +136,84 @@ def test_oracle_bit_matches_compiled_reference_on_a_mesh_slice():
     np.testing.assert_array_equal(rhs, rhs_ref)
 
 
+def test_diluted_oracle_bit_matches_compiled_reference():
+    """or_K_dK_red_dil against compute_K_dK_Hcurl_red_dil (ctools.c:1565-1759)
+    compiled from the reference, with a z-dependent dilution."""
+    R = orc.ref_lib()
+    if R is None:
+        pytest.skip('oracle/_ref not built (reference sources absent)')
+    import ctypes
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    dp = Dipole(h=0.2)
+    m = dp.mesh
+    sel = np.nonzero(m.tags == 1)[0][:300]
+    rel = _bh_rel()
+    ts, n_int, cf, kn = model_tables(rel)
+    w, q = fe.quadrature_rule(2)
+    Q = 4
+    phi = fe.lagrange_basis(q)
+    dil = np.array([0.8, 0.3, 1.1, -0.2])     # c_num, m_num, c_den, m_den
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(m.num_edges) * 3e-2
+    Bs = rng.standard_normal(len(sel) * Q * 3)
+    r, c, vK, vdK, rhs = orc.element_triplets(m.nodes, m.cells[sel], m.glob_ids[sel], m.orient[sel],
+                                              fe.curl_table(Q), fe.lagrange_basis_derivative(q), w,
+                                              (ts, n_int, cf, kn), x, Bs, phi=phi, dilution=dil)
+    ne, nt = len(sel), 36 * len(sel)
+    rk, ck, vk = np.zeros(nt, np.int32), np.zeros(nt, np.int32), np.zeros(nt)
+    rd, cd, vd = np.zeros(nt, np.int32), np.zeros(nt, np.int32), np.zeros(nt)
+    nodes = np.ascontiguousarray(m.nodes.reshape(-1))
+    cells = np.ascontiguousarray(m.cells[sel].reshape(-1))
+    gids = np.ascontiguousarray(m.glob_ids[sel].reshape(-1))
+    orient = np.ascontiguousarray(m.orient[sel])
+    cf, kn = np.ascontiguousarray(cf), np.ascontiguousarray(kn)
+    curls = np.ascontiguousarray(fe.curl_table(Q))
+    phic = np.ascontiguousarray(phi.reshape(-1))
+    dphi = np.ascontiguousarray(fe.lagrange_basis_derivative(q).reshape(-1))
+    qq, ww = np.ascontiguousarray(q), np.ascontiguousarray(w)
+    rhs_ref = np.zeros(m.num_edges)
+    R.compute_K_dK_Hcurl_red_dil(R.make_triplet_list(nt, P(rk), P(ck), P(vk)),
+                                 R.make_triplet_list(nt, P(rd), P(cd), P(vd)), P(rhs_ref),
+                                 R.make_dilution(*dil), P(gids),
+                                 R.make_reluctance(ts, n_int, P(cf), P(kn)),
+                                 R.make_mesh(m.num_nodes, ne, 4, P(nodes), P(cells)),
+                                 R.make_quadrature_rule(Q, P(qq), P(ww)), P(curls), P(phic),
+                                 P(dphi), P(orient), P(x), P(np.ascontiguousarray(Bs)))
+    np.testing.assert_array_equal(r, rk)
+    np.testing.assert_array_equal(c, ck)
+    np.testing.assert_array_equal(vK, vk)
+    np.testing.assert_array_equal(vdK, vd)
+    np.testing.assert_array_equal(rhs, rhs_ref)
+
+
+def test_constant_dilution_equals_rescaled_pchip_material():
+    """A constant dilution s makes the iron behave as the material
+    s nu(s B): the PCHIP pieces with knots x/s and coefficients scaled by
+    s^4, s^3, s^2, s (cubic to constant) -- the diluted kernel must equal the
+    plain kernel with that material."""
+    dp = Dipole(h=0.2)
+    m = dp.mesh
+    sel = np.nonzero(m.tags == 1)[0][:200]
+    ts, n_int, cf, kn = model_tables(_bh_rel())
+    assert ts == 1
+    sc = 0.7
+    c4 = np.asarray(cf).reshape(4, -1)
+    cs = np.concatenate([c4[0] * sc ** 4, c4[1] * sc ** 3, c4[2] * sc ** 2, c4[3] * sc])
+    w, q = fe.quadrature_rule(2)
+    Q = 4
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(m.num_edges) * 3e-2
+    Bs = rng.standard_normal(len(sel) * Q * 3)
+    args = (m.nodes, m.cells[sel], m.glob_ids[sel], m.orient[sel], fe.curl_table(Q),
+            fe.lagrange_basis_derivative(q), w)
+    _, _, vK1, vdK1, rhs1 = orc.element_triplets(*args, (ts, n_int, cf, kn), x, Bs,
+                                                 phi=fe.lagrange_basis(q),
+                                                 dilution=[sc, 0.0, 1.0, 0.0])
+    _, _, vK2, vdK2, rhs2 = orc.element_triplets(*args, (ts, n_int, cs, np.asarray(kn) / sc), x, Bs)
+    for a, b in ((vK1, vK2), (vdK1, vdK2), (rhs1, rhs2)):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
 # ---------------------------------------------------------------- closed forms
 
 def test_quadrature_rules_exact_and_sum_to_volume():
