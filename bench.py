@@ -53,10 +53,14 @@ def parse():
     ap.add_argument('--ams-weight', type=float, default=None)
     ap.add_argument('--ams-omega', type=float, default=None)
     ap.add_argument('--ams-theta', type=float, default=None)
-    ap.add_argument('--workload', default='dipole1m', choices=['dipole1m', 'batch64', 'chain6'],
+    ap.add_argument('--workload', default='dipole1m',
+                    choices=['dipole1m', 'batch64', 'chain6', 'warmstart'],
                     help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
                          'batched vs one at a time; chain6: 6-yoke chain')
     ap.add_argument('--variants', type=int, default=64)
+    ap.add_argument('--ws-train', type=int, default=512, help='warm start: training snapshots')
+    ap.add_argument('--ws-test', type=int, default=32, help='warm start: held-out variants')
+    ap.add_argument('--ws-modes', type=int, default=128, help='warm start: POD modes')
     ap.add_argument('--copy-calib', action='store_true',
                     help='only run a plain 256 MiB device copy 5 times (HBM counter calibration)')
     ap.add_argument('--quick', action='store_true',
@@ -135,6 +139,8 @@ def main():
         return bench_batch(args)
     if args.workload == 'chain6':
         return bench_chain(args)
+    if args.workload == 'warmstart':
+        return bench_warmstart(args)
     from snoopy_amd import _native as nat
     from snoopy_amd.magnets import Dipole, mesh_size_for
     from snoopy_amd.red_mvp_solver import _CG
@@ -361,6 +367,93 @@ def bench_batch(args):
                    'edges_total': bs.n, 'nnz_total': bs.asm.nnz, 'precond': args.precond},
         'batched_s': t_batch, 'one_at_a_time_s': t_seq, 'speedup': t_seq / t_batch,
         'converged': conv, 'newton_steps': bs.stats['newton_steps']}), flush=True)
+
+
+def bench_warmstart(args):
+    """Learned warm start on the morph family (fixed topology, 205k tets):
+    snapshots from the solver, POD, the BF16 MLP trained by this project's
+    kernels, then held-out seeds solved from A = 0 and from the residual-
+    judged prediction (mixed precision, 1e-8)."""
+    import torch
+    from snoopy_amd.warm_start import FamilySolver, MorphFamily, WarmStart, sample_params
+    fam = MorphFamily()
+    fs = FamilySolver(fam)
+    log('warm start family: %d tets, %d edges' % (fam.mesh.num_elements, fam.mesh.num_edges))
+    ws = WarmStart.train(fam, n_train=args.ws_train, r=min(args.ws_modes, args.ws_train),
+                         epochs=2000, rtol=1e-7, fsolver=fs, log=log)
+    log('trained: %s' % ws.train_stats)
+    rows = []
+    for seed in range(5000, 5000 + args.ws_test):
+        p = sample_params(seed)
+        fs.set_variant(p)
+        drv = fs.solver.newton_driver('mixed', 8, 32, 'ams')
+        drv.retarget(fs.solver.source_fields_device())
+        fs.solver.solve_newton('mixed', rtol=args.rtol, precond='ams')   # warm the caches
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fs.solver.solve_newton('mixed', rtol=args.rtol, precond='ams')
+        torch.cuda.synchronize()
+        t0, s0 = time.perf_counter() - t, dict(fs.solver.stats)
+        t = time.perf_counter()
+        x0 = ws.predict_family(p)
+        r_pred = drv.residual(x0) / drv.r0()
+        use = r_pred < 1.0
+        fs.solver.solve_newton('mixed', rtol=args.rtol, x0=x0 if use else None, precond='ams')
+        torch.cuda.synchronize()
+        t1, s1 = time.perf_counter() - t, dict(fs.solver.stats)
+        rows.append((r_pred, use, s0['newton_steps'], s1['newton_steps'],
+                     sum(s0['cg_iterations']), sum(s1['cg_iterations']), t0, t1))
+        log('seed %d: predicted |R|/|R0| %.3e kept %s newton %d -> %d cg %d -> %d  %.3f -> %.3f s'
+            % ((seed,) + rows[-1]))
+    a = np.array([[r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]] for r in rows], dtype=float)
+    gemm = gemm_rates([(args.ws_train, 512, 512), (8192, 512, 512), (8192, 2048, 2048)])
+    print(json.dumps({
+        'metric': 'warm start on %d held-out variants (205k-tet morph family), Newton to %g'
+                  % (len(rows), args.rtol),
+        'value': float(a[:, 6].sum() / a[:, 7].sum()), 'unit': 'speedup (time from A=0 / '
+        'time with the judged prediction, prediction included)', 'n_gpus': 1,
+        'config': {'workload': 'learned warm start', 'train_snapshots': args.ws_train,
+                   'pod_modes': min(args.ws_modes, args.ws_train), 'hidden': [512] * 4},
+        'train_stats': {k: (float(v) if isinstance(v, (int, float, np.floating)) else v)
+                        for k, v in ws.train_stats.items()},
+        'pred_residual_median': float(np.median(a[:, 0])), 'kept_fraction': float(a[:, 1].mean()),
+        'newton_from_zero': float(a[:, 2].mean()), 'newton_warm': float(a[:, 3].mean()),
+        'cg_from_zero': float(a[:, 4].mean()), 'cg_warm': float(a[:, 5].mean()),
+        's_from_zero': float(a[:, 6].mean()), 's_warm': float(a[:, 7].mean()),
+        'gemm_bf16': gemm}), flush=True)
+
+
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
+
+
+def gemm_rates(shapes, reps=50):
+    """TFLOP/s of the BF16 MFMA GEMM (C = A B, GELU epilogue as in the MLP's
+    hidden layers) at the given (M, N, K), timed with events on its stream."""
+    import torch
+    from snoopy_amd import _native as nat
+    out = []
+    for M, N, K in shapes:
+        A = torch.randn(M, K, device='cuda').to(torch.bfloat16)
+        B = torch.randn(K, N, device='cuda').to(torch.bfloat16)
+        bias = torch.zeros(N, device='cuda')
+        Z = torch.empty(M, N, device='cuda')
+        Cb = torch.empty(M, N, device='cuda', dtype=torch.bfloat16)
+        call = lambda: nat.check(nat.load().sn_gemm_bf16(  # noqa: E731
+            0, 0, 1, M, N, K, nat.ptr(A), K, nat.ptr(B), N, nat.ptr(bias), None, nat.ptr(Z),
+            None, nat.ptr(Cb), N, nat.stream_handle()))
+        for _ in range(3):
+            call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        out.append({'M': M, 'N': N, 'K': K, 'avg_us': ms * 1e3, 'TFLOP/s': tf,
+                    'frac_of_peak': tf / BF16_PEAK_TFLOPS})
+    return out
 
 
 def bench_chain(args):

@@ -269,11 +269,12 @@ typedef struct sn_ams sn_ams;
 /* Edge side of the preconditioner (host arrays, copied).
  * edge_nodes_host [2*n_edges]: level-0 ids of the (lo, hi) nodes of each
  *   edge; ids >= n_free_nodes mark Dirichlet nodes (their value is 0).
- * edge_tan_host [4*n_edges]: (x_hi - x_lo)/2 and one padding float.
+ * edge_mesh_nodes_host [2*n_edges]: mesh node ids of (lo, hi) (the edge
+ *   tangents are formed from the node coordinates at every sn_ams_update).
  * inc_ptr_host [n_free_nodes+1], inc_edge_host: free edges at each free node.
  * n_el: elements of the Laplacian (the element order of sn_ams_update). */
 int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el,
-                  const int *edge_nodes_host, const float *edge_tan_host,
+                  const int *edge_nodes_host, const int *edge_mesh_nodes_host,
                   const int *inc_ptr_host, const int *inc_edge_host, float omega,
                   float weight, void *stream);
 /* Append a level (host arrays, copied), finest first.  CSR pattern
@@ -287,9 +288,10 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el,
 int sn_ams_add_level(sn_ams *ams, int n, int nnz, const int *indptr_host,
                      const int *indices_host, const int *gather_ptr_host,
                      const int *gather_list_host, const int *agg_ptr_host, void *stream);
-/* Numeric refresh for new element reluctivities: element Laplacians,
- * every level's values and diagonal, dense inverse of the coarsest level
- * (at most 1024 rows).  cells_dev [4*n_el] uses the ids of nodes_dev. */
+/* Numeric refresh for new element reluctivities (and node coordinates):
+ * edge tangents, element Laplacians, every level's values and diagonal,
+ * dense inverse of the coarsest level (at most 1024 rows).  cells_dev
+ * [4*n_el] and the edge mesh nodes use the ids of nodes_dev. */
 int sn_ams_update(sn_ams *ams, const double *nodes_dev, const int *cells_dev,
                   const float *nu_el_dev, void *stream);
 /* z = M r (float64 vectors; entries with mask == 0 set to 0). */

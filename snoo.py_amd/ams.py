@@ -184,6 +184,7 @@ class AmsPlan:
         tan = np.zeros((len(edges), 4), dtype=np.float32)
         tan[:, :3] = 0.5 * (nodes[edges[:, 1]] - nodes[edges[:, 0]])
         self.edge_tan = tan
+        self.edge_mesh = np.ascontiguousarray(edges.astype(np.int32))
         fe = np.nonzero(free_edge)[0]
         inc_node = np.concatenate([node_new[edges[fe, 0]], node_new[edges[fe, 1]]])
         inc_edge = np.concatenate([fe, fe])
@@ -219,7 +220,7 @@ class AmsPreconditioner:
         lib = nat.load()
         self.handle = ctypes.c_void_p()
         nat.check(lib.sn_ams_create(ctypes.byref(self.handle), p.n_edges, p.n_free, p.n_el,
-                                    nat.ptr_np(p.edge_nodes), nat.ptr_np(p.edge_tan),
+                                    nat.ptr_np(p.edge_nodes), nat.ptr_np(p.edge_mesh),
                                     nat.ptr_np(p.inc_ptr), nat.ptr_np(p.inc_edge),
                                     float(omega), float(weight), nat.stream_handle()))
         for l, lv in enumerate(p.levels):

@@ -118,6 +118,26 @@ __global__ __launch_bounds__(kAmsBlock) void ams_lap_local_kernel(
         }
 }
 
+// edge tangents (x_hi - x_lo)/2 from the current node coordinates, per edge
+// and per incidence (the incidence copy keeps the restriction's reads in order)
+__global__ void ams_tangent_kernel(int n_edges, const double *__restrict__ nodes,
+                                   const int2 *__restrict__ edge_mesh, float4 *__restrict__ tan) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_edges) return;
+    const int2 en = edge_mesh[e];
+    tan[e] = make_float4((float)(0.5 * (nodes[3 * en.y] - nodes[3 * en.x])),
+                         (float)(0.5 * (nodes[3 * en.y + 1] - nodes[3 * en.x + 1])),
+                         (float)(0.5 * (nodes[3 * en.y + 2] - nodes[3 * en.x + 2])), 0.f);
+}
+
+__global__ void ams_inc_tangent_kernel(int n_inc, const int *__restrict__ inc_edge,
+                                       const float4 *__restrict__ tan,
+                                       float4 *__restrict__ inc_tan) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_inc) return;
+    inc_tan[k] = tan[inc_edge[k]];
+}
+
 __global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
                                   const int *__restrict__ glist, const float *__restrict__ in,
                                   float *__restrict__ out) {
@@ -377,9 +397,10 @@ using namespace sn;
 extern "C" {
 
 int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const int *edge_nodes,
-                  const float *edge_tan, const int *inc_ptr, const int *inc_edge, float omega,
+                  const int *edge_mesh_nodes, const int *inc_ptr, const int *inc_edge, float omega,
                   float weight, void *stream) {
-    SN_REQUIRE(out && edge_nodes && edge_tan && inc_ptr && inc_edge, "sn_ams_create: null argument");
+    SN_REQUIRE(out && edge_nodes && edge_mesh_nodes && inc_ptr && inc_edge,
+               "sn_ams_create: null argument");
     SN_REQUIRE(n_edges > 0 && n_free_nodes > 0 && n_el > 0, "sn_ams_create: empty mesh");
     SN_REQUIRE((long long)n_el * 10 < (1LL << 31), "sn_ams_create: too many elements");
     hipStream_t s = as_stream(stream);
@@ -389,8 +410,10 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const i
     A->n_el = n_el;
     A->omega = omega;
     A->weight = weight;
-    for (int e = 0; e < 2 * n_edges; ++e)
+    for (int e = 0; e < 2 * n_edges; ++e) {
         SN_REQUIRE(edge_nodes[e] >= 0, "sn_ams_create: negative node id at edge %d", e / 2);
+        SN_REQUIRE(edge_mesh_nodes[e] >= 0, "sn_ams_create: negative mesh node at edge %d", e / 2);
+    }
     for (int a = 0; a < n_free_nodes; ++a)
         SN_REQUIRE(inc_ptr[a] <= inc_ptr[a + 1], "sn_ams_create: incidence pointers not ascending");
     const int ninc = inc_ptr[n_free_nodes];
@@ -398,14 +421,13 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const i
         SN_REQUIRE(inc_edge[k] >= 0 && inc_edge[k] < n_edges, "sn_ams_create: edge id out of range");
     int rc;
     if ((rc = upload(&A->edge_nodes, reinterpret_cast<const int2 *>(edge_nodes), n_edges, s))) return rc;
-    if ((rc = upload(&A->edge_tan, reinterpret_cast<const float4 *>(edge_tan), n_edges, s))) return rc;
+    if ((rc = upload(&A->edge_mesh, reinterpret_cast<const int2 *>(edge_mesh_nodes), n_edges, s)))
+        return rc;
+    SN_HIP(hipMalloc(&A->edge_tan, (size_t)n_edges * sizeof(float4)));
     if ((rc = upload(&A->inc_ptr, inc_ptr, (size_t)n_free_nodes + 1, s))) return rc;
     if ((rc = upload(&A->inc_edge, inc_edge, (size_t)ninc, s))) return rc;
-    std::vector<float> itan((size_t)4 * (ninc > 0 ? ninc : 1), 0.0f);
-    for (int k = 0; k < ninc; ++k)
-        for (int c = 0; c < 4; ++c) itan[4 * (size_t)k + c] = edge_tan[4 * (size_t)inc_edge[k] + c];
-    if ((rc = upload(&A->inc_tan, reinterpret_cast<const float4 *>(itan.data()), (size_t)ninc, s)))
-        return rc;
+    SN_HIP(hipMalloc(&A->inc_tan, (size_t)(ninc > 0 ? ninc : 1) * sizeof(float4)));
+    A->n_inc = ninc;
     SN_HIP(hipMalloc(&A->loc, (size_t)10 * n_el * sizeof(float)));
     SN_HIP(hipStreamSynchronize(s));
     *out = A;
@@ -481,6 +503,11 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
         SN_HIP(hipMalloc(&A->dense, (size_t)nc * nc * sizeof(double)));
         SN_HIP(hipMalloc(&A->cinv, (size_t)nc * nc * sizeof(float)));
     }
+    hipLaunchKernelGGL(ams_tangent_kernel, dim3(grid_for(A->n_edges, 256)), dim3(256), 0, s,
+                       A->n_edges, nodes, A->edge_mesh, A->edge_tan);
+    if (A->n_inc > 0)
+        hipLaunchKernelGGL(ams_inc_tangent_kernel, dim3(grid_for(A->n_inc, 256)), dim3(256), 0, s,
+                           A->n_inc, A->inc_edge, A->edge_tan, A->inc_tan);
     hipLaunchKernelGGL(ams_lap_local_kernel, dim3(grid_for(A->n_el, kAmsBlock)), dim3(kAmsBlock),
                        0, s, A->n_el, nodes, cells, nu_el, A->loc);
     for (int l = 0; l < L; ++l) {
@@ -589,8 +616,8 @@ void sn_ams_destroy(sn_ams *A) {
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }
-    void *bufs[] = {A->edge_nodes, A->edge_tan, A->inc_ptr, A->inc_edge, A->inc_tan, A->loc,
-                    A->dense, A->cinv};
+    void *bufs[] = {A->edge_nodes, A->edge_mesh, A->edge_tan, A->inc_ptr, A->inc_edge, A->inc_tan,
+                    A->loc, A->dense, A->cinv};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     delete A;

@@ -5,6 +5,8 @@
 // mesh transfer of edge DOFs).  No counterpart in the reference: its solve
 // always starts from A = 0 or a caller-supplied x_0
 // (snoopy/red_mvp_solver.py:146-147,174); this fills x_0.
+#include <cstdint>
+
 #include "sn_common.hpp"
 
 namespace sn {
@@ -101,6 +103,136 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     }
 }
 
+// Large-tile variant: 128x128 tile, BK = 32, 4 waves in 2x2, each wave a
+// 64x64 sub-tile of 2x2 MFMA 32x32 tiles (8 MFMAs per wave per k-tile);
+// global -> register -> LDS staging with 16-byte loads along the contiguous
+// dimension, double-buffered LDS so one barrier per k-tile separates the
+// writes of the next tile from the reads of the current one.  VEC = 16-byte
+// loads allowed (leading dimensions multiples of 8, 16-byte aligned bases);
+// vectors that cross the matrix edge are loaded element by element.
+constexpr int tBM = 128, tBN = 128, tBK = 32, tPad = 8;
+
+template <bool VEC>
+__device__ __forceinline__ bf16x8 load8(const __bf16 *__restrict__ p, size_t off, int n_ok) {
+    bf16x8 v;
+    if (VEC && n_ok >= 8) {
+        v = *reinterpret_cast<const bf16x8 *>(p + off);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (i < n_ok) ? p[off + i] : to_bf16(0.0f);
+    }
+    return v;
+}
+
+template <int TA, int TB, int EPI, bool VEC>
+__global__ __launch_bounds__(256) void gemm_bf16_big_kernel(
+    int M, int N, int K, const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
+    int ldb, const float *__restrict__ bias, const float *__restrict__ Zin,
+    float *__restrict__ Zout, float *__restrict__ Cf, __bf16 *__restrict__ Cb, int ldc) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][tBM][tBK + tPad];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][tBN][tBK + tPad];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * tBM, n0 = blockIdx.x * tBN;
+    f32x16 acc[2][2] = {};
+    bf16x8 ra[2], rb[2];
+    // 512 vectors of 8 per operand tile, two per thread
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int v = tid + 256 * s;
+            if (TA == 0) {           // rows m, k contiguous
+                const int m = v >> 2, kv = (v & 3) * 8, gm = m0 + m, gk = k0 + kv;
+                const int ok = (gm < M) ? (K - gk < 8 ? (K - gk > 0 ? K - gk : 0) : 8) : 0;
+                ra[s] = load8<VEC>(A, (size_t)gm * lda + gk, ok);
+            } else {                 // rows k, m contiguous
+                const int k = v >> 4, mv = (v & 15) * 8, gk = k0 + k, gm = m0 + mv;
+                const int ok = (gk < K) ? (M - gm < 8 ? (M - gm > 0 ? M - gm : 0) : 8) : 0;
+                ra[s] = load8<VEC>(A, (size_t)gk * lda + gm, ok);
+            }
+            if (TB == 1) {           // rows n, k contiguous
+                const int n = v >> 2, kv = (v & 3) * 8, gn = n0 + n, gk = k0 + kv;
+                const int ok = (gn < N) ? (K - gk < 8 ? (K - gk > 0 ? K - gk : 0) : 8) : 0;
+                rb[s] = load8<VEC>(B, (size_t)gn * ldb + gk, ok);
+            } else {                 // rows k, n contiguous
+                const int k = v >> 4, nv = (v & 15) * 8, gk = k0 + k, gn = n0 + nv;
+                const int ok = (gk < K) ? (N - gn < 8 ? (N - gn > 0 ? N - gn : 0) : 8) : 0;
+                rb[s] = load8<VEC>(B, (size_t)gk * ldb + gn, ok);
+            }
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int v = tid + 256 * s;
+            if (TA == 0) {
+                *reinterpret_cast<bf16x8 *>(&As[buf][v >> 2][(v & 3) * 8]) = ra[s];
+            } else {
+                const int k = v >> 4, mv = (v & 15) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) As[buf][mv + i][k] = ra[s][i];
+            }
+            if (TB == 1) {
+                *reinterpret_cast<bf16x8 *>(&Bs[buf][v >> 2][(v & 3) * 8]) = rb[s];
+            } else {
+                const int k = v >> 4, nv = (v & 15) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) Bs[buf][nv + i][k] = rb[s][i];
+            }
+        }
+    };
+    const int nk = (K + tBK - 1) / tBK;
+    load_tile(0);
+    store_tile(0);
+    __syncthreads();
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    for (int t = 0; t < nk; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < nk) load_tile((t + 1) * tBK);
+#pragma unroll
+        for (int kk = 0; kk < tBK; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][wm + 32 * i + r][kk + kh]);
+                b[i] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn + 32 * i + r][kk + kh]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0,
+                                                                          0, 0);
+        }
+        if (t + 1 < nk) store_tile(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn + 32 * j + (lane & 31);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm + 32 * i + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                if (row >= M || col >= N) continue;
+                float v = acc[i][j][q];
+                if (bias) v += bias[col];
+                const size_t o = (size_t)row * ldc + col;
+                if (EPI == 1) {
+                    if (Zout) Zout[o] = v;
+                    v = gelu(v);
+                } else if (EPI == 2) {
+                    v *= gelu_grad(Zin[o]);
+                }
+                if (Cf) Cf[o] = v;
+                if (Cb) Cb[o] = to_bf16(v);
+            }
+        }
+}
+
 // dY = 2 (Y - T) * scale, loss partial sums of (Y - T)^2 per block
 __global__ void mse_grad_kernel(int n, const float *__restrict__ Y, const float *__restrict__ T,
                                 float scale, float *__restrict__ dYf, __bf16 *__restrict__ dYb,
@@ -191,14 +323,31 @@ int sn_gemm_bf16(int ta, int tb, int epi, int M, int N, int K, const void *A, in
     SN_REQUIRE(epi != 2 || Zin, "sn_gemm_bf16: GELU backward needs Zin");
     SN_REQUIRE(lda >= (ta ? M : K) && ldb >= (tb ? K : N) && ldc >= N,
                "sn_gemm_bf16: leading dimension too small");
-    const dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM), block(256);
     hipStream_t s = as_stream(stream);
     const __bf16 *a = (const __bf16 *)A;
     const __bf16 *b = (const __bf16 *)B;
     __bf16 *cb = (__bf16 *)Cb;
-#define SN_G(TA, TB, E)                                                                      \
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E>), grid, block, 0, s, M, N, K, a, lda, b, \
-                       ldb, bias, Zin, Zout, Cf, cb, ldc)
+    // large tiles when they still give every CU a workgroup
+    const long long big_tiles = (long long)((N + tBN - 1) / tBN) * ((M + tBM - 1) / tBM);
+    const bool big = big_tiles >= num_cus();
+    const bool vec = (lda % 8 == 0) && (ldb % 8 == 0) &&
+                     (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                     (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+    const dim3 grid = big ? dim3((N + tBN - 1) / tBN, (M + tBM - 1) / tBM)
+                          : dim3((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
+    const dim3 block(256);
+#define SN_G(TA, TB, E)                                                                          \
+    do {                                                                                         \
+        if (!big)                                                                                \
+            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E>), grid, block, 0, s, M, N, K, a,     \
+                               lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                       \
+        else if (vec)                                                                            \
+            hipLaunchKernelGGL((gemm_bf16_big_kernel<TA, TB, E, true>), grid, block, 0, s, M, N,  \
+                               K, a, lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                 \
+        else                                                                                     \
+            hipLaunchKernelGGL((gemm_bf16_big_kernel<TA, TB, E, false>), grid, block, 0, s, M, N, \
+                               K, a, lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                 \
+    } while (0)
 #define SN_GE(TA, TB) \
     if (epi == 0) SN_G(TA, TB, 0); else if (epi == 1) SN_G(TA, TB, 1); else SN_G(TA, TB, 2)
     if (ta == 0 && tb == 0) { SN_GE(0, 0); }

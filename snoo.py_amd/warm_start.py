@@ -138,9 +138,9 @@ class FamilySolver:
         self.solver._src = None
         return d
 
-    def solve(self, p, rtol=1e-6, precision='mixed', x0=None):
+    def solve(self, p, rtol=1e-6, precision='mixed', x0=None, precond='ams'):
         self.set_variant(p)
-        return self.solver.solve_newton(precision, rtol=rtol, x0=x0)
+        return self.solver.solve_newton(precision, rtol=rtol, x0=x0, precond=precond)
 
 
 def generate(family_solver, seeds, rtol=1e-6, log=None):

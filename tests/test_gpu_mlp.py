@@ -9,7 +9,9 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize('ta,tb', [(0, 0), (0, 1), (1, 0), (1, 1)])
-@pytest.mark.parametrize('M,N,K', [(64, 64, 16), (100, 70, 37), (2048, 512, 8), (48, 256, 512)])
+@pytest.mark.parametrize('M,N,K', [(64, 64, 16), (100, 70, 37), (2048, 512, 8), (48, 256, 512),
+                                   # large-tile kernel: aligned, and ragged edges in all dims
+                                   (2048, 2048, 96), (2050, 2049, 37)])
 def test_gemm_bf16_matches_float64(ta, tb, M, N, K):
     import torch
     from snoopy_amd import _native as nat
