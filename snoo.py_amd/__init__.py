@@ -8,6 +8,7 @@ kernels in libsnoo_hip.so (C ABI: include/snoo_hip.h).
 """
 from .brick_field import BrickField
 from .curl_curl_assembler import CurlCurlAssembler, CurlCurlAssemblerDiluted
+from .evaluators import evaluate_fem_solution_node_interpolation
 from .finite_element import FiniteElement
 from .materials import (BSplineReluctance, ConstantReluctance, Dilution, Reluctance,
                         tanh_reluctance)
@@ -20,4 +21,4 @@ __all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctanc
            'ConstantReluctance', 'Reluctance', 'tanh_reluctance', 'TetMesh', 'TetModel',
            'kuhn_mesh', 'get_cotree_dofs', 'get_edge_boundary_dofs', 'RacetrackCoil',
            'NewtonDriver', 'RedMVPSolver', 'RedMVPSolverDiluted', 'CurlCurlAssemblerDiluted',
-           'Dilution']
+           'Dilution', 'evaluate_fem_solution_node_interpolation']
