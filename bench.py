@@ -53,6 +53,8 @@ def parse():
     ap.add_argument('--ams-weight', type=float, default=None)
     ap.add_argument('--ams-omega', type=float, default=None)
     ap.add_argument('--ams-theta', type=float, default=None)
+    ap.add_argument('--ams-passes', type=int, default=None)
+    ap.add_argument('--ams-coarse', type=int, default=None)
     ap.add_argument('--workload', default='dipole1m',
                     choices=['dipole1m', 'batch64', 'chain6', 'warmstart'],
                     help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
@@ -165,7 +167,8 @@ def main():
         log('warm start unavailable:', e)
 
     ams_opts = {k: v for k, v in (('weight', args.ams_weight), ('omega', args.ams_omega),
-                                   ('theta', args.ams_theta)) if v is not None}
+                                   ('theta', args.ams_theta), ('passes', args.ams_passes),
+                                   ('coarse_max', args.ams_coarse)) if v is not None}
 
     def one_solve(precision, rtol, use_warm, precond=None):
         precond = precond or args.precond

@@ -94,7 +94,7 @@ class AmsPlan:
     """Host arrays of the preconditioner for one mesh and Dirichlet mask."""
 
     def __init__(self, nodes, edges, cells, free_edge, nu_e, theta=0.08, coarse_max=128,
-                 max_levels=12):
+                 max_levels=12, passes=1):
         t0 = time.perf_counter()
         nodes = np.asarray(nodes, dtype=np.float64).reshape(-1, 3)
         edges = np.asarray(edges).reshape(-1, 2)
@@ -127,6 +127,14 @@ class AmsPlan:
                 # too few strong connections (anisotropic or weakly coupled
                 # rows): aggregate over every connection instead
                 agg, na = aggregate(Al, 0.0)
+            for _ in range(passes - 1):
+                # aggregates of aggregates: fewer, larger levels
+                fr = np.repeat(np.arange(Al.shape[0]), np.diff(Al.indptr))
+                At = _coo_to_csr(na, agg[fr], agg[Al.indices], Al.data)
+                agg2, na2 = aggregate(At, theta)
+                if na2 > 0.5 * na:
+                    agg2, na2 = aggregate(At, 0.0)
+                agg, na = agg2[agg], na2
             if na >= Al.shape[0]:
                 break
             fr = np.repeat(np.arange(Al.shape[0]), np.diff(Al.indptr))
@@ -209,13 +217,13 @@ class AmsPreconditioner:
     """Device handle of the preconditioner for one assembler and mask."""
 
     def __init__(self, asm, free_edge, nu_el_dev, omega=0.7, weight=2.0, theta=0.08,
-                 coarse_max=128):
+                 coarse_max=128, passes=1):
         d = asm._device_state()
         tm = asm.tet_mesh
         cells = tm.cells[d['order']]
         nu_host = nu_el_dev.cpu().numpy().astype(np.float64)
         self.plan = AmsPlan(tm.nodes, tm.edges, cells, free_edge, nu_host, theta=theta,
-                            coarse_max=coarse_max)
+                            coarse_max=coarse_max, passes=passes)
         p = self.plan
         lib = nat.load()
         self.handle = ctypes.c_void_p()
