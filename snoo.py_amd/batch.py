@@ -272,3 +272,55 @@ class BatchedSolver:
         """Per-variant host vectors of a joined solution."""
         h = x.cpu().numpy()
         return [h[self.row0[v]:self.row0[v + 1]] for v in range(self.nvar)]
+
+
+class ConcurrentSolver:
+    """Many variants solved independently on several HIP streams at once.
+
+    Each worker thread owns a stream (default priority) and solves the
+    variants handed to it one after another with the single-variant Newton
+    solver; the GPU overlaps the kernels of the streams, which fills the
+    device where one small variant alone is launch- or latency-bound.  The
+    host waits of one stream release the interpreter lock (they happen inside
+    the C library), so the threads' host work overlaps too."""
+
+    def __init__(self, dipoles, workers=4, quad_order=2, precision='mixed', precond='ams',
+                 lanes_per_row=8):
+        import torch
+        self.dipoles = list(dipoles)
+        self.workers = max(1, int(workers))
+        self.kw = dict(precision=precision, precond=precond, lanes_per_row=lanes_per_row)
+        self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
+        self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
+
+    def _run(self, w, idx, rtol, out):
+        import torch
+        with torch.cuda.stream(self.streams[w]):
+            for i in idx:
+                s = self.solvers[i]
+                s._src = None
+                x = s.solve_newton(self.kw['precision'], rtol=rtol,
+                                   lanes_per_row=self.kw['lanes_per_row'],
+                                   precond=self.kw['precond'])
+                out[i] = (x, dict(s.stats))
+            torch.cuda.current_stream().synchronize()
+
+    def solve(self, rtol=1e-8):
+        """Solutions (device tensors) and stats of every variant, in order."""
+        import threading
+        import torch
+        out = [None] * len(self.dipoles)
+        # largest variants first, dealt round-robin, so the streams finish together
+        order = np.argsort([-d.num_tets for d in self.dipoles])
+        parts = [order[w::self.workers] for w in range(self.workers)]
+        torch.cuda.synchronize()
+        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out))
+                   for w in range(self.workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+        if any(o is None for o in out):
+            raise RuntimeError('a worker failed; see the thread traceback above')
+        return out

@@ -514,9 +514,12 @@ static void share_blocks(const std::vector<int> &row0, int target, int rows_per_
     }
 }
 
-static int upload_ints(int **dst, const std::vector<int> &h) {
+// Copies go through the caller's stream (never the legacy stream, which
+// would have to wait for streams another thread may be capturing).
+static int upload_ints(int **dst, const std::vector<int> &h, hipStream_t s) {
     SN_HIP(hipMalloc(dst, h.size() * sizeof(int)));
-    SN_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+    SN_HIP(hipMemcpyAsync(*dst, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    SN_HIP(hipStreamSynchronize(s));
     return SN_OK;
 }
 
@@ -641,11 +644,11 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
     C->grid_rows = blkA0[nseg];
     C->grid_vec = blkB0[nseg];
     int rc;
-    if ((rc = upload_ints(&C->row0, row0))) return rc;
-    if ((rc = upload_ints(&C->blkA0, blkA0))) return rc;
-    if ((rc = upload_ints(&C->blkB0, blkB0))) return rc;
-    if ((rc = upload_ints(&C->segA, segA))) return rc;
-    if ((rc = upload_ints(&C->segB, segB))) return rc;
+    if ((rc = upload_ints(&C->row0, row0, as_stream(stream)))) return rc;
+    if ((rc = upload_ints(&C->blkA0, blkA0, as_stream(stream)))) return rc;
+    if ((rc = upload_ints(&C->blkB0, blkB0, as_stream(stream)))) return rc;
+    if ((rc = upload_ints(&C->segA, segA, as_stream(stream)))) return rc;
+    if ((rc = upload_ints(&C->segB, segB, as_stream(stream)))) return rc;
     const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
     SN_HIP(hipMalloc(&C->r, vb));
     SN_HIP(hipMalloc(&C->z, vb));
@@ -792,7 +795,8 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     SN_HIP(hipEventElapsedTime(&ms, e0, e1));
     SN_HIP(hipEventDestroy(e0));
     SN_HIP(hipEventDestroy(e1));
-    SN_HIP(hipMemcpy(h.data(), C->st, C->nseg * sizeof(CgState), hipMemcpyDeviceToHost));
+    SN_HIP(hipMemcpyAsync(h.data(), C->st, C->nseg * sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SN_HIP(hipStreamSynchronize(s));
     for (const auto &e : h)
         SN_REQUIRE(e.done == 0, "sn_cg_time_spmv: the timed launches stopped early (done=%d)",
                    e.done);

@@ -72,7 +72,7 @@ class RedMVPSolver:
             src = [source_field_device(self.coil_list, p).reshape(-1) for p in pts]
             src += [None] * len(self.air_tags)
             import torch
-            torch.cuda.synchronize()
+            torch.cuda.current_stream().synchronize()
             self.stats['t_source'] = time.perf_counter() - t0
             self.stats['num_targets'] = int(sum(p.shape[0] for p in pts))
             self._src = src
@@ -412,7 +412,7 @@ class NewtonDriver:
             if log is not None:
                 log('  newton %d: |R|/|R0| = %.3e, cg %d, eta %.1e, step %.3g, %.3f s'
                     % (k, rn / r0, its, eta, step, time.perf_counter() - t0))
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
         stats.update({'residual_history': hist, 'cg_iterations': cg_its, 'etas': etas,
                       'cg_modes': modes,
                       'newton_steps': k, 'converged': converged, 'r0': r0,
