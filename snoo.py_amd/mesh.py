@@ -243,8 +243,62 @@ def _boundary_faces(g):
     return np.concatenate(tris)
 
 
+def locate_points_binned(mesh, pts, bins_per_axis=None):
+    """Element containing each point of any tetrahedral mesh (-1 outside):
+    elements are hashed into a uniform grid of bins by their bounding boxes,
+    and each point tests the candidates of its bin by barycentric
+    coordinates, keeping the element whose smallest coordinate is largest
+    (the containing one; ties on shared faces go to the first best)."""
+    pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+    X = mesh.nodes[mesh.cells]                                    # [m,4,3]
+    lo_e, hi_e = X.min(axis=1), X.max(axis=1)
+    lo, hi = mesh.nodes.min(axis=0), mesh.nodes.max(axis=0)
+    if bins_per_axis is None:
+        bins_per_axis = max(1, int(round((mesh.num_elements / 4.0) ** (1.0 / 3.0))))
+    nb = np.full(3, bins_per_axis)
+    size = (hi - lo) / nb * (1 + 1e-12) + 1e-300
+    b0 = np.clip(((lo_e - lo) / size).astype(np.int64), 0, nb - 1)
+    b1 = np.clip(((hi_e - lo) / size).astype(np.int64), 0, nb - 1)
+    span = b1 - b0 + 1
+    cnt = span.prod(axis=1)
+    el = np.repeat(np.arange(mesh.num_elements), cnt)
+    off = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    sp = span[el]
+    i = b0[el, 0] + off % sp[:, 0]
+    j = b0[el, 1] + (off // sp[:, 0]) % sp[:, 1]
+    k = b0[el, 2] + off // (sp[:, 0] * sp[:, 1])
+    key = (k * nb[1] + j) * nb[0] + i
+    order = np.argsort(key, kind='stable')
+    key, el = key[order], el[order]
+    starts = np.searchsorted(key, np.arange(nb.prod() + 1))
+    out = np.full(len(pts), -1, dtype=np.int64)
+    best = np.full(len(pts), -1e-12)
+    inside = np.all((pts >= lo - 1e-12) & (pts <= hi + 1e-12), axis=1)
+    pb = np.clip(((pts - lo) / size).astype(np.int64), 0, nb - 1)
+    pkey = (pb[:, 2] * nb[1] + pb[:, 1]) * nb[0] + pb[:, 0]
+    ncand = np.where(inside, starts[pkey + 1] - starts[pkey], 0)
+    pid = np.repeat(np.arange(len(pts)), ncand)
+    cidx = np.arange(ncand.sum()) - np.repeat(np.cumsum(ncand) - ncand, ncand)
+    cand = el[starts[pkey[pid]] + cidx]
+    Xc = X[cand]
+    M = np.stack([Xc[:, 1] - Xc[:, 0], Xc[:, 2] - Xc[:, 0], Xc[:, 3] - Xc[:, 0]], axis=2)
+    lam = np.linalg.solve(M, (pts[pid] - Xc[:, 0])[:, :, None])[:, :, 0]
+    score = np.minimum(1.0 - lam.sum(axis=1), lam.min(axis=1))
+    # best candidate of each point: first of its group after sorting by -score
+    order = np.lexsort((-score, pid))
+    first = np.ones(len(order), dtype=bool)
+    first[1:] = pid[order][1:] != pid[order][:-1]
+    q = order[first]
+    keep = score[q] > best[pid[q]]
+    out[pid[q][keep]] = cand[q][keep]
+    return out
+
+
 def locate_points(mesh, pts):
-    """Element containing each point of a structured Kuhn mesh (-1 outside)."""
+    """Element containing each point (-1 outside): by the structured grid of
+    a Kuhn mesh, or by element bins for any other mesh."""
+    if getattr(mesh, 'grid', None) is None:
+        return locate_points_binned(mesh, pts)
     g = mesh.grid
     pts = np.asarray(pts, dtype=np.float64)
     out = np.full(pts.shape[0], -1, dtype=np.int64)

@@ -362,3 +362,24 @@ def test_diluted_assembly_and_solve(dipole):
     Bd = asm.element_B(xd.cpu().numpy())
     assert _rel(B1, B0) < 1e-8
     assert _rel(Bd, B0) > 1e-4      # the dilution changes the field
+
+
+def test_msh_mesh_solves_like_the_original(dipole, solver, tmp_path):
+    """A mesh written to and read back from a gmsh 4.1 file (elements
+    regrouped by volume, point location by element bins) gives the original
+    solution's B at probe points."""
+    from snoopy_amd.evaluators import evaluate_fem_solution_node_interpolation
+    from snoopy_amd.msh_io import read_msh, write_msh
+    from snoopy_amd.red_mvp_solver import RedMVPSolver
+    p = tmp_path / 'dipole.msh'
+    write_msh(str(p), dipole.mesh)
+    model = read_msh(str(p))
+    s2 = RedMVPSolver(model, dipole.coils, [1], [2], [dipole.reluctance], quad_order=2,
+                      verbose=False)
+    x1 = solver.solve_newton('fp64', rtol=1e-10).cpu().numpy()
+    x2 = s2.solve_newton('fp64', rtol=1e-10).cpu().numpy()
+    probes = dipole.probes(1000, seed=8)
+    B1 = evaluate_fem_solution_node_interpolation(dipole.model, probes, solver, dipole.coils, x1,
+                                                  [1, 2])
+    B2 = evaluate_fem_solution_node_interpolation(model, probes, s2, dipole.coils, x2, [1, 2])
+    assert _rel(B2, B1) < 1e-7
