@@ -16,9 +16,15 @@ from .mesh import TetMesh, TetModel, kuhn_mesh
 from .mesh_tools import get_cotree_dofs, get_edge_boundary_dofs
 from .racetrack_coil import RacetrackCoil
 from .red_mvp_solver import NewtonDriver, RedMVPSolver, RedMVPSolverDiluted
+from .geometry_tools import compute_area_polygon
+from .SHiP_functions import (compute_J_opt, compute_prices, get_NI, get_vector_field_mag_1,
+                             get_vector_field_mag_2, get_vector_field_mag_3,
+                             get_vector_field_mag_4)
 
 __all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctance',
            'ConstantReluctance', 'Reluctance', 'tanh_reluctance', 'TetMesh', 'TetModel',
            'kuhn_mesh', 'get_cotree_dofs', 'get_edge_boundary_dofs', 'RacetrackCoil',
            'NewtonDriver', 'RedMVPSolver', 'RedMVPSolverDiluted', 'CurlCurlAssemblerDiluted',
-           'Dilution', 'evaluate_fem_solution_node_interpolation']
+           'Dilution', 'evaluate_fem_solution_node_interpolation', 'compute_area_polygon',
+           'compute_J_opt', 'compute_prices', 'get_NI', 'get_vector_field_mag_1',
+           'get_vector_field_mag_2', 'get_vector_field_mag_3', 'get_vector_field_mag_4']

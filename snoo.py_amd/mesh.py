@@ -195,17 +195,10 @@ def kuhn_mesh(xs, ys, zs, tag_fn):
     hexahedron by hexahedron (i fastest), 6 per hexahedron.
     """
     g = StructuredGrid(xs, ys, zs)
-    nx, ny, nz = g.nx, g.ny, g.nz
     X, Y, Z = np.meshgrid(g.xs, g.ys, g.zs, indexing='ij')
     nodes = np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(),
                       Z.transpose(2, 1, 0).ravel()], axis=1)
-    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij')
-    I, J, K = (v.transpose(2, 1, 0).ravel() for v in (I, J, K))
-    cells = np.empty((I.size, 6, 4), dtype=np.int64)
-    for t, verts in enumerate(_KUHN):
-        for a, (di, dj, dk) in enumerate(verts):
-            cells[:, t, a] = g.node_id(I + di, J + dj, K + dk)
-    cells = cells.reshape(-1, 4)
+    I, J, K, cells = _kuhn_cells(g)
     xc = 0.5 * (g.xs[I] + g.xs[I + 1])
     yc = 0.5 * (g.ys[J] + g.ys[J + 1])
     zc = 0.5 * (g.zs[K] + g.zs[K + 1])
@@ -214,10 +207,52 @@ def kuhn_mesh(xs, ys, zs, tag_fn):
     return TetMesh(nodes, cells, tags, boundary_faces=faces, grid=g)
 
 
-def _boundary_faces(g):
+def _kuhn_cells(g):
+    """Hexahedron indices (I, J, K) and the 6 Kuhn tetrahedra of each, as
+    node ids of the grid; hexahedra ordered i fastest."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing='ij')
+    I, J, K = (v.transpose(2, 1, 0).ravel() for v in (I, J, K))
+    cells = np.empty((I.size, 6, 4), dtype=np.int64)
+    for t, verts in enumerate(_KUHN):
+        for a, (di, dj, dk) in enumerate(verts):
+            cells[:, t, a] = g.node_id(I + di, J + dj, K + dk)
+    return I, J, K, cells.reshape(-1, 4)
+
+
+ALL_SIDES = ('x0', 'x1', 'y0', 'y1', 'z0', 'z1')
+
+
+def mapped_kuhn_mesh(nodes, shape, hex_tags, sides=ALL_SIDES):
+    """Kuhn split of a structured hexahedral grid whose nodes are placed
+    freely: node (i, j, k) of the (nx+1) x (ny+1) x (nz+1) lattice is
+    ``nodes[i + (nx+1) (j + (ny+1) k)]``.  Used for grids bent to follow
+    tapered faces.  ``hex_tags`` [nx*ny*nz] (i fastest) are the volume tags;
+    ``sides`` names the outer faces that carry the Dirichlet condition
+    ('x0' is i = 0, 'x1' is i = nx, and so on).  The mapping must keep every
+    tetrahedron positively oriented; a ValueError says otherwise.  The mesh
+    has no tensor grid for point location (element bins are used)."""
+    nx, ny, nz = shape
+    g = StructuredGrid(np.arange(nx + 1.0), np.arange(ny + 1.0), np.arange(nz + 1.0))
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
+    if nodes.shape[0] != (nx + 1) * (ny + 1) * (nz + 1):
+        raise ValueError('mapped_kuhn_mesh: %d nodes for a %dx%dx%d grid'
+                         % (nodes.shape[0], nx, ny, nz))
+    _, _, _, cells = _kuhn_cells(g)
+    X = nodes[cells]
+    det = np.einsum('ij,ij->i', X[:, 1] - X[:, 0],
+                    np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))
+    if not np.all(det > 0):
+        raise ValueError('mapped_kuhn_mesh: %d tetrahedra are not positively oriented '
+                         '(the mapping folds the grid)' % int(np.sum(det <= 0)))
+    tags = np.repeat(np.asarray(hex_tags, dtype=np.int32).ravel(), 6)
+    return TetMesh(nodes, cells, tags, boundary_faces=_boundary_faces(g, sides), grid=None)
+
+
+def _boundary_faces(g, sides=ALL_SIDES):
     """Triangles of the outer surface (each grid face quad split in two along
     the diagonal the Kuhn split uses; every edge of the face plane is
-    covered either way)."""
+    covered either way), on the named sides only."""
     tris = []
     nx, ny, nz = g.nx, g.ny, g.nz
 
@@ -237,10 +272,11 @@ def _boundary_faces(g):
         tris.append(np.stack([n00, n10, n11], axis=1))
         tris.append(np.stack([n00, n01, n11], axis=1))
 
-    quads(0, 0); quads(0, nx)
-    quads(1, 0); quads(1, ny)
-    quads(2, 0); quads(2, nz)
-    return np.concatenate(tris)
+    for name, axis, val in (('x0', 0, 0), ('x1', 0, nx), ('y0', 1, 0), ('y1', 1, ny),
+                            ('z0', 2, 0), ('z1', 2, nz)):
+        if name in sides:
+            quads(axis, val)
+    return np.concatenate(tris) if tris else np.zeros((0, 3), dtype=np.int64)
 
 
 def locate_points_binned(mesh, pts, bins_per_axis=None):

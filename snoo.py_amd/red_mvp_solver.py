@@ -336,8 +336,9 @@ class NewtonDriver:
     def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=None, max_cg=-1,
               line_search=True, stats=None, log=None, stall_steps=3):
         """Newton steps until ||R|| <= rtol ||R(0)||, ``max_newton`` steps, or
-        ``stall_steps`` consecutive steps that reduce ||R|| by less than 10%
-        (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
+        ``stall_steps`` consecutive stalled steps: full steps that reduce ||R||
+        by less than 10%, or steps that no length of the line search makes
+        reduce it (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
         linear solve at min(10 n, 50000) iterations; a float32 solve is cut at
         4x the previous step's count (at least 1500, 300 with the auxiliary-
         space preconditioner) and, if it stalls there,
@@ -404,7 +405,11 @@ class NewtonDriver:
                     x_try = x + step * dx
                     rn_try = self.residual(x_try)
                     tries += 1
-            stalls = stalls + 1 if rn_try > 0.9 * rn else 0
+            # stalled: no step length lowers ||R||, or the full step gains
+            # less than 10%; damped steps that make progress (the first
+            # steps into saturation) are not stalls
+            stalled = rn_try >= rn or (step == 1.0 and rn_try > 0.9 * rn)
+            stalls = stalls + 1 if stalled else 0
             rn_prev, rn, x = rn, rn_try, x_try
             hist.append(rn / r0)
             k += 1

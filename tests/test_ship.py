@@ -1,0 +1,203 @@
+"""CPU: the SHiP templates' host side -- geometry, mesh, coils and cost model.
+
+Known answers are the figures the reference's own template tests assert
+(tests/test_mag_1.py:79-82, test_mag_1_dil.py:142-145, test_mag_3.py:66-69,
+test_mag_3_dil.py:140-143, test_mag_4.py:138-141).  Coil mass, power and
+current density depend only on the coil layout and the cost model, so they
+must agree to the bit; the iron mass comes from gmsh's OCC volume there and
+from the exact ruled-solid volume here, so it agrees to rounding.
+"""
+import importlib
+import json
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from conftest import GOLDEN
+
+SHIP = os.path.join(GOLDEN, 'ship')
+MAT = os.path.join(SHIP, 'materials')
+
+# test, template, diluted, (M_i, M_c, Q, J) as the reference tests assert them
+KNOWN = [
+    ('test_mag_1', 1, False, (241892319.99999994, 447739.94824795314, 88448.62816951754,
+                              10000000.0)),
+    ('test_mag_1_dil', 1, True, (262326858.94666672, 140580.8687436417, 7591.366912156654,
+                                 5228345.170351609)),
+    ('test_mag_3', 3, False, (89058843.98683298, 260537.6988856567, 51467.82667718888,
+                              10000000.0)),
+    ('test_mag_3_dil', 3, True, (211835719.68, 219941.53877284116, 11876.843093733421,
+                                 5228345.170351607)),
+    ('test_mag_4', 4, True, (265423620.0, 231182.49808090203, 12483.854896368712,
+                             5228345.170351609)),
+]
+
+
+def sf():
+    return importlib.import_module('snoopy_amd.SHiP_functions')
+
+
+def params(test):
+    return pd.read_csv(os.path.join(SHIP, test, 'parameters.csv'))
+
+
+@pytest.mark.parametrize('test,kind,dil,known', KNOWN, ids=[k[0] for k in KNOWN])
+def test_template_figures_match_reference_asserts(test, kind, dil, known):
+    tp = sf().build_template(kind, params(test), 0, lc=0.4, materials_directory=MAT,
+                             use_diluted_steel=dil)
+    M_i, M_c, Q, J = known
+    assert tp.M_coil == M_c
+    assert tp.Q == Q
+    assert tp.current_density == J
+    assert tp.M_iron == pytest.approx(M_i, rel=1e-15)
+    # the mesh resolves the iron exactly: its volume is the ruled-solid volume
+    tm = tp.model.tet_mesh
+    vol = tm.element_volumes()
+    dens = json.load(open(os.path.join(MAT, params(test)['material'][0])))['density(g/m3)']
+    v_iron = sum(vol[tm.tags == t].sum() for t in tp.iron_tags)
+    assert 4 * v_iron * dens == pytest.approx(M_i, rel=1e-12)
+    assert np.all(vol > 0)
+
+
+def _iron_at(p, kind, pts, dil):
+    """Analytic membership of points in the iron of a template (breaks
+    interpolated linearly in z), independent of the mesher."""
+    x, y, z = pts.T
+    s = (z - p['Z_pos']) / p['Z_len']
+    inz = (s > 0) & (s < 1)
+    lin = lambda k: (1 - s) * p[k + '_1'] + s * p[k + '_2']  # noqa: E731
+    if kind == 4:
+        X = {n: lin('X_' + n) for n in 'ABCDEF'}
+        yc, yy = lin('Y_core'), lin('Y_yoke')
+        core = (x > X['C']) & (x < X['D']) & (y < yc)
+        yoke = (x > X['A']) & (x < X['F']) & (y < yy) & ~((x > X['B']) & (x < X['E']) & (y < yc))
+        return inz & core, inz & yoke
+    X = {n: lin('X_' + n) for n in ('mgap', 'core', 'void', 'yoke')}
+    yc, yy = lin('Y_core'), lin('Y_yoke')
+    core = (x > X['mgap']) & (x < X['core']) & (y < yc)
+    top = (x > X['mgap']) & (x < X['yoke']) & (y > yc) & (y < yy)
+    leg = (x > X['void']) & (x < X['yoke']) & (y < (lin('Y_void') if dil else yc))
+    if dil:
+        return inz & (core | top), inz & leg & ~(core | top)
+    return inz & (core | top | leg), np.zeros_like(inz)
+
+
+@pytest.mark.parametrize('test,kind,dil', [(k[0], k[1], k[2]) for k in KNOWN],
+                         ids=[k[0] for k in KNOWN])
+def test_mesh_tags_follow_the_tapered_solid(test, kind, dil):
+    S = sf()
+    p = S._mag4_row(params(test), 0) if kind == 4 else S._row(params(test), 0)
+    tp = S.build_template(kind, params(test), 0, lc=0.4, materials_directory=MAT,
+                          use_diluted_steel=dil)
+    tm = tp.model.tet_mesh
+    cen = tm.nodes[tm.cells].mean(axis=1)
+    a, b = _iron_at(p, kind, cen, dil)
+    if len(tp.iron_tags) == 1:
+        np.testing.assert_array_equal(tm.tags == 1, a)
+    else:
+        np.testing.assert_array_equal(tm.tags == 1, a)
+        np.testing.assert_array_equal(tm.tags == 2, b)
+
+
+def test_dirichlet_edges_on_the_box_but_not_the_symmetry_plane():
+    S = sf()
+    tp = S.build_template(3, params('test_mag_3'), 0, lc=0.4, materials_directory=MAT)
+    tm = tp.model.tet_mesh
+    lo, hi = tm.nodes.min(axis=0), tm.nodes.max(axis=0)
+    on = lambda P: ((np.abs(P[:, 0] - lo[0]) < 1e-12) | (np.abs(P[:, 0] - hi[0]) < 1e-12)  # noqa
+                    | (np.abs(P[:, 1] - hi[1]) < 1e-12) | (np.abs(P[:, 2] - lo[2]) < 1e-12)
+                    | (np.abs(P[:, 2] - hi[2]) < 1e-12))
+    e = tm.edges[tm.boundary_edges]
+    a, b = tm.nodes[e[:, 0]], tm.nodes[e[:, 1]]
+    assert np.all(on(a) & on(b))
+    # every edge with both ends on such a plane and in it is Dirichlet
+    ea, eb = tm.nodes[tm.edges[:, 0]], tm.nodes[tm.edges[:, 1]]
+    same = np.zeros(tm.num_edges, dtype=bool)
+    for ax, v in ((0, lo[0]), (0, hi[0]), (1, hi[1]), (2, lo[2]), (2, hi[2])):
+        same |= (np.abs(ea[:, ax] - v) < 1e-12) & (np.abs(eb[:, ax] - v) < 1e-12)
+    assert set(np.nonzero(same)[0]) == set(tm.boundary_edges.tolist())
+    # y = 0 keeps the natural condition
+    y0 = (np.abs(ea[:, 1]) < 1e-12) & (np.abs(eb[:, 1]) < 1e-12) & ~same
+    assert y0.sum() > 0 and not np.any(np.isin(np.nonzero(y0)[0], tm.boundary_edges))
+
+
+def test_cost_model():
+    S = sf()
+    p = params('test_mag_1')
+    c_iron, c_coil, c_edf = S.compute_prices(p, 0, 1e6, 1e3, 100.0, materials_directory=MAT)
+    # aisi1010: 2 + 6 CHF/kg; water-cooled copper: 20 + 250 CHF/kg; peanut price 0.5
+    assert c_iron == pytest.approx(1e3 * 8 * 0.5, rel=1e-15)
+    assert c_coil == pytest.approx(1.0 * 270 * 0.5, rel=1e-15)
+    assert c_edf == pytest.approx(100.0 * 5.0 * 0.5, rel=1e-15)
+    cu = json.load(open(os.path.join(MAT, 'copper_water_cooled.json')))
+    J = S.compute_J_opt(cu, 5.0)
+    # the optimum balances conductor cost against electricity: at J_opt the
+    # derivative of kappa_cu dens A + kappa_elec rho I^2 / A in A is zero
+    A = 1.0 / (J * 1e6)
+    kappa_cu = 270.0 * 1e-3
+    d_cost = kappa_cu * cu['density(g/m3)'] - 5.0 * cu['resistivity(Ohm.m)'] / A ** 2
+    assert abs(d_cost) < 1e-9 * kappa_cu * cu['density(g/m3)']
+
+
+def test_get_NI_circuit():
+    S = sf()
+    p = params('test_mag_1')
+    ni, ok = S.get_NI(1.7, p, 0, materials_directory=MAT)
+    ni2, _ = S.get_NI(1.8, p, 0, materials_directory=MAT)
+    assert 0 < ni < ni2                       # more field needs more ampere-turns
+    # In test_mag_1's geometry the core cut (x 0..0.5), the plate cut
+    # (y 1.19..1.69) and the leg cut (x 0.52..1.02) have the same area, so
+    # every section carries B_goal and NI = H(B_goal) times the path
+    # l1 + l2 + l3 = 1.44 + 1.04 + 1.44 m.  At a tabulated B the PCHIP of
+    # nu = H/B returns the table's H exactly.
+    bh = json.load(open(os.path.join(MAT, 'aisi1010.json')))['BH_data']
+    for k in (30, 35, 40):
+        B, H = bh['B(T)'][k], bh['H(A/m)'][k]
+        ni_k, _ = S.get_NI(B, p, 0, materials_directory=MAT)
+        assert ni_k == pytest.approx(H * (1.44 + 1.04 + 1.44), rel=1e-12)
+    with pytest.raises(ValueError):
+        q = p.copy()
+        q.loc[0, 'yoke_type'] = 'Mag4_1'
+        S.get_NI(1.0, q, 0, materials_directory=MAT)
+
+
+def test_structured_mesher_rejects_breaks_that_touch_at_one_end():
+    gt = importlib.import_module('snoopy_amd.geometry_tools')
+    with pytest.raises(ValueError):
+        gt.ruled_quarter_mesh({'a': (0.0, 0.2), 'b': (0.3, 0.5)}, {'c': (0.4, 0.4)}, 0.0, 1.0,
+                              1.0, 1.0, 0.5, [(1, [('a', 'b', '0', 'c')])], 2, 0.1)
+    m = gt.ruled_quarter_mesh({'a': (0.1, 0.2), 'b': (0.3, 0.5)}, {'c': (0.4, 0.4)}, 0.0, 1.0,
+                              1.0, 1.0, 0.5, [(1, [('a', 'b', '0', 'c')])], 2, 0.1)
+    assert np.all(m.tet_mesh.element_volumes() > 0)
+
+
+def mag2_params():
+    """A superconducting-template row (no reference test holds one): tapered
+    core, void and yoke, on test_mag_1's coil and material columns."""
+    p = params('test_mag_1').copy()
+    p.loc[0, 'yoke_type'] = 'Mag2'
+    geo = {'Xcore1(m)': 0.3, 'Xcore2(m)': 0.35, 'Xvoid1(m)': 0.5, 'Xvoid2(m)': 0.55,
+           'Xyoke1(m)': 1.0, 'Xyoke2(m)': 1.1, 'Ycore1(m)': 0.4, 'Ycore2(m)': 0.45,
+           'Yvoid1(m)': 0.6, 'Yvoid2(m)': 0.6, 'Yyoke1(m)': 1.0, 'Yyoke2(m)': 1.05,
+           'Z_pos(m)': -1.0, 'Z_len(m)': 2.0, 'NI(A)': 2e5}
+    for k, v in geo.items():
+        p.loc[0, k] = v
+    return p
+
+
+def test_mag2_template_volume_and_tags():
+    S = sf()
+    tp = S.build_template(2, mag2_params(), 0, lc=0.4, materials_directory=MAT)
+    tm = tp.model.tet_mesh
+    vol = tm.element_volumes()
+    # core [0, Xc] x [0, Yc] and yoke [0, Xy] x [0, Yy] less [0, Xv] x [0, Yv],
+    # all linear in z: exact volumes by the mean of end and mid sections
+    sec = lambda s: ((0.3 + 0.05 * s) * (0.4 + 0.05 * s)  # noqa: E731
+                     + (1.0 + 0.1 * s) * (1.0 + 0.05 * s) - (0.5 + 0.05 * s) * 0.6)
+    v_exact = 2.0 / 6.0 * (sec(0) + 4 * sec(0.5) + sec(1))
+    assert vol[tm.tags <= 2].sum() == pytest.approx(v_exact, rel=1e-12)
+    assert tp.M_iron == pytest.approx(4 * v_exact * 7.87e6, rel=1e-14)
+    assert tp.iron_tags == [1, 2] and tp.air_tags == [3]
+    assert len(tp.coils) == 1
