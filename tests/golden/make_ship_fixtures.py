@@ -8,9 +8,9 @@ reference's own test inputs and outputs.
   (files/materials/{aisi1010,iron_pw,copper_water_cooled}.json), copied;
 * <test>/parameters.csv: the template parameters (files/tests/<test>/parameters.csv);
 * <test>/ref.npz: ``B_eval`` = B_eval_ref.csv (x, y, z, Bx, By, Bz at the
-  probe points the test evaluates by nodal interpolation), and ``map_sample``
-  = every k-th row of map.npy (quadrature-point positions of the reference's
-  gmsh mesh and B there), at most 2000 rows.
+  probe points the test evaluates by nodal interpolation).  The field maps
+  (map.npy) sit at the quadrature points of the reference's own gmsh mesh
+  and are not copied.
 
     python tests/golden/make_ship_fixtures.py
 """
@@ -37,12 +37,8 @@ def main():
         shutil.copyfile(os.path.join(SRC, 'tests', t, 'parameters.csv'),
                         os.path.join(d, 'parameters.csv'))
         b_eval = pd.read_csv(os.path.join(SRC, 'tests', t, 'B_eval_ref.csv')).values
-        mp = np.load(os.path.join(SRC, 'tests', t, 'map.npy'))
-        step = max(1, int(np.ceil(mp.shape[0] / 2000)))
-        np.savez_compressed(os.path.join(d, 'ref.npz'), B_eval=b_eval.astype(np.float64),
-                            map_sample=mp[::step].astype(np.float64), map_rows=mp.shape[0],
-                            map_step=step)
-        print(t, b_eval.shape, mp.shape, 'step', step)
+        np.savez_compressed(os.path.join(d, 'ref.npz'), B_eval=b_eval.astype(np.float64))
+        print(t, b_eval.shape)
 
 
 if __name__ == '__main__':
