@@ -239,8 +239,16 @@ def main():
                                          ctypes.byref(ms), nat.stream_handle()))
     n, nnz = asm.num_dofs, asm.nnz
     s = 4  # float32 values and vectors
-    spmv_bytes = nnz * (s + 4) + 4 * (n + 1) + 4 * s * n + n
+    ib = nat.load().sn_cg_index_bytes(drv.cg.h)   # 2: 16-bit column offsets, 4: int32
+    spmv_bytes = nnz * (s + ib) + 4 * (n + 1) + 4 * s * n + n
     spmv_gbs = spmv_bytes / (ms.value * 1e-3) / 1e9
+    ms32 = ctypes.c_double(ms.value)
+    if ib == 2:
+        # the same launches on int32 column indices, for comparison
+        nat.check(nat.load().sn_cg_set_col_offsets(drv.cg.h, 0))
+        nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
+                                             ctypes.byref(ms32), nat.stream_handle()))
+        nat.check(nat.load().sn_cg_set_col_offsets(drv.cg.h, 1))
     # element kernel (mixed, residual mode) timed with torch events on its stream
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record()
@@ -261,7 +269,8 @@ def main():
     # (PMC counts of the same kernel at this mesh and lane count; see profiles/)
     if os.path.exists(tfile):
         tr = json.load(open(tfile))
-        if tr.get('tets') == N and tr.get('lanes_per_row') == args.lanes:
+        if (tr.get('tets') == N and tr.get('lanes_per_row') == args.lanes
+                and tr.get('index_bytes', 4) == ib):
             traffic = tr['hbm_bytes_per_launch']
     cpu = None
     if not args.no_cpu:
@@ -310,7 +319,8 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': traffic,
                      'kernel': 'cg_spmv_kernel<float,%d>' % args.lanes,
-                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': spmv_bytes},
+                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': spmv_bytes,
+                     'index_bytes': ib, 'avg_ms_int32_indices': ms32.value},
         'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
                             'GB/s': asm_bytes / (asm_ms * 1e-3) / 1e9},
         'cpu_baseline': cpu,

@@ -248,6 +248,18 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
  * them.  Default 0. */
 int sn_cg_set_schedule(sn_cg *cg, int schedule);
 
+/* Column storage of the CG SpMV (no counterpart in the reference, whose
+ * scipy CG multiplies a CSR matrix with int32 indices).  At creation the
+ * workspace stores every column as a 16-bit offset j - i when all of them
+ * fit (|j - i| <= 32767, true for the edge numbering of structured meshes up
+ * to a few million elements), halving the SpMV's index stream; results are
+ * bit-identical either way.  enable = 0 goes back to the int32 indices;
+ * enable = 1 fails with SN_ERR_ARG when the offsets do not fit. */
+int sn_cg_set_col_offsets(sn_cg *cg, int enable);
+
+/* Bytes per column index the SpMV reads: 2 (offsets) or 4. */
+int sn_cg_index_bytes(const sn_cg *cg);
+
 /* Average duration (ms) of the CG SpMV kernel (kernel A: p = z + beta p,
  * q = A p, p.q) over `reps` back-to-back launches on `stream`, timed with HIP
  * events; run after a solve on the same workspace (it overwrites the CG

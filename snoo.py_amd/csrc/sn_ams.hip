@@ -37,6 +37,7 @@
 //   edge_prolong     z = D_J^-1 r + w Pi y0, partial sums of r.z
 // Pre- and post-smoothing are the same damped Jacobi step, so the cycle is a
 // symmetric operator and M is symmetric positive definite on the range of J.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -118,24 +119,24 @@ __global__ __launch_bounds__(kAmsBlock) void ams_lap_local_kernel(
         }
 }
 
-// edge tangents (x_hi - x_lo)/2 from the current node coordinates, per edge
-// and per incidence (the incidence copy keeps the restriction's reads in order)
-__global__ void ams_tangent_kernel(int n_edges, const double *__restrict__ nodes,
-                                   const int2 *__restrict__ edge_mesh, float4 *__restrict__ tan) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_edges) return;
-    const int2 en = edge_mesh[e];
-    tan[e] = make_float4((float)(0.5 * (nodes[3 * en.y] - nodes[3 * en.x])),
-                         (float)(0.5 * (nodes[3 * en.y + 1] - nodes[3 * en.x + 1])),
-                         (float)(0.5 * (nodes[3 * en.y + 2] - nodes[3 * en.x + 2])), 0.f);
+// node coordinates in float, in the preconditioner's node order; the edge
+// tangents (x_hi - x_lo)/2 are formed from them where they are used
+// (a 16-byte gather from a few MB that stays in cache, instead of a
+// 16-byte tangent streamed per edge and per incidence)
+__global__ void ams_coords_kernel(int n_nodes, const double *__restrict__ nodes,
+                                  const int *__restrict__ node_mesh, float4 *__restrict__ xyz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_nodes) return;
+    const int m = node_mesh[i];
+    xyz[i] = make_float4((float)nodes[3 * m], (float)nodes[3 * m + 1], (float)nodes[3 * m + 2],
+                         0.f);
 }
 
-__global__ void ams_inc_tangent_kernel(int n_inc, const int *__restrict__ inc_edge,
-                                       const float4 *__restrict__ tan,
-                                       float4 *__restrict__ inc_tan) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_inc) return;
-    inc_tan[k] = tan[inc_edge[k]];
+// half the difference of two coordinates: both the restriction and the
+// prolongation form a tangent as half_diff(x_hi, x_lo), so the two use the
+// same rounded values and stay exact transposes
+__device__ __forceinline__ float4 half_diff(const float4 &hi, const float4 &lo) {
+    return make_float4(0.5f * (hi.x - lo.x), 0.5f * (hi.y - lo.y), 0.5f * (hi.z - lo.z), 0.f);
 }
 
 __global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
@@ -188,20 +189,26 @@ __global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
 }
 
 // ------------------------------------------------------------------- apply
-// LPR lanes per free node: b0 = Pi^T r (tangents stored per incidence, so the
-// incidence stream is read in order) and the pre-smoothed x0 = w D^-1 b0.
+// LPR lanes per free node a: b0 = Pi^T r over the incidences of a (edge id
+// and other end point streamed in order) and the pre-smoothed x0 = w D^-1 b0.
 template <typename V, int LPR>
 __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     int n_free, const int *__restrict__ inc_ptr, const int *__restrict__ inc_edge,
-    const float4 *__restrict__ inc_tan, const V *__restrict__ r, const float *__restrict__ dinv0,
-    float omega, float4 *__restrict__ b0, float4 *__restrict__ x0, const int *__restrict__ done) {
+    const int *__restrict__ inc_other, const float4 *__restrict__ xyz, const V *__restrict__ r,
+    const float *__restrict__ dinv0, float omega, float4 *__restrict__ b0,
+    float4 *__restrict__ x0, const int *__restrict__ done) {
     if (done && *done) return;
     const int a = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
     float4 acc = f4_zero();
-    if (a < n_free)
-        for (int k = inc_ptr[a] + lane; k < inc_ptr[a + 1]; k += LPR)
-            f4_fma(acc, (float)r[inc_edge[k]], inc_tan[k]);
+    if (a < n_free) {
+        const float4 xa = xyz[a];
+        for (int k = inc_ptr[a] + lane; k < inc_ptr[a + 1]; k += LPR) {
+            const int o = inc_other[k];
+            const float4 xb = xyz[o & 0x7fffffff];
+            f4_fma(acc, (float)r[inc_edge[k]], o < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
+        }
+    }
     f4_reduce<LPR>(acc);
     if (a < n_free && lane == 0) {
         b0[a] = acc;
@@ -296,7 +303,7 @@ template <typename V>
 __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
     const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
-    const float4 *__restrict__ tan, const float4 *__restrict__ y0, float weight,
+    const float4 *__restrict__ xyz, const float4 *__restrict__ y0, float weight,
     V *__restrict__ z, double *__restrict__ partials, int stride, const int *__restrict__ row0,
     const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
     const int *__restrict__ done) {
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
             float4 s = f4_zero();
             if (en.x < n_free) s = f4_add(s, y0[en.x]);
             if (en.y < n_free) s = f4_add(s, y0[en.y]);
-            const float4 t = tan[i];
+            const float4 t = half_diff(xyz[en.y], xyz[en.x]);
             const float aux = t.x * s.x + t.y * s.y + t.z * s.z;
             const V ri = r[i];
             zi = dinv[i] * ri + (V)(weight * aux);
@@ -342,8 +349,8 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     constexpr int kIncLanes = 8;
     hipLaunchKernelGGL((ams_edge_restrict_kernel<V, kIncLanes>),
                        dim3(grid_for((long long)A->n_free * kIncLanes, kAmsBlock)),
-                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->inc_tan, r,
-                       l0.dinv, A->omega, l0.b, l0.x, done);
+                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->inc_other,
+                       A->xyz, r, l0.dinv, A->omega, l0.b, l0.x, done);
     const int waves_per_block = kAmsBlock / kWave;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
@@ -364,7 +371,7 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
                            l0.y, done);
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
-                       A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->edge_tan, l0.y,
+                       A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->xyz, l0.y,
                        A->weight, z, partials, stride, seg ? seg->row0 : nullptr,
                        seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done);
 }
@@ -419,14 +426,54 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const i
     const int ninc = inc_ptr[n_free_nodes];
     for (int k = 0; k < ninc; ++k)
         SN_REQUIRE(inc_edge[k] >= 0 && inc_edge[k] < n_edges, "sn_ams_create: edge id out of range");
+    // node ids: the free nodes keep theirs; every fixed mesh node (ids >=
+    // n_free, possibly one shared id) gets its own, in mesh order, so the
+    // coordinates of both end points of every edge are at hand
+    std::vector<int> node_mesh(n_free_nodes, -1), fixed_mesh;
+    for (int e = 0; e < 2 * n_edges; ++e) {
+        const int v = edge_nodes[e], m = edge_mesh_nodes[e];
+        if (v < n_free_nodes) {
+            SN_REQUIRE(node_mesh[v] < 0 || node_mesh[v] == m,
+                       "sn_ams_create: node %d stands for two mesh nodes", v);
+            node_mesh[v] = m;
+        } else {
+            fixed_mesh.push_back(m);
+        }
+    }
+    for (int v = 0; v < n_free_nodes; ++v)
+        SN_REQUIRE(node_mesh[v] >= 0, "sn_ams_create: free node %d on no edge", v);
+    std::sort(fixed_mesh.begin(), fixed_mesh.end());
+    fixed_mesh.erase(std::unique(fixed_mesh.begin(), fixed_mesh.end()), fixed_mesh.end());
+    std::vector<int2> en(n_edges);
+    for (int e = 0; e < n_edges; ++e) {
+        int v[2];
+        for (int j = 0; j < 2; ++j) {
+            v[j] = edge_nodes[2 * e + j];
+            if (v[j] >= n_free_nodes)
+                v[j] = n_free_nodes + (int)(std::lower_bound(fixed_mesh.begin(), fixed_mesh.end(),
+                                                             edge_mesh_nodes[2 * e + j]) -
+                                            fixed_mesh.begin());
+        }
+        en[e] = make_int2(v[0], v[1]);
+    }
+    node_mesh.insert(node_mesh.end(), fixed_mesh.begin(), fixed_mesh.end());
+    A->n_nodes = (int)node_mesh.size();
+    // incidences: the other end point of the edge, sign bit set when this node is hi
+    std::vector<int> other(ninc > 0 ? ninc : 1, 0);
+    for (int a = 0; a < n_free_nodes; ++a)
+        for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+            const int2 v = en[inc_edge[k]];
+            SN_REQUIRE(v.x == a || v.y == a, "sn_ams_create: incidence %d: edge %d misses node %d",
+                       k, inc_edge[k], a);
+            other[k] = (v.x == a) ? v.y : (int)((unsigned)v.x | 0x80000000u);
+        }
     int rc;
-    if ((rc = upload(&A->edge_nodes, reinterpret_cast<const int2 *>(edge_nodes), n_edges, s))) return rc;
-    if ((rc = upload(&A->edge_mesh, reinterpret_cast<const int2 *>(edge_mesh_nodes), n_edges, s)))
-        return rc;
-    SN_HIP(hipMalloc(&A->edge_tan, (size_t)n_edges * sizeof(float4)));
+    if ((rc = upload(&A->edge_nodes, en.data(), (size_t)n_edges, s))) return rc;
+    if ((rc = upload(&A->node_mesh, node_mesh.data(), node_mesh.size(), s))) return rc;
+    SN_HIP(hipMalloc(&A->xyz, node_mesh.size() * sizeof(float4)));
     if ((rc = upload(&A->inc_ptr, inc_ptr, (size_t)n_free_nodes + 1, s))) return rc;
     if ((rc = upload(&A->inc_edge, inc_edge, (size_t)ninc, s))) return rc;
-    SN_HIP(hipMalloc(&A->inc_tan, (size_t)(ninc > 0 ? ninc : 1) * sizeof(float4)));
+    if ((rc = upload(&A->inc_other, other.data(), (size_t)ninc, s))) return rc;
     A->n_inc = ninc;
     SN_HIP(hipMalloc(&A->loc, (size_t)10 * n_el * sizeof(float)));
     SN_HIP(hipStreamSynchronize(s));
@@ -503,11 +550,8 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
         SN_HIP(hipMalloc(&A->dense, (size_t)nc * nc * sizeof(double)));
         SN_HIP(hipMalloc(&A->cinv, (size_t)nc * nc * sizeof(float)));
     }
-    hipLaunchKernelGGL(ams_tangent_kernel, dim3(grid_for(A->n_edges, 256)), dim3(256), 0, s,
-                       A->n_edges, nodes, A->edge_mesh, A->edge_tan);
-    if (A->n_inc > 0)
-        hipLaunchKernelGGL(ams_inc_tangent_kernel, dim3(grid_for(A->n_inc, 256)), dim3(256), 0, s,
-                           A->n_inc, A->inc_edge, A->edge_tan, A->inc_tan);
+    hipLaunchKernelGGL(ams_coords_kernel, dim3(grid_for(A->n_nodes, 256)), dim3(256), 0, s,
+                       A->n_nodes, nodes, A->node_mesh, A->xyz);
     hipLaunchKernelGGL(ams_lap_local_kernel, dim3(grid_for(A->n_el, kAmsBlock)), dim3(kAmsBlock),
                        0, s, A->n_el, nodes, cells, nu_el, A->loc);
     for (int l = 0; l < L; ++l) {
@@ -616,7 +660,7 @@ void sn_ams_destroy(sn_ams *A) {
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }
-    void *bufs[] = {A->edge_nodes, A->edge_mesh, A->edge_tan, A->inc_ptr, A->inc_edge, A->inc_tan,
+    void *bufs[] = {A->edge_nodes, A->node_mesh, A->xyz, A->inc_ptr, A->inc_edge, A->inc_other,
                     A->loc, A->dense, A->cinv};
     for (void *p : bufs)
         if (p) (void)hipFree(p);

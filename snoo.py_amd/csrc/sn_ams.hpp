@@ -24,12 +24,13 @@ struct AmsLevel {
 
 struct sn_ams {
     int n_edges = 0, n_free = 0, n_el = 0;
-    int2 *edge_nodes = nullptr;  // [n_edges] level-0 ids of (lo, hi); >= n_free: fixed node
-    int2 *edge_mesh = nullptr;   // [n_edges] mesh node ids of (lo, hi)
-    float4 *edge_tan = nullptr;  // [n_edges] (x_hi - x_lo) / 2, refreshed by sn_ams_update
+    int n_nodes = 0;             // free nodes [0, n_free), then fixed nodes
+    int2 *edge_nodes = nullptr;  // [n_edges] node ids of (lo, hi); >= n_free: fixed node
+    int *node_mesh = nullptr;    // [n_nodes] mesh node id of each node
+    float4 *xyz = nullptr;       // [n_nodes] coordinates (float), refreshed by sn_ams_update
     int *inc_ptr = nullptr;      // [n_free+1] free edges at each free node
-    int *inc_edge = nullptr;
-    float4 *inc_tan = nullptr;   // tangent of each incidence's edge, in incidence order
+    int *inc_edge = nullptr;     // [n_inc] edge of each incidence
+    int *inc_other = nullptr;    // [n_inc] the edge's other node; sign bit: this node is hi
     int n_inc = 0;
     float *loc = nullptr;        // [10*n_el] element Laplacian pair values
     std::vector<AmsLevel> lv;

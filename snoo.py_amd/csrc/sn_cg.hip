@@ -141,10 +141,19 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
 }
 
 // ---------------------------------------------------------------- kernel A
+// Column of nonzero k in row i: int32 indices, or 16-bit offsets j - i when
+// every row's columns lie within 32767 of the row (edge numberings of
+// structured meshes up to a few million elements), which cuts the index
+// stream of the SpMV in half.
+__device__ __forceinline__ int col_at(const int *__restrict__ c, int k, int) { return c[k]; }
+__device__ __forceinline__ int col_at(const short *__restrict__ c, int k, int i) {
+    return i + (int)c[k];
+}
+
 // parity = iteration & 1: reads rho[parity ^ 1] (previous), writes rho[parity].
-template <typename V, typename M, int LPR>
+template <typename V, typename M, int LPR, typename I>
 __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
-    SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
+    SegMap sm, const int *__restrict__ indptr, const I *__restrict__ indices,
     const M *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
     const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
     const double *__restrict__ partB, double *__restrict__ partA, CgState *__restrict__ st,
@@ -196,7 +205,7 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
         V acc = 0;
         const int k1 = indptr[i + 1];
         for (int k = indptr[i] + lane; k < k1; k += LPR) {
-            const int j = indices[k];
+            const int j = col_at(indices, k, i);
             acc += (V)vals[k] * (z[j] + beta * pold[j]);
         }
 #pragma unroll
@@ -270,6 +279,27 @@ __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
             S->it = S->it + 1;
         }
     }
+}
+
+// 16-bit column offsets j - i (thread per row) and, per wave, the largest
+// |j - i| seen, so the host can tell whether the offsets are exact
+__global__ __launch_bounds__(256) void col_offset_kernel(int n, const int *__restrict__ indptr,
+                                                         const int *__restrict__ indices,
+                                                         short *__restrict__ dcol,
+                                                         int *__restrict__ wave_max_out) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)gid;
+    int m = 0;
+    if (gid < n) {
+        for (int k = indptr[i]; k < indptr[i + 1]; ++k) {
+            const int d = indices[k] - i;
+            const int ad = d < 0 ? -d : d;
+            m = ad > m ? ad : m;
+            dcol[k] = (short)(ad <= 32767 ? d : 0);
+        }
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_max_out[gid / kWave] = m;
 }
 
 // x64 += xs (f32 correction) on free rows
@@ -393,6 +423,8 @@ struct sn_cg {
     void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
     float *xs = nullptr;
     double *partA = nullptr, *partB = nullptr, *partbb = nullptr;
+    short *dcol = nullptr;       // [nnz] 16-bit column offsets j - i, when they are exact
+    bool use_dcol = true;        // SpMV reads dcol (when present) instead of the indices
     sn_ams *ams = nullptr;       // auxiliary-space preconditioner (null: Jacobi)
     CgState *st = nullptr;       // [nseg]
     CgState *st_host = nullptr;  // [nseg], pinned
@@ -418,9 +450,41 @@ static void launch_spmv(sn_cg *C, const M *vals, const unsigned char *mask, int 
     const sn_pattern *P = C->pat;
     V *pold = (V *)(parity ? C->p1 : C->p0);
     V *pnew = (V *)(parity ? C->p0 : C->p1);
-    hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR>), dim3(C->grid_rows), dim3(kRowBlock), 0, s,
-                       C->map(), P->indptr, P->indices, vals, mask, (const V *)C->z, pold, pnew,
-                       (V *)C->q, C->partB, C->partA, C->st, parity, C->sched);
+    if (C->dcol && C->use_dcol)
+        hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR, short>), dim3(C->grid_rows),
+                           dim3(kRowBlock), 0, s, C->map(), P->indptr, C->dcol, vals, mask,
+                           (const V *)C->z, pold, pnew, (V *)C->q, C->partB, C->partA, C->st,
+                           parity, C->sched);
+    else
+        hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR, int>), dim3(C->grid_rows),
+                           dim3(kRowBlock), 0, s, C->map(), P->indptr, P->indices, vals, mask,
+                           (const V *)C->z, pold, pnew, (V *)C->q, C->partB, C->partA, C->st,
+                           parity, C->sched);
+}
+
+// 16-bit column offsets for the SpMV when every |j - i| fits (kept otherwise
+// on int32 indices)
+static int build_col_offsets(sn_cg *C, hipStream_t s) {
+    const sn_pattern *P = C->pat;
+    if (P->nnz == 0 || P->n == 0) return SN_OK;
+    const int grid = grid_for(P->n, 256);
+    const size_t nw = (size_t)grid * (256 / kWave);
+    int *wmax = nullptr;
+    SN_HIP(hipMalloc(&C->dcol, (size_t)P->nnz * sizeof(short)));
+    SN_HIP(hipMalloc(&wmax, nw * sizeof(int)));
+    hipLaunchKernelGGL(col_offset_kernel, dim3(grid), dim3(256), 0, s, P->n, P->indptr,
+                       P->indices, C->dcol, wmax);
+    std::vector<int> h(nw);
+    SN_HIP(hipMemcpyAsync(h.data(), wmax, nw * sizeof(int), hipMemcpyDeviceToHost, s));
+    SN_HIP(hipStreamSynchronize(s));
+    (void)hipFree(wmax);
+    int m = 0;
+    for (int v : h) m = v > m ? v : m;
+    if (m > 32767) {
+        (void)hipFree(C->dcol);
+        C->dcol = nullptr;
+    }
+    return SN_OK;
 }
 
 template <typename V, typename M>
@@ -666,6 +730,7 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
     std::memset(C->st_host, 0, nseg * sizeof(CgState));
     SN_HIP(hipMemcpyAsync(C->st, C->st_host, nseg * sizeof(CgState), hipMemcpyHostToDevice,
                           as_stream(stream)));
+    if ((rc = build_col_offsets(C, as_stream(stream)))) return rc;
     SN_HIP(hipStreamSynchronize(as_stream(stream)));
     *out = C;
     return SN_OK;
@@ -686,6 +751,19 @@ int sn_cg_set_schedule(sn_cg *C, int sched) {
     return SN_OK;
 }
 
+int sn_cg_set_col_offsets(sn_cg *C, int enable) {
+    SN_REQUIRE(C, "sn_cg_set_col_offsets: null workspace");
+    SN_REQUIRE(!enable || C->dcol, "sn_cg_set_col_offsets: some |column - row| exceeds 32767");
+    if (C->graph) {
+        SN_HIP(hipGraphExecDestroy(C->graph));
+        C->graph = nullptr;
+    }
+    C->use_dcol = enable != 0;
+    return SN_OK;
+}
+
+int sn_cg_index_bytes(const sn_cg *C) { return (C && C->dcol && C->use_dcol) ? 2 : 4; }
+
 int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
     SN_REQUIRE(C, "sn_cg_set_ams: null workspace");
     SN_REQUIRE(!A || A->n_edges == C->pat->n, "sn_cg_set_ams: %d edges, matrix has %d rows",
@@ -701,8 +779,9 @@ int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
 void sn_cg_destroy(sn_cg *C) {
     if (!C) return;
     if (C->graph) (void)hipGraphExecDestroy(C->graph);
-    void *bufs[] = {C->r, C->z, C->p0, C->p1, C->q, C->dinv, C->xs, C->partA, C->partB,
-                    C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0, C->blkB0, C->segA, C->segB};
+    void *bufs[] = {C->r,    C->z,        C->p0, C->p1,   C->q,     C->dinv,  C->xs,
+                    C->partA, C->partB,   C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0,
+                    C->blkB0, C->segA,    C->segB, C->dcol};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     (void)hipHostFree(C->st_host);

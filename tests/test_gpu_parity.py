@@ -222,6 +222,35 @@ def test_cg_matches_oracle_pcg(solver, oprob):
     assert it == 0 and float(x[mask == 1].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize('precision', [64, 32])
+def test_cg_column_offsets_are_bit_identical(solver, precision):
+    """16-bit column offsets (j - i) and int32 indices address the same
+    columns, so the CG iterates agree to the bit."""
+    torch = _torch()
+    from snoopy_amd import _native as nat
+    from snoopy_amd.red_mvp_solver import _CG
+    asm = solver.curl_curl_factory
+    K, dK, J, rhs = asm.assemble_device(np.zeros(asm.num_dofs), 2, solver.source_fields_device())
+    if precision == 32:
+        J = J.float()
+    mask = solver.mask_device()
+    lib = nat.load()
+    ip, ix = asm.csr_pattern_host()
+    rows = np.repeat(np.arange(asm.num_dofs), np.diff(ip))
+    assert np.abs(ix - rows).max() <= 32767
+    out = []
+    for enable in (1, 0):
+        cg = _CG(asm, precision)
+        assert lib.sn_cg_index_bytes(cg.h) == 2          # chosen at creation
+        nat.check(lib.sn_cg_set_col_offsets(cg.h, enable))
+        assert lib.sn_cg_index_bytes(cg.h) == (2 if enable else 4)
+        x = torch.zeros(asm.num_dofs, dtype=torch.float64, device='cuda')
+        it, rn, rc = cg.solve(J, mask, rhs, x, rtol=1e-6, atol=0.0, maxiter=400)
+        out.append((it, rn, x.cpu().numpy()))
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
 def test_cg_mixed_precision_refinement(solver):
     torch = _torch()
     from snoopy_amd.red_mvp_solver import _CG
