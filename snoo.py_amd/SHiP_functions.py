@@ -28,7 +28,7 @@ import numpy as np
 
 from .evaluators import evaluate_fem_solution_node_interpolation
 from .geometry_tools import (compute_area_polygon, mag_1_3_quarter, mag_2_quarter,
-                             mag_4_quarter, ruled_volume)
+                             mag_4_quarter, ncsc_quarter, ruled_volume)
 from .materials import Dilution, Reluctance
 from .racetrack_coil import RacetrackCoil
 from .red_mvp_solver import RedMVPSolver, RedMVPSolverDiluted
@@ -336,6 +336,65 @@ def build_template(kind, parameters, df_index=0, lc=0.2, materials_directory='fi
                     dv['current'])
 
 
+def build_ncsc(parameters, df_index=0, lc=0.2, materials_directory='files/materials', h=None,
+               hz=None):
+    """Mesh, coils and figures of the normal-conducting / superconducting
+    pair of rows df_index and df_index + 1 (snoopy/SHiP_functions.py:2139-2502):
+    the material, conductor and target current density come from the second
+    row; iron mass, coil mass and power count the superconducting magnet
+    only; the coils keep yoke spacer + insulation (no conductor radius)
+    from the iron."""
+    i, j = df_index, df_index + 1
+    rel = Reluctance(os.path.join(materials_directory, parameters['material'][j]))
+    iron = _load(materials_directory, parameters['material'][j])
+    conductor = _load(materials_directory, parameters['coil_material'][j])
+    J_tar = parameters['J_tar(A/mm2)'][j] * 1e6
+    p1, p2 = _row(parameters, i), _row(parameters, j)
+    d1, d2 = _drive(parameters, i), _drive(parameters, j)
+    lim_x = max([max([p1['X_yoke_1'], p1['X_yoke_2']]) + d1['delta_x'],
+                 max([p2['X_yoke_1'], p2['X_yoke_2']]) + d2['delta_x']])
+    lim_y = max([max([p1['Y_yoke_1'], p1['Y_yoke_2']]) + d1['delta_y'],
+                 max([p2['Y_yoke_1'], p2['Y_yoke_2']]) + d2['delta_y']])
+    h, hz = _sizes(lc, h, hz)
+    model = ncsc_quarter(p1, p2, lim_x, lim_y, d1['delta_z'], d2['delta_z'], h, hz)
+    dens = iron['density(g/m3)']
+    lin = lambda k, s: (1.0 - s) * p2[k + '_1'] + s * p2[k + '_2']  # noqa: E731
+    core = lambda s: lin('X_core', s) * lin('Y_core', s)  # noqa: E731
+    yoke = lambda s: lin('X_yoke', s) * lin('Y_yoke', s) - lin('X_void', s) * lin('Y_void', s)  # noqa: E731
+    M_iron = 4 * ruled_volume(core, p2['Z_len']) * dens
+    M_iron += 4 * ruled_volume(yoke, p2['Z_len']) * dens
+    coils = []
+    for p, dv in ((p1, d1), (p2, d2)):
+        _, num_cond, y = _levels(p['Y_core_1'], p['Y_core_2'], dv['coil_radius'], dv['ins'],
+                                 dv['max_turns'])
+        kp = _around(-p['X_core_1'], p['X_core_1'], -p['X_core_2'], p['X_core_2'],
+                     dict(dv, coil_radius=0.0), p['Z_pos'], p['Z_len'])
+        coils.append(RacetrackCoil(kp, y, dv['coil_radius'], dv['current'] / num_cond))
+    turn_perimeter = coils[1].get_length()
+    current = d2['current']
+    A_cu = abs(current) / J_tar
+    M_coil = A_cu * turn_perimeter * conductor['density(g/m3)']
+    Q = abs(current * current) * turn_perimeter * conductor['resistivity(Ohm.m)'] / A_cu
+    return Template(model, [1, 2, 3], [4], [rel, rel, rel], None, coils, M_iron, M_coil, Q,
+                    J_tar, A_cu / conductor['filling_factor'], turn_perimeter,
+                    max([d1['field_density'], d2['field_density']]), current)
+
+
+def get_vector_field_ncsc(parameters, df_index=0, lc=0.2, geo_th=1e-5, run_gmsh=False,
+                          plot_geo=False, plot_result=False, result_directory='none',
+                          result_spec='', eval_pos=np.zeros((0, 3)),
+                          materials_directory='files/materials', quad_order=6, h=None, hz=None,
+                          method='reference', verbose=True):
+    """Cross-talk of a normal-conducting magnet (row df_index) and a
+    superconducting one (row df_index + 1) in one air box
+    (snoopy/SHiP_functions.py:2139-2620); returns
+    [points, B, M_iron, M_coil, Q, J_tar]."""
+    tp = build_ncsc(parameters, df_index, lc, materials_directory, h, hz)
+    return _solve_template(tp, parameters, df_index, result_directory, result_spec, eval_pos,
+                           quad_order, 50, method, verbose, time.time(),
+                           eval_tags=[1, 2, 4, 3])
+
+
 def _mag4_row(parameters, i):
     """Template 4 takes its six x key points from rows i and i + 1
     (snoopy/SHiP_functions.py:1795-1808)."""
@@ -363,6 +422,15 @@ def _run(kind, parameters, df_index, lc, result_directory, result_spec, eval_pos
     t_s = time.time()
     tp = build_template(kind, parameters, df_index, lc, materials_directory,
                         use_diluted_steel, kappa_elec, h, hz)
+    return _solve_template(tp, parameters, df_index, result_directory, result_spec, eval_pos,
+                           quad_order, max_newton, method, verbose, t_s)
+
+
+def _solve_template(tp, parameters, df_index, result_directory, result_spec, eval_pos,
+                    quad_order, max_newton, method, verbose, t_s, eval_tags=None):
+    """Solve a built template, form the field map and the probe fields, write
+    the reference's result files; eval_tags orders the domains for the probe
+    interpolation (default: iron then air)."""
     if tp.dilutions is not None:
         solver = RedMVPSolverDiluted(tp.model, tp.coils, tp.iron_tags, tp.air_tags,
                                      tp.reluctances, tp.dilutions, quad_order=quad_order,
@@ -392,7 +460,8 @@ def _run(kind, parameters, df_index, lc, result_directory, result_spec, eval_pos
     ret_vals = [points, B_i + B_coil, tp.M_iron, tp.M_coil, tp.Q, tp.current_density]
     if eval_pos.shape[0] > 0:
         B_tot = evaluate_fem_solution_node_interpolation(
-            tp.model, eval_pos, solver, tp.coils, x, tp.iron_tags + tp.air_tags)
+            tp.model, eval_pos, solver, tp.coils, x,
+            eval_tags if eval_tags is not None else tp.iron_tags + tp.air_tags)
         if result_directory != 'none':
             fn = os.path.join(result_directory, 'B' + result_spec + '.csv')
         else:

@@ -19,7 +19,7 @@ from .red_mvp_solver import NewtonDriver, RedMVPSolver, RedMVPSolverDiluted
 from .geometry_tools import compute_area_polygon
 from .SHiP_functions import (compute_J_opt, compute_prices, get_NI, get_vector_field_mag_1,
                              get_vector_field_mag_2, get_vector_field_mag_3,
-                             get_vector_field_mag_4)
+                             get_vector_field_mag_4, get_vector_field_ncsc)
 
 __all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctance',
            'ConstantReluctance', 'Reluctance', 'tanh_reluctance', 'TetMesh', 'TetModel',
@@ -27,4 +27,5 @@ __all__ = ['BrickField', 'CurlCurlAssembler', 'FiniteElement', 'BSplineReluctanc
            'NewtonDriver', 'RedMVPSolver', 'RedMVPSolverDiluted', 'CurlCurlAssemblerDiluted',
            'Dilution', 'evaluate_fem_solution_node_interpolation', 'compute_area_polygon',
            'compute_J_opt', 'compute_prices', 'get_NI', 'get_vector_field_mag_1',
-           'get_vector_field_mag_2', 'get_vector_field_mag_3', 'get_vector_field_mag_4']
+           'get_vector_field_mag_2', 'get_vector_field_mag_3', 'get_vector_field_mag_4',
+           'get_vector_field_ncsc']

@@ -131,3 +131,23 @@ def test_mag2_driver_solves(tmp_path, monkeypatch):
     b = pd.read_csv('B.csv').values[0, 3:]
     os.remove('B.csv')
     assert b[1] > 0.5 and abs(b[0]) < 0.1 * b[1]
+
+
+def test_ncsc_driver_solves(tmp_path):
+    """The normal-conducting / superconducting pair has no reference test: it
+    must solve, return the superconducting magnet's figures and J_tar, and
+    drive +y through both cores (the coils' currents have the same sign)."""
+    from snoopy_amd import SHiP_functions as sf
+    from test_ship import ncsc_params
+    p = ncsc_params()
+    probe = np.array([[0.1, 0.1, -2.25], [0.1, 0.1, 0.0]])
+    pts, B, M_i, M_c, Q, J = sf.get_vector_field_ncsc(p, 0, lc=0.4, eval_pos=probe,
+                                                      materials_directory=MAT,
+                                                      result_directory=str(tmp_path),
+                                                      verbose=False)
+    tp = sf.build_ncsc(p, 0, lc=0.4, materials_directory=MAT)
+    assert (M_i, M_c, Q, J) == (tp.M_iron, tp.M_coil, tp.Q, 20e6)
+    assert np.isfinite(B).all()
+    b = pd.read_csv(tmp_path / 'B.csv').values[:, 3:]
+    print('B in the two cores:', b, flush=True)
+    assert b[0, 1] > 0.3 and b[1, 1] > 0.3

@@ -201,3 +201,62 @@ def test_mag2_template_volume_and_tags():
     assert tp.M_iron == pytest.approx(4 * v_exact * 7.87e6, rel=1e-14)
     assert tp.iron_tags == [1, 2] and tp.air_tags == [3]
     assert len(tp.coils) == 1
+
+
+def ncsc_params():
+    """A normal-conducting / superconducting pair (no reference test holds
+    one): a template-1 row followed along z by a template-2 row."""
+    p = pd.concat([params('test_mag_1'), mag2_params()], ignore_index=True)
+    geo1 = {'Xcore1(m)': 0.28, 'Xcore2(m)': 0.28, 'Xvoid1(m)': 0.33, 'Xvoid2(m)': 0.34,
+            'Xyoke1(m)': 0.9, 'Xyoke2(m)': 0.95, 'Ycore1(m)': 0.5, 'Ycore2(m)': 0.5,
+            'Yvoid1(m)': 0.5, 'Yvoid2(m)': 0.5, 'Yyoke1(m)': 0.8, 'Yyoke2(m)': 0.8,
+            'Z_pos(m)': -3.0, 'Z_len(m)': 1.5, 'NI(A)': 3e4}
+    for k, v in geo1.items():
+        p.loc[0, k] = v
+    p.loc[1, 'J_tar(A/mm2)'] = 20.0
+    return p
+
+
+def test_ncsc_template_volume_tags_and_figures():
+    S = sf()
+    p = ncsc_params()
+    tp = S.build_ncsc(p, 0, lc=0.4, materials_directory=MAT)
+    tm = tp.model.tet_mesh
+    vol = tm.element_volumes()
+    assert np.all(vol > 0)
+    # iron mass counts the superconducting magnet only (its two volumes)
+    v_sc = vol[(tm.tags == 2) | (tm.tags == 3)].sum()
+    assert 4 * v_sc * 7.87e6 == pytest.approx(tp.M_iron, rel=1e-12)
+    sec = lambda s: ((0.3 + 0.05 * s) * (0.4 + 0.05 * s)  # noqa: E731
+                     + (1.0 + 0.1 * s) * (1.0 + 0.05 * s) - (0.5 + 0.05 * s) * 0.6)
+    assert v_sc == pytest.approx(2.0 / 6.0 * (sec(0) + 4 * sec(0.5) + sec(1)), rel=1e-12)
+    # the normal-conducting yoke: core, leg and top plate of template 1
+    sec1 = lambda s: ((0.9 + 0.05 * s) * 0.8 - (0.33 + 0.01 * s - 0.28) * 0.5)  # noqa: E731
+    assert vol[tm.tags == 1].sum() == pytest.approx(1.5 / 6.0 * (sec1(0) + 4 * sec1(0.5)
+                                                                 + sec1(1)), rel=1e-12)
+    # tags follow each magnet's own z range
+    cen = tm.nodes[tm.cells].mean(axis=1)
+    assert np.all((cen[tm.tags == 1, 2] > -3.0) & (cen[tm.tags == 1, 2] < -1.5))
+    sc = (tm.tags == 2) | (tm.tags == 3)
+    assert np.all((cen[sc, 2] > -1.0) & (cen[sc, 2] < 1.0))
+    # coil figures: the superconducting coil's mass and power at J_tar
+    cu = json.load(open(os.path.join(MAT, 'copper_water_cooled.json')))
+    A_cu = 2e5 / 20e6
+    assert tp.M_coil == A_cu * tp.turn_perimeter * cu['density(g/m3)']
+    assert tp.Q == abs(2e5 * 2e5) * tp.turn_perimeter * cu['resistivity(Ohm.m)'] / A_cu
+    assert tp.current_density == 20e6
+    assert len(tp.coils) == 2
+    with pytest.raises(ValueError):
+        q = p.copy()
+        q.loc[1, 'Z_pos(m)'] = -2.0        # overlaps the first magnet
+        S.build_ncsc(q, 0, lc=0.4, materials_directory=MAT)
+    # a break of one magnet that meets a break of the other between them
+    # (both cores at x = 0.3 there) shares one grid line with it
+    q = p.copy()
+    q.loc[0, 'Xcore1(m)'] = q.loc[0, 'Xcore2(m)'] = 0.3
+    tq = S.build_ncsc(q, 0, lc=0.4, materials_directory=MAT)
+    vq = tq.model.tet_mesh.element_volumes()
+    assert np.all(vq > 0)
+    sec1 = lambda s: ((0.9 + 0.05 * s) * 0.8 - (0.33 + 0.01 * s - 0.3) * 0.5)  # noqa: E731
+    assert vq[tq.model.tet_mesh.tags == 1].sum() == pytest.approx(
+        1.5 / 6.0 * (sec1(0) + 4 * sec1(0.5) + sec1(1)), rel=1e-12)
