@@ -304,7 +304,8 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
     const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
     const float4 *__restrict__ xyz, const float4 *__restrict__ y0, float weight,
-    V *__restrict__ z, double *__restrict__ partials, int stride, const int *__restrict__ row0,
+    V *__restrict__ z, int zs, double *__restrict__ partials, int stride,
+    const int *__restrict__ row0,
     const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
     const int *__restrict__ done) {
     __shared__ double lds[kAmsEdgeBlock / kWave];
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
             zi = dinv[i] * ri + (V)(weight * aux);
             rz += (double)ri * (double)zi;
         }
-        z[i] = zi;
+        z[i * zs] = zi;
     }
     const double sum = block_sum(rz, lds);
     if (threadIdx.x == 0 && partials) partials[(long long)stride * blockIdx.x] = sum;
@@ -342,8 +343,8 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
 
 template <typename V>
 static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
-                           double *partials, int stride, int grid, const AmsSegments *seg,
-                           const int *done, hipStream_t s) {
+                           int zs, double *partials, int stride, int grid,
+                           const AmsSegments *seg, const int *done, hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
     constexpr int kIncLanes = 8;
@@ -372,20 +373,22 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->xyz, l0.y,
-                       A->weight, z, partials, stride, seg ? seg->row0 : nullptr,
+                       A->weight, z, zs, partials, stride, seg ? seg->row0 : nullptr,
                        seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done);
 }
 
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
-                     const unsigned char *mask, void *z, double *partials, int stride, int grid,
-                     const AmsSegments *seg, const int *done, hipStream_t s) {
+                     const unsigned char *mask, void *z, int zstride, double *partials,
+                     int stride, int grid, const AmsSegments *seg, const int *done,
+                     hipStream_t s) {
     SN_REQUIRE(A && A->ready, "AMS preconditioner used before sn_ams_update");
+    SN_REQUIRE(zstride >= 1, "ams_launch_apply: zstride must be >= 1");
     if (v64)
         launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
-                               partials, stride, grid, seg, done, s);
+                               zstride, partials, stride, grid, seg, done, s);
     else
         launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
-                              partials, stride, grid, seg, done, s);
+                              zstride, partials, stride, grid, seg, done, s);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -573,7 +576,7 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
 int sn_ams_apply(sn_ams *A, const unsigned char *mask, const double *r, const double *dinv,
                  double *z, void *stream) {
     SN_REQUIRE(A && r && dinv && z, "sn_ams_apply: null argument");
-    return ams_launch_apply(A, true, r, dinv, mask, z, nullptr, 0,
+    return ams_launch_apply(A, true, r, dinv, mask, z, 1, nullptr, 0,
                             grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr, nullptr,
                             as_stream(stream));
 }

@@ -50,13 +50,15 @@ struct AmsSegments {
     const int *seg_of_blk;  // [grid]
 };
 
-// z = dinv .* r + weight * Pi V(Pi^T r) on the free edges (0 elsewhere), with
-// per-block partial sums of r.z written to partials[stride*b], b < grid
-// (blocks of 1024 threads striding over the edges, or over the rows of
-// their segment when `seg` is given).  `done` (may be null) points to a
-// device flag that turns every kernel into a no-op when set.
+// z = dinv .* r + weight * Pi V(Pi^T r) on the free edges (0 elsewhere),
+// stored at z[zstride * i], with per-block partial sums of r.z written to
+// partials[stride*b], b < grid (blocks of 1024 threads striding over the
+// edges, or over the rows of their segment when `seg` is given).  `done`
+// (may be null) points to a device flag that turns every kernel into a
+// no-op when set.
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
-                     const unsigned char *mask, void *z, double *partials, int stride, int grid,
-                     const AmsSegments *seg, const int *done, hipStream_t s);
+                     const unsigned char *mask, void *z, int zstride, double *partials,
+                     int stride, int grid, const AmsSegments *seg, const int *done,
+                     hipStream_t s);
 
 }  // namespace sn

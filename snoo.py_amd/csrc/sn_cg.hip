@@ -58,6 +58,15 @@ __device__ __forceinline__ double reduce_range(const double *__restrict__ partia
     return block_sum(s, lds);
 }
 
+// z and the search direction p live interleaved, w[2i] = z_i, w[2i+1] = p_i,
+// so the SpMV fetches both at a column with one 8- or 16-byte load.  Two
+// such arrays alternate: the SpMV of iteration k reads {z, p_old} from one
+// and writes p_new into the other, where the update (or the preconditioner)
+// then puts the next z.
+template <typename V> struct Pair;
+template <> struct Pair<float> { using T = float2; };
+template <> struct Pair<double> { using T = double2; };
+
 // ---------------------------------------------------------------- init
 // r = b - A_c x0 (x0 masked; the f32 solve starts from 0), dinv, z, p_old = 0,
 // per-block partials: (r.z, r.r) in the layout kernel B uses, b.b separately.
@@ -66,7 +75,7 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
     SegMap sm, const int *__restrict__ indptr, const int *__restrict__ indices,
     const int *__restrict__ diag, const M *__restrict__ vals, const unsigned char *__restrict__ mask,
     const double *__restrict__ b, const double *__restrict__ x0, V *__restrict__ r,
-    V *__restrict__ z, V *__restrict__ dinv, V *__restrict__ pold, float *__restrict__ xs,
+    V *__restrict__ w, V *__restrict__ w_other, V *__restrict__ dinv, float *__restrict__ xs,
     double *__restrict__ partB, double *__restrict__ partbb) {
     __shared__ double lds[kVecBlock / kWave];
     const int seg = sm.segB[blockIdx.x];
@@ -100,9 +109,13 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
             }
             const V zi = di * ri;
             r[i] = ri;
-            z[i] = zi;
+            w[2 * (long long)i] = zi;
+            w[2 * (long long)i + 1] = 0;
+            // the other array's z of fixed rows stays 0 for the whole solve
+            // (only the free rows' z is written by the iterations)
+            w_other[2 * (long long)i] = 0;
+            w_other[2 * (long long)i + 1] = 0;
             dinv[i] = di;
-            pold[i] = 0;
             if (xs) xs[i] = 0;
             rr += (double)ri * (double)ri;
             rz += (double)ri * (double)zi;
@@ -154,10 +167,12 @@ __device__ __forceinline__ int col_at(const short *__restrict__ c, int k, int i)
 template <typename V, typename M, int LPR, typename I>
 __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     SegMap sm, const int *__restrict__ indptr, const I *__restrict__ indices,
-    const M *__restrict__ vals, const unsigned char *__restrict__ mask, const V *__restrict__ z,
-    const V *__restrict__ pold, V *__restrict__ pnew, V *__restrict__ q,
+    const M *__restrict__ vals, const unsigned char *__restrict__ mask,
+    const V *__restrict__ wold, V *__restrict__ wnew, V *__restrict__ q,
     const double *__restrict__ partB, double *__restrict__ partA, CgState *__restrict__ st,
     int parity, int sched) {
+    using V2 = typename Pair<V>::T;
+    const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);   // {z, p_old}
     __shared__ double lds[kRowBlock / kWave];
     const int seg = sm.segA[blockIdx.x];
     CgState *S = st + seg;
@@ -205,16 +220,17 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
         V acc = 0;
         const int k1 = indptr[i + 1];
         for (int k = indptr[i] + lane; k < k1; k += LPR) {
-            const int j = col_at(indices, k, i);
-            acc += (V)vals[k] * (z[j] + beta * pold[j]);
+            const V2 w = zp[col_at(indices, k, i)];
+            acc += (V)vals[k] * (w.x + beta * w.y);
         }
 #pragma unroll
         for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
         if (lane == 0) {
             const bool free_row = mask ? mask[i] != 0 : true;
-            const V pi = free_row ? z[i] + beta * pold[i] : (V)0;
+            const V2 w = zp[i];
+            const V pi = free_row ? w.x + beta * w.y : (V)0;
             const V qi = free_row ? acc : (V)0;
-            pnew[i] = pi;
+            wnew[2 * row + 1] = pi;
             q[i] = qi;
             pq += (double)pi * (double)qi;
         }
@@ -234,8 +250,8 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
 // the auxiliary-space apply that follows, and only r.r is written.
 template <typename V, typename X, bool JAC>
 __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
-    SegMap sm, const unsigned char *__restrict__ mask, const V *__restrict__ pnew,
-    const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r, V *__restrict__ z,
+    SegMap sm, const unsigned char *__restrict__ mask, V *__restrict__ w,
+    const V *__restrict__ q, const V *__restrict__ dinv, V *__restrict__ r,
     X *__restrict__ x, const double *__restrict__ partA, double *__restrict__ partB,
     CgState *__restrict__ st, int parity) {
     __shared__ double lds[kVecBlock / kWave];
@@ -259,12 +275,12 @@ __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
          i += (long long)nbl * blockDim.x) {
         const bool free_row = mask ? mask[i] != 0 : true;
         if (!free_row) continue;
-        x[i] += (X)(alpha * pnew[i]);
+        x[i] += (X)(alpha * w[2 * i + 1]);
         const V ri = r[i] - alpha * q[i];
         r[i] = ri;
         if (JAC) {
             const V zi = dinv[i] * ri;
-            z[i] = zi;
+            w[2 * i] = zi;
             rz += (double)ri * (double)zi;
         }
         rr += (double)ri * (double)ri;
@@ -320,13 +336,13 @@ __global__ void zero_free_kernel(int n, const unsigned char *__restrict__ mask,
 }
 
 template <typename V>
-__global__ void fill_free_kernel(int n, const unsigned char *__restrict__ mask, V *__restrict__ z,
-                                 V *__restrict__ p, V v) {
+__global__ void fill_free_kernel(int n, const unsigned char *__restrict__ mask, V *__restrict__ w,
+                                 V v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool f = mask ? mask[i] != 0 : true;
-    z[i] = f ? v : (V)0;
-    p[i] = (V)0;
+    w[2 * (long long)i] = f ? v : (V)0;      // z
+    w[2 * (long long)i + 1] = (V)0;          // p
 }
 
 // partials that reduce to (rho = 1, ||r||^2 = 1) in every segment, for the
@@ -420,7 +436,8 @@ struct sn_cg {
     std::vector<int> row0_h;            // [nseg+1]
     int *row0 = nullptr, *blkA0 = nullptr, *blkB0 = nullptr, *segA = nullptr, *segB = nullptr;
     double *rtol_dev = nullptr;         // [nseg]
-    void *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *dinv = nullptr;
+    void *r = nullptr, *q = nullptr, *dinv = nullptr;
+    void *w0 = nullptr, *w1 = nullptr;   // interleaved {z, p}: even iterations read w0
     float *xs = nullptr;
     double *partA = nullptr, *partB = nullptr, *partbb = nullptr;
     short *dcol = nullptr;       // [nnz] 16-bit column offsets j - i, when they are exact
@@ -448,18 +465,16 @@ template <typename V, typename M, int LPR>
 static void launch_spmv(sn_cg *C, const M *vals, const unsigned char *mask, int parity,
                         hipStream_t s) {
     const sn_pattern *P = C->pat;
-    V *pold = (V *)(parity ? C->p1 : C->p0);
-    V *pnew = (V *)(parity ? C->p0 : C->p1);
+    const V *wold = (const V *)(parity ? C->w1 : C->w0);
+    V *wnew = (V *)(parity ? C->w0 : C->w1);
     if (C->dcol && C->use_dcol)
         hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR, short>), dim3(C->grid_rows),
                            dim3(kRowBlock), 0, s, C->map(), P->indptr, C->dcol, vals, mask,
-                           (const V *)C->z, pold, pnew, (V *)C->q, C->partB, C->partA, C->st,
-                           parity, C->sched);
+                           wold, wnew, (V *)C->q, C->partB, C->partA, C->st, parity, C->sched);
     else
         hipLaunchKernelGGL((cg_spmv_kernel<V, M, LPR, int>), dim3(C->grid_rows),
                            dim3(kRowBlock), 0, s, C->map(), P->indptr, P->indices, vals, mask,
-                           (const V *)C->z, pold, pnew, (V *)C->q, C->partB, C->partA, C->st,
-                           parity, C->sched);
+                           wold, wnew, (V *)C->q, C->partB, C->partA, C->st, parity, C->sched);
 }
 
 // 16-bit column offsets for the SpMV when every |j - i| fits (kept otherwise
@@ -501,17 +516,18 @@ template <typename V, typename M>
 static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask, void *xdst,
                              int parity, hipStream_t s) {
     launch_spmv_lpr<V, M>(C, vals, mask, parity, s);
-    V *pnew = (V *)(parity ? C->p0 : C->p1);
+    V *wnew = (V *)(parity ? C->w0 : C->w1);
 #define SN_LAUNCH_UPD(X, JAC)                                                                 \
     hipLaunchKernelGGL((cg_update_kernel<V, X, JAC>), dim3(C->grid_vec), dim3(kVecBlock), 0, s, \
-                       C->map(), mask, pnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,  \
-                       (V *)C->z, (X *)xdst, C->partA, C->partB, C->st, parity)
+                       C->map(), mask, wnew, (const V *)C->q, (const V *)C->dinv, (V *)C->r,  \
+                       (X *)xdst, C->partA, C->partB, C->st, parity)
     if (C->ams) {
         if (C->precision == 64) SN_LAUNCH_UPD(double, false);
         else SN_LAUNCH_UPD(float, false);
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
-        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB,
-                               2, C->grid_vec, &seg, C->nseg == 1 ? C->done_flag() : nullptr, s);
+        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, wnew, 2,
+                               C->partB, 2, C->grid_vec, &seg,
+                               C->nseg == 1 ? C->done_flag() : nullptr, s);
     } else {
         if (C->precision == 64) SN_LAUNCH_UPD(double, true);
         else SN_LAUNCH_UPD(float, true);
@@ -525,7 +541,7 @@ static void launch_init(sn_cg *C, const M *vals, const unsigned char *mask, cons
     const sn_pattern *P = C->pat;
     hipLaunchKernelGGL((cg_init_kernel<V, M, LPR>), dim3(C->grid_vec), dim3(kVecBlock), 0, s,
                        C->map(), P->indptr, P->indices, P->diag, vals, mask, b, x0, (V *)C->r,
-                       (V *)C->z, (V *)C->dinv, (V *)C->p0,
+                       (V *)C->w0, (V *)C->w1, (V *)C->dinv,
                        C->precision == 32 ? C->xs : (float *)nullptr, C->partB, C->partbb);
 }
 
@@ -541,8 +557,8 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
                        C->partbb, C->st, C->rtol_dev, atol, maxiter);
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
-        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->z, C->partB, 2,
-                               C->grid_vec, &seg, nullptr, s);
+        (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->w0, 2,
+                               C->partB, 2, C->grid_vec, &seg, nullptr, s);
     }
 }
 
@@ -715,9 +731,8 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
     if ((rc = upload_ints(&C->segB, segB, as_stream(stream)))) return rc;
     const size_t vb = (size_t)P->n * (precision == 64 ? 8 : 4);
     SN_HIP(hipMalloc(&C->r, vb));
-    SN_HIP(hipMalloc(&C->z, vb));
-    SN_HIP(hipMalloc(&C->p0, vb));
-    SN_HIP(hipMalloc(&C->p1, vb));
+    SN_HIP(hipMalloc(&C->w0, 2 * vb));
+    SN_HIP(hipMalloc(&C->w1, 2 * vb));
     SN_HIP(hipMalloc(&C->q, vb));
     SN_HIP(hipMalloc(&C->dinv, vb));
     if (precision == 32) SN_HIP(hipMalloc(&C->xs, (size_t)P->n * 4));
@@ -779,7 +794,7 @@ int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
 void sn_cg_destroy(sn_cg *C) {
     if (!C) return;
     if (C->graph) (void)hipGraphExecDestroy(C->graph);
-    void *bufs[] = {C->r,    C->z,        C->p0, C->p1,   C->q,     C->dinv,  C->xs,
+    void *bufs[] = {C->r,    C->w0,       C->w1, C->q,     C->dinv,  C->xs,
                     C->partA, C->partB,   C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0,
                     C->blkB0, C->segA,    C->segB, C->dcol};
     for (void *p : bufs)
@@ -849,16 +864,16 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
                        C->map(), C->grid_vec, C->partB);
     if (C->precision == 64)
         hipLaunchKernelGGL(fill_free_kernel<double>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
-                           P->n, mask, (double *)C->z, (double *)C->p1, 1.0);
+                           P->n, mask, (double *)C->w1, 1.0);
     else
         hipLaunchKernelGGL(fill_free_kernel<float>, dim3(grid_for(P->n, 256)), dim3(256), 0, s,
-                           P->n, mask, (float *)C->z, (float *)C->p1, 1.0f);
+                           P->n, mask, (float *)C->w1, 1.0f);
     SN_CHECK_LAUNCH();
     hipEvent_t e0, e1;
     SN_HIP(hipEventCreate(&e0));
     SN_HIP(hipEventCreate(&e1));
     SN_HIP(hipEventRecord(e0, s));
-    // parity 1 reads p1 (zero) and rho[0], writes p0 and rho[1]: the state stays
+    // parity 1 reads {z, p} of w1 (p zero) and rho[0], writes p into w0 and rho[1]: the state stays
     // the same for every repetition
     for (int r = 0; r < reps; ++r) {
         if (C->precision == 64 && !C->mat32)
