@@ -376,7 +376,8 @@ def main():
         'cg_iterations_fp64': st64['cg_iterations'],
         'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': 'cg_spmv_kernel<float,%d>' % args.lanes,
+                     'kernel': 'cg_spmv_kernel<float,float,%d,%s>'
+                               % (args.lanes, 'short' if ib == 2 else 'int'),
                      'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': spmv_bytes,
                      'index_bytes': ib, 'avg_ms_int32_indices': ms32.value},
         'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
