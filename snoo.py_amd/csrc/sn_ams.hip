@@ -123,13 +123,17 @@ __global__ __launch_bounds__(kAmsBlock) void ams_lap_local_kernel(
 // tangents (x_hi - x_lo)/2 are formed from them where they are used
 // (a 16-byte gather from a few MB that stays in cache, instead of a
 // 16-byte tangent streamed per edge and per incidence)
-__global__ void ams_coords_kernel(int n_nodes, const double *__restrict__ nodes,
-                                  const int *__restrict__ node_mesh, float4 *__restrict__ xyz) {
+__global__ void ams_coords_kernel(int n_nodes, int n_free, const double *__restrict__ nodes,
+                                  const int *__restrict__ node_mesh, float4 *__restrict__ xyz,
+                                  float4 *__restrict__ yx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_nodes) return;
     const int m = node_mesh[i];
-    xyz[i] = make_float4((float)nodes[3 * m], (float)nodes[3 * m + 1], (float)nodes[3 * m + 2],
-                         0.f);
+    const float4 c = make_float4((float)nodes[3 * m], (float)nodes[3 * m + 1],
+                                 (float)nodes[3 * m + 2], 0.f);
+    xyz[i] = c;
+    yx[2 * i + 1] = c;
+    if (i >= n_free) yx[2 * i] = f4_zero();   // fixed nodes: y stays 0
 }
 
 // half the difference of two coordinates: both the restriction and the
@@ -193,8 +197,8 @@ __global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
 // and other end point streamed in order) and the pre-smoothed x0 = w D^-1 b0.
 template <typename V, int LPR>
 __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
-    int n_free, const int *__restrict__ inc_ptr, const int *__restrict__ inc_edge,
-    const int *__restrict__ inc_other, const float4 *__restrict__ xyz, const V *__restrict__ r,
+    int n_free, const int *__restrict__ inc_ptr, const int2 *__restrict__ inc,
+    const float4 *__restrict__ xyz, const V *__restrict__ r,
     const float *__restrict__ dinv0, float omega, float4 *__restrict__ b0,
     float4 *__restrict__ x0, const int *__restrict__ done) {
     if (done && *done) return;
@@ -204,9 +208,9 @@ __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     if (a < n_free) {
         const float4 xa = xyz[a];
         for (int k = inc_ptr[a] + lane; k < inc_ptr[a + 1]; k += LPR) {
-            const int o = inc_other[k];
-            const float4 xb = xyz[o & 0x7fffffff];
-            f4_fma(acc, (float)r[inc_edge[k]], o < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
+            const int2 ie = inc[k];   // {edge, other end | sign}
+            const float4 xb = xyz[ie.y & 0x7fffffff];
+            f4_fma(acc, (float)r[ie.x], ie.y < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
         }
     }
     f4_reduce<LPR>(acc);
@@ -246,7 +250,8 @@ __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
 
 // wave per row I of the coarsest level: e_I = (C^-1 b)_I, then prolongation
 // into the finer level's pre-smoothed iterate, x_f[i] += e_I for the rows i
-// of aggregate I (or, with a single level, y = e)
+// of aggregate I (or, with a single level, y = e, stored as y[2I] of the
+// per-node {y, coordinates} pairs)
 __global__ __launch_bounds__(kAmsBlock) void ams_coarse_kernel(
     int nc, const float *__restrict__ cinv, const float4 *__restrict__ b,
     const int *__restrict__ agg_ptr, float4 *__restrict__ x_f, float4 *__restrict__ y,
@@ -262,13 +267,14 @@ __global__ __launch_bounds__(kAmsBlock) void ams_coarse_kernel(
         for (int i = agg_ptr[wid] + lane; i < agg_ptr[wid + 1]; i += kWave)
             x_f[i] = f4_add(x_f[i], acc);
     } else if (lane == 0) {
-        y[wid] = acc;
+        y[2 * wid] = acc;
     }
 }
 
 // LPR lanes per row of level l, whose x already holds x + P e_(l+1):
 // y = x + w D^-1 (b - A x); then y is prolongated into the finer level
 // (x_f[i] += y for the rows i of this row's aggregate), or stored at level 0
+// as y[2 row] of the per-node {y, coordinates} pairs
 template <int LPR>
 __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
         for (int i = agg_ptr[row] + lane; i < agg_ptr[row + 1]; i += LPR)
             x_f[i] = f4_add(x_f[i], yi);
     } else if (lane == 0) {
-        y[row] = yi;
+        y[2 * row] = yi;
     }
 }
 
@@ -303,7 +309,7 @@ template <typename V>
 __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
     const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
-    const float4 *__restrict__ xyz, const float4 *__restrict__ y0, float weight,
+    const float4 *__restrict__ yx, float weight,
     V *__restrict__ z, int zs, double *__restrict__ partials, int stride,
     const int *__restrict__ row0,
     const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
@@ -326,10 +332,11 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
         V zi = 0;
         if (f) {
             const int2 en = edge_nodes[i];
-            float4 s = f4_zero();
-            if (en.x < n_free) s = f4_add(s, y0[en.x]);
-            if (en.y < n_free) s = f4_add(s, y0[en.y]);
-            const float4 t = half_diff(xyz[en.y], xyz[en.x]);
+            // {y, coordinates} of both end points; y is 0 at fixed nodes
+            const float4 y_lo = yx[2 * (long long)en.x], c_lo = yx[2 * (long long)en.x + 1];
+            const float4 y_hi = yx[2 * (long long)en.y], c_hi = yx[2 * (long long)en.y + 1];
+            const float4 s = f4_add(f4_add(f4_zero(), y_lo), y_hi);
+            const float4 t = half_diff(c_hi, c_lo);
             const float aux = t.x * s.x + t.y * s.y + t.z * s.z;
             const V ri = r[i];
             zi = dinv[i] * ri + (V)(weight * aux);
@@ -350,8 +357,8 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     constexpr int kIncLanes = 8;
     hipLaunchKernelGGL((ams_edge_restrict_kernel<V, kIncLanes>),
                        dim3(grid_for((long long)A->n_free * kIncLanes, kAmsBlock)),
-                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc_edge, A->inc_other,
-                       A->xyz, r, l0.dinv, A->omega, l0.b, l0.x, done);
+                       dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc, A->xyz, r, l0.dinv,
+                       A->omega, l0.b, l0.x, done);
     const int waves_per_block = kAmsBlock / kWave;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
@@ -363,16 +370,16 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     AmsLevel &lc = A->lv[L - 1];
     hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)), dim3(kAmsBlock),
                        0, s, lc.n, A->cinv, lc.b, lc.agg_ptr,
-                       L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, l0.y, done);
+                       L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, A->yx, done);
     for (int l = L - 2; l >= 0; --l) {
         AmsLevel &f = A->lv[l];
         hipLaunchKernelGGL((ams_up_kernel<8>), dim3(grid_for((long long)f.n * 8, kAmsBlock)),
                            dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv, f.b,
                            f.x, A->omega, f.agg_ptr, l > 0 ? A->lv[l - 1].x : (float4 *)nullptr,
-                           l0.y, done);
+                           A->yx, done);
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
-                       A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->xyz, l0.y,
+                       A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->yx,
                        A->weight, z, zs, partials, stride, seg ? seg->row0 : nullptr,
                        seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done);
 }
@@ -461,22 +468,23 @@ int sn_ams_create(sn_ams **out, int n_edges, int n_free_nodes, int n_el, const i
     }
     node_mesh.insert(node_mesh.end(), fixed_mesh.begin(), fixed_mesh.end());
     A->n_nodes = (int)node_mesh.size();
-    // incidences: the other end point of the edge, sign bit set when this node is hi
-    std::vector<int> other(ninc > 0 ? ninc : 1, 0);
+    // incidences: {edge, other end point with the sign bit set when this node is hi}
+    std::vector<int2> inc(ninc > 0 ? ninc : 1, make_int2(0, 0));
     for (int a = 0; a < n_free_nodes; ++a)
         for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
             const int2 v = en[inc_edge[k]];
             SN_REQUIRE(v.x == a || v.y == a, "sn_ams_create: incidence %d: edge %d misses node %d",
                        k, inc_edge[k], a);
-            other[k] = (v.x == a) ? v.y : (int)((unsigned)v.x | 0x80000000u);
+            inc[k] = make_int2(inc_edge[k],
+                               (v.x == a) ? v.y : (int)((unsigned)v.x | 0x80000000u));
         }
     int rc;
     if ((rc = upload(&A->edge_nodes, en.data(), (size_t)n_edges, s))) return rc;
     if ((rc = upload(&A->node_mesh, node_mesh.data(), node_mesh.size(), s))) return rc;
     SN_HIP(hipMalloc(&A->xyz, node_mesh.size() * sizeof(float4)));
+    SN_HIP(hipMalloc(&A->yx, 2 * node_mesh.size() * sizeof(float4)));
     if ((rc = upload(&A->inc_ptr, inc_ptr, (size_t)n_free_nodes + 1, s))) return rc;
-    if ((rc = upload(&A->inc_edge, inc_edge, (size_t)ninc, s))) return rc;
-    if ((rc = upload(&A->inc_other, other.data(), (size_t)ninc, s))) return rc;
+    if ((rc = upload(&A->inc, inc.data(), (size_t)ninc, s))) return rc;
     A->n_inc = ninc;
     SN_HIP(hipMalloc(&A->loc, (size_t)10 * n_el * sizeof(float)));
     SN_HIP(hipStreamSynchronize(s));
@@ -533,7 +541,6 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     SN_HIP(hipMalloc(&lv.dinv, (size_t)n * sizeof(float)));
     SN_HIP(hipMalloc(&lv.b, (size_t)n * sizeof(float4)));
     SN_HIP(hipMalloc(&lv.x, (size_t)n * sizeof(float4)));
-    if (L == 0) SN_HIP(hipMalloc(&lv.y, (size_t)n * sizeof(float4)));
     SN_HIP(hipStreamSynchronize(s));
     A->lv.push_back(lv);
     A->ready = false;
@@ -554,7 +561,7 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
         SN_HIP(hipMalloc(&A->cinv, (size_t)nc * nc * sizeof(float)));
     }
     hipLaunchKernelGGL(ams_coords_kernel, dim3(grid_for(A->n_nodes, 256)), dim3(256), 0, s,
-                       A->n_nodes, nodes, A->node_mesh, A->xyz);
+                       A->n_nodes, A->n_free, nodes, A->node_mesh, A->xyz, A->yx);
     hipLaunchKernelGGL(ams_lap_local_kernel, dim3(grid_for(A->n_el, kAmsBlock)), dim3(kAmsBlock),
                        0, s, A->n_el, nodes, cells, nu_el, A->loc);
     for (int l = 0; l < L; ++l) {
@@ -659,11 +666,11 @@ void sn_ams_destroy(sn_ams *A) {
     if (!A) return;
     for (AmsLevel &v : A->lv) {
         void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.vals,
-                        v.dinv, v.b, v.x, v.y};
+                        v.dinv, v.b, v.x};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }
-    void *bufs[] = {A->edge_nodes, A->node_mesh, A->xyz, A->inc_ptr, A->inc_edge, A->inc_other,
+    void *bufs[] = {A->edge_nodes, A->node_mesh, A->xyz, A->yx, A->inc_ptr, A->inc,
                     A->loc, A->dense, A->cinv};
     for (void *p : bufs)
         if (p) (void)hipFree(p);

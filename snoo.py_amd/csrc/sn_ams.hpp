@@ -19,7 +19,6 @@ struct AmsLevel {
     float *dinv = nullptr;  // [n]
     float4 *b = nullptr;    // right-hand side of the cycle at this level
     float4 *x = nullptr;    // pre-smoothed iterate, then + prolongated coarse correction
-    float4 *y = nullptr;    // level 0 only: the cycle's result
 };
 
 struct sn_ams {
@@ -28,9 +27,12 @@ struct sn_ams {
     int2 *edge_nodes = nullptr;  // [n_edges] node ids of (lo, hi); >= n_free: fixed node
     int *node_mesh = nullptr;    // [n_nodes] mesh node id of each node
     float4 *xyz = nullptr;       // [n_nodes] coordinates (float), refreshed by sn_ams_update
+    float4 *yx = nullptr;        // [2*n_nodes] per node: the cycle's result y (0 at fixed
+                                 // nodes), then the coordinates: what the prolongation
+                                 // reads at an end point, in one 32-byte piece
     int *inc_ptr = nullptr;      // [n_free+1] free edges at each free node
-    int *inc_edge = nullptr;     // [n_inc] edge of each incidence
-    int *inc_other = nullptr;    // [n_inc] the edge's other node; sign bit: this node is hi
+    int2 *inc = nullptr;         // [n_inc] {edge, the edge's other node | sign bit when this
+                                 // node is the edge's hi end}
     int n_inc = 0;
     float *loc = nullptr;        // [10*n_el] element Laplacian pair values
     std::vector<AmsLevel> lv;
