@@ -201,17 +201,19 @@ __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     const float4 *__restrict__ xyz, const V *__restrict__ r,
     const float *__restrict__ dinv0, float omega, float4 *__restrict__ b0,
     float4 *__restrict__ x0, const int *__restrict__ done) {
-    if (done && *done) return;
     const int a = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
+    // the node's incidence range and coordinates and the stop flag:
+    // independent loads, issued together
+    const bool in = a < n_free;
+    const int k0 = in ? inc_ptr[a] : 0, k1 = in ? inc_ptr[a + 1] : 0;
+    const float4 xa = in ? xyz[a] : f4_zero();
+    if (done && *done) return;
     float4 acc = f4_zero();
-    if (a < n_free) {
-        const float4 xa = xyz[a];
-        for (int k = inc_ptr[a] + lane; k < inc_ptr[a + 1]; k += LPR) {
-            const int2 ie = inc[k];   // {edge, other end | sign}
-            const float4 xb = xyz[ie.y & 0x7fffffff];
-            f4_fma(acc, (float)r[ie.x], ie.y < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
-        }
+    for (int k = k0 + lane; k < k1; k += LPR) {
+        const int2 ie = inc[k];   // {edge, other end | sign}
+        const float4 xb = xyz[ie.y & 0x7fffffff];
+        f4_fma(acc, (float)r[ie.x], ie.y < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
     }
     f4_reduce<LPR>(acc);
     if (a < n_free && lane == 0) {
@@ -225,19 +227,21 @@ __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
 // (x = w D^-1 b already stored for this level), and x_c = w D_c^-1 b_c unless
 // the next level is the coarsest
 __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
-    int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ indptr,
+    int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ agg_kp,
     const int *__restrict__ indices, const float *__restrict__ vals, const float4 *__restrict__ b,
     const float4 *__restrict__ x, const float *__restrict__ dinv_c, float omega,
     float4 *__restrict__ bc, float4 *__restrict__ xc, const int *__restrict__ done) {
-    if (done && *done) return;
     const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (wid >= n_agg) return;  // uniform over the wave
+    // the aggregate's rows and nonzero range (agg_kp = indptr at agg_ptr) and
+    // the stop flag are independent loads, issued together
     const int r0 = agg_ptr[wid], r1 = agg_ptr[wid + 1];
+    const int k0 = agg_kp[wid], k1 = agg_kp[wid + 1];
+    if (done && *done) return;
     float4 s = f4_zero();
     for (int i = r0 + lane; i < r1; i += kWave) s = f4_add(s, b[i]);
-    const int k1 = indptr[r1];
-    for (int k = indptr[r0] + lane; k < k1; k += kWave) f4_fma(s, -vals[k], x[indices[k]]);
+    for (int k = k0 + lane; k < k1; k += kWave) f4_fma(s, -vals[k], x[indices[k]]);
     f4_reduce<kWave>(s);
     if (lane == 0) {
         bc[wid] = s;
@@ -281,17 +285,19 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     const float *__restrict__ vals, const float *__restrict__ dinv, const float4 *__restrict__ b,
     const float4 *__restrict__ x, float omega, const int *__restrict__ agg_ptr,
     float4 *__restrict__ x_f, float4 *__restrict__ y, const int *__restrict__ done) {
-    if (done && *done) return;
     const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
+    // the row's range, its own values and the stop flag: independent loads
+    const bool in = row < n;
+    const int k0 = in ? indptr[row] : 0, k1 = in ? indptr[row + 1] : 0;
+    const float4 xi = in ? x[row] : f4_zero(), bi = in ? b[row] : f4_zero();
+    const float di = in ? dinv[row] : 0.f;
+    if (done && *done) return;
     float4 acc = f4_zero();
-    if (row < n)
-        for (int k = indptr[row] + lane; k < indptr[row + 1]; k += LPR)
-            f4_fma(acc, vals[k], x[indices[k]]);
+    for (int k = k0 + lane; k < k1; k += LPR) f4_fma(acc, vals[k], x[indices[k]]);
     f4_reduce<LPR>(acc);
-    if (row >= n) return;
-    const float4 xi = x[row], bi = b[row];
-    const float d = omega * dinv[row];
+    if (!in) return;
+    const float d = omega * di;
     const float4 yi = make_float4(xi.x + d * (bi.x - acc.x), xi.y + d * (bi.y - acc.y),
                                   xi.z + d * (bi.z - acc.z), 0.f);
     if (x_f) {
@@ -363,7 +369,7 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
         hipLaunchKernelGGL(ams_down_kernel, dim3(grid_for(c.n, waves_per_block)),
-                           dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, f.indptr, f.indices, f.vals,
+                           dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, c.agg_kp, f.indices, f.vals,
                            f.b, f.x, c.dinv, A->omega, c.b,
                            (l + 2 < L) ? c.x : (float4 *)nullptr, done);
     }
@@ -529,14 +535,23 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     AmsLevel lv;
     lv.n = n;
     lv.nnz = nnz;
+    lv.indptr_h.assign(indptr, indptr + n + 1);
+    std::vector<int> agg_kp;
+    if (L > 0) {
+        const std::vector<int> &fp = A->lv[L - 1].indptr_h;
+        agg_kp.resize((size_t)n + 1);
+        for (int I = 0; I <= n; ++I) agg_kp[I] = fp[agg_ptr[I]];
+    }
     int rc;
     if ((rc = upload(&lv.indptr, indptr, (size_t)n + 1, s))) return rc;
     if ((rc = upload(&lv.indices, indices, (size_t)nnz, s))) return rc;
     if ((rc = upload(&lv.diagpos, diag.data(), (size_t)n, s))) return rc;
     if ((rc = upload(&lv.gptr, gather_ptr, (size_t)nnz + 1, s))) return rc;
     if ((rc = upload(&lv.glist, gather_list, (size_t)ng, s))) return rc;
-    if (L > 0)
+    if (L > 0) {
         if ((rc = upload(&lv.agg_ptr, agg_ptr, (size_t)n + 1, s))) return rc;
+        if ((rc = upload(&lv.agg_kp, agg_kp.data(), (size_t)n + 1, s))) return rc;
+    }
     SN_HIP(hipMalloc(&lv.vals, (size_t)nnz * sizeof(float)));
     SN_HIP(hipMalloc(&lv.dinv, (size_t)n * sizeof(float)));
     SN_HIP(hipMalloc(&lv.b, (size_t)n * sizeof(float4)));
@@ -665,8 +680,8 @@ int sn_amg_aggregate(int n, const int *indptr, const int *indices, const double 
 void sn_ams_destroy(sn_ams *A) {
     if (!A) return;
     for (AmsLevel &v : A->lv) {
-        void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.vals,
-                        v.dinv, v.b, v.x};
+        void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.agg_kp,
+                        v.vals, v.dinv, v.b, v.x};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }

@@ -15,6 +15,8 @@ struct AmsLevel {
     int *gptr = nullptr;    // [nnz+1] gather plan of vals
     int *glist = nullptr;   // level 0: slot*n_el + e of the element values; else nnz of level-1
     int *agg_ptr = nullptr; // [n+1] rows of level-1 forming each row here (null at level 0)
+    int *agg_kp = nullptr;  // [n+1] level-1 indptr at agg_ptr: each aggregate's nonzeros
+    std::vector<int> indptr_h;  // host copy of indptr (forms the next level's agg_kp)
     float *vals = nullptr;  // [nnz]
     float *dinv = nullptr;  // [n]
     float4 *b = nullptr;    // right-hand side of the cycle at this level

@@ -176,7 +176,9 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     __shared__ double lds[kRowBlock / kWave];
     const int seg = sm.segA[blockIdx.x];
     CgState *S = st + seg;
-    if (S->done) return;
+    // the stop flag is read together with the partials (independent loads)
+    // and tested after them
+    const int was_done = S->done;
     const int nbl = sm.blkA0[seg + 1] - sm.blkA0[seg];
     int lb = blockIdx.x - sm.blkA0[seg];
     // schedule bit 1: workgroups are dealt round-robin to the 8 XCDs;
@@ -186,6 +188,7 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     const int b0 = sm.blkB0[seg], b1 = sm.blkB0[seg + 1];
     const double rho = reduce_range<kRowBlock>(partB, b0, b1, 2, 0, lds);
     const double rr = reduce_range<kRowBlock>(partB, b0, b1, 2, 1, lds);
+    if (was_done) return;
     const int it = S->it;
     // scipy: the iteration cap ends the loop before the residual test
     const int stop = (it >= S->maxiter) ? 2 : ((sqrt(rr) < S->tol) ? 1 : 0);
@@ -257,9 +260,10 @@ __global__ __launch_bounds__(kVecBlock) void cg_update_kernel(
     __shared__ double lds[kVecBlock / kWave];
     const int seg = sm.segB[blockIdx.x];
     CgState *S = st + seg;
-    if (S->done) return;
+    const int was_done = S->done;   // read with the partials, tested after them
     const int lb = blockIdx.x - sm.blkB0[seg], nbl = sm.blkB0[seg + 1] - sm.blkB0[seg];
     const double pq = reduce_range<kVecBlock>(partA, sm.blkA0[seg], sm.blkA0[seg + 1], 1, 0, lds);
+    if (was_done) return;
     if (!(pq == pq) || pq == 0.0 || isinf(pq)) {
         if (lb == 0 && threadIdx.x == 0) {
             S->pq = pq;
