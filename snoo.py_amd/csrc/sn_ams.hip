@@ -162,11 +162,20 @@ __global__ void ams_dinv_kernel(int n, const int *__restrict__ diagpos,
 
 // One block: dense inverse of the coarsest matrix by Gauss-Jordan elimination
 // in place (symmetric positive definite, so no pivoting is needed).
+// A coarsest matrix of up to kAmsLdsCoarse rows is eliminated in LDS (one
+// pivot step costs a few LDS round trips instead of L2 ones); larger ones
+// in the global scratch `a_glob`.  Same arithmetic either way.
+// 80^2 doubles = 50 KB of dynamic LDS, with the 8 KB column buffer within
+// the 64 KB a workgroup may allocate
+constexpr int kAmsLdsCoarse = 80;
+
 __global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
     int nc, const int *__restrict__ indptr, const int *__restrict__ indices,
-    const float *__restrict__ vals, double *__restrict__ a, float *__restrict__ cinv) {
+    const float *__restrict__ vals, double *__restrict__ a_glob, float *__restrict__ cinv) {
     __shared__ double col[kAmsMaxCoarse];
     __shared__ double piv;
+    extern __shared__ double a_lds[];
+    double *a = (nc <= kAmsLdsCoarse) ? a_lds : a_glob;
     const int t = threadIdx.x, T = blockDim.x;
     for (int k = t; k < nc * nc; k += T) a[k] = 0.0;
     __syncthreads();
@@ -588,7 +597,8 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
                            v.diagpos, v.vals, v.dinv);
     }
     AmsLevel &c = A->lv[L - 1];
-    hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(1024), 0, s, nc, c.indptr,
+    const size_t lds = (nc <= kAmsLdsCoarse) ? (size_t)nc * nc * sizeof(double) : 0;
+    hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(1024), lds, s, nc, c.indptr,
                        c.indices, c.vals, A->dense, A->cinv);
     SN_CHECK_LAUNCH();
     A->ready = true;
