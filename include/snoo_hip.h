@@ -158,6 +158,15 @@ int sn_dof_gather_f64(const sn_pattern *pat, const double *loc6_dev,
  * float64 or float32 output) and one local vector in float64 (6):
  *   mode 0: b_e = dK_e x_e + rhs_e  (b = Jx - Kx + rhs, red_mvp_solver.py:186)
  *   mode 1: R_e = K_e x_e - rhs_e   (Newton residual R(x) = K(x)x - rhs(x))
+ *   mode 2: R_e as mode 1; instead of the 21 pairs, locJ_dev receives the
+ *           Jacobian in reference-curl form, 6 values per element at
+ *           u*num_el_total + e (u over 00, 01, 02, 11, 12, 22 of the
+ *           symmetric 3x3 Mr), such that J_e[j][k] = s_j s_k c_j . Mr c_k
+ *           with c_j the integer reference curl of local edge j (rows
+ *           (0,-1,1), (1,0,-1), (-1,1,0), (0,0,1), (0,-1,0), (1,0,0) for the
+ *           edges (0,1), (0,2), (0,3), (1,2), (1,3), (2,3)) and s_j the edge's
+ *           orientation sign (-1 where bit j of orient is set).  The operand
+ *           of sn_cg_set_element_operator.
  * Element arithmetic is float64 in both precisions.  Bs_dev == NULL marks a
  * source-free (air) domain.  nu_el_dev (may be NULL) receives the element's
  * mean reluctivity sum_q nu_q w_q / sum_q w_q (float32, indexed like the
@@ -325,6 +334,30 @@ void sn_ams_destroy(sn_ams *ams);
  * workspace the preconditioner is built on the joined (block-diagonal) mesh,
  * so its cycle never couples two segments. */
 int sn_cg_set_ams(sn_cg *cg, sn_ams *ams);
+
+/* Element operator for the float32 CG.  No counterpart in the reference,
+ * whose CG multiplies the assembled CSR Jacobian (snoopy/red_mvp_solver.py:
+ * 195-205); this is the same matrix, J = sum_e P_e^T S_e C^T Mr_e C S_e P_e,
+ * applied as a sum of element products from the reference-curl form of
+ * sn_element_newton mode 2.  Rounding Mr to float32 keeps J's gradient null
+ * space exact, which rounding the assembled CSR values does not: the float32
+ * CG on the rounded CSR matrix can stall far above the tolerance of a tight
+ * Newton step.  After enable = 1 every sn_cg_solve on this workspace takes
+ * vals_dev = Mr (float32 [6*num_el], layout of mode 2) and starts from 0;
+ * enable = 0 returns to CSR values.  glob_ids_dev [6*num_el] and orient_dev
+ * [num_el] as given to the element kernels (num_el of the pattern).  Float32
+ * workspaces (precision 32) with one segment only. */
+int sn_cg_set_element_operator(sn_cg *cg, const int *glob_ids_dev, const int *orient_dev,
+                               int enable, void *stream);
+/* 1 when the workspace multiplies with the element operator, else 0. */
+int sn_cg_element_operator(const sn_cg *cg);
+/* y = J x with the element operator in float64 (Mr_dev float64, layout of
+ * sn_element_newton mode 2; rows and columns with mask == 0 removed, mask may
+ * be NULL); work_dev [6*num_el] receives the element products.  No
+ * counterpart in the reference (a check of the element operator). */
+int sn_element_apply_f64(const sn_pattern *pat, const int *glob_ids_dev, const int *orient_dev,
+                         const double *Mr_dev, const unsigned char *mask_dev,
+                         const double *x_dev, double *work_dev, double *y_dev, void *stream);
 
 /* ------------------------------------------------- learned warm start
  * No counterpart in the reference (it starts Newton from A = 0 or a given

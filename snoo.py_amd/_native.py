@@ -76,6 +76,9 @@ SIGNATURES = {
     'sn_cg_set_schedule': (_c_int, [_vp, _c_int]),
     'sn_cg_set_col_offsets': (_c_int, [_vp, _c_int]),
     'sn_cg_index_bytes': (_c_int, [_vp]),
+    'sn_cg_set_element_operator': (_c_int, [_vp, _vp, _vp, _c_int, _vp]),
+    'sn_cg_element_operator': (_c_int, [_vp]),
+    'sn_element_apply_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 SN_OK, SN_ERR_ARG, SN_ERR_HIP, SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN = 0, 1, 2, 3, 4

@@ -22,7 +22,16 @@ constexpr int kElBlock = 256;
 constexpr int kMaxLdsIntervals = 512;
 constexpr double kNu0 = 1.0 / 4.0 / 3.141592653589793238462643383279502984 * 1e7;
 
-enum { MODE_LOCAL = 0, MODE_NEWTON_B = 1, MODE_NEWTON_R = 2 };
+// MODE_NEWTON_C: the residual of MODE_NEWTON_R, and instead of the 21 pair
+// values the Jacobian in reference-curl form.  With c_j = 2 c^_j the curl of
+// local edge j on the reference element (c^_j has entries 0 and +-1), s_j the
+// edge's orientation sign and Jm the element map's Jacobian, g_j = s_j Jm c_j,
+// so J_jk = s_j s_k c^_j . Mr c^_k with the symmetric 3x3
+//   Mr = 4 Jm^T (kscale I + sum_q a_q B_q B_q^T) Jm.
+// A product of this form keeps the gradient null space of J exact whatever
+// the rounding of Mr: c^ applied to a gradient's edge values sums them
+// around faces, which cancels exactly.
+enum { MODE_LOCAL = 0, MODE_NEWTON_B = 1, MODE_NEWTON_R = 2, MODE_NEWTON_C = 3 };
 
 // Diluted steel (fem_c/ctools.c:124-128, 1565-1759): at a quadrature point
 // with physical z, s = (c_num + z m_num) / (c_den + z m_den) scales the
@@ -139,10 +148,11 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
     double sum_nu_w = 0.0, sum_w = 0.0;
     double dK[21];
     double rhs[6];
+    double dM[6];   // MODE_NEWTON_C: sum_q a_q t t^T, t = Jm^T B_q (upper triangle)
 #pragma unroll
     for (int p = 0; p < 21; ++p) dK[p] = 0.0;
 #pragma unroll
-    for (int j = 0; j < 6; ++j) rhs[j] = 0.0;
+    for (int j = 0; j < 6; ++j) rhs[j] = dM[j] = 0.0;
 
     const double *bs = Bs ? Bs + (size_t)le * 3 * Q : nullptr;
     for (int q = 0; q < Q; ++q) {
@@ -169,7 +179,18 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
         const double wq = w[q];
         sum_nu_w += nu * wq;
         sum_w += wq;
-        if (bm >= 1e-14) {
+        if (MODE == MODE_NEWTON_C && bm >= 1e-14) {
+            const double a = dnu * wq * inv_det / bm;
+            double t[3];
+#pragma unroll
+            for (int l = 0; l < 3; ++l) t[l] = J[0][l] * b0 + J[1][l] * b1 + J[2][l] * b2;
+            dM[0] += a * t[0] * t[0];
+            dM[1] += a * t[0] * t[1];
+            dM[2] += a * t[0] * t[2];
+            dM[3] += a * t[1] * t[1];
+            dM[4] += a * t[1] * t[2];
+            dM[5] += a * t[2] * t[2];
+        } else if (MODE != MODE_NEWTON_C && bm >= 1e-14) {
             const double a = dnu * wq * inv_det / bm;
             double P[6];
 #pragma unroll
@@ -204,7 +225,8 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
                 reinterpret_cast<double *>(out_a)[slot] = kv;
                 out_b[slot] = dv;
             } else {
-                store_as<TJ>(out_a, slot, dv + kv);  // J = dK + K (curl_curl_assembler.py:932)
+                // J = dK + K (curl_curl_assembler.py:932)
+                if (MODE != MODE_NEWTON_C) store_as<TJ>(out_a, slot, dv + kv);
                 Kx[j] += kv * xv[k];
                 dKx[j] += dv * xv[k];
                 if (k != j) {
@@ -214,6 +236,18 @@ __global__ __launch_bounds__(kElBlock) void element_kernel(
             }
             ++p;
         }
+    }
+    if (MODE == MODE_NEWTON_C) {
+        // Mr = 4 (kscale Jm^T Jm + dM), upper triangle (00, 01, 02, 11, 12, 22)
+        int u = 0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m2 = l; m2 < 3; ++m2) {
+                const double jtj = J[0][l] * J[0][m2] + J[1][l] * J[1][m2] + J[2][l] * J[2][m2];
+                store_as<TJ>(out_a, (long long)u * num_el_total + e, 4.0 * (kscale * jtj + dM[u]));
+                ++u;
+            }
     }
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
@@ -340,7 +374,7 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
     int rc = check_elem_args(num_el_total, el_begin, el_count, Q, type_spec, n_int, nodes, cells,
                              gids, orient, curls, w, coeffs, knots, x);
     if (rc) return rc;
-    SN_REQUIRE(mode == 0 || mode == 1, "sn_element_newton: mode must be 0 or 1");
+    SN_REQUIRE(mode >= 0 && mode <= 2, "sn_element_newton: mode must be 0, 1 or 2");
     if (el_count == 0) return SN_OK;
     SN_REQUIRE(locJ && loc6, "sn_element_newton: null output");
     Dilution dil;
@@ -354,9 +388,12 @@ int sn_element_newton(int out_f32, int mode, int num_el_total, int el_begin, int
     if (mode == 0) {
         if (out_f32) SN_LAUNCH_EL(MODE_NEWTON_B, float);
         else SN_LAUNCH_EL(MODE_NEWTON_B, double);
-    } else {
+    } else if (mode == 1) {
         if (out_f32) SN_LAUNCH_EL(MODE_NEWTON_R, float);
         else SN_LAUNCH_EL(MODE_NEWTON_R, double);
+    } else {
+        if (out_f32) SN_LAUNCH_EL(MODE_NEWTON_C, float);
+        else SN_LAUNCH_EL(MODE_NEWTON_C, double);
     }
 #undef SN_LAUNCH_EL
     SN_CHECK_LAUNCH();

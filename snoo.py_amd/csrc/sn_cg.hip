@@ -248,6 +248,233 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
     }
 }
 
+// ------------------------------------------------- element operator (A1, A2)
+// The Jacobian as a sum of element products, A = sum_e P_e^T S_e c^T Mr_e c S_e P_e
+// (P_e the element's 6 DOFs, S_e their orientation signs, c the 3x6 integer
+// reference curls, Mr_e the symmetric 3x3 of sn_element_newton mode 2).  With
+// float32 values this keeps A's gradient null space exact, which the rounded
+// CSR values of the assembled matrix do not: on meshes of ~1M elements the
+// float32 CG on the rounded CSR matrix stalls near a relative residual of
+// 1e-2, this one converges like the float64 CG.
+//
+// A1 (thread per element): kernel A's stop test; p = z + beta p_old at the
+//    6 DOFs; u = c (S p), the circulations of the three faces at local node 0;
+//    m = Mr u; the element products y_j = s_j c_j . m at slot j*n_el + e.
+// A2 (thread per DOF): q_i = sum of DOF i's element products in ascending
+//    element order (the pattern's DOF contribution lists); p_i written to the
+//    other {z, p} array; per-block partials of p.q, in kernel A's layout.
+template <typename V>
+__global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
+    int n_el, const int *__restrict__ el_dofs, const float *__restrict__ Mr,
+    const V *__restrict__ wold, V *__restrict__ yloc, const double *__restrict__ partB, int nbB,
+    CgState *__restrict__ st, int parity) {
+    using V2 = typename Pair<V>::T;
+    const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);
+    __shared__ double lds[kRowBlock / kWave];
+    CgState *S = st;
+    const int was_done = S->done;
+    const double rho = reduce_range<kRowBlock>(partB, 0, nbB, 2, 0, lds);
+    const double rr = reduce_range<kRowBlock>(partB, 0, nbB, 2, 1, lds);
+    if (was_done) return;
+    const int it = S->it;
+    const int stop = (it >= S->maxiter) ? 2 : ((sqrt(rr) < S->tol) ? 1 : 0);
+    if (stop) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->rr = rr;
+            S->done = stop;
+        }
+        return;
+    }
+    const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n_el;
+         e += (long long)gridDim.x * blockDim.x) {
+        int d[6];
+        V xh[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) d[j] = el_dofs[j * (long long)n_el + e];
+        V a[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a[k] = (V)Mr[k * (long long)n_el + e];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const V2 w = zp[d[j] & 0x7fffffff];
+            const V pj = w.x + beta * w.y;
+            xh[j] = d[j] < 0 ? -pj : pj;
+        }
+        const V u0 = xh[1] - xh[2] + xh[5];
+        const V u1 = xh[2] - xh[0] - xh[4];
+        const V u2 = xh[0] - xh[1] + xh[3];
+        const V m0 = a[0] * u0 + a[1] * u1 + a[2] * u2;
+        const V m1 = a[1] * u0 + a[3] * u1 + a[4] * u2;
+        const V m2 = a[2] * u0 + a[4] * u1 + a[5] * u2;
+        const V yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
+#pragma unroll
+        for (int j = 0; j < 6; ++j) yloc[j * (long long)n_el + e] = d[j] < 0 ? -yh[j] : yh[j];
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kRowBlock) void cg_edge_kernel(
+    int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
+    const V *__restrict__ yloc, const unsigned char *__restrict__ mask,
+    const V *__restrict__ wold, V *__restrict__ wnew, V *__restrict__ q,
+    const double *__restrict__ partB, int nbB, double *__restrict__ partA,
+    CgState *__restrict__ st, int parity) {
+    using V2 = typename Pair<V>::T;
+    const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);
+    __shared__ double lds[kRowBlock / kWave];
+    CgState *S = st;
+    const int was_done = S->done;   // set by A1 when the solve stops here
+    const double rho = reduce_range<kRowBlock>(partB, 0, nbB, 2, 0, lds);
+    const double rr = reduce_range<kRowBlock>(partB, 0, nbB, 2, 1, lds);
+    if (was_done) return;
+    const int it = S->it;
+    const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    double pq = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const bool free_row = mask ? mask[i] != 0 : true;
+        const int k0 = dof_ptr[i], k1 = dof_ptr[i + 1];
+        const V2 w = zp[i];
+        V acc = 0;
+        if (free_row)
+            for (int k = k0; k < k1; ++k) acc += yloc[dof_contrib[k]];
+        const V pi = free_row ? w.x + beta * w.y : (V)0;
+        const V qi = free_row ? acc : (V)0;
+        wnew[2 * i + 1] = pi;
+        q[i] = qi;
+        pq += (double)pi * (double)qi;
+    }
+    const double s = block_sum(pq, lds);
+    if (threadIdx.x == 0) {
+        partA[blockIdx.x] = s;
+        if (blockIdx.x == 0) {
+            S->rho[parity] = rho;
+            S->rr = rr;
+        }
+    }
+}
+
+// c_j . Mr c_j for local edge j (Mr upper triangle 00 01 02 11 12 22)
+__device__ __forceinline__ double elem_diag(const float *__restrict__ Mr, long long n_el,
+                                            long long e, int j) {
+    const auto m = [&](int k) { return (double)Mr[k * n_el + e]; };
+    switch (j) {
+        case 0: return m(3) + m(5) - 2.0 * m(4);
+        case 1: return m(0) + m(5) - 2.0 * m(2);
+        case 2: return m(0) + m(3) - 2.0 * m(1);
+        case 3: return m(5);
+        case 4: return m(3);
+        default: return m(0);
+    }
+}
+
+// init of the element operator's solve (one segment, start from 0): r = b,
+// the diagonal from the element matrices, dinv, z = dinv r, p_old = 0, and
+// the partials of cg_init_kernel
+template <typename V>
+__global__ __launch_bounds__(kVecBlock) void cg_init_elem_kernel(
+    int n, int n_el, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
+    const float *__restrict__ Mr, const unsigned char *__restrict__ mask,
+    const double *__restrict__ b, V *__restrict__ r, V *__restrict__ w, V *__restrict__ w_other,
+    V *__restrict__ dinv, float *__restrict__ xs, double *__restrict__ partB,
+    double *__restrict__ partbb) {
+    __shared__ double lds[kVecBlock / kWave];
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const bool free_row = mask ? mask[i] != 0 : true;
+        V ri = 0, di = 0;
+        if (free_row) {
+            double dg = 0.0;
+            for (int k = dof_ptr[i]; k < dof_ptr[i + 1]; ++k) {
+                const int slot = dof_contrib[k];
+                const int j = slot / n_el;
+                dg += elem_diag(Mr, n_el, slot - (long long)j * n_el, j);
+            }
+            const double bi = b[i];
+            bb += bi * bi;
+            ri = (V)bi;
+            di = dg != 0.0 ? (V)(1.0 / dg) : (V)0;
+        }
+        const V zi = di * ri;
+        r[i] = ri;
+        w[2 * i] = zi;
+        w[2 * i + 1] = 0;
+        w_other[2 * i] = 0;
+        w_other[2 * i + 1] = 0;
+        dinv[i] = di;
+        if (xs) xs[i] = 0;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    const double sbb = block_sum(bb, lds);
+    const double srz = block_sum(rz, lds);
+    const double srr = block_sum(rr, lds);
+    if (threadIdx.x == 0) {
+        partB[2 * blockIdx.x] = srz;
+        partB[2 * blockIdx.x + 1] = srr;
+        partbb[blockIdx.x] = sbb;
+    }
+}
+
+// float64 product with the element operator (the arithmetic of A1 on x,
+// masked: fixed DOFs carry 0), element products into work[j*n_el + e]
+__global__ void elem_apply_f64_kernel(int n_el, const int *__restrict__ gids,
+                                      const int *__restrict__ orient,
+                                      const double *__restrict__ Mr,
+                                      const unsigned char *__restrict__ mask,
+                                      const double *__restrict__ x, double *__restrict__ work) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    const int o = orient[e];
+    double xh[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int g = gids[6 * e + j];
+        const double v = (mask && !mask[g]) ? 0.0 : x[g];
+        xh[j] = ((o >> j) & 1) ? -v : v;
+    }
+    double a[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a[k] = Mr[k * (long long)n_el + e];
+    const double u0 = xh[1] - xh[2] + xh[5];
+    const double u1 = xh[2] - xh[0] - xh[4];
+    const double u2 = xh[0] - xh[1] + xh[3];
+    const double m0 = a[0] * u0 + a[1] * u1 + a[2] * u2;
+    const double m1 = a[1] * u0 + a[3] * u1 + a[4] * u2;
+    const double m2 = a[2] * u0 + a[4] * u1 + a[5] * u2;
+    const double yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) work[j * (long long)n_el + e] = ((o >> j) & 1) ? -yh[j] : yh[j];
+}
+
+__global__ void elem_gather_f64_kernel(int n, const int *__restrict__ dof_ptr,
+                                       const int *__restrict__ dof_contrib,
+                                       const unsigned char *__restrict__ mask,
+                                       const double *__restrict__ work, double *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    if (!mask || mask[i])
+        for (int k = dof_ptr[i]; k < dof_ptr[i + 1]; ++k) s += work[dof_contrib[k]];
+    y[i] = s;
+}
+
+// signed DOF ids of the element operator, slot j*n_el + e: id | bit 31 when
+// the edge's orientation is flipped in element e
+__global__ void el_dofs_kernel(int n_el, const int *__restrict__ gids,
+                               const int *__restrict__ orient, int *__restrict__ el_dofs) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    const int o = orient[e];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int g = gids[6 * e + j];
+        el_dofs[j * (long long)n_el + e] = ((o >> j) & 1) ? (int)(g | 0x80000000u) : g;
+    }
+}
+
 // ---------------------------------------------------------------- kernel B
 // JAC: z = dinv r here (Jacobi); otherwise z and the r.z partials come from
 // the auxiliary-space apply that follows, and only r.r is written.
@@ -447,6 +674,13 @@ struct sn_cg {
     short *dcol = nullptr;       // [nnz] 16-bit column offsets j - i, when they are exact
     bool use_dcol = true;        // SpMV reads dcol (when present) instead of the indices
     sn_ams *ams = nullptr;       // auxiliary-space preconditioner (null: Jacobi)
+    // element operator (sn_cg_set_element_operator): solves take the element
+    // matrices Mr instead of CSR values
+    bool el_mode = false;
+    int n_el = 0;
+    int grid_el = 0;
+    int *el_dofs = nullptr;      // [6*n_el] signed DOF ids, slot j*n_el + e
+    void *yloc = nullptr;        // [6*n_el] element products
     CgState *st = nullptr;       // [nseg]
     CgState *st_host = nullptr;  // [nseg], pinned
     // cached graph of one chunk (2*chunk kernels), keyed by its pointers
@@ -516,10 +750,10 @@ static void launch_spmv_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     }
 }
 
-template <typename V, typename M>
-static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask, void *xdst,
-                             int parity, hipStream_t s) {
-    launch_spmv_lpr<V, M>(C, vals, mask, parity, s);
+// kernel B and the preconditioner, after the product of an iteration
+template <typename V>
+static void launch_update_precond(sn_cg *C, const unsigned char *mask, void *xdst, int parity,
+                                  hipStream_t s) {
     V *wnew = (V *)(parity ? C->w0 : C->w1);
 #define SN_LAUNCH_UPD(X, JAC)                                                                 \
     hipLaunchKernelGGL((cg_update_kernel<V, X, JAC>), dim3(C->grid_vec), dim3(kVecBlock), 0, s, \
@@ -537,6 +771,27 @@ static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask,
         else SN_LAUNCH_UPD(float, true);
     }
 #undef SN_LAUNCH_UPD
+}
+
+template <typename V, typename M>
+static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask, void *xdst,
+                             int parity, hipStream_t s) {
+    launch_spmv_lpr<V, M>(C, vals, mask, parity, s);
+    launch_update_precond<V>(C, mask, xdst, parity, s);
+}
+
+// the two kernels of the element operator's product (float32 workspaces)
+static void launch_elem_product(sn_cg *C, const float *Mr, const unsigned char *mask, int parity,
+                                hipStream_t s) {
+    const sn_pattern *P = C->pat;
+    const float *wold = (const float *)(parity ? C->w1 : C->w0);
+    float *wnew = (float *)(parity ? C->w0 : C->w1);
+    hipLaunchKernelGGL(cg_elem_kernel<float>, dim3(C->grid_el), dim3(kRowBlock), 0, s, C->n_el,
+                       C->el_dofs, Mr, wold, (float *)C->yloc, C->partB, C->grid_vec, C->st,
+                       parity);
+    hipLaunchKernelGGL(cg_edge_kernel<float>, dim3(C->grid_rows), dim3(kRowBlock), 0, s, P->n,
+                       P->dof_ptr, P->dof_contrib, (const float *)C->yloc, mask, wold, wnew,
+                       (float *)C->q, C->partB, C->grid_vec, C->partA, C->st, parity);
 }
 
 template <typename V, typename M, int LPR>
@@ -566,11 +821,30 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     }
 }
 
+static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mask,
+                             const double *b, double atol, int maxiter, hipStream_t s) {
+    const sn_pattern *P = C->pat;
+    hipLaunchKernelGGL(cg_init_elem_kernel<float>, dim3(C->grid_vec), dim3(kVecBlock), 0, s, P->n,
+                       C->n_el, P->dof_ptr, P->dof_contrib, Mr, mask, b, (float *)C->r,
+                       (float *)C->w0, (float *)C->w1, (float *)C->dinv, C->xs, C->partB,
+                       C->partbb);
+    hipLaunchKernelGGL(cg_init_state_kernel, dim3(1), dim3(kVecBlock), 0, s, C->map(), C->partbb,
+                       C->st, C->rtol_dev, atol, maxiter);
+    if (C->ams) {
+        const AmsSegments seg{C->row0, C->blkB0, C->segB};
+        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, C->w0, 2, C->partB, 2,
+                               C->grid_vec, &seg, nullptr, s);
+    }
+}
+
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
                          hipStream_t s) {
     // an even number of iterations keeps the p buffers and rho slots in step
     for (int k = 0; k < C->chunk; ++k) {
-        if (C->precision == 64 && !C->mat32)
+        if (C->el_mode) {
+            launch_elem_product(C, (const float *)vals, mask, k & 1, s);
+            launch_update_precond<float>(C, mask, xdst, k & 1, s);
+        } else if (C->precision == 64 && !C->mat32)
             launch_iteration<double, double>(C, (const double *)vals, mask, xdst, k & 1, s);
         else if (C->precision == 64)
             launch_iteration<double, float>(C, (const float *)vals, mask, xdst, k & 1, s);
@@ -626,7 +900,9 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
                           hipMemcpyHostToDevice, s));
     void *xdst = (C->precision == 64) ? (void *)x : (void *)C->xs;
     const double *x0 = (C->precision == 64) ? x : nullptr;
-    if (C->precision == 64 && !C->mat32)
+    if (C->el_mode)
+        launch_init_elem(C, (const float *)vals, mask, b, atol, maxiter, s);
+    else if (C->precision == 64 && !C->mat32)
         launch_init_lpr<double, double>(C, (const double *)vals, mask, b, x0, atol, maxiter, s);
     else if (C->precision == 64)
         launch_init_lpr<double, float>(C, (const float *)vals, mask, b, x0, atol, maxiter, s);
@@ -795,12 +1071,69 @@ int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
     return SN_OK;
 }
 
+int sn_cg_set_element_operator(sn_cg *C, const int *gids, const int *orient, int enable,
+                               void *stream) {
+    SN_REQUIRE(C, "sn_cg_set_element_operator: null workspace");
+    if (enable) {
+        SN_REQUIRE(gids && orient, "sn_cg_set_element_operator: null element data");
+        SN_REQUIRE(C->precision == 32 && C->mat32,
+                   "sn_cg_set_element_operator: needs a float32 workspace (precision 32)");
+        SN_REQUIRE(C->nseg == 1, "sn_cg_set_element_operator: one segment only");
+        SN_REQUIRE(C->pat->dof_ptr && C->pat->dof_contrib && C->pat->num_el > 0,
+                   "sn_cg_set_element_operator: pattern without element contribution lists");
+        SN_REQUIRE((long long)C->pat->num_el * 6 < (1LL << 31),
+                   "sn_cg_set_element_operator: too many elements");
+    }
+    if (C->graph) {
+        SN_HIP(hipGraphExecDestroy(C->graph));
+        C->graph = nullptr;
+    }
+    if (!enable) {
+        C->el_mode = false;
+        return SN_OK;
+    }
+    hipStream_t s = as_stream(stream);
+    const int ne = C->pat->num_el;
+    if (C->n_el != ne || !C->el_dofs) {
+        if (C->el_dofs) (void)hipFree(C->el_dofs);
+        if (C->yloc) (void)hipFree(C->yloc);
+        C->el_dofs = nullptr;
+        C->yloc = nullptr;
+        SN_HIP(hipMalloc(&C->el_dofs, (size_t)6 * ne * sizeof(int)));
+        SN_HIP(hipMalloc(&C->yloc, (size_t)6 * ne * sizeof(float)));
+        C->n_el = ne;
+    }
+    hipLaunchKernelGGL(el_dofs_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, s, ne, gids, orient,
+                       C->el_dofs);
+    SN_CHECK_LAUNCH();
+    SN_HIP(hipStreamSynchronize(s));
+    C->grid_el = grid_for(ne, kRowBlock);
+    C->el_mode = true;
+    return SN_OK;
+}
+
+int sn_cg_element_operator(const sn_cg *C) { return (C && C->el_mode) ? 1 : 0; }
+
+int sn_element_apply_f64(const sn_pattern *P, const int *gids, const int *orient,
+                         const double *Mr, const unsigned char *mask, const double *x,
+                         double *work, double *y, void *stream) {
+    SN_REQUIRE(P && gids && orient && Mr && x && work && y, "sn_element_apply_f64: null argument");
+    SN_REQUIRE(P->dof_ptr && P->dof_contrib, "sn_element_apply_f64: pattern without DOF lists");
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(elem_apply_f64_kernel, dim3(grid_for(P->num_el, 256)), dim3(256), 0, s,
+                       P->num_el, gids, orient, Mr, mask, x, work);
+    hipLaunchKernelGGL(elem_gather_f64_kernel, dim3(grid_for(P->n, 256)), dim3(256), 0, s, P->n,
+                       P->dof_ptr, P->dof_contrib, mask, work, y);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
 void sn_cg_destroy(sn_cg *C) {
     if (!C) return;
     if (C->graph) (void)hipGraphExecDestroy(C->graph);
     void *bufs[] = {C->r,    C->w0,       C->w1, C->q,     C->dinv,  C->xs,
                     C->partA, C->partB,   C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0,
-                    C->blkB0, C->segA,    C->segB, C->dcol};
+                    C->blkB0, C->segA,    C->segB, C->dcol, C->el_dofs, C->yloc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     (void)hipHostFree(C->st_host);
@@ -880,7 +1213,9 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     // parity 1 reads {z, p} of w1 (p zero) and rho[0], writes p into w0 and rho[1]: the state stays
     // the same for every repetition
     for (int r = 0; r < reps; ++r) {
-        if (C->precision == 64 && !C->mat32)
+        if (C->el_mode)
+            launch_elem_product(C, (const float *)vals, mask, 1, s);
+        else if (C->precision == 64 && !C->mat32)
             launch_spmv_lpr<double, double>(C, (const double *)vals, mask, 1, s);
         else if (C->precision == 64)
             launch_spmv_lpr<double, float>(C, (const float *)vals, mask, 1, s);

@@ -143,34 +143,41 @@ class RedMVPSolver:
 
     def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
                      eta_min=None, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True,
-                     log=None, precond='jacobi'):
+                     log=None, precond='jacobi', tight='element', jacobian=None):
         """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
 
         Returns the device solution (float64) and fills ``self.stats`` with the
-        residual history, per-step CG iteration counts and timings."""
-        drv = self.newton_driver(precision, lanes_per_row, chunk, precond)
+        residual history, per-step CG iteration counts and timings.
+        ``jacobian`` and ``tight`` (mixed precision): the operator of the
+        float32 CG, "element" (default) or "csr", and with "csr" the solver of
+        the steps too tight for it, "element" or "fp64" (NewtonDriver)."""
+        drv = self.newton_driver(precision, lanes_per_row, chunk, precond, tight, jacobian)
         drv.retarget(self.source_fields_device())
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
                          stats=self.stats, log=log)
 
-    def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32, precond='jacobi'):
+    def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32, precond='jacobi',
+                      tight='element', jacobian=None):
+        if jacobian is None:
+            jacobian = 'element' if precision == 'mixed' else 'csr'
         if lanes_per_row == 0:
-            lanes_per_row = self.tuned_lanes(precision)
-        key = ('drv', precision, lanes_per_row, chunk, precond)
+            lanes_per_row = self.tuned_lanes(precision) if jacobian == 'csr' else 8
+        key = ('drv', precision, lanes_per_row, chunk, precond, tight, jacobian)
         if key not in self.__dict__.setdefault('_drivers', {}):
             self._drivers[key] = NewtonDriver(self.curl_curl_factory, self.source_fields_device(),
                                               self.mask_device(), self.quad_order, precision,
-                                              lanes_per_row, chunk, precond)
+                                              lanes_per_row, chunk, precond, tight, jacobian)
         return self._drivers[key]
 
 
 def _tuned_lanes(solver, precision):
+    """Lanes per CSR row of the SpMV, timed on the Jacobian at x = 0."""
     import torch
     from .autotune import tune_lanes
     asm = solver.curl_curl_factory
     d = asm._device_state()
-    drv = solver.newton_driver(precision, 8, 32)
+    drv = solver.newton_driver(precision, 8, 32, jacobian='csr')
     drv.retarget(solver.source_fields_device())
     drv.residual(torch.zeros(asm.num_dofs, dtype=torch.float64, device=device()))
     best, times = tune_lanes(d['pattern'], asm.num_dofs, d['nnz'], drv.J, drv.mask,
@@ -192,10 +199,15 @@ def _absmax(t):
 class _CG:
     """Owning wrapper of an sn_cg workspace."""
 
-    def __init__(self, asm, precision, lanes_per_row=8, chunk=32):
+    def __init__(self, asm, precision, lanes_per_row=8, chunk=32, element_operator=False):
         self.h = ctypes.c_void_p()
         nat.check(nat.load().sn_cg_create(ctypes.byref(self.h), asm.pattern, int(precision),
                                           int(lanes_per_row), int(chunk), nat.stream_handle()))
+        if element_operator:
+            # solves take the element matrices of sn_element_newton mode 2
+            d = asm._device_state()
+            nat.check(nat.load().sn_cg_set_element_operator(
+                self.h, nat.ptr(d['gids']), nat.ptr(d['orient']), 1, nat.stream_handle()))
 
     def solve(self, vals, mask, b, x, rtol, atol, maxiter=-1, allow_cap=True):
         it = ctypes.c_int()
@@ -227,16 +239,36 @@ class NewtonDriver:
     precond "jacobi": the reference's diagonal preconditioner; "ams": the
     auxiliary-space preconditioner (ams.py), refreshed with the element
     reluctivities of every Jacobian.
+    jacobian (mixed only): "element" (default): the Jacobian of every step as
+    float32 element matrices in reference-curl form and the float32 CG on the
+    element operator (sn_cg_set_element_operator), which keeps the gradient
+    null space of J exact under rounding; "csr": the float32 CSR values of the
+    assembled J (a rounding that breaks the null space and can stall the CG
+    at tight forcing terms), with ``tight`` choosing the solver of the steps it
+    cannot reach: "element", or "fp64" (float64 Jacobian and CG).  A float32
+    step that stalls at its iteration cap is redone in float64.
     """
 
     def __init__(self, asm, src_dev, mask_dev, quad_order, precision='fp64', lanes_per_row=8,
-                 chunk=32, precond='jacobi'):
+                 chunk=32, precond='jacobi', tight='element', jacobian=None):
         import torch
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be "fp64" or "mixed"')
         if precond not in ('jacobi', 'ams'):
             raise ValueError('precond must be "jacobi" or "ams"')
+        if tight not in ('element', 'fp64'):
+            raise ValueError('tight must be "element" or "fp64"')
+        if jacobian is None:
+            jacobian = 'element' if precision == 'mixed' else 'csr'
+        if jacobian not in ('csr', 'element'):
+            raise ValueError('jacobian must be "csr" or "element"')
+        if jacobian == 'element' and precision != 'mixed':
+            raise ValueError('the element operator is a float32 operator (precision "mixed")')
         self.precond = precond
+        self.tight = tight
+        self.jacobian = jacobian
+        self.el_all = jacobian == 'element'
+        self.lanes_per_row, self.chunk = lanes_per_row, chunk
         self.asm, self.src, self.mask, self.quad_order = asm, src_dev, mask_dev, quad_order
         self.precision = precision
         self.f32 = precision == 'mixed'
@@ -245,17 +277,24 @@ class NewtonDriver:
         N, n, nnz = d['num_el'], asm.num_dofs, d['nnz']
         jt = torch.float32 if self.f32 else torch.float64
         dev = device()
-        self.locJ = torch.empty(21 * N, dtype=jt, device=dev)
         self.loc6 = torch.empty(6 * N, dtype=torch.float64, device=dev)
-        self.J = torch.empty(nnz, dtype=jt, device=dev)
         self.R = torch.empty(n, dtype=torch.float64, device=dev)
         self.b = torch.empty(n, dtype=torch.float64, device=dev)
         self.dx = torch.empty(n, dtype=torch.float64, device=dev)
-        self.cg = _CG(asm, 32 if self.f32 else 64, lanes_per_row, chunk)
-        # float64 Jacobian and CG for the steps where the float32 CG stalls:
-        # float32 rounding of J breaks its exact gradient null space, which
-        # can stall CG on large meshes at tight forcing terms
-        self.cg_safe = _CG(asm, 64, lanes_per_row, chunk) if self.f32 else None
+        # float32 CG on the element operator: every step with jacobian
+        # "element", the tight steps of "csr" (built on first use there)
+        self.cg_el = None
+        self.Mr = None
+        if self.el_all:
+            self.locJ = self.J = self.cg = None
+            self.Mr = torch.empty(6 * N, dtype=torch.float32, device=dev)
+            self.cg_el = _CG(asm, 32, lanes_per_row, chunk, element_operator=True)
+        else:
+            self.locJ = torch.empty(21 * N, dtype=jt, device=dev)
+            self.J = torch.empty(nnz, dtype=jt, device=dev)
+            self.cg = _CG(asm, 32 if self.f32 else 64, lanes_per_row, chunk)
+        # float64 Jacobian and CG of the mixed solver's fallback (built on first use)
+        self.cg_safe = None
         self._J64 = None
         self.eta_fp64 = 0.0   # steps with eta <= this go straight to float64
         self._r0 = None
@@ -263,9 +302,10 @@ class NewtonDriver:
         self.ams = None
         self.ams_options = {}   # omega, weight, theta, coarse_max of AmsPreconditioner
         if precond == 'ams' and self.f32:
-            # with the auxiliary-space preconditioner the float32 CG reaches
-            # a few 1e-3 before the rounding of J stalls it; tighter forcing
-            # terms go to float64 from the start
+            # with the auxiliary-space preconditioner the float32 CG on the
+            # rounded CSR matrix reaches a few 1e-3 before the rounding of J
+            # stalls it; tighter forcing terms go to the tight solver from
+            # the start
             self.eta_fp64 = 4.9e-3
 
     def _refresh_preconditioner(self):
@@ -277,7 +317,7 @@ class NewtonDriver:
             from .ams import AmsPreconditioner
             free = self.mask.cpu().numpy().astype(bool)
             self.ams = AmsPreconditioner(self.asm, free, self.nu_el, **self.ams_options)
-            for cg in (self.cg, self.cg_safe):
+            for cg in (self.cg, self.cg_safe, self.cg_el):
                 if cg is not None:
                     nat.check(nat.load().sn_cg_set_ams(cg.h, self.ams.handle))
         else:
@@ -295,10 +335,14 @@ class NewtonDriver:
     def residual(self, x_dev):
         """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2."""
         lib = nat.load()
-        self.asm.newton_local_device(x_dev, self.quad_order, self.src, 1, self.f32, self.locJ,
-                                     self.loc6, nu_el=self.nu_el)
-        nat.check(lib.sn_csr_gather_one(self.d['pattern'], int(self.f32), nat.ptr(self.locJ),
-                                        nat.ptr(self.J), nat.stream_handle()))
+        if self.el_all:
+            self.asm.newton_local_device(x_dev, self.quad_order, self.src, 2, True, self.Mr,
+                                         self.loc6, nu_el=self.nu_el)
+        else:
+            self.asm.newton_local_device(x_dev, self.quad_order, self.src, 1, self.f32,
+                                         self.locJ, self.loc6, nu_el=self.nu_el)
+            nat.check(lib.sn_csr_gather_one(self.d['pattern'], int(self.f32), nat.ptr(self.locJ),
+                                            nat.ptr(self.J), nat.stream_handle()))
         nat.check(lib.sn_dof_gather_f64(self.d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
                                         nat.stream_handle()))
         out = ctypes.c_double()
@@ -316,6 +360,9 @@ class NewtonDriver:
             self._locJ64 = torch.empty(21 * N, dtype=torch.float64, device=device())
             self._J64 = torch.empty(self.d['nnz'], dtype=torch.float64, device=device())
             self._loc6b = torch.empty_like(self.loc6)
+            self.cg_safe = _CG(self.asm, 64, self.lanes_per_row, self.chunk)
+            if self.ams is not None:
+                nat.check(lib.sn_cg_set_ams(self.cg_safe.h, self.ams.handle))
         self.asm.newton_local_device(x, self.quad_order, self.src, 1, False, self._locJ64,
                                      self._loc6b)
         nat.check(lib.sn_csr_gather_one(self.d['pattern'], 0, nat.ptr(self._locJ64),
@@ -324,6 +371,28 @@ class NewtonDriver:
         its, _, _ = self.cg_safe.solve(self._J64, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
                                        maxiter=max_cg)
         return its, x + self.dx
+
+    def _element_step(self, x, eta, maxiter):
+        """One Newton step of the mixed solver on the element operator: the
+        Jacobian at x as float32 element matrices (sn_element_newton mode 2;
+        residual() has formed them already with jacobian "element") and the
+        float32 CG multiplying with them.  Returns (iterations, status, x + dx)."""
+        import torch
+        if self.cg_el is None:
+            N = self.d['num_el']
+            self.Mr = torch.empty(6 * N, dtype=torch.float32, device=device())
+            self.cg_el = _CG(self.asm, 32, self.lanes_per_row, self.chunk, element_operator=True)
+            if self.ams is not None:
+                nat.check(nat.load().sn_cg_set_ams(self.cg_el.h, self.ams.handle))
+        if not self.el_all:
+            if getattr(self, '_loc6c', None) is None:
+                self._loc6c = torch.empty_like(self.loc6)
+            self.asm.newton_local_device(x, self.quad_order, self.src, 2, True, self.Mr,
+                                         self._loc6c)
+        x_try = x.clone()
+        its, _, rc = self.cg_el.solve(self.Mr, self.mask, self.b, x_try, rtol=eta, atol=0.0,
+                                      maxiter=maxiter)
+        return its, rc, x_try
 
     def r0(self):
         """||R(0)|| (the norm of the source term), cached."""
@@ -341,9 +410,10 @@ class NewtonDriver:
         reduce it (the floating-point floor of the residual).  ``max_cg`` < 0 caps each
         linear solve at min(10 n, 50000) iterations; a float32 solve is cut at
         4x the previous step's count (at least 1500, 300 with the auxiliary-
-        space preconditioner) and, if it stalls there,
-        the step is redone in float64 (Jacobian and CG), as are later steps
-        with a forcing term at least as small."""
+        space preconditioner) and, if it stalls there, the step is redone by
+        the tight solver (float32 on the element operator, then float64 if
+        that stalls too), as are later steps with a forcing term at least as
+        small."""
         import torch
         stats = stats if stats is not None else {}
         if eta_min is None:
@@ -374,20 +444,35 @@ class NewtonDriver:
             torch.neg(self.R, out=self.b)
             self._refresh_preconditioner()
             mode = 'fp64'
-            if self.f32 and eta > self.eta_fp64:
+            if self.f32:
                 floor = 300 if self.precond == 'ams' else 1500
-                cap32 = max(floor, 4 * max(cg_its[-1] if cg_its else 0, 1))
-                x_try = x.clone()
-                its, _, rc = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta, atol=0.0,
-                                           maxiter=min(cap32, max_cg))
-                mode = 'fp32'
-                if rc == nat.SN_ERR_NOT_CONVERGED:
+                cap32 = min(max(floor, 4 * max(cg_its[-1] if cg_its else 0, 1)), max_cg)
+                its, tight, mode = 0, True, ''
+                if self.el_all:
+                    its, rc, x_try = self._element_step(x, eta, cap32)
+                    mode = 'fp32 element'
+                    tight = rc == nat.SN_ERR_NOT_CONVERGED
+                    if tight:
+                        mode += ' stalled -> '
+                elif eta > self.eta_fp64:
+                    x_try = x.clone()
+                    its, _, rc = self.cg.solve(self.J, self.mask, self.b, x_try, rtol=eta,
+                                               atol=0.0, maxiter=cap32)
+                    tight = rc == nat.SN_ERR_NOT_CONVERGED
+                    mode = 'fp32 stalled -> ' if tight else 'fp32'
+                    if tight:
+                        self.eta_fp64 = max(self.eta_fp64, eta)
+                if tight and self.tight == 'element' and not self.el_all:
+                    more, rc, x_try = self._element_step(x, eta, cap32)
+                    its += more
+                    mode += 'fp32 element'
+                    tight = rc == nat.SN_ERR_NOT_CONVERGED
+                    if tight:
+                        mode += ' stalled -> '
+                if tight:
                     more, x_try = self._fp64_step(x, eta, max_cg)
                     its += more
-                    self.eta_fp64 = max(self.eta_fp64, eta)
-                    mode = 'fp32 stalled -> fp64'
-            elif self.f32:
-                its, x_try = self._fp64_step(x, eta, max_cg)
+                    mode += 'fp64'
             else:
                 self.dx.zero_()
                 its, _, _ = self.cg.solve(self.J, self.mask, self.b, self.dx, rtol=eta, atol=0.0,
