@@ -272,49 +272,55 @@ def main():
                                                probe_B(bd, bs, xr, probes))[0])
         log('batch B error (max over variants): %.2e' % batch_err)
 
-    # roofline of the dominant kernel: the CG SpMV kernel of the mixed solve,
-    # 200 back-to-back launches on the solver's stream between two HIP events
-    # (inside a solve the same kernel runs with a colder cache; the kernel
-    # trace summary in profiles/ gives both figures)
+    # roofline of the dominant operation: the CG's product with the element
+    # operator (cg_elem_kernel, then cg_edge_kernel), 200 back-to-back
+    # products on the solver's stream between two HIP events (the kernel
+    # trace summary in profiles/ gives the in-solve durations too)
     drv = solver.newton_driver('mixed', args.lanes, 32, args.precond)
+    lib = nat.load()
     ms = ctypes.c_double()
-    nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
-                                         ctypes.byref(ms), nat.stream_handle()))
+    nat.check(lib.sn_cg_time_spmv(drv.cg_el.h, nat.ptr(drv.Mr), nat.ptr(drv.mask), 200,
+                                  ctypes.byref(ms), nat.stream_handle()))
     n, nnz = asm.num_dofs, asm.nnz
-    s = 4  # float32 values and vectors
-    ib = nat.load().sn_cg_index_bytes(drv.cg.h)   # 2: 16-bit column offsets, 4: int32
-    spmv_bytes = nnz * (s + ib) + 4 * (n + 1) + 4 * s * n + n
-    spmv_gbs = spmv_bytes / (ms.value * 1e-3) / 1e9
-    ms32 = ctypes.c_double(ms.value)
-    if ib == 2:
-        # the same launches on int32 column indices, for comparison
-        nat.check(nat.load().sn_cg_set_col_offsets(drv.cg.h, 0))
-        nat.check(nat.load().sn_cg_time_spmv(drv.cg.h, nat.ptr(drv.J), nat.ptr(drv.mask), 200,
-                                             ctypes.byref(ms32), nat.stream_handle()))
-        nat.check(nat.load().sn_cg_set_col_offsets(drv.cg.h, 1))
-    # element kernel (mixed, residual mode) timed with torch events on its stream
+    N = dp.num_tets
+    # per element: 6 signed DOF ids, 6 matrix values, 6 products written
+    # (cg_elem_kernel) and read back with their 6 list entries (cg_edge_kernel);
+    # per DOF: the {z, p_old} pair, DOF range, mask, p and q written
+    op_bytes = 4 * 6 * N * 5 + (8 + 4 + 1 + 8 + 4 + 4) * n + 4
+    op_gbs = op_bytes / (ms.value * 1e-3) / 1e9
+    # the CSR SpMV of the same Jacobian rounded after assembly (jacobian
+    # "csr"), for comparison
+    drv_csr = solver.newton_driver('mixed', args.lanes, 32, args.precond, jacobian='csr')
+    drv_csr.retarget(solver.source_fields_device())
+    drv_csr.residual(xm)
+    ms_csr = ctypes.c_double()
+    nat.check(lib.sn_cg_time_spmv(drv_csr.cg.h, nat.ptr(drv_csr.J), nat.ptr(drv_csr.mask), 200,
+                                  ctypes.byref(ms_csr), nat.stream_handle()))
+    ib = lib.sn_cg_index_bytes(drv_csr.cg.h)   # 2: 16-bit column offsets, 4: int32
+    csr_bytes = nnz * (4 + ib) + 4 * (n + 1) + 4 * 4 * n + n
+    # element kernel (residual + element matrices, mode 2) timed with HIP
+    # events on its stream
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record()
     reps = 10
     for _ in range(reps):
-        asm.newton_local_device(xm, 2, solver.source_fields_device(), 1, True, drv.locJ, drv.loc6)
+        asm.newton_local_device(xm, 2, solver.source_fields_device(), 2, True, drv.Mr, drv.loc6)
     ev1.record()
     torch.cuda.synchronize()
     asm_ms = ev0.elapsed_time(ev1) / reps
-    N = dp.num_tets
     n_iron = int(np.sum(dp.mesh.tags == 1))
     asm_bytes = 16 * N + 24 * N + 4 * N + 24 * asm.tet_mesh.num_nodes + 8 * n + \
-        4 * 3 * 8 * n_iron + 21 * 4 * N + 6 * 8 * N
+        4 * 3 * 8 * n_iron + 6 * 4 * N + 6 * 8 * N
 
-    log('roofline: spmv %.4f ms (%.0f GB/s), element kernel %.3f ms' % (ms.value, spmv_gbs, asm_ms))
+    log('roofline: element operator product %.4f ms (%.0f GB/s), csr spmv %.4f ms, '
+        'element kernel %.3f ms' % (ms.value, op_gbs, ms_csr.value, asm_ms))
     traffic = None
-    tfile = os.path.join(ROOT, 'profiles', 'cg_spmv_f32_dipole1M_traffic.json')
-    # (PMC counts of the same kernel at this mesh and lane count; see profiles/)
+    tfile = os.path.join(ROOT, 'profiles', 'cg_element_product_dipole1M_traffic.json')
+    # (PMC counts of the two kernels at this mesh; see profiles/)
     if os.path.exists(tfile):
         tr = json.load(open(tfile))
-        if (tr.get('tets') == N and tr.get('lanes_per_row') == args.lanes
-                and tr.get('index_bytes', 4) == ib):
-            traffic = tr['hbm_bytes_per_launch']
+        if tr.get('tets') == N:
+            traffic = tr['hbm_bytes_per_product']
     cpu = None
     if not args.no_cpu:
         log('cpu baseline leg')
@@ -350,7 +356,8 @@ def main():
         'config': {'workload': workload,
                    'tets': N, 'edges': n, 'nnz': nnz, 'iron_fraction': dp.iron_fraction(),
                    'batch': args.batch, 'streams': args.streams,
-                   'precision': 'mixed (FP32 Jacobian/SpMV/CG, FP64 residual and dots)',
+                   'precision': 'mixed (FP32 element-operator Jacobian and CG, FP64 residual '
+                                'and dots)',
                    'preconditioner': args.precond,
                    'warm_start': warm is not None, 'parallelism': 'single GPU'},
         'single_solve_per_s': 1.0 / t_mixed,
@@ -374,12 +381,15 @@ def main():
         'cg_iterations': stm['cg_iterations'],
         'newton_steps_fp64': st64['newton_steps'],
         'cg_iterations_fp64': st64['cg_iterations'],
-        'roofline': {'bound': 'hbm', 'achieved': spmv_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': spmv_gbs / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': 'cg_spmv_kernel<float,float,%d,%s>'
+        'roofline': {'bound': 'hbm', 'achieved': op_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': op_gbs / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': 'cg_elem_kernel<float> + cg_edge_kernel<float> '
+                               '(element operator product)',
+                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': op_bytes},
+        'csr_spmv': {'kernel': 'cg_spmv_kernel<float,float,%d,%s>'
                                % (args.lanes, 'short' if ib == 2 else 'int'),
-                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': spmv_bytes,
-                     'index_bytes': ib, 'avg_ms_int32_indices': ms32.value},
+                     'avg_ms': ms_csr.value, 'algorithmic_bytes': csr_bytes,
+                     'GB/s': csr_bytes / (ms_csr.value * 1e-3) / 1e9},
         'assembly_kernel': {'avg_ms': asm_ms, 'algorithmic_bytes': asm_bytes,
                             'GB/s': asm_bytes / (asm_ms * 1e-3) / 1e9},
         'cpu_baseline': cpu,

@@ -254,7 +254,8 @@ int sn_cg_solve(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev,
  * to the summation order of p.q): bit 0 = each workgroup takes one
  * contiguous span of rows (else workgroups stride over the rows), bit 1 =
  * workgroups renumbered so each of the 8 XCDs takes one contiguous eighth of
- * them.  Default 0. */
+ * them (also the element operator's two kernels).  Default 0; enabling the
+ * element operator sets 2. */
 int sn_cg_set_schedule(sn_cg *cg, int schedule);
 
 /* Column storage of the CG SpMV (no counterpart in the reference, whose
