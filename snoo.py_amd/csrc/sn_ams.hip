@@ -70,6 +70,20 @@ __device__ __forceinline__ void f4_reduce(float4 &a) {
     }
 }
 
+// Work index of block b of nb.  xcd: workgroups are dealt round-robin to the
+// 8 XCDs; renumber them so that each XCD takes one contiguous eighth of the
+// grid (a multiple of 8 blocks), whose gathers of one mesh neighbourhood then
+// meet in its L2.
+__device__ __forceinline__ long long ams_block(int xcd, int b, int nb) {
+    if (xcd && (nb & 7) == 0) return (long long)(b & 7) * (nb >> 3) + (b >> 3);
+    return b;
+}
+
+inline int ams_grid(long long threads, int block, int xcd) {
+    const int g = grid_for(threads, block);
+    return xcd ? (g + 7) / 8 * 8 : g;
+}
+
 // ------------------------------------------------------------ numeric set-up
 // P1 stiffness of each element with its reluctivity: 10 pair values
 // (a <= b, row-major upper triangle) nu_e vol_e grad(l_a).grad(l_b).
@@ -209,8 +223,9 @@ __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     int n_free, const int *__restrict__ inc_ptr, const int2 *__restrict__ inc,
     const float4 *__restrict__ xyz, const V *__restrict__ r,
     const float *__restrict__ dinv0, float omega, float4 *__restrict__ b0,
-    float4 *__restrict__ x0, const int *__restrict__ done) {
-    const int a = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    float4 *__restrict__ x0, const int *__restrict__ done, int xcd) {
+    const int a =
+        (int)((ams_block(xcd, blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LPR);
     const int lane = threadIdx.x % LPR;
     // the node's incidence range and coordinates and the stop flag:
     // independent loads, issued together
@@ -239,8 +254,9 @@ __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
     int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ agg_kp,
     const int *__restrict__ indices, const float *__restrict__ vals, const float4 *__restrict__ b,
     const float4 *__restrict__ x, const float *__restrict__ dinv_c, float omega,
-    float4 *__restrict__ bc, float4 *__restrict__ xc, const int *__restrict__ done) {
-    const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    float4 *__restrict__ bc, float4 *__restrict__ xc, const int *__restrict__ done, int xcd) {
+    const int wid =
+        (int)((ams_block(xcd, blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     if (wid >= n_agg) return;  // uniform over the wave
     // the aggregate's rows and nonzero range (agg_kp = indptr at agg_ptr) and
@@ -293,8 +309,9 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
     const float *__restrict__ vals, const float *__restrict__ dinv, const float4 *__restrict__ b,
     const float4 *__restrict__ x, float omega, const int *__restrict__ agg_ptr,
-    float4 *__restrict__ x_f, float4 *__restrict__ y, const int *__restrict__ done) {
-    const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+    float4 *__restrict__ x_f, float4 *__restrict__ y, const int *__restrict__ done, int xcd) {
+    const int row =
+        (int)((ams_block(xcd, blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LPR);
     const int lane = threadIdx.x % LPR;
     // the row's range, its own values and the stop flag: independent loads
     const bool in = row < n;
@@ -328,7 +345,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     V *__restrict__ z, int zs, double *__restrict__ partials, int stride,
     const int *__restrict__ row0,
     const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
-    const int *__restrict__ done) {
+    const int *__restrict__ done, int xcd) {
     __shared__ double lds[kAmsEdgeBlock / kWave];
     if (done && *done) return;
     long long e0 = 0, e1 = n_edges;
@@ -340,6 +357,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
         lb = blockIdx.x - blk0[seg];
         nbl = blk0[seg + 1] - blk0[seg];
     }
+    lb = (int)ams_block(xcd, lb, nbl);
     double rz = 0.0;
     for (long long i = e0 + (long long)lb * blockDim.x + threadIdx.x; i < e1;
          i += (long long)nbl * blockDim.x) {
@@ -366,21 +384,21 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
 template <typename V>
 static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
                            int zs, double *partials, int stride, int grid,
-                           const AmsSegments *seg, const int *done, hipStream_t s) {
+                           const AmsSegments *seg, const int *done, int xcd, hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
     constexpr int kIncLanes = 8;
     hipLaunchKernelGGL((ams_edge_restrict_kernel<V, kIncLanes>),
-                       dim3(grid_for((long long)A->n_free * kIncLanes, kAmsBlock)),
+                       dim3(ams_grid((long long)A->n_free * kIncLanes, kAmsBlock, xcd)),
                        dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc, A->xyz, r, l0.dinv,
-                       A->omega, l0.b, l0.x, done);
+                       A->omega, l0.b, l0.x, done, xcd);
     const int waves_per_block = kAmsBlock / kWave;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
-        hipLaunchKernelGGL(ams_down_kernel, dim3(grid_for(c.n, waves_per_block)),
+        hipLaunchKernelGGL(ams_down_kernel, dim3(ams_grid(c.n, waves_per_block, xcd)),
                            dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, c.agg_kp, f.indices, f.vals,
                            f.b, f.x, c.dinv, A->omega, c.b,
-                           (l + 2 < L) ? c.x : (float4 *)nullptr, done);
+                           (l + 2 < L) ? c.x : (float4 *)nullptr, done, xcd);
     }
     AmsLevel &lc = A->lv[L - 1];
     hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)), dim3(kAmsBlock),
@@ -388,29 +406,30 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
                        L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, A->yx, done);
     for (int l = L - 2; l >= 0; --l) {
         AmsLevel &f = A->lv[l];
-        hipLaunchKernelGGL((ams_up_kernel<8>), dim3(grid_for((long long)f.n * 8, kAmsBlock)),
+        hipLaunchKernelGGL((ams_up_kernel<8>), dim3(ams_grid((long long)f.n * 8, kAmsBlock, xcd)),
                            dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv, f.b,
                            f.x, A->omega, f.agg_ptr, l > 0 ? A->lv[l - 1].x : (float4 *)nullptr,
-                           A->yx, done);
+                           A->yx, done, xcd);
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->yx,
                        A->weight, z, zs, partials, stride, seg ? seg->row0 : nullptr,
-                       seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done);
+                       seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done, xcd);
 }
 
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
                      const unsigned char *mask, void *z, int zstride, double *partials,
                      int stride, int grid, const AmsSegments *seg, const int *done,
-                     hipStream_t s) {
+                     int sched, hipStream_t s) {
     SN_REQUIRE(A && A->ready, "AMS preconditioner used before sn_ams_update");
     SN_REQUIRE(zstride >= 1, "ams_launch_apply: zstride must be >= 1");
+    const int xcd = (sched & 2) ? 1 : 0;
     if (v64)
         launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
-                               zstride, partials, stride, grid, seg, done, s);
+                               zstride, partials, stride, grid, seg, done, xcd, s);
     else
         launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
-                              zstride, partials, stride, grid, seg, done, s);
+                              zstride, partials, stride, grid, seg, done, xcd, s);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -610,7 +629,7 @@ int sn_ams_apply(sn_ams *A, const unsigned char *mask, const double *r, const do
     SN_REQUIRE(A && r && dinv && z, "sn_ams_apply: null argument");
     return ams_launch_apply(A, true, r, dinv, mask, z, 1, nullptr, 0,
                             grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr, nullptr,
-                            as_stream(stream));
+                            0, as_stream(stream));
 }
 
 int sn_ams_level_info(const sn_ams *A, int level, int *n, int *nnz) {

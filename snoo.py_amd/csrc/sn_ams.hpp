@@ -59,10 +59,11 @@ struct AmsSegments {
 // partials[stride*b], b < grid (blocks of 1024 threads striding over the
 // edges, or over the rows of their segment when `seg` is given).  `done`
 // (may be null) points to a device flag that turns every kernel into a
-// no-op when set.
+// no-op when set.  sched bit 1: XCD-grouped workgroups (as the CG's
+// schedule knob).
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
                      const unsigned char *mask, void *z, int zstride, double *partials,
-                     int stride, int grid, const AmsSegments *seg, const int *done,
+                     int stride, int grid, const AmsSegments *seg, const int *done, int sched,
                      hipStream_t s);
 
 }  // namespace sn

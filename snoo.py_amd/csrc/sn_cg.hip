@@ -781,7 +781,7 @@ static void launch_update_precond(sn_cg *C, const unsigned char *mask, void *xds
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, wnew, 2,
                                C->partB, 2, C->grid_vec, &seg,
-                               C->nseg == 1 ? C->done_flag() : nullptr, s);
+                               C->nseg == 1 ? C->done_flag() : nullptr, C->sched, s);
     } else {
         if (C->precision == 64) SN_LAUNCH_UPD(double, true);
         else SN_LAUNCH_UPD(float, true);
@@ -833,7 +833,7 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->w0, 2,
-                               C->partB, 2, C->grid_vec, &seg, nullptr, s);
+                               C->partB, 2, C->grid_vec, &seg, nullptr, C->sched, s);
     }
 }
 
@@ -849,7 +849,7 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, C->w0, 2, C->partB, 2,
-                               C->grid_vec, &seg, nullptr, s);
+                               C->grid_vec, &seg, nullptr, C->sched, s);
     }
 }
 
