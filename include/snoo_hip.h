@@ -273,7 +273,8 @@ int sn_cg_index_bytes(const sn_cg *cg);
 /* Average duration (ms) of the CG SpMV kernel (kernel A: p = z + beta p,
  * q = A p, p.q) over `reps` back-to-back launches on `stream`, timed with HIP
  * events; run after a solve on the same workspace (it overwrites the CG
- * scratch vectors). */
+ * scratch vectors).  With the element operator: its two kernels, the product
+ * with the CG update of x and r. */
 int sn_cg_time_spmv(sn_cg *cg, const void *vals_dev, const unsigned char *mask_dev, int reps,
                     double *avg_ms_host, void *stream);
 

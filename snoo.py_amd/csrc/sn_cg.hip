@@ -257,16 +257,22 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
 // float32 CG on the rounded CSR matrix stalls near a relative residual of
 // 1e-2, this one converges like the float64 CG.
 //
+// Two kernels per iteration do the work of kernels A and B:
 // A1 (thread per element): kernel A's stop test; p = z + beta p_old at the
 //    6 DOFs; u = c (S p), the circulations of the three faces at local node 0;
-//    m = Mr u; the element products y_j = s_j c_j . m at slot j*n_el + e.
-// A2 (thread per DOF): q_i = sum of DOF i's element products in ascending
-//    element order (the pattern's DOF contribution lists); p_i written to the
-//    other {z, p} array; per-block partials of p.q, in kernel A's layout.
-// Measured on the 1M-tet dipole (in-solve averages): A1 15.9 us, A2 19.1 us
-// with these planes of slots j*n_el + e.  Storing the 6 products of an
-// element side by side (6e + j) made A2 35 us; storing m (16 bytes per
-// element) and forming c_j . m in A2 made it 25 us.
+//    m = Mr u; the element products y_j = s_j c_j . m at slot j*n_el + e; and
+//    per-block partials of p.q = p^T A p = sum_e u . m (the element energies:
+//    no pass over q is needed for it).
+// A2 (thread per DOF, kernel B's grid): alpha = rho / p.q from A1's
+//    partials; q_i = sum of DOF i's element products in ascending element
+//    order (the pattern's DOF contribution lists); p_i = z_i + beta p_old_i
+//    written to the other {z, p} array; x_i += alpha p_i; r_i -= alpha q_i;
+//    (Jacobi: z_i = dinv_i r_i); per-block partials of r.r (and r.z) in
+//    kernel B's layout.  q itself is never stored.
+// Measured on the 1M-tet dipole: with the products in planes of slots
+// j*n_el + e the DOF pass took 19 us (as a product, before the update moved
+// into it); storing an element's 6 products side by side (6e + j) made it
+// 35 us, storing m (16 bytes per element) and forming c_j . m there 25 us.
 // Work index of this block.  Schedule bit 1: workgroups are dealt
 // round-robin to the 8 XCDs, so renumber them to give each XCD one
 // contiguous eighth of the grid (needs a multiple of 8 blocks).
@@ -280,7 +286,7 @@ template <typename V>
 __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
     int n_el, const int *__restrict__ el_dofs, const float *__restrict__ Mr,
     const V *__restrict__ wold, V *__restrict__ yloc, const double *__restrict__ partB,
-    int nbB, CgState *__restrict__ st, int parity, int sched) {
+    int nbB, double *__restrict__ partE, CgState *__restrict__ st, int parity, int sched) {
     using V2 = typename Pair<V>::T;
     const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);
     __shared__ double lds[kRowBlock / kWave];
@@ -300,6 +306,7 @@ __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
         return;
     }
     const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    double pq = 0.0;
     for (long long e = lb * blockDim.x + threadIdx.x; e < n_el;
          e += (long long)gridDim.x * blockDim.x) {
         int d[6];
@@ -321,51 +328,82 @@ __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
         const V m0 = a[0] * u0 + a[1] * u1 + a[2] * u2;
         const V m1 = a[1] * u0 + a[3] * u1 + a[4] * u2;
         const V m2 = a[2] * u0 + a[4] * u1 + a[5] * u2;
+        pq += (double)u0 * (double)m0 + (double)u1 * (double)m1 + (double)u2 * (double)m2;
         const V yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
 #pragma unroll
         for (int j = 0; j < 6; ++j) yloc[j * (long long)n_el + e] = d[j] < 0 ? -yh[j] : yh[j];
     }
+    const double s = block_sum(pq, lds);
+    if (threadIdx.x == 0) {
+        partE[blockIdx.x] = s;
+        if (blockIdx.x == 0) {
+            S->rho[parity] = rho;
+            S->rr = rr;
+        }
+    }
 }
 
-template <typename V>
-__global__ __launch_bounds__(kRowBlock) void cg_edge_kernel(
+template <typename V, typename X, bool JAC>
+__global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
     int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const V *__restrict__ yloc, const unsigned char *__restrict__ mask,
-    const V *__restrict__ wold, V *__restrict__ wnew, V *__restrict__ q,
-    const double *__restrict__ partB, int nbB, double *__restrict__ partA,
-    CgState *__restrict__ st, int parity, int sched) {
+    const V *__restrict__ dinv, const V *__restrict__ wold, V *__restrict__ wnew,
+    V *__restrict__ r, X *__restrict__ x, const double *__restrict__ partE, int nbE,
+    double *__restrict__ partB, CgState *__restrict__ st, int parity, int sched) {
     using V2 = typename Pair<V>::T;
     const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);
-    __shared__ double lds[kRowBlock / kWave];
+    __shared__ double lds[kVecBlock / kWave];
     CgState *S = st;
     const int was_done = S->done;   // set by A1 when the solve stops here
     const long long lb = xcd_block(sched);
-    const double rho = reduce_range<kRowBlock>(partB, 0, nbB, 2, 0, lds);
-    const double rr = reduce_range<kRowBlock>(partB, 0, nbB, 2, 1, lds);
+    const double pq = reduce_range<kVecBlock>(partE, 0, nbE, 1, 0, lds);
     if (was_done) return;
+    if (!(pq == pq) || pq == 0.0 || isinf(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S->pq = pq;
+            S->done = 3;
+        }
+        return;
+    }
     const int it = S->it;
+    const double rho = S->rho[parity];   // written by A1
     const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
-    double pq = 0.0;
+    const V alpha = (V)(rho / pq);
+    double rz = 0.0, rr = 0.0;
     for (long long i = lb * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const bool free_row = mask ? mask[i] != 0 : true;
         const int k0 = dof_ptr[i], k1 = dof_ptr[i + 1];
         const V2 w = zp[i];
-        V acc = 0;
-        if (free_row)
-            for (int k = k0; k < k1; ++k) acc += yloc[dof_contrib[k]];
-        const V pi = free_row ? w.x + beta * w.y : (V)0;
-        const V qi = free_row ? acc : (V)0;
-        wnew[2 * i + 1] = pi;
-        q[i] = qi;
-        pq += (double)pi * (double)qi;
+        if (!free_row) {
+            // p (and, with Jacobi, z) of a fixed row stays 0
+            if (JAC) wnew[2 * i] = (V)0;
+            wnew[2 * i + 1] = (V)0;
+            continue;
+        }
+        V qi = 0;
+        for (int k = k0; k < k1; ++k) qi += yloc[dof_contrib[k]];
+        const V pi = w.x + beta * w.y;
+        x[i] += (X)(alpha * pi);
+        const V ri = r[i] - alpha * qi;
+        r[i] = ri;
+        rr += (double)ri * (double)ri;
+        if (JAC) {
+            const V zi = dinv[i] * ri;
+            reinterpret_cast<V2 *>(wnew)[i] = V2{zi, pi};
+            rz += (double)ri * (double)zi;
+        } else {
+            wnew[2 * i + 1] = pi;
+        }
     }
-    const double s = block_sum(pq, lds);
+    const double s1 = JAC ? block_sum(rz, lds) : 0.0;
+    const double s2 = block_sum(rr, lds);
     if (threadIdx.x == 0) {
-        partA[blockIdx.x] = s;
+        if (JAC) partB[2 * blockIdx.x] = s1;
+        partB[2 * blockIdx.x + 1] = s2;
         if (blockIdx.x == 0) {
-            S->rho[parity] = rho;
-            S->rr = rr;
+            S->pq = pq;
+            S->it = it + 1;
         }
     }
 }
@@ -796,18 +834,39 @@ static void launch_iteration(sn_cg *C, const M *vals, const unsigned char *mask,
     launch_update_precond<V>(C, mask, xdst, parity, s);
 }
 
-// the two kernels of the element operator's product (float32 workspaces)
-static void launch_elem_product(sn_cg *C, const float *Mr, const unsigned char *mask, int parity,
-                                hipStream_t s) {
+// A1 and A2 of one iteration on the element operator (float32 workspaces):
+// the product with the CG update (and the Jacobi z when there is no
+// auxiliary-space preconditioner).  A1's p.q partials go to partA
+// (grid_el <= grid_rows blocks).
+static void launch_elem_product(sn_cg *C, const float *Mr, const unsigned char *mask, void *xdst,
+                                int parity, hipStream_t s) {
     const sn_pattern *P = C->pat;
     const float *wold = (const float *)(parity ? C->w1 : C->w0);
     float *wnew = (float *)(parity ? C->w0 : C->w1);
     hipLaunchKernelGGL(cg_elem_kernel<float>, dim3(C->grid_el), dim3(kRowBlock), 0, s, C->n_el,
-                       C->el_dofs, Mr, wold, C->yloc, C->partB, C->grid_vec, C->st, parity,
-                       C->sched);
-    hipLaunchKernelGGL(cg_edge_kernel<float>, dim3(C->grid_rows), dim3(kRowBlock), 0, s, P->n,
-                       P->dof_ptr, P->dof_contrib, C->yloc, mask, wold, wnew, (float *)C->q,
-                       C->partB, C->grid_vec, C->partA, C->st, parity, C->sched);
+                       C->el_dofs, Mr, wold, C->yloc, C->partB, C->grid_vec, C->partA, C->st,
+                       parity, C->sched);
+#define SN_LAUNCH_EDGE(JAC)                                                                    \
+    hipLaunchKernelGGL((cg_edge_kernel<float, float, JAC>), dim3(C->grid_vec), dim3(kVecBlock), \
+                       0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask,                  \
+                       (const float *)C->dinv, wold, wnew, (float *)C->r, (float *)xdst,       \
+                       C->partA, C->grid_el, C->partB, C->st, parity, C->sched)
+    if (C->ams) SN_LAUNCH_EDGE(false);
+    else SN_LAUNCH_EDGE(true);
+#undef SN_LAUNCH_EDGE
+}
+
+// one iteration on the element operator: A1, A2, then the auxiliary-space
+// preconditioner (z and the r.z partials)
+static void launch_elem_iteration(sn_cg *C, const float *Mr, const unsigned char *mask,
+                                  void *xdst, int parity, hipStream_t s) {
+    launch_elem_product(C, Mr, mask, xdst, parity, s);
+    if (C->ams) {
+        float *wnew = (float *)(parity ? C->w0 : C->w1);
+        const AmsSegments seg{C->row0, C->blkB0, C->segB};
+        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, wnew, 2, C->partB, 2,
+                               C->grid_vec, &seg, C->done_flag(), C->sched, s);
+    }
 }
 
 template <typename V, typename M, int LPR>
@@ -857,10 +916,9 @@ static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, 
                          hipStream_t s) {
     // an even number of iterations keeps the p buffers and rho slots in step
     for (int k = 0; k < C->chunk; ++k) {
-        if (C->el_mode) {
-            launch_elem_product(C, (const float *)vals, mask, k & 1, s);
-            launch_update_precond<float>(C, mask, xdst, k & 1, s);
-        } else if (C->precision == 64 && !C->mat32)
+        if (C->el_mode)
+            launch_elem_iteration(C, (const float *)vals, mask, xdst, k & 1, s);
+        else if (C->precision == 64 && !C->mat32)
             launch_iteration<double, double>(C, (const double *)vals, mask, xdst, k & 1, s);
         else if (C->precision == 64)
             launch_iteration<double, float>(C, (const float *)vals, mask, xdst, k & 1, s);
@@ -1123,7 +1181,11 @@ int sn_cg_set_element_operator(sn_cg *C, const int *gids, const int *orient, int
                        C->el_dofs);
     SN_CHECK_LAUNCH();
     SN_HIP(hipStreamSynchronize(s));
-    C->grid_el = (grid_for(ne, kRowBlock) + 7) / 8 * 8;   // a multiple of the 8 XCDs
+    // A1 strides over the elements with at most grid_rows blocks (its p.q
+    // partials go to partA), a multiple of the 8 XCDs
+    C->grid_el = grid_for(ne, kRowBlock);
+    if (C->grid_el > C->grid_rows) C->grid_el = C->grid_rows;
+    if (C->grid_el >= 8) C->grid_el = C->grid_el / 8 * 8;
     // XCD-grouped blocks: the element and DOF ranges of one XCD are
     // neighbours in the mesh, so their shared gathers meet in its L2
     // (1M-tet dipole: 43.3 -> 41.0 us per product, back to back)
@@ -1233,8 +1295,8 @@ int sn_cg_time_spmv(sn_cg *C, const void *vals, const unsigned char *mask, int r
     // parity 1 reads {z, p} of w1 (p zero) and rho[0], writes p into w0 and rho[1]: the state stays
     // the same for every repetition
     for (int r = 0; r < reps; ++r) {
-        if (C->el_mode)
-            launch_elem_product(C, (const float *)vals, mask, 1, s);
+        if (C->el_mode)   // A1 and A2, whose update moves x and r each time
+            launch_elem_product(C, (const float *)vals, mask, C->xs, 1, s);
         else if (C->precision == 64 && !C->mat32)
             launch_spmv_lpr<double, double>(C, (const double *)vals, mask, 1, s);
         else if (C->precision == 64)
