@@ -273,9 +273,9 @@ def main():
         log('batch B error (max over variants): %.2e' % batch_err)
 
     # roofline of the dominant operation: the CG's product with the element
-    # operator (cg_elem_kernel, then cg_edge_kernel), 200 back-to-back
-    # products on the solver's stream between two HIP events (the kernel
-    # trace summary in profiles/ gives the in-solve durations too)
+    # operator and the CG update (cg_elem_kernel, then cg_edge_kernel), 200
+    # back-to-back launch pairs on the solver's stream between two HIP events
+    # (the kernel trace summary in profiles/ gives the in-solve durations too)
     drv = solver.newton_driver('mixed', args.lanes, 32, args.precond)
     lib = nat.load()
     ms = ctypes.c_double()
@@ -285,8 +285,9 @@ def main():
     N = dp.num_tets
     # per element: 6 signed DOF ids, 6 matrix values, 6 products written
     # (cg_elem_kernel) and read back with their 6 list entries (cg_edge_kernel);
-    # per DOF: the {z, p_old} pair, DOF range, mask, p and q written
-    op_bytes = 4 * 6 * N * 5 + (8 + 4 + 1 + 8 + 4 + 4) * n + 4
+    # per DOF: the {z, p_old} pair in each kernel, DOF range, mask, p written,
+    # r and x read and written
+    op_bytes = 4 * 6 * N * 5 + (8 + 8 + 4 + 1 + 4 + 8 + 8) * n + 4
     op_gbs = op_bytes / (ms.value * 1e-3) / 1e9
     # the CSR SpMV of the same Jacobian rounded after assembly (jacobian
     # "csr"), for comparison
@@ -383,8 +384,8 @@ def main():
         'cg_iterations_fp64': st64['cg_iterations'],
         'roofline': {'bound': 'hbm', 'achieved': op_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': op_gbs / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': 'cg_elem_kernel<float> + cg_edge_kernel<float> '
-                               '(element operator product)',
+                     'kernel': 'cg_elem_kernel<float> + cg_edge_kernel<float,float,false> '
+                               '(element operator product with the CG update)',
                      'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': op_bytes},
         'csr_spmv': {'kernel': 'cg_spmv_kernel<float,float,%d,%s>'
                                % (args.lanes, 'short' if ib == 2 else 'int'),

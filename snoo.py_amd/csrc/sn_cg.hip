@@ -33,6 +33,7 @@ namespace sn {
 struct CgState {
     double rho[2];   // rho of even / odd iterations (written by the segment's first A block)
     double pq, rr, tol, bnorm;
+    double beta;     // element operator: this iteration's beta, written by A1 for A2
     int it, maxiter, done, pad;  // done: 0 running, 1 converged, 2 iteration cap, 3 breakdown
 };
 
@@ -305,7 +306,8 @@ __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
         }
         return;
     }
-    const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    const double beta_d = it > 0 ? rho / S->rho[parity ^ 1] : 0.0;
+    const V beta = (V)beta_d;
     double pq = 0.0;
     for (long long e = lb * blockDim.x + threadIdx.x; e < n_el;
          e += (long long)gridDim.x * blockDim.x) {
@@ -339,6 +341,7 @@ __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
         if (blockIdx.x == 0) {
             S->rho[parity] = rho;
             S->rr = rr;
+            S->beta = beta_d;
         }
     }
 }
@@ -365,9 +368,11 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
         }
         return;
     }
-    const int it = S->it;
-    const double rho = S->rho[parity];   // written by A1
-    const V beta = (V)(it > 0 ? rho / S->rho[parity ^ 1] : 0.0);
+    // rho and beta as A1 formed them.  Not S->it: block 0 advances it at its
+    // end, and a block that starts later (other streams can hold the CUs)
+    // would read the advanced count
+    const double rho = S->rho[parity];
+    const V beta = (V)S->beta;
     const V alpha = (V)(rho / pq);
     double rz = 0.0, rr = 0.0;
     for (long long i = lb * blockDim.x + threadIdx.x; i < n;
@@ -403,7 +408,7 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
         partB[2 * blockIdx.x + 1] = s2;
         if (blockIdx.x == 0) {
             S->pq = pq;
-            S->it = it + 1;
+            S->it = S->it + 1;   // no other block writes it
         }
     }
 }

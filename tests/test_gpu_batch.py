@@ -69,16 +69,21 @@ def test_batched_newton_matches_single_newton(variants, precision, precond):
         assert np.abs(Bb - Bs).max() <= 1e-6 * np.abs(Bs).max()
 
 
-def test_concurrent_streams_match_single_solves(variants):
-    """Variants solved on two streams at once give the solutions of the same
-    solves run one at a time (every solve is deterministic)."""
+@pytest.mark.parametrize('workers', [2, 3])
+def test_concurrent_streams_match_single_solves(variants, workers):
+    """Variants solved on several streams at once give the solutions of the
+    same solves run one at a time (every solve is deterministic).  With the
+    element operator's CG, concurrent streams delay some workgroups of a
+    kernel behind others: a kernel that read a counter its first workgroup
+    advances broke down there (regression)."""
     from snoopy_amd.batch import ConcurrentSolver
-    cs = ConcurrentSolver(variants, workers=2, precision='mixed', precond='ams')
+    cs = ConcurrentSolver(variants, workers=workers, precision='mixed', precond='ams')
     res = cs.solve(rtol=1e-8)
     for v, d in enumerate(variants):
         assert res[v][1]['converged']
         s = d.solver(quad_order=2, verbose=False)
         xv = s.solve_newton('mixed', rtol=1e-8, precond='ams')
+        assert res[v][1]['cg_iterations'] == s.stats['cg_iterations']
         asm = s.curl_curl_factory
         Bc = asm.element_B(res[v][0].cpu().numpy())
         Bs = asm.element_B(xv.cpu().numpy())
