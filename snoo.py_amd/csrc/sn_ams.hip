@@ -162,8 +162,20 @@ __global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
                                   float *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_out) return;
+    // four list entries, then their four values, in flight at once; the sum
+    // keeps the list order
     double s = 0.0;
-    for (int k = gptr[i]; k < gptr[i + 1]; ++k) s += (double)in[glist[k]];
+    int k = gptr[i];
+    const int k1 = gptr[i + 1];
+    for (; k + 4 <= k1; k += 4) {
+        const int g0 = glist[k], g1 = glist[k + 1], g2 = glist[k + 2], g3 = glist[k + 3];
+        const float v0 = in[g0], v1 = in[g1], v2 = in[g2], v3 = in[g3];
+        s += (double)v0;
+        s += (double)v1;
+        s += (double)v2;
+        s += (double)v3;
+    }
+    for (; k < k1; ++k) s += (double)in[glist[k]];
     out[i] = (float)s;
 }
 

@@ -297,8 +297,21 @@ __global__ void dof_gather_kernel(int n, const int *__restrict__ dof_ptr,
                                   const double *__restrict__ loc6, double *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // four list entries, then their four values, in flight at once; the sum
+    // keeps the list order
     double s = 0.0;
-    for (int k = dof_ptr[i]; k < dof_ptr[i + 1]; ++k) s += loc6[dof_contrib[k]];
+    int k = dof_ptr[i];
+    const int k1 = dof_ptr[i + 1];
+    for (; k + 4 <= k1; k += 4) {
+        const int c0 = dof_contrib[k], c1 = dof_contrib[k + 1];
+        const int c2 = dof_contrib[k + 2], c3 = dof_contrib[k + 3];
+        const double v0 = loc6[c0], v1 = loc6[c1], v2 = loc6[c2], v3 = loc6[c3];
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
+    }
+    for (; k < k1; ++k) s += loc6[dof_contrib[k]];
     out[i] = s;
 }
 

@@ -386,8 +386,20 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
             wnew[2 * i + 1] = (V)0;
             continue;
         }
+        // four list entries, then their four products, in flight at once;
+        // the sum keeps the list order
         V qi = 0;
-        for (int k = k0; k < k1; ++k) qi += yloc[dof_contrib[k]];
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            const int c0 = dof_contrib[k], c1 = dof_contrib[k + 1];
+            const int c2 = dof_contrib[k + 2], c3 = dof_contrib[k + 3];
+            const V y0 = yloc[c0], y1 = yloc[c1], y2 = yloc[c2], y3 = yloc[c3];
+            qi += y0;
+            qi += y1;
+            qi += y2;
+            qi += y3;
+        }
+        for (; k < k1; ++k) qi += yloc[dof_contrib[k]];
         const V pi = w.x + beta * w.y;
         x[i] += (X)(alpha * pi);
         const V ri = r[i] - alpha * qi;
