@@ -191,36 +191,66 @@ __global__ void ams_dinv_kernel(int n, const int *__restrict__ diagpos,
 // A coarsest matrix of up to kAmsLdsCoarse rows is eliminated in LDS (one
 // pivot step costs a few LDS round trips instead of L2 ones); larger ones
 // in the global scratch `a_glob`.  Same arithmetic either way.
-// 80^2 doubles = 50 KB of dynamic LDS, with the 8 KB column buffer within
-// the 64 KB a workgroup may allocate
+// Dynamic LDS: the matrix (when it fits, 80^2 doubles = 50 KB) followed by the
+// staged pivot column and row (2 nc doubles), within the 64 KB a workgroup
+// may allocate
 constexpr int kAmsLdsCoarse = 80;
 
-__global__ __launch_bounds__(1024) void ams_coarse_invert_kernel(
+// One pivot step on entry (i, j), from the old pivot row and column staged
+// in `row` and `col` and pk = 1 / a_kk: the pivot row is scaled, the pivot
+// column becomes -a_ik pk, every other entry loses a_ik (a_kj pk).
+__device__ __forceinline__ double gj_entry(double a_ij, int i, int j, int k, double pk,
+                                           const double *col, const double *row) {
+    if (i == k) return (j == k) ? pk : row[j] * pk;
+    if (j == k) return -col[i] * pk;
+    return a_ij - col[i] * (row[j] * pk);
+}
+
+// launched with 1024 threads: entries per thread of an LDS-resident matrix
+constexpr int kAmsInvThreads = 1024;
+constexpr int kAmsLdsEntries = (kAmsLdsCoarse * kAmsLdsCoarse + kAmsInvThreads - 1) / kAmsInvThreads;
+
+__global__ __launch_bounds__(kAmsInvThreads) void ams_coarse_invert_kernel(
     int nc, const int *__restrict__ indptr, const int *__restrict__ indices,
     const float *__restrict__ vals, double *__restrict__ a_glob, float *__restrict__ cinv) {
-    __shared__ double col[kAmsMaxCoarse];
-    __shared__ double piv;
-    extern __shared__ double a_lds[];
-    double *a = (nc <= kAmsLdsCoarse) ? a_lds : a_glob;
+    extern __shared__ double dyn[];
+    const bool in_lds = nc <= kAmsLdsCoarse;
+    double *a = in_lds ? dyn : a_glob;
+    double *col = dyn + (in_lds ? nc * nc : 0);
+    double *row = col + nc;
     const int t = threadIdx.x, T = blockDim.x;
     for (int k = t; k < nc * nc; k += T) a[k] = 0.0;
     __syncthreads();
     for (int i = t; i < nc; i += T)
         for (int k = indptr[i]; k < indptr[i + 1]; ++k) a[(size_t)i * nc + indices[k]] = vals[k];
     __syncthreads();
+    // a thread's entries of an LDS-resident matrix: (row, column) formed once
+    int ei[kAmsLdsEntries], ej[kAmsLdsEntries];
+#pragma unroll
+    for (int m = 0; m < kAmsLdsEntries; ++m) {
+        const int idx = t + m * T;
+        ei[m] = (in_lds && idx < nc * nc) ? idx / nc : -1;
+        ej[m] = (in_lds && idx < nc * nc) ? idx % nc : 0;
+    }
+    // two barriers per pivot: stage the old pivot row and column, then
+    // update every entry from them
     for (int k = 0; k < nc; ++k) {
-        if (t == 0) piv = 1.0 / a[(size_t)k * nc + k];
-        for (int i = t; i < nc; i += T) col[i] = a[(size_t)i * nc + k];
+        for (int i = t; i < nc; i += T) {
+            col[i] = a[(size_t)i * nc + k];
+            row[i] = a[(size_t)k * nc + i];
+        }
         __syncthreads();
-        const double pk = piv;
-        for (int j = t; j < nc; j += T) a[(size_t)k * nc + j] = (j == k) ? pk : a[(size_t)k * nc + j] * pk;
-        __syncthreads();
-        for (int idx = t; idx < nc * nc; idx += T) {
-            const int i = idx / nc, j = idx % nc;
-            if (i == k) continue;
-            const double f = col[i];
-            if (j == k) a[idx] = -f * pk;
-            else a[idx] -= f * a[(size_t)k * nc + j];
+        const double pk = 1.0 / row[k];
+        if (in_lds) {
+#pragma unroll
+            for (int m = 0; m < kAmsLdsEntries; ++m) {
+                if (ei[m] < 0) continue;
+                const int idx = t + m * T;
+                a[idx] = gj_entry(a[idx], ei[m], ej[m], k, pk, col, row);
+            }
+        } else {
+            for (int idx = t; idx < nc * nc; idx += T)
+                a[idx] = gj_entry(a[idx], idx / nc, idx % nc, k, pk, col, row);
         }
         __syncthreads();
     }
@@ -628,8 +658,10 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
                            v.diagpos, v.vals, v.dinv);
     }
     AmsLevel &c = A->lv[L - 1];
-    const size_t lds = (nc <= kAmsLdsCoarse) ? (size_t)nc * nc * sizeof(double) : 0;
-    hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(1024), lds, s, nc, c.indptr,
+    const size_t lds = ((nc <= kAmsLdsCoarse) ? (size_t)nc * nc + 2 * nc : 2 * (size_t)nc) *
+                       sizeof(double);
+    hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(kAmsInvThreads), lds, s, nc,
+                       c.indptr,
                        c.indices, c.vals, A->dense, A->cinv);
     SN_CHECK_LAUNCH();
     A->ready = true;
