@@ -36,10 +36,18 @@ int num_cus() {
 constexpr int kBsBlock = 256;
 constexpr int kBsTile = 256;
 
+// Per segment, staged in LDS once per tile: start and end point, direction
+// n = end - start, 1/|n|^2, and a sphere around the segment's midpoint that
+// holds every point within the core radius of it, with a margin.  A target
+// outside that sphere is farther than the core radius from the segment and
+// from both end points, so the core test of the reference is false there and
+// is skipped; inside it the test runs as the reference writes it.
+constexpr int kBsSegVals = 14;
+
 __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
     const double *__restrict__ src, int num_src, const double *__restrict__ tar,
     int num_tar, double scale, double rad, double *__restrict__ B, int accumulate) {
-    __shared__ double seg[kBsTile * 6];
+    __shared__ double seg[kBsTile * kBsSegVals];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double tx = 0.0, ty = 0.0, tz = 0.0;
     if (i < num_tar) {
@@ -52,23 +60,42 @@ __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
     for (int base = 0; base < num_src; base += kBsTile) {
         const int cnt = min(kBsTile, num_src - base);
         __syncthreads();
-        for (int k = threadIdx.x; k < cnt * 6; k += blockDim.x) seg[k] = src[6 * base + k];
+        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const double *g = src + 6 * (long long)(base + j);
+            double *s = &seg[kBsSegVals * j];
+            const double x0 = g[0], y0 = g[1], z0 = g[2], x1 = g[3], y1 = g[4], z1 = g[5];
+            const double nx = x1 - x0, ny = y1 - y0, nz = z1 - z0;
+            const double nn = nx * nx + ny * ny + nz * nz;
+            const double half = 0.5 * sqrt(nn);
+            const double R = (half + rad) * (1.0 + 1e-6) + 1e-12;
+            s[0] = x0; s[1] = y0; s[2] = z0;
+            s[3] = x1; s[4] = y1; s[5] = z1;
+            s[6] = nx; s[7] = ny; s[8] = nz;
+            s[9] = nn;
+            s[10] = 0.5 * (x0 + x1); s[11] = 0.5 * (y0 + y1); s[12] = 0.5 * (z0 + z1);
+            s[13] = R * R;
+        }
         __syncthreads();
         if (i < num_tar) {
             for (int j = 0; j < cnt; ++j) {
-                const double *s = &seg[6 * j];
-                const double nx = s[3] - s[0], ny = s[4] - s[1], nz = s[5] - s[2];
-                const double nn = nx * nx + ny * ny + nz * nz;
+                const double *s = &seg[kBsSegVals * j];
+                const double nx = s[6], ny = s[7], nz = s[8];
                 const double d1x = s[0] - tx, d1y = s[1] - ty, d1z = s[2] - tz;
                 const double d2x = s[3] - tx, d2y = s[4] - ty, d2z = s[5] - tz;
                 const double l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
                 const double l2 = sqrt(d2x * d2x + d2y * d2y + d2z * d2z);
-                const double tp = -1.0 * (d1x * nx + d1y * ny + d1z * nz) / nn;
-                const double px = tx - (s[0] + tp * nx);
-                const double py = ty - (s[1] + tp * ny);
-                const double pz = tz - (s[2] + tp * nz);
-                const double dd = px * px + py * py + pz * pz;
-                if ((dd < r2 && tp <= 1.0 && tp >= 0.0) || l1 < rad || l2 < rad) {
+                const double mx = tx - s[10], my = ty - s[11], mz = tz - s[12];
+                bool core = false;
+                double px = 0.0, py = 0.0, pz = 0.0, dd = 0.0;
+                if (mx * mx + my * my + mz * mz <= s[13]) {
+                    const double tp = -1.0 * (d1x * nx + d1y * ny + d1z * nz) / s[9];
+                    px = tx - (s[0] + tp * nx);
+                    py = ty - (s[1] + tp * ny);
+                    pz = tz - (s[2] + tp * nz);
+                    dd = px * px + py * py + pz * pz;
+                    core = (dd < r2 && tp <= 1.0 && tp >= 0.0) || l1 < rad || l2 < rad;
+                }
+                if (core) {
                     // inside the filament core: mu0 I r/(2 pi R^2) along n x p
                     const double bx = ny * pz - nz * py;
                     const double by = nz * px - nx * pz;
@@ -81,7 +108,7 @@ __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
                 } else {
                     // finite straight segment (Russenschuck 5.51)
                     const double d12 = d1x * d2x + d1y * d2y + d1z * d2z;
-                    const double f = (l1 + l2) / (l1 * l2 + d12) / l1 / l2;
+                    const double f = (l1 + l2) / ((l1 * l2 + d12) * l1 * l2);
                     ax += (d1y * d2z - d1z * d2y) * f;
                     ay += (d1z * d2x - d1x * d2z) * f;
                     az += (d1x * d2y - d1y * d2x) * f;
