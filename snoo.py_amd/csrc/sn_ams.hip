@@ -376,6 +376,68 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     }
 }
 
+// The coarse solve and the first up step in one kernel: level f is the one
+// above the coarsest level c (nc rows; b_c formed by ams_down_kernel).  Every
+// block forms, in LDS, e = C^-1 b_c (thread per coarse row) and
+// x_f' = x_f + P e for all nf rows of f (thread per aggregate), then smooths
+// its own rows of f as ams_up_kernel does, from x_f' in LDS:
+// y = x_f' + w D^-1 (b - A x_f'), prolongated into the next finer level's x
+// (or stored as y[2 row] when f is level 0).  The redundant part is nc^2
+// multiply-adds and nf loads per block, all independent of each other; it
+// replaces a launch that costs its latency (about 4.5 us) more than its
+// work.  (Forming b_c here too, wave per aggregate as the down kernel does,
+// made every block walk all aggregates one after another: 39 instead of 32 ms
+// per 1M-tet solve.)
+constexpr int kAmsTailRows = 2048;
+
+template <int LPR>
+__global__ __launch_bounds__(kAmsBlock) void ams_tail_kernel(
+    int nf, int nc, const int *__restrict__ indptr, const int *__restrict__ indices,
+    const float *__restrict__ vals, const float *__restrict__ dinv, const float4 *__restrict__ b,
+    const float4 *__restrict__ x, float omega, const float4 *__restrict__ bc,
+    const int *__restrict__ c_agg_ptr, const float *__restrict__ cinv,
+    const int *__restrict__ agg_ptr, float4 *__restrict__ x_f, float4 *__restrict__ y,
+    const int *__restrict__ done, int xcd) {
+    __shared__ float4 s_bc[kAmsLdsCoarse];
+    __shared__ float4 s_e[kAmsLdsCoarse];
+    __shared__ float4 s_x[kAmsTailRows];
+    if (done && *done) return;   // the same flag for every thread
+    const int t = threadIdx.x;
+    for (int I = t; I < nc; I += blockDim.x) s_bc[I] = bc[I];
+    __syncthreads();
+    for (int I = t; I < nc; I += blockDim.x) {
+        float4 acc = f4_zero();
+        for (int j = 0; j < nc; ++j) f4_fma(acc, cinv[(size_t)I * nc + j], s_bc[j]);
+        s_e[I] = acc;
+    }
+    __syncthreads();
+    for (int I = t; I < nc; I += blockDim.x) {
+        const float4 e = s_e[I];
+        for (int i = c_agg_ptr[I]; i < c_agg_ptr[I + 1]; ++i) s_x[i] = f4_add(x[i], e);
+    }
+    __syncthreads();
+    const int row =
+        (int)((ams_block(xcd, blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / LPR);
+    const int lane = threadIdx.x % LPR;
+    const bool in = row < nf;
+    const int k0 = in ? indptr[row] : 0, k1 = in ? indptr[row + 1] : 0;
+    const float4 xi = in ? s_x[row] : f4_zero(), bi = in ? b[row] : f4_zero();
+    const float di = in ? dinv[row] : 0.f;
+    float4 acc = f4_zero();
+    for (int k = k0 + lane; k < k1; k += LPR) f4_fma(acc, vals[k], s_x[indices[k]]);
+    f4_reduce<LPR>(acc);
+    if (!in) return;
+    const float d = omega * di;
+    const float4 yi = make_float4(xi.x + d * (bi.x - acc.x), xi.y + d * (bi.y - acc.y),
+                                  xi.z + d * (bi.z - acc.z), 0.f);
+    if (x_f) {
+        for (int i = agg_ptr[row] + lane; i < agg_ptr[row + 1]; i += LPR)
+            x_f[i] = f4_add(x_f[i], yi);
+    } else if (lane == 0) {
+        y[2 * row] = yi;
+    }
+}
+
 // Blocks of a batched solve serve one row segment each (seg_of_blk, blk0,
 // row0 as in the CG's vector kernel), so every partial sum of r.z belongs to
 // one segment; without segments the grid strides over all edges.
@@ -435,6 +497,10 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
                        dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc, A->xyz, r, l0.dinv,
                        A->omega, l0.b, l0.x, done, xcd);
     const int waves_per_block = kAmsBlock / kWave;
+    AmsLevel &lc = A->lv[L - 1];
+    // the coarse solve and the first up step in one kernel when the two
+    // lowest levels fit its LDS
+    const bool tail = L >= 2 && lc.n <= kAmsLdsCoarse && A->lv[L - 2].n <= kAmsTailRows;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
         hipLaunchKernelGGL(ams_down_kernel, dim3(ams_grid(c.n, waves_per_block, xcd)),
@@ -442,16 +508,24 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
                            f.b, f.x, c.dinv, A->omega, c.b,
                            (l + 2 < L) ? c.x : (float4 *)nullptr, done, xcd);
     }
-    AmsLevel &lc = A->lv[L - 1];
-    hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)), dim3(kAmsBlock),
-                       0, s, lc.n, A->cinv, lc.b, lc.agg_ptr,
-                       L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, A->yx, done);
+    if (!tail)
+        hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)),
+                           dim3(kAmsBlock), 0, s, lc.n, A->cinv, lc.b, lc.agg_ptr,
+                           L > 1 ? A->lv[L - 2].x : (float4 *)nullptr, A->yx, done);
     for (int l = L - 2; l >= 0; --l) {
         AmsLevel &f = A->lv[l];
+        float4 *x_next = l > 0 ? A->lv[l - 1].x : (float4 *)nullptr;
+        if (tail && l == L - 2) {
+            hipLaunchKernelGGL((ams_tail_kernel<8>),
+                               dim3(ams_grid((long long)f.n * 8, kAmsBlock, xcd)),
+                               dim3(kAmsBlock), 0, s, f.n, lc.n, f.indptr, f.indices, f.vals,
+                               f.dinv, f.b, f.x, A->omega, lc.b, lc.agg_ptr, A->cinv,
+                               f.agg_ptr, x_next, A->yx, done, xcd);
+            continue;
+        }
         hipLaunchKernelGGL((ams_up_kernel<8>), dim3(ams_grid((long long)f.n * 8, kAmsBlock, xcd)),
                            dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv, f.b,
-                           f.x, A->omega, f.agg_ptr, l > 0 ? A->lv[l - 1].x : (float4 *)nullptr,
-                           A->yx, done, xcd);
+                           f.x, A->omega, f.agg_ptr, x_next, A->yx, done, xcd);
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->yx,
