@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "sn_ams.hpp"
@@ -446,7 +447,8 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     int n_edges, int n_free, const unsigned char *__restrict__ mask, const V *__restrict__ r,
     const V *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
     const float4 *__restrict__ yx, float weight,
-    V *__restrict__ z, int zs, double *__restrict__ partials, int stride,
+    V *__restrict__ z, int zs, const V *__restrict__ pin, double *__restrict__ partials,
+    int stride,
     const int *__restrict__ row0,
     const int *__restrict__ blk0, const int *__restrict__ seg_of_blk,
     const int *__restrict__ done, int xcd) {
@@ -479,7 +481,14 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
             zi = dinv[i] * ri + (V)(weight * aux);
             rz += (double)ri * (double)zi;
         }
-        z[i * zs] = zi;
+        if (pin) {
+            // the whole pair {z_i, pin_i}: full lines, no refetch of the
+            // other half
+            using V2 = typename std::conditional<sizeof(V) == 4, float2, double2>::type;
+            reinterpret_cast<V2 *>(z)[i] = V2{zi, pin[i]};
+        } else {
+            z[i * zs] = zi;
+        }
     }
     const double sum = block_sum(rz, lds);
     if (threadIdx.x == 0 && partials) partials[(long long)stride * blockIdx.x] = sum;
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
 
 template <typename V>
 static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
-                           int zs, double *partials, int stride, int grid,
+                           int zs, const V *pin, double *partials, int stride, int grid,
                            const AmsSegments *seg, const int *done, int xcd, hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
@@ -529,23 +538,26 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     }
     hipLaunchKernelGGL((ams_edge_prolong_kernel<V>), dim3(grid), dim3(kAmsEdgeBlock), 0, s,
                        A->n_edges, A->n_free, mask, r, dinv, A->edge_nodes, A->yx,
-                       A->weight, z, zs, partials, stride, seg ? seg->row0 : nullptr,
+                       A->weight, z, zs, pin, partials, stride, seg ? seg->row0 : nullptr,
                        seg ? seg->blk0 : nullptr, seg ? seg->seg_of_blk : nullptr, done, xcd);
 }
 
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
-                     const unsigned char *mask, void *z, int zstride, double *partials,
-                     int stride, int grid, const AmsSegments *seg, const int *done,
-                     int sched, hipStream_t s) {
+                     const unsigned char *mask, void *z, int zstride, const void *pin,
+                     double *partials, int stride, int grid, const AmsSegments *seg,
+                     const int *done, int sched, hipStream_t s) {
     SN_REQUIRE(A && A->ready, "AMS preconditioner used before sn_ams_update");
     SN_REQUIRE(zstride >= 1, "ams_launch_apply: zstride must be >= 1");
+    SN_REQUIRE(!pin || zstride == 2, "ams_launch_apply: pairs need zstride 2");
     const int xcd = (sched & 2) ? 1 : 0;
     if (v64)
         launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
-                               zstride, partials, stride, grid, seg, done, xcd, s);
+                               zstride, (const double *)pin, partials, stride, grid, seg, done,
+                               xcd, s);
     else
         launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
-                              zstride, partials, stride, grid, seg, done, xcd, s);
+                              zstride, (const float *)pin, partials, stride, grid, seg, done,
+                              xcd, s);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -745,7 +757,7 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
 int sn_ams_apply(sn_ams *A, const unsigned char *mask, const double *r, const double *dinv,
                  double *z, void *stream) {
     SN_REQUIRE(A && r && dinv && z, "sn_ams_apply: null argument");
-    return ams_launch_apply(A, true, r, dinv, mask, z, 1, nullptr, 0,
+    return ams_launch_apply(A, true, r, dinv, mask, z, 1, nullptr, nullptr, 0,
                             grid_for(A->n_edges, kAmsEdgeBlock, 4 * num_cus()), nullptr, nullptr,
                             0, as_stream(stream));
 }

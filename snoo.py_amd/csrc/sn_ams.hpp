@@ -57,13 +57,14 @@ struct AmsSegments {
 // z = dinv .* r + weight * Pi V(Pi^T r) on the free edges (0 elsewhere),
 // stored at z[zstride * i], with per-block partial sums of r.z written to
 // partials[stride*b], b < grid (blocks of 1024 threads striding over the
-// edges, or over the rows of their segment when `seg` is given).  `done`
-// (may be null) points to a device flag that turns every kernel into a
-// no-op when set.  sched bit 1: XCD-grouped workgroups (as the CG's
-// schedule knob).
+// edges, or over the rows of their segment when `seg` is given).  With `pin`
+// (zstride 2 only) the pair {z_i, pin[i]} is stored at z[2i], whole lines
+// instead of every other value.  `done` (may be null) points to a device
+// flag that turns every kernel into a no-op when set.  sched bit 1:
+// XCD-grouped workgroups (as the CG's schedule knob).
 int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
-                     const unsigned char *mask, void *z, int zstride, double *partials,
-                     int stride, int grid, const AmsSegments *seg, const int *done, int sched,
-                     hipStream_t s);
+                     const unsigned char *mask, void *z, int zstride, const void *pin,
+                     double *partials, int stride, int grid, const AmsSegments *seg,
+                     const int *done, int sched, hipStream_t s);
 
 }  // namespace sn

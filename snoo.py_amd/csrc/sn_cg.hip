@@ -266,10 +266,12 @@ __global__ __launch_bounds__(kRowBlock) void cg_spmv_kernel(
 //    no pass over q is needed for it).
 // A2 (thread per DOF, kernel B's grid): alpha = rho / p.q from A1's
 //    partials; q_i = sum of DOF i's element products in ascending element
-//    order (the pattern's DOF contribution lists); p_i = z_i + beta p_old_i
-//    written to the other {z, p} array; x_i += alpha p_i; r_i -= alpha q_i;
-//    (Jacobi: z_i = dinv_i r_i); per-block partials of r.r (and r.z) in
-//    kernel B's layout.  q itself is never stored.
+//    order (the pattern's DOF contribution lists); p_i = z_i + beta p_old_i;
+//    x_i += alpha p_i; r_i -= alpha q_i; per-block partials of r.r (and r.z)
+//    in kernel B's layout.  With Jacobi, z_i = dinv_i r_i and the pair
+//    {z_i, p_i} goes to the other {z, p} array; with the auxiliary-space
+//    preconditioner p_i goes to a plain array and the preconditioner's last
+//    kernel writes the pair {z_i, p_i}.  q itself is never stored.
 // Measured on the 1M-tet dipole: with the products in planes of slots
 // j*n_el + e the DOF pass took 19 us (as a product, before the update moved
 // into it); storing an element's 6 products side by side (6e + j) made it
@@ -351,8 +353,9 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
     int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const V *__restrict__ yloc, const unsigned char *__restrict__ mask,
     const V *__restrict__ dinv, const V *__restrict__ wold, V *__restrict__ wnew,
-    V *__restrict__ r, X *__restrict__ x, const double *__restrict__ partE, int nbE,
-    double *__restrict__ partB, CgState *__restrict__ st, int parity, int sched) {
+    V *__restrict__ pout, V *__restrict__ r, X *__restrict__ x,
+    const double *__restrict__ partE, int nbE, double *__restrict__ partB,
+    CgState *__restrict__ st, int parity, int sched) {
     using V2 = typename Pair<V>::T;
     const V2 *__restrict__ zp = reinterpret_cast<const V2 *>(wold);
     __shared__ double lds[kVecBlock / kWave];
@@ -382,8 +385,8 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
         const V2 w = zp[i];
         if (!free_row) {
             // p (and, with Jacobi, z) of a fixed row stays 0
-            if (JAC) wnew[2 * i] = (V)0;
-            wnew[2 * i + 1] = (V)0;
+            if (JAC) reinterpret_cast<V2 *>(wnew)[i] = V2{(V)0, (V)0};
+            else pout[i] = (V)0;
             continue;
         }
         // four list entries, then their four products, in flight at once;
@@ -410,7 +413,10 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
             reinterpret_cast<V2 *>(wnew)[i] = V2{zi, pi};
             rz += (double)ri * (double)zi;
         } else {
-            wnew[2 * i + 1] = pi;
+            // a plain array: the preconditioner writes the whole {z, p} pair
+            // of the next iteration from it (half-pair stores made both
+            // kernels refetch the other half of every line)
+            pout[i] = pi;
         }
     }
     const double s1 = JAC ? block_sum(rz, lds) : 0.0;
@@ -738,6 +744,7 @@ struct sn_cg {
     std::vector<int> row0_h;            // [nseg+1]
     int *row0 = nullptr, *blkA0 = nullptr, *blkB0 = nullptr, *segA = nullptr, *segB = nullptr;
     double *rtol_dev = nullptr;         // [nseg]
+    // q: [n] the product of kernel A (CSR); with the element operator, A2's p
     void *r = nullptr, *q = nullptr, *dinv = nullptr;
     void *w0 = nullptr, *w1 = nullptr;   // interleaved {z, p}: even iterations read w0
     float *xs = nullptr;
@@ -835,7 +842,7 @@ static void launch_update_precond(sn_cg *C, const unsigned char *mask, void *xds
         else SN_LAUNCH_UPD(float, false);
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, wnew, 2,
-                               C->partB, 2, C->grid_vec, &seg,
+                               nullptr, C->partB, 2, C->grid_vec, &seg,
                                C->nseg == 1 ? C->done_flag() : nullptr, C->sched, s);
     } else {
         if (C->precision == 64) SN_LAUNCH_UPD(double, true);
@@ -866,8 +873,8 @@ static void launch_elem_product(sn_cg *C, const float *Mr, const unsigned char *
 #define SN_LAUNCH_EDGE(JAC)                                                                    \
     hipLaunchKernelGGL((cg_edge_kernel<float, float, JAC>), dim3(C->grid_vec), dim3(kVecBlock), \
                        0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask,                  \
-                       (const float *)C->dinv, wold, wnew, (float *)C->r, (float *)xdst,       \
-                       C->partA, C->grid_el, C->partB, C->st, parity, C->sched)
+                       (const float *)C->dinv, wold, wnew, (float *)C->q, (float *)C->r,       \
+                       (float *)xdst, C->partA, C->grid_el, C->partB, C->st, parity, C->sched)
     if (C->ams) SN_LAUNCH_EDGE(false);
     else SN_LAUNCH_EDGE(true);
 #undef SN_LAUNCH_EDGE
@@ -879,9 +886,11 @@ static void launch_elem_iteration(sn_cg *C, const float *Mr, const unsigned char
                                   void *xdst, int parity, hipStream_t s) {
     launch_elem_product(C, Mr, mask, xdst, parity, s);
     if (C->ams) {
+        // the pairs {z, p} of the next iteration: z from the preconditioner,
+        // p from A2's plain array
         float *wnew = (float *)(parity ? C->w0 : C->w1);
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
-        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, wnew, 2, C->partB, 2,
+        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, wnew, 2, C->q, C->partB, 2,
                                C->grid_vec, &seg, C->done_flag(), C->sched, s);
     }
 }
@@ -909,7 +918,7 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->w0, 2,
-                               C->partB, 2, C->grid_vec, &seg, nullptr, C->sched, s);
+                               nullptr, C->partB, 2, C->grid_vec, &seg, nullptr, C->sched, s);
     }
 }
 
@@ -924,8 +933,8 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
                        C->st, C->rtol_dev, atol, maxiter);
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
-        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, C->w0, 2, C->partB, 2,
-                               C->grid_vec, &seg, nullptr, C->sched, s);
+        (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, C->w0, 2, nullptr, C->partB,
+                               2, C->grid_vec, &seg, nullptr, C->sched, s);
     }
 }
 
