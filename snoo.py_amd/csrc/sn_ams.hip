@@ -158,6 +158,16 @@ __device__ __forceinline__ float4 half_diff(const float4 &hi, const float4 &lo) 
     return make_float4(0.5f * (hi.x - lo.x), 0.5f * (hi.y - lo.y), 0.5f * (hi.z - lo.z), 0.f);
 }
 
+// (Pi y)_e = t_e . (y_lo + y_hi) from the {y, coordinates} pairs of the
+// edge's end points {lo, hi}; y is 0 at fixed nodes
+__device__ __forceinline__ float edge_aux(const float4 *__restrict__ yx, int2 en) {
+    const float4 y_lo = yx[2 * (long long)en.x], c_lo = yx[2 * (long long)en.x + 1];
+    const float4 y_hi = yx[2 * (long long)en.y], c_hi = yx[2 * (long long)en.y + 1];
+    const float4 s = f4_add(y_lo, y_hi);
+    const float4 t = half_diff(c_hi, c_lo);
+    return t.x * s.x + t.y * s.y + t.z * s.z;
+}
+
 __global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
                                   const int *__restrict__ glist, const float *__restrict__ in,
                                   float *__restrict__ out) {
@@ -277,10 +287,16 @@ __global__ __launch_bounds__(kAmsBlock) void ams_edge_restrict_kernel(
     const float4 xa = in ? xyz[a] : f4_zero();
     if (done && *done) return;
     float4 acc = f4_zero();
-    for (int k = k0 + lane; k < k1; k += LPR) {
-        const int2 ie = inc[k];   // {edge, other end | sign}
-        const float4 xb = xyz[ie.y & 0x7fffffff];
-        f4_fma(acc, (float)r[ie.x], ie.y < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
+    // two incidences per trip (k and k + LPR, the order of a one-at-a-time
+    // loop), both chains inc -> {coordinates, r} in flight at once; a
+    // missing second one reloads the first and is dropped
+    for (int k = k0 + lane; k < k1; k += 2 * LPR) {
+        const bool h2 = k + LPR < k1;
+        const int2 ie = inc[k], ie2 = inc[h2 ? k + LPR : k];   // {edge, other end | sign}
+        const float4 xb = xyz[ie.y & 0x7fffffff], xb2 = xyz[ie2.y & 0x7fffffff];
+        const float rv = (float)r[ie.x], rv2 = (float)r[ie2.x];
+        f4_fma(acc, rv, ie.y < 0 ? half_diff(xa, xb) : half_diff(xb, xa));
+        if (h2) f4_fma(acc, rv2, ie2.y < 0 ? half_diff(xa, xb2) : half_diff(xb2, xa));
     }
     f4_reduce<LPR>(acc);
     if (a < n_free && lane == 0) {
@@ -309,7 +325,17 @@ __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
     if (done && *done) return;
     float4 s = f4_zero();
     for (int i = r0 + lane; i < r1; i += kWave) s = f4_add(s, b[i]);
-    for (int k = k0 + lane; k < k1; k += kWave) f4_fma(s, -vals[k], x[indices[k]]);
+    // two nonzeros per trip (k and k + kWave, the order of a one-at-a-time
+    // loop), both index -> x chains in flight at once; a missing second one
+    // reloads the first and is dropped
+    for (int k = k0 + lane; k < k1; k += 2 * kWave) {
+        const bool h2 = k + kWave < k1;
+        const int k2 = h2 ? k + kWave : k;
+        const float v = vals[k], v2 = vals[k2];
+        const float4 xc = x[indices[k]], xc2 = x[indices[k2]];
+        f4_fma(s, -v, xc);
+        if (h2) f4_fma(s, -v2, xc2);
+    }
     f4_reduce<kWave>(s);
     if (lane == 0) {
         bc[wid] = s;
@@ -363,7 +389,15 @@ __global__ __launch_bounds__(kAmsBlock) void ams_up_kernel(
     const float di = in ? dinv[row] : 0.f;
     if (done && *done) return;
     float4 acc = f4_zero();
-    for (int k = k0 + lane; k < k1; k += LPR) f4_fma(acc, vals[k], x[indices[k]]);
+    // two nonzeros per trip, as in ams_down_kernel
+    for (int k = k0 + lane; k < k1; k += 2 * LPR) {
+        const bool h2 = k + LPR < k1;
+        const int k2 = h2 ? k + LPR : k;
+        const float v = vals[k], v2 = vals[k2];
+        const float4 xc = x[indices[k]], xc2 = x[indices[k2]];
+        f4_fma(acc, v, xc);
+        if (h2) f4_fma(acc, v2, xc2);
+    }
     f4_reduce<LPR>(acc);
     if (!in) return;
     const float d = omega * di;
@@ -464,27 +498,19 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
         nbl = blk0[seg + 1] - blk0[seg];
     }
     lb = (int)ams_block(xcd, lb, nbl);
+    using V2 = typename std::conditional<sizeof(V) == 4, float2, double2>::type;
     double rz = 0.0;
     for (long long i = e0 + (long long)lb * blockDim.x + threadIdx.x; i < e1;
          i += (long long)nbl * blockDim.x) {
         const bool f = mask ? mask[i] != 0 : true;
         V zi = 0;
         if (f) {
-            const int2 en = edge_nodes[i];
-            // {y, coordinates} of both end points; y is 0 at fixed nodes
-            const float4 y_lo = yx[2 * (long long)en.x], c_lo = yx[2 * (long long)en.x + 1];
-            const float4 y_hi = yx[2 * (long long)en.y], c_hi = yx[2 * (long long)en.y + 1];
-            const float4 s = f4_add(f4_add(f4_zero(), y_lo), y_hi);
-            const float4 t = half_diff(c_hi, c_lo);
-            const float aux = t.x * s.x + t.y * s.y + t.z * s.z;
             const V ri = r[i];
-            zi = dinv[i] * ri + (V)(weight * aux);
+            zi = dinv[i] * ri + (V)(weight * edge_aux(yx, edge_nodes[i]));
             rz += (double)ri * (double)zi;
         }
         if (pin) {
-            // the whole pair {z_i, pin_i}: full lines, no refetch of the
-            // other half
-            using V2 = typename std::conditional<sizeof(V) == 4, float2, double2>::type;
+            // the whole pair {z, p}: full lines, no refetch of the other half
             reinterpret_cast<V2 *>(z)[i] = V2{zi, pin[i]};
         } else {
             z[i * zs] = zi;
