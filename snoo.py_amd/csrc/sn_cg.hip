@@ -378,45 +378,69 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
     const V beta = (V)S->beta;
     const V alpha = (V)(rho / pq);
     double rz = 0.0, rr = 0.0;
-    for (long long i = lb * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const bool free_row = mask ? mask[i] != 0 : true;
-        const int k0 = dof_ptr[i], k1 = dof_ptr[i + 1];
-        const V2 w = zp[i];
-        if (!free_row) {
-            // p (and, with Jacobi, z) of a fixed row stays 0
-            if (JAC) reinterpret_cast<V2 *>(wnew)[i] = V2{(V)0, (V)0};
-            else pout[i] = (V)0;
-            continue;
+    // Two DOFs per trip (i and i + step, the grid-stride order), and the
+    // first kHead list entries of both, then their products, loaded at once
+    // (predicated; a missing entry adds 0, which leaves the sum unchanged):
+    // the DOF range -> list -> product chain is this kernel's latency.  The
+    // sum of each DOF keeps the list order.
+    constexpr int kHead = 8;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long i0 = lb * blockDim.x + threadIdx.x; i0 < n; i0 += 2 * step) {
+        const bool two = i0 + step < n;
+        const long long id[2] = {i0, two ? i0 + step : i0};
+        bool fr[2];
+        int k0[2], k1[2];
+        V2 w[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            fr[u] = (u == 0 || two) && (mask ? mask[id[u]] != 0 : true);
+            k0[u] = dof_ptr[id[u]];
+            k1[u] = dof_ptr[id[u] + 1];
+            w[u] = zp[id[u]];
         }
-        // four list entries, then their four products, in flight at once;
-        // the sum keeps the list order
-        V qi = 0;
-        int k = k0;
-        for (; k + 4 <= k1; k += 4) {
-            const int c0 = dof_contrib[k], c1 = dof_contrib[k + 1];
-            const int c2 = dof_contrib[k + 2], c3 = dof_contrib[k + 3];
-            const V y0 = yloc[c0], y1 = yloc[c1], y2 = yloc[c2], y3 = yloc[c3];
-            qi += y0;
-            qi += y1;
-            qi += y2;
-            qi += y3;
+        int c[2][kHead];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int m = 0; m < kHead; ++m)
+                c[u][m] = (fr[u] && k0[u] + m < k1[u]) ? dof_contrib[k0[u] + m] : -1;
+        V q[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            V y[kHead];
+#pragma unroll
+            for (int m = 0; m < kHead; ++m) y[m] = c[u][m] >= 0 ? yloc[c[u][m]] : (V)0;
+            q[u] = 0;
+#pragma unroll
+            for (int m = 0; m < kHead; ++m) q[u] += y[m];
         }
-        for (; k < k1; ++k) qi += yloc[dof_contrib[k]];
-        const V pi = w.x + beta * w.y;
-        x[i] += (X)(alpha * pi);
-        const V ri = r[i] - alpha * qi;
-        r[i] = ri;
-        rr += (double)ri * (double)ri;
-        if (JAC) {
-            const V zi = dinv[i] * ri;
-            reinterpret_cast<V2 *>(wnew)[i] = V2{zi, pi};
-            rz += (double)ri * (double)zi;
-        } else {
-            // a plain array: the preconditioner writes the whole {z, p} pair
-            // of the next iteration from it (half-pair stores made both
-            // kernels refetch the other half of every line)
-            pout[i] = pi;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (u == 1 && !two) break;
+            const long long i = id[u];
+            if (!fr[u]) {
+                // p (and, with Jacobi, z) of a fixed row stays 0
+                if (JAC) reinterpret_cast<V2 *>(wnew)[i] = V2{(V)0, (V)0};
+                else pout[i] = (V)0;
+                continue;
+            }
+            V qi = q[u];
+            for (int k = k0[u] + kHead; k < k1[u]; ++k) qi += yloc[dof_contrib[k]];
+            const V pi = w[u].x + beta * w[u].y;
+            x[i] += (X)(alpha * pi);
+            const V ri = r[i] - alpha * qi;
+            r[i] = ri;
+            rr += (double)ri * (double)ri;
+            if (JAC) {
+                const V zi = dinv[i] * ri;
+                reinterpret_cast<V2 *>(wnew)[i] = V2{zi, pi};
+                rz += (double)ri * (double)zi;
+            } else {
+                // a plain array: the preconditioner writes the whole {z, p}
+                // pair of the next iteration from it (half-pair stores made
+                // both kernels refetch the other half of every line)
+                pout[i] = pi;
+            }
         }
     }
     const double s1 = JAC ? block_sum(rz, lds) : 0.0;
