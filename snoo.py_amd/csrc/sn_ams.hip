@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
             rz += (double)ri * (double)zi;
         }
         if (pin) {
-            // the whole pair {z, p}: full lines, no refetch of the other half
+            // the whole pair {z, p}: no half-written sectors
             reinterpret_cast<V2 *>(z)[i] = V2{zi, pin[i]};
         } else {
             z[i * zs] = zi;

@@ -437,8 +437,9 @@ __global__ __launch_bounds__(kVecBlock) void cg_edge_kernel(
                 rz += (double)ri * (double)zi;
             } else {
                 // a plain array: the preconditioner writes the whole {z, p}
-                // pair of the next iteration from it (half-pair stores made
-                // both kernels refetch the other half of every line)
+                // pair of the next iteration from it (a store to every other
+                // float writes whole sectors: this kernel wrote 18.8 MB for
+                // 14.3 MB of data at 1M tets, now 14.1 MB)
                 pout[i] = pi;
             }
         }
