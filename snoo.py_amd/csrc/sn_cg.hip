@@ -311,46 +311,31 @@ __global__ __launch_bounds__(kRowBlock) void cg_elem_kernel(
     const double beta_d = it > 0 ? rho / S->rho[parity ^ 1] : 0.0;
     const V beta = (V)beta_d;
     double pq = 0.0;
-    // two elements per trip (e and e + step, the grid-stride order): the DOF
-    // ids of both, then the 12 {z, p_old} gathers, in flight at once.  A
-    // missing second element reloads the first and is dropped.
-    const long long step = (long long)gridDim.x * blockDim.x;
-    for (long long e0 = lb * blockDim.x + threadIdx.x; e0 < n_el; e0 += 2 * step) {
-        const bool two = e0 + step < n_el;
-        const long long ee[2] = {e0, two ? e0 + step : e0};
-        int d[2][6];
-        V a[2][6], xh[2][6];
+    for (long long e = lb * blockDim.x + threadIdx.x; e < n_el;
+         e += (long long)gridDim.x * blockDim.x) {
+        int d[6];
+        V xh[6];
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int j = 0; j < 6; ++j) d[j] = el_dofs[j * (long long)n_el + e];
+        V a[6];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) d[u][j] = el_dofs[j * (long long)n_el + ee[u]];
+        for (int k = 0; k < 6; ++k) a[k] = (V)Mr[k * (long long)n_el + e];
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int k = 0; k < 6; ++k) a[u][k] = (V)Mr[k * (long long)n_el + ee[u]];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int j = 0; j < 6; ++j) {
-                const V2 w = zp[d[u][j] & 0x7fffffff];
-                const V pj = w.x + beta * w.y;
-                xh[u][j] = d[u][j] < 0 ? -pj : pj;
-            }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (u == 1 && !two) break;
-            const V u0 = xh[u][1] - xh[u][2] + xh[u][5];
-            const V u1 = xh[u][2] - xh[u][0] - xh[u][4];
-            const V u2 = xh[u][0] - xh[u][1] + xh[u][3];
-            const V m0 = a[u][0] * u0 + a[u][1] * u1 + a[u][2] * u2;
-            const V m1 = a[u][1] * u0 + a[u][3] * u1 + a[u][4] * u2;
-            const V m2 = a[u][2] * u0 + a[u][4] * u1 + a[u][5] * u2;
-            pq += (double)u0 * (double)m0 + (double)u1 * (double)m1 + (double)u2 * (double)m2;
-            const V yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-                yloc[j * (long long)n_el + ee[u]] = d[u][j] < 0 ? -yh[j] : yh[j];
+        for (int j = 0; j < 6; ++j) {
+            const V2 w = zp[d[j] & 0x7fffffff];
+            const V pj = w.x + beta * w.y;
+            xh[j] = d[j] < 0 ? -pj : pj;
         }
+        const V u0 = xh[1] - xh[2] + xh[5];
+        const V u1 = xh[2] - xh[0] - xh[4];
+        const V u2 = xh[0] - xh[1] + xh[3];
+        const V m0 = a[0] * u0 + a[1] * u1 + a[2] * u2;
+        const V m1 = a[1] * u0 + a[3] * u1 + a[4] * u2;
+        const V m2 = a[2] * u0 + a[4] * u1 + a[5] * u2;
+        pq += (double)u0 * (double)m0 + (double)u1 * (double)m1 + (double)u2 * (double)m2;
+        const V yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
+#pragma unroll
+        for (int j = 0; j < 6; ++j) yloc[j * (long long)n_el + e] = d[j] < 0 ? -yh[j] : yh[j];
     }
     const double s = block_sum(pq, lds);
     if (threadIdx.x == 0) {
