@@ -10,10 +10,13 @@ A solve is complete: source field at the iron quadrature points
 (Biot-Savart) plus Newton from the start vector to ||R|| <= 1e-8 ||R(0)||.
 Mesh, CSR pattern and element-to-nonzero map are set up before the timed
 region (inputs resident in HBM).  One step is one batch: --batch variants of
-the dipole (coil current density 1.2e6..1.8e6 A/m^2) solved concurrently on
---streams HIP streams in the chosen form (mixed precision with the AMS
-preconditioner, plus the learned warm start when a trained model is
-present); value = solves per second of the batch.  With --batch 0 a step is
+the dipole (coil current density 1.2e6..1.8e6 A/m^2) solved on --streams HIP
+streams in the chosen form (mixed precision with the AMS preconditioner) with
+the anchor warm start (--batch-warm): the middle variant is solved from zero,
+then the others concurrently from its solution scaled by the current-density
+ratio, each to ||R|| <= 1e-8 ||R(0)|| of its own; value = solves per second
+of the batch.  The same batch with every variant from zero is timed beside
+it (batch_cold_solves_per_s).  With --batch 0 a step is
 one solve.  The single solve of the chosen form and the plain FP64 GPU solve
 are timed beside it, and the relative B error at 10k aperture probes is
 reported against FP64 solves at 1e-11 (for the single solve and for every
@@ -68,6 +71,9 @@ def parse():
     ap.add_argument('--batch', type=int, default=8,
                     help='dipole1m: variants solved concurrently in one step (0: one solve '
                          'per step)')
+    ap.add_argument('--batch-warm', default='anchor', choices=['none', 'anchor'],
+                    help='dipole1m batch: anchor = the middle variant solved from zero first, '
+                         'the others started from its solution scaled by the current ratio')
     ap.add_argument('--streams', type=int, default=None,
                     help='concurrent HIP streams (default 8 for dipole1m, 4 for batch64)')
     ap.add_argument('--ws-train', type=int, default=512, help='warm start: training snapshots')
@@ -230,11 +236,13 @@ def main():
     # 1.2e6..1.8e6 A/m^2, the metric's 1.5e6 in the middle), each a complete
     # solve, run concurrently on --streams HIP streams
     t_batch, bout, bdps, cs = None, None, None, None
+    t_batch_cold, cold_its = None, None
     if args.batch > 0:
         from snoopy_amd.batch import ConcurrentSolver
         bdps = [Dipole(h=h, J=float(J)) for J in np.linspace(1.2e6, 1.8e6, args.batch)]
         cs = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
-                              precond=args.precond, lanes_per_row=args.lanes)
+                              precond=args.precond, lanes_per_row=args.lanes,
+                              warm_start=None if args.batch_warm == 'none' else args.batch_warm)
         for _ in range(max(args.warmup, 1)):
             cs.solve(args.rtol)
         torch.cuda.synchronize()
@@ -243,12 +251,30 @@ def main():
             bout = cs.solve(args.rtol)
         torch.cuda.synchronize()
         t_batch = (time.perf_counter() - t) / args.steps
-        log('batch: %d variants on %d streams, %.3f s per batch (%.1f solves/s), cg %s'
-            % (args.batch, args.streams, t_batch, args.batch / t_batch,
+        log('batch (%s): %d variants on %d streams, %.3f s per batch (%.1f solves/s), cg %s'
+            % (args.batch_warm, args.batch, args.streams, t_batch, args.batch / t_batch,
                [int(sum(o[1]['cg_iterations'])) for o in bout]))
+        if args.batch_warm != 'none':
+            # the same batch with every variant from zero, for comparison
+            cs_cold = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
+                                       precond=args.precond, lanes_per_row=args.lanes)
+            cs_cold.solve(args.rtol)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                cold_out = cs_cold.solve(args.rtol)
+            torch.cuda.synchronize()
+            t_batch_cold = (time.perf_counter() - t) / args.steps
+            cold_its = [int(sum(o[1]['cg_iterations'])) for o in cold_out]
+            del cs_cold, cold_out
+            log('batch (cold): %.3f s per batch (%.1f solves/s), cg %s'
+                % (t_batch_cold, args.batch / t_batch_cold, cold_its))
+        else:
+            t_batch_cold, cold_its = t_batch, None
     if args.quick:
         print(json.dumps({'quick': True, 'fp64_s': t_fp64, 'mixed_s': t_mixed,
                           'fp64_jacobi_s': t_jac, 'batch_s': t_batch,
+                          'batch_cold_s': t_batch_cold,
                           'fp64_stats': {k: v for k, v in st64.items()},
                           'mixed_stats': {k: v for k, v in stm.items()}}), flush=True)
         return
@@ -335,8 +361,12 @@ def main():
         value, step_ms = args.batch / t_batch, 1e3 * t_batch
         workload = ('%d variants (coil current density 1.2e6-1.8e6 A/m^2) of the window-frame '
                     'dipole, %d tets, %d edge DOFs, nnz %d each, Q=4, B-spline-in-B^2 iron, '
-                    'Newton to 1e-8, solved concurrently on %d HIP streams; one step = the '
-                    'batch' % (args.batch, N, n, nnz, args.streams))
+                    'Newton to 1e-8 of each variant\'s own |R(0)|, on %d HIP streams%s; one '
+                    'step = the batch'
+                    % (args.batch, N, n, nnz, args.streams,
+                       '' if args.batch_warm == 'none' else
+                       ', anchor warm start (middle variant from zero, the others concurrently '
+                       'from its solution scaled by the current-density ratio)'))
     else:
         value, step_ms = 1.0 / t_mixed, 1e3 * t_mixed
         workload = ('window-frame dipole, %d tets, %d edge DOFs, nnz %d, Q=4, '
@@ -360,9 +390,17 @@ def main():
                    'precision': 'mixed (FP32 element-operator Jacobian and CG, FP64 residual '
                                 'and dots)',
                    'preconditioner': args.precond,
-                   'warm_start': warm is not None, 'parallelism': 'single GPU'},
+                   'warm_start': warm is not None,
+                   'batch_warm_start': (None if args.batch == 0 or args.batch_warm == 'none'
+                                        else 'anchor: the middle variant solved from zero, the '
+                                             'others started from its solution scaled by the '
+                                             'current-density ratio; each to |R| <= 1e-8 |R(0)| '
+                                             'of its own'),
+                   'parallelism': 'single GPU'},
         'single_solve_per_s': 1.0 / t_mixed,
         'batch_B_rel_err_max': batch_err,
+        'batch_cold_solves_per_s': (args.batch / t_batch_cold) if t_batch_cold else None,
+        'batch_cold_cg_iterations': cold_its,
         'batch_cg_iterations': ([int(sum(o[1]['cg_iterations'])) for o in bout]
                                 if bout is not None else None),
         'fp64_solves_per_s': 1.0 / t_fp64,

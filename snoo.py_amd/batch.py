@@ -282,24 +282,43 @@ class ConcurrentSolver:
     solver; the GPU overlaps the kernels of the streams, which fills the
     device where one small variant alone is launch- or latency-bound.  The
     host waits of one stream release the interpreter lock (they happen inside
-    the C library), so the threads' host work overlaps too."""
+    the C library), so the threads' host work overlaps too.
+
+    ``warm_start='anchor'`` (variants on one mesh that differ in the coil
+    current density ``p['J']`` only): the middle variant of the list is solved
+    from zero first; every other variant then starts from its solution scaled
+    by the ratio of the current densities, the solution of the linear part of
+    the problem.  Each variant still runs Newton to ||R|| <= rtol ||R(0)|| of
+    its own, R(0) its residual at zero."""
 
     def __init__(self, dipoles, workers=4, quad_order=2, precision='mixed', precond='ams',
-                 lanes_per_row=8):
+                 lanes_per_row=8, warm_start=None):
         import torch
         self.dipoles = list(dipoles)
         self.workers = max(1, int(workers))
         self.kw = dict(precision=precision, precond=precond, lanes_per_row=lanes_per_row)
+        if warm_start not in (None, 'anchor'):
+            raise ValueError("warm_start must be None or 'anchor'")
+        if warm_start == 'anchor':
+            m0 = self.dipoles[0].mesh
+            for d in self.dipoles:
+                if (d.mesh.num_edges != m0.num_edges or d.mesh.cells.shape != m0.cells.shape
+                        or not np.array_equal(d.mesh.cells, m0.cells)):
+                    raise ValueError('anchor warm start: every variant must share the mesh')
+                if not d.p.get('J'):
+                    raise ValueError('anchor warm start: every variant needs a current density')
+        self.warm_start = warm_start
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
 
-    def _run(self, w, idx, rtol, out):
+    def _run(self, w, idx, rtol, out, x0s=None):
         import torch
         with torch.cuda.stream(self.streams[w]):
             for i in idx:
                 s = self.solvers[i]
                 s._src = None
                 x = s.solve_newton(self.kw['precision'], rtol=rtol,
+                                   x0=None if x0s is None else x0s.get(int(i)),
                                    lanes_per_row=self.kw['lanes_per_row'],
                                    precond=self.kw['precond'])
                 out[i] = (x, dict(s.stats))
@@ -311,10 +330,18 @@ class ConcurrentSolver:
         import torch
         out = [None] * len(self.dipoles)
         # largest variants first, dealt round-robin, so the streams finish together
-        order = np.argsort([-d.num_tets for d in self.dipoles])
+        order = np.argsort([-d.num_tets for d in self.dipoles], kind='stable')
+        x0s = None
+        if self.warm_start == 'anchor' and len(self.dipoles) > 1:
+            a = len(self.dipoles) // 2
+            torch.cuda.synchronize()
+            self._run(0, [a], rtol, out)
+            xa, ja = out[a][0], float(self.dipoles[a].p['J'])
+            x0s = {i: xa * (float(d.p['J']) / ja) for i, d in enumerate(self.dipoles) if i != a}
+            order = np.array([i for i in order if i != a])
         parts = [order[w::self.workers] for w in range(self.workers)]
         torch.cuda.synchronize()
-        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out))
+        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out, x0s))
                    for w in range(self.workers)]
         for t in threads:
             t.start()

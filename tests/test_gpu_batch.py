@@ -88,3 +88,39 @@ def test_concurrent_streams_match_single_solves(variants, workers):
         Bc = asm.element_B(res[v][0].cpu().numpy())
         Bs = asm.element_B(xv.cpu().numpy())
         assert np.abs(Bc - Bs).max() <= 1e-12 * np.abs(Bs).max()
+
+
+def test_anchor_warm_start_matches_cold_solves():
+    """Variants of one dipole that differ in the coil current density: with the
+    anchor warm start (middle variant from zero, the others from its solution
+    scaled by the current ratio) every variant converges to ||R|| <= 1e-8
+    ||R(0)|| of its own and gives the field of its solve from zero, with fewer
+    CG iterations than from zero."""
+    from snoopy_amd.batch import ConcurrentSolver
+    from snoopy_amd.magnets import Dipole
+    fam = [Dipole(h=0.1, J=J) for J in (1.2e6, 1.5e6, 1.8e6)]
+    print('case: anchor warm start, 3 variants of', fam[0].num_tets, 'tets', flush=True)
+    warm = ConcurrentSolver(fam, workers=2, precision='mixed', precond='ams',
+                            warm_start='anchor').solve(rtol=1e-8)
+    cold = ConcurrentSolver(fam, workers=2, precision='mixed', precond='ams').solve(rtol=1e-8)
+    asm = fam[0].solver(quad_order=2, verbose=False).curl_curl_factory
+    for v in range(3):
+        assert warm[v][1]['converged'] and cold[v][1]['converged']
+        assert warm[v][1]['residual_history'][-1] <= 1e-8
+        Bw = asm.element_B(warm[v][0].cpu().numpy())
+        Bc = asm.element_B(cold[v][0].cpu().numpy())
+        err = np.abs(Bw - Bc).max() / np.abs(Bc).max()
+        iw, ic = sum(warm[v][1]['cg_iterations']), sum(cold[v][1]['cg_iterations'])
+        print('variant', v, 'B diff %.2e' % err, 'cg warm', iw, 'cold', ic, flush=True)
+        assert err <= 1e-6
+        if v != 1:
+            assert iw < ic
+    # the anchor itself is the solve from zero
+    assert warm[1][1]['cg_iterations'] == cold[1][1]['cg_iterations']
+
+
+def test_anchor_warm_start_needs_one_mesh():
+    from snoopy_amd.batch import ConcurrentSolver
+    from snoopy_amd.magnets import Dipole
+    with pytest.raises(ValueError):
+        ConcurrentSolver([Dipole(h=0.12), Dipole(h=0.15)], workers=2, warm_start='anchor')
