@@ -300,13 +300,18 @@ class ConcurrentSolver:
         if warm_start not in (None, 'anchor'):
             raise ValueError("warm_start must be None or 'anchor'")
         if warm_start == 'anchor':
-            m0 = self.dipoles[0].mesh
+            m0, d0 = self.dipoles[0].mesh, self.dipoles[0]
             for d in self.dipoles:
                 if (d.mesh.num_edges != m0.num_edges or d.mesh.cells.shape != m0.cells.shape
-                        or not np.array_equal(d.mesh.cells, m0.cells)):
+                        or not np.array_equal(d.mesh.cells, m0.cells)
+                        or not np.array_equal(d.mesh.nodes, m0.nodes)):
                     raise ValueError('anchor warm start: every variant must share the mesh')
                 if not d.p.get('J'):
                     raise ValueError('anchor warm start: every variant needs a current density')
+                if _without_J(d.p) != _without_J(d0.p) or \
+                        not _same_reluctance(d.reluctance, d0.reluctance):
+                    raise ValueError('anchor warm start: the variants may differ in the coil '
+                                     'current density J only')
         self.warm_start = warm_start
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
@@ -333,12 +338,15 @@ class ConcurrentSolver:
         order = np.argsort([-d.num_tets for d in self.dipoles], kind='stable')
         x0s = None
         if self.warm_start == 'anchor' and len(self.dipoles) > 1:
-            a = len(self.dipoles) // 2
+            a = anchor_index([float(d.p['J']) for d in self.dipoles])
             torch.cuda.synchronize()
             self._run(0, [a], rtol, out)
             xa, ja = out[a][0], float(self.dipoles[a].p['J'])
-            x0s = {i: xa * (float(d.p['J']) / ja) for i, d in enumerate(self.dipoles) if i != a}
             order = np.array([i for i in order if i != a])
+            if anchor_usable(out[a][1], xa):
+                x0s = {i: xa * (float(d.p['J']) / ja) for i, d in enumerate(self.dipoles)
+                       if i != a}
+            # else: the anchor failed, every other variant starts from zero
         parts = [order[w::self.workers] for w in range(self.workers)]
         torch.cuda.synchronize()
         threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out, x0s))
@@ -351,3 +359,32 @@ class ConcurrentSolver:
         if any(o is None for o in out):
             raise RuntimeError('a worker failed; see the thread traceback above')
         return out
+
+
+def _without_J(p):
+    return {k: v for k, v in p.items() if k != 'J'}
+
+
+def _same_reluctance(a, b):
+    """Two reluctivity models with identical tables (or the same object)."""
+    if a is b:
+        return True
+    from .materials import model_tables
+    ta, tb = model_tables(a), model_tables(b)
+    return len(ta) == len(tb) and all(
+        (np.array_equal(np.asarray(u), np.asarray(v)) if not np.isscalar(u) else u == v)
+        for u, v in zip(ta, tb))
+
+
+def anchor_index(currents):
+    """The variant of median current density (the lower median for an even
+    count), the anchor whose solution seeds the others."""
+    order = np.argsort(np.asarray(currents, dtype=float), kind='stable')
+    return int(order[(len(order) - 1) // 2])
+
+
+def anchor_usable(stats, x):
+    """The anchor's solution seeds the other variants only if its Newton
+    converged and every entry is finite; otherwise they start from zero."""
+    import torch
+    return bool(stats.get('converged')) and bool(torch.isfinite(x).all().item())

@@ -394,6 +394,123 @@ def test_biot_savart_closed_forms():
     assert abs(Bc[0, 2] - 2 * np.sqrt(2) * MU0 * I / (np.pi * a)) < 1e-12 * abs(Bc[0, 2])
 
 
+# Brick-field oracle (BrickField.compute_B, snoopy/brick_field.py:400-433 with
+# is_inside, :165-194), pinned by geometry that does not go through its face
+# tables: the field of a brick is Phi / (the area of its cross-section across
+# the flux), Phi is one flux through all three bricks, a point is inside a
+# brick when it is inside the convex hull of its eight corners, points on a
+# face count as inside (only proj > 0 excludes), and the first brick that
+# holds a point sets its field.
+
+def _shoelace(P):
+    P = np.asarray(P, dtype=float)
+    return 0.5 * abs(np.dot(P[:, 0], np.roll(P[:, 1], -1)) - np.dot(P[:, 1], np.roll(P[:, 0], -1)))
+
+
+def _brick_case(taper, btype=1):
+    """BrickField arguments and, from the magnet's geometry alone, the corner
+    sets of (core, top yoke, return yoke) and the cross-section area of each
+    across its flux direction (y, x, y)."""
+    X1 = dict(mgap=0.0, core=0.5, void=0.625, yoke=1.0)
+    Y1 = dict(void=1.25, yoke=1.75)
+    X2 = {k: v + (0.125 if taper and k != 'mgap' else 0.0) for k, v in X1.items()}
+    Y2 = {k: v + (0.25 if taper else 0.0) for k, v in Y1.items()}
+    Zl, Z0 = 4.0, -2.0
+    args = (X1['mgap'], X1['core'], X1['void'], X1['yoke'], X2['mgap'], X2['core'], X2['void'],
+            X2['yoke'], Y1['void'], Y1['yoke'], Y2['void'], Y2['yoke'], Zl)
+    ends = [(X1, Y1, Z0), (X2, Y2, Z0 + Zl)]
+    core = [[(X['mgap'], 0, z), (X['core'], 0, z), (X['core'], Y['void'], z),
+             (X['mgap'], Y['yoke'], z)] for X, Y, z in ends]
+    top = [[(X['mgap'], Y['yoke'], z), (X['core'], Y['void'], z), (X['void'], Y['void'], z),
+            (X['yoke'], Y['yoke'], z)] for X, Y, z in ends]
+    ret = [[(X['void'], 0, z), (X['yoke'], 0, z), (X['yoke'], Y['yoke'], z),
+            (X['void'], Y['void'], z)] for X, Y, z in ends]
+    corners = [np.array(c[0] + c[1], dtype=float) for c in (core, top, ret)]
+    # flux cross-sections: the y = 0 faces of core and return yoke (x-z
+    # polygons) and the top yoke's outer face projected on y-z
+    A1 = _shoelace([(X1['mgap'], Z0), (X1['core'], Z0), (X2['core'], Z0 + Zl),
+                    (X2['mgap'], Z0 + Zl)])
+    A3 = _shoelace([(X1['void'], Z0), (X1['yoke'], Z0), (X2['yoke'], Z0 + Zl),
+                    (X2['void'], Z0 + Zl)])
+    A2 = _shoelace([(Y1['void'], Z0), (Y1['yoke'], Z0), (Y2['yoke'], Z0 + Zl),
+                    (Y2['void'], Z0 + Zl)])
+    return args, Z0, corners, (A1, A2, A3)
+
+
+@pytest.mark.parametrize('taper,btype', [(False, 1), (True, 1), (True, 2)])
+def test_brick_field_oracle_centroids_and_flux(taper, btype):
+    from snoopy_amd.brick_field import BrickField
+    args, Z0, corners, (A1, A2, A3) = _brick_case(taper)
+    Bg = 1.5
+    bf = BrickField(*args, Z_pos=Z0, B_goal=Bg, type=btype)
+    Phi = Bg * (A1 if btype == 1 else A3)
+    cen = np.array([c.mean(axis=0) for c in corners])
+    B = orc.brick_field(bf, cen)
+    expect = np.array([[0.0, Phi / A1, 0.0], [Phi / A2, 0.0, 0.0], [0.0, -Phi / A3, 0.0]])
+    np.testing.assert_allclose(B, expect, rtol=1e-14, atol=0)
+    # one flux through the three iron paths
+    flux = [abs(B[0, 1]) * A1, abs(B[1, 0]) * A2, abs(B[2, 1]) * A3]
+    np.testing.assert_allclose(flux, Phi, rtol=1e-14)
+    # the goal field sits where the type says
+    assert abs(abs(B[0 if btype == 1 else 2, 1]) - Bg) < 1e-14
+    # nothing beyond the bricks' bounding box
+    lo, hi = np.min(np.vstack(corners), axis=0), np.max(np.vstack(corners), axis=0)
+    rng = np.random.default_rng(5)
+    far = rng.uniform(lo - 1.0, hi + 1.0, size=(3000, 3))
+    far = far[np.any((far < lo - 1e-9) | (far > hi + 1e-9), axis=1)]
+    assert len(far) > 1000
+    assert not np.any(orc.brick_field(bf, far))
+
+
+def test_brick_field_oracle_inside_test_is_the_convex_hull():
+    """Point in brick by the hull of its corners (scipy), on an untapered
+    magnet whose faces are planes; points within 1e-9 of a face are left to
+    the on-face test below."""
+    from scipy.spatial import ConvexHull
+    from snoopy_amd.brick_field import BrickField
+    args, Z0, corners, (A1, A2, A3) = _brick_case(False)
+    bf = BrickField(*args, Z_pos=Z0, B_goal=1.5, type=1)
+    Phi = 1.5 * A1
+    vals = [np.array([0.0, Phi / A1, 0.0]), np.array([Phi / A2, 0.0, 0.0]),
+            np.array([0.0, -Phi / A3, 0.0])]
+    rng = np.random.default_rng(9)
+    pts = rng.uniform([-0.2, -0.2, -2.3], [1.2, 2.0, 2.3], size=(6000, 3))
+    expect = np.zeros_like(pts)
+    keep = np.ones(len(pts), dtype=bool)
+    done = np.zeros(len(pts), dtype=bool)
+    for c, v in zip(corners, vals):
+        eq = ConvexHull(c).equations
+        s = (pts @ eq[:, :3].T + eq[:, 3]).max(axis=1)     # > 0 outside
+        keep &= np.abs(s) > 1e-9
+        inside = (s <= 0) & ~done
+        expect[inside] = v
+        done |= inside
+    assert done[keep].sum() > 1000 and (~done[keep]).sum() > 1000
+    np.testing.assert_array_equal(orc.brick_field(bf, pts[keep]), expect[keep])
+
+
+def test_brick_field_oracle_faces_count_as_inside_and_first_brick_wins():
+    from snoopy_amd.brick_field import BrickField
+    args, Z0, corners, (A1, A2, A3) = _brick_case(False)
+    bf = BrickField(*args, Z_pos=Z0, B_goal=1.5, type=1)
+    Phi = 1.5 * A1
+    # on the core's end face z = Z0 (exact plane) inside, just beyond it outside
+    on = np.array([[0.25, 0.5, Z0], [0.25, 0.5, Z0 - 1e-9], [0.25, 0.5, Z0 + 4.0],
+                   [0.25, 0.5, Z0 + 4.0 + 1e-9]])
+    B = orc.brick_field(bf, on)
+    np.testing.assert_array_equal(B[:, 1], [Phi / A1, 0.0, Phi / A1, 0.0])
+    # the centre of the slanted face shared by core and top yoke (all
+    # coordinates dyadic, so both bricks' face centres are this exact point):
+    # inside both, the core (listed first) sets the field
+    shared = np.array([[0.25, 1.5, 0.0]])
+    np.testing.assert_array_equal(orc.brick_field(bf, shared), [[0.0, Phi / A1, 0.0]])
+    # the return yoke's inner face x = 0.625 at y < 1.25 belongs to it alone
+    np.testing.assert_array_equal(orc.brick_field(bf, np.array([[0.625, 0.5, 0.0]])),
+                                  [[0.0, -Phi / A3, 0.0]])
+    np.testing.assert_array_equal(orc.brick_field(bf, np.array([[0.625 - 1e-9, 0.5, 0.0]])),
+                                  [[0.0, 0.0, 0.0]])
+
+
 def test_pcg_matches_scipy_cg(small_problem):
     from scipy.sparse import diags
     from scipy.sparse.linalg import cg
