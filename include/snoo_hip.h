@@ -88,6 +88,42 @@ int sn_p1_interp_f64(int n_pts, const double *pts_dev, const int *el_dev, const 
                      int n_nodes, const double *nodes_dev, const int *cells_dev,
                      const double *nodal_dev, double *out_dev, void *stream);
 
+/* ------------------------------------------- the reference's own tables
+ * Element kernels that take the arrays the reference's FFI receives,
+ * unchanged apart from living on the device: 0-based cells [num_el*4] and
+ * DOF ids [num_el*6], a per-element orientation number orient [num_el] in
+ * [0, n_orient), the curl table of every orientation curls
+ * [n_orient*Q*6*3] (c_{j,q} of orientation o at ((o*Q + q)*6 + j)*3), the P1
+ * gradients d_phi [Q*4*3] and weights w [Q].  The element map and the curls
+ * are evaluated at every quadrature point as the tables give them.  These
+ * calls synchronise the stream: an orientation outside [0, n_orient) makes
+ * them return SN_ERR_ARG (its element is skipped, never read out of
+ * bounds). */
+
+/* Local K, dK (21 upper-triangle pairs each, loc[pr*num_el + e]) and rhs
+ * (loc6[j*num_el + e]; 0 without a source field) of one domain.  Replaces
+ * compute_K_dK_Hcurl_red (fem_c/ctools.c:1195-1388, Bs_dev != NULL,
+ * [num_el*Q*3]) and compute_K_dK_Hcurl (fem_c/ctools.c:960-1193, Bs_dev ==
+ * NULL) as called by compute_stiffness_and_jacobi_matrix_curl_curl
+ * (fem_c/interface_c.pyx:162-374, from snoopy/curl_curl_assembler.py:915-927);
+ * sn_pattern_create on glob_ids_dev and sn_csr_gather_f64 /
+ * sn_dof_gather_f64 then give the CSR matrices and rhs that call returns. */
+int sn_element_table_f64(int num_el, const double *nodes_dev, const int *cells_dev,
+                         const int *glob_ids_dev, const int *orient_dev, int n_orient,
+                         const double *curls_dev, const double *d_phi_dev, const double *w_dev,
+                         int Q, int type_spec, int n_int, const double *coeffs_dev,
+                         const double *knots_dev, const double *x_dev, const double *Bs_dev,
+                         double *locK_dev, double *locdK_dev, double *loc6_dev, void *stream);
+
+/* Quadrature points and curl A, points_dev/B_dev [num_el*Q*3]; phi_dev [Q*4].
+ * Replaces compute_B_Hcurl (fem_c/ctools.c:1500-1563) as bound by
+ * fem_c/interface_c.pyx:555-696 with the reference's tables. */
+int sn_B_Hcurl_table_f64(int num_el, const double *nodes_dev, const int *cells_dev,
+                         const int *glob_ids_dev, const int *orient_dev, int n_orient,
+                         const double *curls_dev, const double *phi_dev,
+                         const double *d_phi_dev, int Q, const double *x_dev,
+                         double *points_dev, double *B_dev, void *stream);
+
 /* ---------------------------------------------------- pattern and scatter */
 
 typedef struct sn_pattern sn_pattern;

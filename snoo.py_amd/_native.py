@@ -23,6 +23,10 @@ SIGNATURES = {
     'sn_brick_field_f64': (_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_int, _vp]),
     'sn_quad_points_f64': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp]),
     'sn_B_Hcurl_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_element_table_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int,
+                                      _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_B_Hcurl_table_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int,
+                                      _vp, _vp, _vp, _vp]),
     'sn_pattern_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _vp, _vp]),
     'sn_pattern_destroy': (None, [_vp]),
     'sn_pattern_info': (_c_int, [_vp, _P_ll, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
