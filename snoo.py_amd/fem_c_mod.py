@@ -165,13 +165,18 @@ def compute_stiffness_and_jacobi_matrix_curl_curl(nodes, cells, glob_ids, curls,
     locK = torch.empty(21 * num_el, dtype=torch.float64, device=dev)
     locdK = torch.empty_like(locK)
     loc6 = torch.empty(6 * num_el, dtype=torch.float64, device=dev)
+    # device copies held in names until the kernel has run (a temporary
+    # freed early would hand its memory to the next copy)
     Bs_dev = as_dev_f64(B_s.reshape(-1)) if with_src else None
+    nodes_d = as_dev_f64(np.asarray(nodes, dtype=np.float64).reshape(-1))
+    cells_d, orient_d = as_dev_i32(cells), as_dev_i32(orient)
+    curls_d, dphi_d, w_d = as_dev_f64(curls), as_dev_f64(d_phi), as_dev_f64(w)
+    c_d, xk_d, x_d = as_dev_f64(c), as_dev_f64(xk), as_dev_f64(x)
     nat.check(lib.sn_element_table_f64(
-        num_el, nat.ptr(as_dev_f64(np.asarray(nodes, dtype=np.float64).reshape(-1))),
-        nat.ptr(as_dev_i32(cells)), nat.ptr(gids_dev), nat.ptr(as_dev_i32(orient)), n_orient,
-        nat.ptr(as_dev_f64(curls)), nat.ptr(as_dev_f64(d_phi)), nat.ptr(as_dev_f64(w)), Q, ts,
-        n_int, nat.ptr(as_dev_f64(c)), nat.ptr(as_dev_f64(xk)), nat.ptr(as_dev_f64(x)),
-        nat.ptr(Bs_dev), nat.ptr(locK), nat.ptr(locdK), nat.ptr(loc6), nat.stream_handle()))
+        num_el, nat.ptr(nodes_d), nat.ptr(cells_d), nat.ptr(gids_dev), nat.ptr(orient_d),
+        n_orient, nat.ptr(curls_d), nat.ptr(dphi_d), nat.ptr(w_d), Q, ts, n_int, nat.ptr(c_d),
+        nat.ptr(xk_d), nat.ptr(x_d), nat.ptr(Bs_dev), nat.ptr(locK), nat.ptr(locdK),
+        nat.ptr(loc6), nat.stream_handle()))
     K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
     dK = torch.empty_like(K)
     rhs = torch.empty(num_dofs, dtype=torch.float64, device=dev)
@@ -204,11 +209,12 @@ def compute_B_Hcurl(nodes, cells, glob_ids, curls, phi, d_phi, orientations, x, 
     pts = torch.empty(num_el * Q * 3, dtype=torch.float64, device=dev)
     B = torch.empty_like(pts)
     if num_el:
+        hold = [as_dev_f64(np.asarray(nodes, dtype=np.float64).reshape(-1)), as_dev_i32(cells),
+                as_dev_i32(gids.reshape(-1)), as_dev_i32(orient), as_dev_f64(curls),
+                as_dev_f64(phi), as_dev_f64(d_phi), as_dev_f64(x)]
         nat.check(lib.sn_B_Hcurl_table_f64(
-            num_el, nat.ptr(as_dev_f64(np.asarray(nodes, dtype=np.float64).reshape(-1))),
-            nat.ptr(as_dev_i32(cells)), nat.ptr(as_dev_i32(gids.reshape(-1))),
-            nat.ptr(as_dev_i32(orient)), n_orient, nat.ptr(as_dev_f64(curls)),
-            nat.ptr(as_dev_f64(phi)), nat.ptr(as_dev_f64(d_phi)), Q, nat.ptr(as_dev_f64(x)),
+            num_el, nat.ptr(hold[0]), nat.ptr(hold[1]), nat.ptr(hold[2]), nat.ptr(hold[3]),
+            n_orient, nat.ptr(hold[4]), nat.ptr(hold[5]), nat.ptr(hold[6]), Q, nat.ptr(hold[7]),
             nat.ptr(pts), nat.ptr(B), nat.stream_handle()))
     return pts.cpu().numpy().reshape(-1, 3), B.cpu().numpy().reshape(-1, 3)
 
