@@ -240,6 +240,18 @@ int sn_absmax_f64(int n, const double *a_dev, const unsigned char *mask_dev,
 int sn_seg_dot_f64(int nseg, const int *row0_dev, const double *a_dev, const double *b_dev,
                    const unsigned char *mask_dev, double *out_host, void *stream);
 
+/* Allocation-free variant of sn_seg_dot_f64.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-segment sums go to a file-scope device array of 4096 doubles, one
+ * per device, instead of a buffer allocated per call; the call allocates and
+ * frees nothing and synchronises the stream to copy the sums out.  At most
+ * 4096 segments (more return SN_ERR_ARG).  Calls that overlap on one device
+ * share that array and race: serialise them (one stream, or events). */
+int sn_seg_dot_f64_noalloc(int nseg, const int *row0_dev, const double *a_dev,
+                           const double *b_dev, const unsigned char *mask_dev,
+                           double *out_host, void *stream);
+
 /* ------------------------------------------------------------------- PCG */
 
 typedef struct sn_cg sn_cg;

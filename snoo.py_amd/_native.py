@@ -44,6 +44,7 @@ SIGNATURES = {
     'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),
     'sn_seg_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'sn_seg_dot_f64_noalloc': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_absmax_f64': (_c_int, [_c_int, _vp, _vp, _P_dbl, _vp]),
     'sn_cg_create': (_c_int, [ctypes.POINTER(_vp), _vp, _c_int, _c_int, _c_int, _vp]),
     'sn_cg_destroy': (None, [_vp]),

@@ -19,6 +19,7 @@
 // stop flag once per chunk of iterations; each chunk is captured once as a
 // hipGraph and replayed.
 #include <climits>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -1410,6 +1411,50 @@ int sn_seg_dot_f64(int nseg, const int *row0_dev, const double *a, const double 
     SN_CHECK_LAUNCH();
     SN_HIP(hipMemcpyAsync(out_host, dev, nseg * sizeof(double), hipMemcpyDeviceToHost, s));
     SN_HIP(hipFreeAsync(dev, s));
+    SN_HIP(hipStreamSynchronize(s));
+    return SN_OK;
+}
+
+// Allocation-free segmented dot: the per-segment sums go to a file-scope
+// device array instead of a buffer allocated per call.  The array's host-side
+// name is not a device address: its address on each device is looked up once
+// per device (hipGetSymbolAddress, with that device current) and only that
+// address is passed to kernels and copies.
+constexpr int kSegDotScratchMax = 4096;
+static __device__ __align__(256) double scratch[kSegDotScratchMax];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[64] = {nullptr};
+
+static int seg_dot_scratch(double **out) {
+    int dev = 0;
+    SN_HIP(hipGetDevice(&dev));
+    SN_REQUIRE(dev >= 0 && dev < 64, "sn_seg_dot_f64_noalloc: device %d beyond 64", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        SN_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        SN_REQUIRE(ptr, "sn_seg_dot_f64_noalloc: no device address for the scratch array");
+        g_scratch_dev[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch_dev[dev];
+    return SN_OK;
+}
+
+int sn_seg_dot_f64_noalloc(int nseg, const int *row0_dev, const double *a, const double *b,
+                           const unsigned char *mask, double *out_host, void *stream) {
+    SN_REQUIRE(nseg >= 1 && row0_dev && a && b && out_host,
+               "sn_seg_dot_f64_noalloc: bad argument");
+    SN_REQUIRE(nseg <= kSegDotScratchMax,
+               "sn_seg_dot_f64_noalloc: %d segments exceed the scratch maximum of %d", nseg,
+               kSegDotScratchMax);
+    double *sc = nullptr;
+    int rc = seg_dot_scratch(&sc);
+    if (rc) return rc;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(seg_dot_kernel, dim3(nseg), dim3(kVecBlock), 0, s, row0_dev, a, b, mask, sc);
+    SN_CHECK_LAUNCH();
+    SN_HIP(hipMemcpyAsync(out_host, sc, nseg * sizeof(double), hipMemcpyDeviceToHost, s));
     SN_HIP(hipStreamSynchronize(s));
     return SN_OK;
 }
