@@ -434,6 +434,22 @@ int sn_adam_f32(int n, float *p_dev, const float *g_dev, float *m1_dev, float *m
                 void *pb_dev, float lr, float b1, float b2, float eps, int step, float wd,
                 void *stream);
 int sn_f32_to_bf16(int n, const float *x_dev, void *y_dev, void *stream);
+/* Reduced (POD-Galerkin) Newton step on the first r POD modes U [r][n]
+ * (mode k at U[k*n .. k*n+n)), with the Jacobian in the reference-curl form
+ * of sn_element_newton mode 2 (Mr float32 [6*num_el], element order of
+ * glob_ids/orient): H = U^T J U = sum_e W_e^T Mr_e W_e with W_e the 3 x r
+ * face circulations of the modes on element e.
+ * sn_pod_element_modes: W [num_el][3][r] (float32), once per model and mesh.
+ * sn_pod_reduced_jacobian: H (float64, r*r, row-major) to the host; r in
+ *   {32, 64, 128}; synchronises the stream.
+ * sn_pod_project: g[k] = sum_i U[k*n + i] v[i] (float64) to the host;
+ *   synchronises the stream. */
+int sn_pod_element_modes(int num_el, int n, int r, const float *U_dev, const int *glob_ids_dev,
+                         const int *orient_dev, float *W_dev, void *stream);
+int sn_pod_reduced_jacobian(int num_el, int r, const float *W_dev, const float *Mr_dev,
+                            double *H_host, void *stream);
+int sn_pod_project(int n, int r, const float *U_dev, const double *v_dev, double *g_host,
+                   void *stream);
 /* out[i] = mean[i] + sum_j U[j*n + i] c[j]  (mean may be NULL) */
 int sn_pod_expand(int n, int r, const float *U_dev, const float *c_dev, const double *mean_dev,
                   double *out_dev, void *stream);

@@ -63,6 +63,9 @@ SIGNATURES = {
                              ctypes.c_float, ctypes.c_float, _c_int, ctypes.c_float, _vp]),
     'sn_f32_to_bf16': (_c_int, [_c_int, _vp, _vp, _vp]),
     'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_pod_element_modes': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_pod_reduced_jacobian': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pod_project': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_edge_transfer': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_ams_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                                ctypes.c_float, ctypes.c_float, _vp]),
@@ -135,9 +138,15 @@ def check(rc):
 
 
 def ptr(t):
-    """Device (or host) address of a torch tensor, None for None."""
+    """Device (or host) address of a torch tensor, None for None.  The kernels
+    read dense row-major arrays: a strided view (a transpose, a column slice,
+    or a tensor made from a Fortran-ordered numpy array) raises instead of
+    handing its storage over in the wrong order."""
     if t is None:
         return None
+    if not t.is_contiguous():
+        raise ValueError('a strided tensor (shape {}, strides {}) cannot be passed to the C ABI; '
+                         'call .contiguous() first'.format(tuple(t.shape), t.stride()))
     return ctypes.c_void_p(t.data_ptr())
 
 

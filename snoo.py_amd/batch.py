@@ -289,16 +289,45 @@ class ConcurrentSolver:
     from zero first; every other variant then starts from its solution scaled
     by the ratio of the current densities, the solution of the linear part of
     the problem.  Each variant still runs Newton to ||R|| <= rtol ||R(0)|| of
-    its own, R(0) its residual at zero."""
+    its own, R(0) its residual at zero.  The anchor is the variant of median
+    current density; when it does not converge every variant starts from zero.
+
+    ``warm_start='learned'`` with ``model`` (a trained WarmStart whose family
+    topology every variant shares): the network predicts every variant's POD
+    coefficients in one batched pass, then each worker expands its variants'
+    predictions, takes one reduced Newton step in the leading modes and keeps
+    the result if its residual is below the residual at zero (WarmStart.start);
+    Newton then runs to the same tolerance."""
 
     def __init__(self, dipoles, workers=4, quad_order=2, precision='mixed', precond='ams',
-                 lanes_per_row=8, warm_start=None):
+                 lanes_per_row=8, warm_start=None, model=None):
         import torch
         self.dipoles = list(dipoles)
         self.workers = max(1, int(workers))
         self.kw = dict(precision=precision, precond=precond, lanes_per_row=lanes_per_row)
-        if warm_start not in (None, 'anchor'):
-            raise ValueError("warm_start must be None or 'anchor'")
+        self.set_warm_start(warm_start, model)      # checks the variants first
+        self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
+        self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
+
+    def set_warm_start(self, warm_start, model=None):
+        """Choose the start of the next solves: None (A = 0), 'anchor' or
+        'learned' (with ``model``); the variants are checked for it."""
+        if warm_start not in (None, 'anchor', 'learned'):
+            raise ValueError("warm_start must be None, 'anchor' or 'learned'")
+        precision, precond = self.kw['precision'], self.kw['precond']
+        if warm_start == 'learned':
+            if model is None:
+                raise ValueError("learned warm start: pass the trained model (model=WarmStart)")
+            if precision != 'mixed' or precond != 'ams':
+                raise ValueError('learned warm start: the reduced Newton step needs the mixed '
+                                 'solver with the auxiliary-space preconditioner')
+            fm = model.family.mesh
+            for d in self.dipoles:
+                if d.mesh.cells.shape != fm.cells.shape or not np.array_equal(d.mesh.cells,
+                                                                             fm.cells):
+                    raise ValueError('learned warm start: every variant must be on the '
+                                     "model's family topology")
+        self.model = model
         if warm_start == 'anchor':
             m0, d0 = self.dipoles[0].mesh, self.dipoles[0]
             for d in self.dipoles:
@@ -313,20 +342,26 @@ class ConcurrentSolver:
                     raise ValueError('anchor warm start: the variants may differ in the coil '
                                      'current density J only')
         self.warm_start = warm_start
-        self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
-        self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
 
-    def _run(self, w, idx, rtol, out, x0s=None):
+    def _run(self, w, idx, rtol, out, x0s=None, coeffs=None):
         import torch
         with torch.cuda.stream(self.streams[w]):
             for i in idx:
                 s = self.solvers[i]
                 s._src = None
-                x = s.solve_newton(self.kw['precision'], rtol=rtol,
-                                   x0=None if x0s is None else x0s.get(int(i)),
+                x0, info = None if x0s is None else x0s.get(int(i)), None
+                if coeffs is not None:
+                    from .warm_start import PARAM_NAMES
+                    p = {k: self.dipoles[i].p[k] for k in PARAM_NAMES}
+                    x0, info = self.model.start(s, p, coeffs=coeffs[i],
+                                                lanes_per_row=self.kw['lanes_per_row'])
+                x = s.solve_newton(self.kw['precision'], rtol=rtol, x0=x0,
                                    lanes_per_row=self.kw['lanes_per_row'],
                                    precond=self.kw['precond'])
-                out[i] = (x, dict(s.stats))
+                st = dict(s.stats)
+                if info is not None:
+                    st['warm_start'] = info
+                out[i] = (x, st)
             torch.cuda.current_stream().synchronize()
 
     def solve(self, rtol=1e-8):
@@ -347,9 +382,17 @@ class ConcurrentSolver:
                 x0s = {i: xa * (float(d.p['J']) / ja) for i, d in enumerate(self.dipoles)
                        if i != a}
             # else: the anchor failed, every other variant starts from zero
+        coeffs = None
+        if self.warm_start == 'learned':
+            # one batched forward pass of the network for every variant (the
+            # network's buffers are not shared across threads)
+            from .warm_start import PARAM_NAMES
+            coeffs = self.model.coefficients([{k: d.p[k] for k in PARAM_NAMES}
+                                              for d in self.dipoles])
+            self.model.to_device(self.solvers[0].curl_curl_factory)
         parts = [order[w::self.workers] for w in range(self.workers)]
         torch.cuda.synchronize()
-        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out, x0s))
+        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out, x0s, coeffs))
                    for w in range(self.workers)]
         for t in threads:
             t.start()

@@ -1,0 +1,213 @@
+// Reduced (POD-Galerkin) Newton step of the learned warm start.  No
+// counterpart in the reference, whose Newton starts from A = 0 or a given x_0
+// (snoopy/red_mvp_solver.py:146-147,174).
+//
+// With U the first r POD modes (U[k*n + i], mode k contiguous) and the
+// Jacobian in the reference-curl form of the element kernel (J_e[j][l] =
+// s_j s_l c_j . Mr_e c_l, Mr_e a symmetric 3x3 per element), the reduced
+// Jacobian is
+//     H = U^T J U = sum_e W_e^T Mr_e W_e,   W_e[:, k] = sum_j s_j c_j U[k][g_ej],
+// where W_e (3 x r) are the face circulations of the modes on element e.  W
+// depends on the mesh topology and the modes only, so it is formed once per
+// trained model; every correction then streams W and the 6 Mr values per
+// element once.  Arithmetic: FP32 products with FP32 accumulation inside a
+// workgroup's element range, partial sums combined in FP64 in a fixed order.
+#include "sn_common.hpp"
+
+namespace sn {
+
+// rows of the integer reference curls c_j of the local edges (0,1), (0,2),
+// (0,3), (1,2), (1,3), (2,3), the convention of the element kernels
+__constant__ int kCurl[6][3] = {{0, -1, 1}, {1, 0, -1}, {-1, 1, 0},
+                                {0, 0, 1},  {0, -1, 0}, {1, 0, 0}};
+
+__global__ void pod_element_modes_kernel(int num_el, int n, int r, const float *__restrict__ U,
+                                         const int *__restrict__ gids,
+                                         const int *__restrict__ orient, float *__restrict__ W) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)num_el * r) return;
+    const int e = (int)(t % num_el);     // element fastest: neighbouring DOFs per wave
+    const int k = (int)(t / num_el);
+    const int o = orient[e];
+    float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const float u = U[(size_t)k * n + gids[6 * e + j]];
+        const float su = ((o >> j) & 1) ? -u : u;
+        w0 += kCurl[j][0] * su;
+        w1 += kCurl[j][1] * su;
+        w2 += kCurl[j][2] * su;
+    }
+    float *dst = W + (size_t)e * 3 * r;
+    dst[k] = w0;
+    dst[r + k] = w1;
+    dst[2 * r + k] = w2;
+}
+
+constexpr int kGalBlock = 256;
+constexpr int kGalEB = 16;   // elements staged in LDS per step
+
+// Partial H of a contiguous element range per workgroup; thread t owns the
+// TK x TK block of rows (t / 16) * TK.. and columns (t % 16) * TK.. of H.
+template <int R>
+__global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
+    int num_el, const float *__restrict__ W, const float *__restrict__ Mr,
+    float *__restrict__ part) {
+    constexpr int TK = R / 16;
+    __shared__ float sW[kGalEB][3][R];
+    __shared__ float sZ[kGalEB][3][R];
+    __shared__ float sM[kGalEB][6];
+    const int tid = threadIdx.x;
+    const int kb = (tid / 16) * TK, lb = (tid % 16) * TK;
+    float acc[TK][TK];
+#pragma unroll
+    for (int a = 0; a < TK; ++a)
+#pragma unroll
+        for (int b = 0; b < TK; ++b) acc[a][b] = 0.0f;
+    const int nb = (num_el + kGalEB - 1) / kGalEB;
+    const int per = (nb + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per, b1 = min(nb, b0 + per);
+    for (int bt = b0; bt < b1; ++bt) {
+        const int e0 = bt * kGalEB;
+        for (int v = tid; v < kGalEB * 3 * R; v += kGalBlock) {
+            const int le = v / (3 * R);
+            const int e = e0 + le;
+            (&sW[0][0][0])[v] = (e < num_el) ? W[(size_t)e0 * 3 * R + v] : 0.0f;
+        }
+        if (tid < kGalEB * 6) {
+            const int le = tid / 6, u = tid % 6, e = e0 + le;
+            sM[le][u] = (e < num_el) ? Mr[(size_t)u * num_el + e] : 0.0f;
+        }
+        __syncthreads();
+        // Z_e = Mr_e W_e (Mr upper triangle 00 01 02 11 12 22)
+        for (int v = tid; v < kGalEB * R; v += kGalBlock) {
+            const int le = v / R, l = v % R;
+            const float *m = sM[le];
+            const float x0 = sW[le][0][l], x1 = sW[le][1][l], x2 = sW[le][2][l];
+            sZ[le][0][l] = m[0] * x0 + m[1] * x1 + m[2] * x2;
+            sZ[le][1][l] = m[1] * x0 + m[3] * x1 + m[4] * x2;
+            sZ[le][2][l] = m[2] * x0 + m[4] * x1 + m[5] * x2;
+        }
+        __syncthreads();
+        for (int le = 0; le < kGalEB; ++le)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float wk[TK], zl[TK];
+#pragma unroll
+                for (int q = 0; q < TK; ++q) {
+                    wk[q] = sW[le][a][kb + q];
+                    zl[q] = sZ[le][a][lb + q];
+                }
+#pragma unroll
+                for (int p = 0; p < TK; ++p)
+#pragma unroll
+                    for (int q = 0; q < TK; ++q) acc[p][q] += wk[p] * zl[q];
+            }
+        __syncthreads();
+    }
+    float *dst = part + (size_t)blockIdx.x * R * R;
+#pragma unroll
+    for (int p = 0; p < TK; ++p)
+#pragma unroll
+        for (int q = 0; q < TK; ++q) dst[(kb + p) * R + lb + q] = acc[p][q];
+}
+
+__global__ void sum_partials_kernel(int G, int m, const float *__restrict__ part,
+                                    double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int g = 0; g < G; ++g) s += (double)part[(size_t)g * m + i];   // fixed order
+    out[i] = s;
+}
+
+// g[k] = sum_i U[k*n + i] v[i], one workgroup per mode
+__global__ __launch_bounds__(256) void pod_project_kernel(int n, const float *__restrict__ U,
+                                                          const double *__restrict__ v,
+                                                          double *__restrict__ g) {
+    __shared__ double lds[256 / kWave];
+    const int k = blockIdx.x;
+    const float *u = U + (size_t)k * n;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += (double)u[i] * v[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) g[k] = s;
+}
+
+}  // namespace sn
+
+using namespace sn;
+
+extern "C" {
+
+int sn_pod_element_modes(int num_el, int n, int r, const float *U, const int *gids,
+                         const int *orient, float *W, void *stream) {
+    SN_REQUIRE(num_el > 0 && n > 0 && r > 0 && U && gids && orient && W,
+               "sn_pod_element_modes: bad argument");
+    const long long tot = (long long)num_el * r;
+    hipLaunchKernelGGL(pod_element_modes_kernel, dim3(grid_for(tot, 256, 1 << 30)), dim3(256), 0,
+                       as_stream(stream), num_el, n, r, U, gids, orient, W);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, double *H_host,
+                            void *stream) {
+    SN_REQUIRE(num_el > 0 && W && Mr && H_host, "sn_pod_reduced_jacobian: bad argument");
+    SN_REQUIRE(r == 32 || r == 64 || r == 128,
+               "sn_pod_reduced_jacobian: r must be 32, 64 or 128 (r=%d)", r);
+    hipStream_t s = as_stream(stream);
+    const int nb = (num_el + kGalEB - 1) / kGalEB;
+    const int G = nb < 1024 ? nb : 1024;
+    const int m = r * r;
+    float *part = nullptr;
+    double *H = nullptr;
+    SN_HIP(hipMallocAsync((void **)&part, (size_t)G * m * sizeof(float), s));
+    SN_HIP(hipMallocAsync((void **)&H, (size_t)m * sizeof(double), s));
+    if (r == 32)
+        hipLaunchKernelGGL(pod_reduced_jacobian_kernel<32>, dim3(G), dim3(kGalBlock), 0, s,
+                           num_el, W, Mr, part);
+    else if (r == 64)
+        hipLaunchKernelGGL(pod_reduced_jacobian_kernel<64>, dim3(G), dim3(kGalBlock), 0, s,
+                           num_el, W, Mr, part);
+    else
+        hipLaunchKernelGGL(pod_reduced_jacobian_kernel<128>, dim3(G), dim3(kGalBlock), 0, s,
+                           num_el, W, Mr, part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sum_partials_kernel, dim3((m + 255) / 256), dim3(256), 0, s, G, m,
+                           part, H);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(H_host, H, (size_t)m * sizeof(double),
+                                            hipMemcpyDeviceToHost, s);
+    hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(H, s);
+    if (e == hipSuccess) e = f1 != hipSuccess ? f1 : f2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        set_error("sn_pod_reduced_jacobian: %s", hipGetErrorString(e));
+        return SN_ERR_HIP;
+    }
+    return SN_OK;
+}
+
+int sn_pod_project(int n, int r, const float *U, const double *v, double *g_host, void *stream) {
+    SN_REQUIRE(n > 0 && r > 0 && U && v && g_host, "sn_pod_project: bad argument");
+    hipStream_t s = as_stream(stream);
+    double *g = nullptr;
+    SN_HIP(hipMallocAsync((void **)&g, (size_t)r * sizeof(double), s));
+    hipLaunchKernelGGL(pod_project_kernel, dim3(r), dim3(256), 0, s, n, U, v, g);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(g_host, g, (size_t)r * sizeof(double),
+                                            hipMemcpyDeviceToHost, s);
+    hipError_t f = hipFreeAsync(g, s);
+    if (e == hipSuccess) e = f;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        set_error("sn_pod_project: %s", hipGetErrorString(e));
+        return SN_ERR_HIP;
+    }
+    return SN_OK;
+}
+
+}  // extern "C"

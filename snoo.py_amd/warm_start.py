@@ -65,16 +65,45 @@ def _fixed_axis(seg_bounds, counts, pad, n_pad, ratio):
     return np.concatenate(parts)
 
 
+def _breaks(p, pad):
+    """Geometry break points of each axis of a single-yoke dipole, outer
+    boundary (yoke bounds +- pad) included."""
+    bx = [-p['W'] / 2 - pad, -p['W'] / 2, -p['ax'] / 2, p['ax'] / 2, p['W'] / 2, p['W'] / 2 + pad]
+    by = [-p['H'] / 2 - pad, -p['H'] / 2, -p['ay'] / 2, p['ay'] / 2, p['H'] / 2, p['H'] / 2 + pad]
+    bz = [-p['L'] / 2 - pad, -p['L'] / 2, p['L'] / 2, p['L'] / 2 + pad]
+    return bx, by, bz
+
+
 class MorphFamily:
     """Fixed-topology dipole meshes over the warm-start parameter ranges.
 
-    counts = (n_pad, n_wall_x, n_ap_x, n_wall_y, n_ap_y, n_len).  The default
-    gives 205 632 tetrahedra."""
+    Two constructions share one rule: every variant's grid lines are those of
+    a reference grid moved so that the geometry's break points (pad
+    boundary, yoke faces, aperture faces) land on the variant's.
+    * ``counts = (n_pad, n_wall_x, n_ap_x, n_wall_y, n_ap_y, n_len)``: a grid
+      with these cell counts per segment (graded pads, uniform inner
+      segments); the default gives 205 632 tetrahedra.
+    * ``base`` (a single-yoke Dipole): that dipole's own mesh is the
+      reference, moved piecewise-linearly between the break points, so the
+      family contains the dipole's mesh exactly (its parameters give its
+      nodes)."""
 
-    def __init__(self, counts=(7, 6, 10, 6, 8, 14), pad=0.5, ratio=1.3):
+    def __init__(self, counts=(7, 6, 10, 6, 8, 14), pad=0.5, ratio=1.3, base=None):
         self.counts = tuple(int(c) for c in counts)
         self.pad, self.ratio = pad, ratio
-        base = self.axes(dict(DEFAULT))
+        self.base = base
+        if base is not None:
+            if len(base.yokes) != 1 or base.p.get('z0', 0.0) != 0.0:
+                raise ValueError('MorphFamily: the base dipole must have one yoke at z0 = 0')
+            self.pad = base.p['pad']
+            self.p0 = {k: base.p[k] for k in PARAM_NAMES}
+            self._ref_axes = base.axes
+            self._ref_breaks = _breaks(base.p, self.pad)
+            self.mesh = base.mesh
+            self.model = base.model
+            return
+        self.p0 = {k: DEFAULT[k] for k in PARAM_NAMES}
+        base_axes = self.axes(dict(DEFAULT))
         p0 = dict(DEFAULT)
 
         def tag_fn(xc, yc, zc):
@@ -82,10 +111,15 @@ class MorphFamily:
                      (np.abs(zc) < p0['L'] / 2)
             in_ap = (np.abs(xc) < p0['ax'] / 2) & (np.abs(yc) < p0['ay'] / 2)
             return np.where(in_box & ~in_ap, IRON_TAG, AIR_TAG)
-        self.mesh = kuhn_mesh(*base, tag_fn)
+        self.mesh = kuhn_mesh(*base_axes, tag_fn)
         self.model = TetModel(self.mesh)
 
     def axes(self, p):
+        p = dict(self.base.p if self.base is not None else DEFAULT, **p)
+        if self.base is not None:
+            new = _breaks(p, self.pad)
+            return tuple(np.interp(a, r, b) for a, r, b in zip(self._ref_axes, self._ref_breaks,
+                                                               new))
         n_pad, nwx, nax, nwy, nay, nl = self.counts
         xs = _fixed_axis([-p['W'] / 2, -p['ax'] / 2, p['ax'] / 2, p['W'] / 2], [nwx, nax, nwx],
                          self.pad, n_pad, self.ratio)
@@ -100,9 +134,11 @@ class MorphFamily:
         return np.stack([X.transpose(2, 1, 0).ravel(), Y.transpose(2, 1, 0).ravel(),
                          Z.transpose(2, 1, 0).ravel()], axis=1)
 
-    def variant(self, p):
-        """Dipole object of parameters p on this topology (coils and reluctivity)."""
-        full = dict(DEFAULT)
+    def variant(self, p, model=None):
+        """Dipole object of parameters p on this topology (coils and reluctivity);
+        ``model`` (a TetModel of a private copy of the family mesh) replaces
+        the shared one."""
+        full = dict(self.base.p if self.base is not None else DEFAULT)
         full.update(p)
         d = Dipole.__new__(Dipole)
         d.p = full
@@ -111,25 +147,34 @@ class MorphFamily:
                         z0=0.0, sign=1.0)]
         d.coils = d._coils_for(d.yokes[0])
         d.reluctance = tanh_reluctance(full['B_sat'], full['mu_r'])
-        d.mesh = self.mesh
-        d.model = self.model
+        d.model = model if model is not None else self.model
+        d.mesh = d.model.tet_mesh
         return d
+
+    def private_model(self):
+        """A TetModel over a copy of the family mesh with its own node array
+        (solvers on several streams hold different variants at once)."""
+        import copy
+        m = copy.copy(self.mesh)
+        m.nodes = self.mesh.nodes.copy()
+        return TetModel(m)
 
 
 class FamilySolver:
     """One RedMVPSolver on the family topology, re-targeted per variant by
     replacing node coordinates, coils and reluctivity on the device."""
 
-    def __init__(self, family, quad_order=2):
+    def __init__(self, family, quad_order=2, private=True):
         self.family = family
-        d0 = family.variant(dict(DEFAULT))
+        self.model = family.private_model() if private else family.model
+        d0 = family.variant(dict(family.p0), self.model)
         self.solver = d0.solver(quad_order=quad_order, verbose=False)
         self.asm = self.solver.curl_curl_factory
         self.asm._device_state()
 
     def set_variant(self, p):
         import torch
-        d = self.family.variant(p)
+        d = self.family.variant(p, self.model)
         nodes = self.family.nodes(d.p)
         self.asm.tet_mesh.nodes[:] = nodes
         self.asm._dev['nodes'].copy_(torch.from_numpy(nodes.reshape(-1)).to(device()))
@@ -143,40 +188,148 @@ class FamilySolver:
         return self.solver.solve_newton(precision, rtol=rtol, x0=x0, precond=precond)
 
 
-def generate(family_solver, seeds, rtol=1e-6, log=None):
-    """Converged edge DOFs (float64, host) of the variants with the given seeds."""
-    X, P = [], []
+class EnergyInner:
+    """The inner product (x, y)_K0 = x^T K0 y with K0 the curl-curl matrix of
+    unit reluctivity on the family's reference geometry (the default
+    parameters).  K0 annihilates discrete gradients (the curl of a gradient
+    is zero whatever the geometry), so this inner product sees only the field
+    B = curl A: two solutions that differ by a gradient, which the Newton
+    solver's iterates pick up from the preconditioner, are the same vector in
+    it.  K0 is held as float64 element matrices in reference-curl form
+    (sn_element_newton mode 2 at A = 0, no source) and applied with
+    sn_element_apply_f64."""
+
+    def __init__(self, fsolver):
+        import torch
+        lib = nat.load()
+        fsolver.set_variant(dict(fsolver.family.p0))
+        asm = fsolver.asm
+        d = asm._device_state()
+        self.d, self.n = d, asm.num_dofs
+        N = d['num_el']
+        T = asm.tables(fsolver.solver.quad_order)
+        dev = device()
+        self.Mr = torch.empty(6 * N, dtype=torch.float64, device=dev)
+        loc6 = torch.empty(6 * N, dtype=torch.float64, device=dev)
+        x0 = torch.zeros(self.n, dtype=torch.float64, device=dev)
+        one = as_dev_f64(np.array([1.0]))
+        knots = as_dev_f64(np.zeros(1))
+        nat.check(lib.sn_element_newton(
+            0, 2, N, 0, N, nat.ptr(d['nodes']), nat.ptr(d['cells']), nat.ptr(d['gids']),
+            nat.ptr(d['orient']), nat.ptr(T.curls), nat.ptr(T.dphi), nat.ptr(T.w), T.Q, 0, -1,
+            nat.ptr(one), nat.ptr(knots), nat.ptr(x0), None, nat.ptr(self.Mr), nat.ptr(loc6), None,
+            nat.ptr(T.phi), None, nat.stream_handle()))
+        self.mask = fsolver.solver.mask_device()
+        torch.cuda.current_stream().synchronize()
+
+    def apply(self, x, out=None, work=None):
+        """K0 x (float64 device vectors; Dirichlet rows and columns removed)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.float64, device=device())
+        if work is None:
+            work = torch.empty(6 * self.d['num_el'], dtype=torch.float64, device=device())
+        nat.check(nat.load().sn_element_apply_f64(
+            self.d['pattern'], nat.ptr(self.d['gids']), nat.ptr(self.d['orient']),
+            nat.ptr(self.Mr), nat.ptr(self.mask), nat.ptr(x), nat.ptr(work), nat.ptr(out),
+            nat.stream_handle()))
+        return out
+
+
+def generate(family_solvers, seeds, rtol=1e-8, energy=None, log=None):
+    """Converged edge DOFs of the variants with the given seeds, float32
+    [len(seeds), n] on the host, and (with ``energy``) K0 x of each.  With
+    several family solvers the variants are shared out over as many threads,
+    each on its own HIP stream (default priority)."""
+    import threading
+    import torch
+    fss = family_solvers if isinstance(family_solvers, (list, tuple)) else [family_solvers]
+    seeds = list(seeds)
+    n = fss[0].asm.num_dofs
+    X = np.empty((len(seeds), n), dtype=np.float32)
+    KX = np.empty((len(seeds), n), dtype=np.float32) if energy is not None else None
+    P = [sample_params(s) for s in seeds]
+    stats = np.zeros((len(seeds), 3))             # newton steps, CG iterations, converged
     t0 = time.perf_counter()
-    for k, s in enumerate(seeds):
-        p = sample_params(s)
-        x = family_solver.solve(p, rtol=rtol)
-        X.append(x.cpu().numpy())
-        P.append(p)
-        if log is not None and (k % max(len(seeds) // 8, 1) == 0):
-            log('  snapshot %d/%d seed %d newton %d (%.1f s)' % (
-                k + 1, len(seeds), s, family_solver.solver.stats['newton_steps'],
-                time.perf_counter() - t0))
-    return np.stack(X), P
+    errors = []
+
+    def work(w):
+        fs = fss[w]
+        stream = torch.cuda.Stream()
+        try:
+            with torch.cuda.stream(stream):
+                buf = torch.empty(6 * fs.asm._device_state()['num_el'], dtype=torch.float64,
+                                  device=device()) if energy is not None else None
+                kx = torch.empty(n, dtype=torch.float64, device=device()) \
+                    if energy is not None else None
+                for k in range(w, len(seeds), len(fss)):
+                    x = fs.solve(P[k], rtol=rtol)
+                    st = fs.solver.stats
+                    stats[k] = (st['newton_steps'], sum(st['cg_iterations']), st['converged'])
+                    X[k] = x.cpu().numpy()
+                    if energy is not None:
+                        energy.apply(x, kx, buf)
+                        KX[k] = kx.cpu().numpy()
+                    if log is not None and w == 0 and k % max(len(seeds) // 8, 1) < len(fss):
+                        log('  snapshot %d/%d newton %d (%.1f s)'
+                            % (k + 1, len(seeds), st['newton_steps'], time.perf_counter() - t0))
+                stream.synchronize()
+        except Exception as e:     # re-raised in the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(w,)) for w in range(len(fss))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return X, P, KX, stats
 
 
 class POD:
-    """x ~ mean + U a, U orthonormal columns (method of snapshots)."""
+    """x ~ mean + U a with U [n, r] (method of snapshots).
 
-    def __init__(self, X, r):
-        X = np.asarray(X, dtype=np.float64)
-        self.mean = X.mean(axis=0)
-        Xc = (X - self.mean).astype(np.float32)
-        C = (Xc @ Xc.T).astype(np.float64)
+    With ``KX`` (K0 x of every snapshot, EnergyInner) the modes are
+    orthonormal in the K0 inner product and a = U^T K0 (x - mean): the
+    decomposition and its coefficients see only B, not the gradient part of
+    x, so no mode is spent on the solver's arbitrary gauge.  Without it, the
+    Euclidean inner product."""
+
+    def __init__(self, X, r, KX=None):
+        X = np.asarray(X, dtype=np.float32)
+        self.mean = X.astype(np.float64).mean(axis=0)
+        Xc = X - self.mean.astype(np.float32)
+        if KX is not None:
+            kmean = np.asarray(KX, dtype=np.float64).mean(axis=0)
+            Kc = np.asarray(KX, dtype=np.float32) - kmean.astype(np.float32)
+            C = (Xc @ Kc.T).astype(np.float64)
+            C = 0.5 * (C + C.T)
+        else:
+            Kc = None
+            C = (Xc @ Xc.T).astype(np.float64)
         lam_all, V = np.linalg.eigh(C)
         order = np.argsort(lam_all)[::-1][:r]
-        lam, V = np.clip(lam_all[order], 1e-300, None), V[:, order]
+        lam, V = lam_all[order], V[:, order]
+        keep = lam > lam[0] * 1e-14
+        lam, V = lam[keep], V[:, keep]
         s = np.sqrt(lam)
         self.U = (Xc.T @ V.astype(np.float32)) / s.astype(np.float32)      # [n, r]
+        self.KU = (Kc.T @ V.astype(np.float32)) / s.astype(np.float32) if Kc is not None else None
         # deterministic signs: largest-magnitude entry of each mode positive
         sign = np.sign(self.U[np.abs(self.U).argmax(axis=0), np.arange(self.U.shape[1])])
         self.U *= sign
+        if self.KU is not None:
+            self.KU *= sign
         self.coeffs = (V * s * sign).astype(np.float64)                    # [n_snap, r]
         self.energy = lam / np.sum(np.clip(lam_all, 0, None))
+        self.lam = lam
+
+    def project(self, x):
+        """Coefficients of a host vector x (in the POD's inner product)."""
+        xc = np.asarray(x, dtype=np.float64) - self.mean
+        B = self.KU if self.KU is not None else self.U
+        return B.T.astype(np.float64) @ xc
 
 
 def transfer_operator(src_mesh, dst_mesh):
@@ -215,54 +368,142 @@ class WarmStart:
     def __init__(self, family, pod, mlp, coeff_std):
         self.family, self.pod, self.mlp, self.coeff_std = family, pod, mlp, coeff_std
         self._U = None
+        self._W = None
+        self.r_galerkin = 0
         self._transfer = {}
 
     @classmethod
-    def train(cls, family, n_train=2048, r=256, seeds0=1000, epochs=3000, rtol=1e-6,
-              hidden=(512, 512, 512, 512), log=None, fsolver=None):
+    def train(cls, family, n_train=2048, r=256, seeds0=1000, epochs=3000, rtol=1e-8,
+              hidden=(512, 512, 512, 512), log=None, fsolver=None, workers=4, energy=True,
+              lr=2e-3):
+        """Snapshots of ``n_train`` seeded variants (Newton to ``rtol``, on
+        ``workers`` streams), the POD (in the K0 inner product with
+        ``energy``), and the MLP trained on the normalised coefficients."""
         import torch
         from .mlp import MLP
         fs = fsolver or FamilySolver(family)
+        fss = [fs] + [FamilySolver(family) for _ in range(max(workers, 1) - 1)]
         t0 = time.perf_counter()
-        X, P = generate(fs, range(seeds0, seeds0 + n_train), rtol=rtol, log=log)
+        K0 = EnergyInner(fs) if energy else None
+        X, P, KX, gst = generate(fss, range(seeds0, seeds0 + n_train), rtol=rtol, energy=K0,
+                                 log=log)
         t_data = time.perf_counter() - t0
         r = min(r, n_train)
-        pod = POD(X, r)
+        pod = POD(X, r, KX)
+        del X, KX
         t_pod = time.perf_counter() - t0 - t_data
+        r = pod.U.shape[1]
         std = pod.coeffs.std(axis=0) + 1e-30
-        T = (pod.coeffs / std).astype(np.float32)
+        T = np.ascontiguousarray(pod.coeffs / std, dtype=np.float32)
         Xin = normalise(P)
         mlp = MLP((8,) + tuple(hidden) + (r,), seed=0)
         hist = mlp.train(torch.from_numpy(Xin).to(device()), torch.from_numpy(T).to(device()),
-                         epochs=epochs, lr=2e-3, log=log)
+                         epochs=epochs, lr=lr, log=log)
         torch.cuda.synchronize()
         ws = cls(family, pod, mlp, std)
+        ws.energy_inner = K0
         ws.train_stats = {'t_data': t_data, 't_pod': t_pod,
                           't_train': time.perf_counter() - t0 - t_data - t_pod,
                           'loss_first': hist[0], 'loss_last': hist[-1], 'n_train': n_train,
-                          'modes': r, 'pod_energy': float(pod.energy[:r].sum())}
+                          'modes': r, 'pod_energy': float(pod.energy[:r].sum()),
+                          'snapshot_newton_mean': float(gst[:, 0].mean()),
+                          'snapshot_cg_mean': float(gst[:, 1].mean()),
+                          'snapshots_converged': int(gst[:, 2].sum()), 'workers': len(fss),
+                          'inner_product': 'K0' if energy else 'euclidean'}
         ws.fsolver = fs
+        ws.fsolvers = fss
         return ws
 
     def coefficients(self, params_list):
+        """POD coefficients [len(params_list), r] (host float64) predicted by
+        the network in one batched forward pass."""
         import torch
         Xin = torch.from_numpy(normalise(params_list)).to(device())
         a = self.mlp.forward(Xin).cpu().numpy().astype(np.float64)
         return a * self.coeff_std
 
-    def predict_family(self, p):
-        """Predicted edge DOFs on the family topology (device, float64)."""
+    def to_device(self, asm=None, r_galerkin=64):
+        """Upload the modes and the mean (once), and, given an assembler of the
+        family topology, the face circulations W of the first ``r_galerkin``
+        modes for the reduced Newton step (sn_pod_element_modes)."""
         import torch
         if self._U is None:
             self._U = torch.from_numpy(np.ascontiguousarray(self.pod.U.T)).to(device())
             self._mean = as_dev_f64(self.pod.mean)
-        c = torch.from_numpy(self.coefficients([p])[0].astype(np.float32)).to(device())
+        sizes = [k for k in (32, 64, 128) if k <= min(self.pod.U.shape[1], r_galerkin)]
+        rg = max(sizes) if sizes else 0
+        if asm is not None and rg and self._W is None:
+            d = asm._device_state()
+            N, n = d['num_el'], asm.num_dofs
+            self._W = torch.empty(N * 3 * rg, dtype=torch.float32, device=device())
+            nat.check(nat.load().sn_pod_element_modes(N, n, rg, nat.ptr(self._U),
+                                                      nat.ptr(d['gids']), nat.ptr(d['orient']),
+                                                      nat.ptr(self._W), nat.stream_handle()))
+            self.r_galerkin = rg
+        return self
+
+    def expand(self, a, out=None):
+        """mean + U a on the family topology (device, float64)."""
+        import torch
+        self.to_device()
+        c = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(device())
         n = self.pod.U.shape[0]
-        out = torch.empty(n, dtype=torch.float64, device=device())
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device=device())
         nat.check(nat.load().sn_pod_expand(n, int(c.numel()), nat.ptr(self._U), nat.ptr(c),
                                            nat.ptr(self._mean), nat.ptr(out),
                                            nat.stream_handle()))
         return out
+
+    def predict_family(self, p):
+        """Predicted edge DOFs on the family topology (device, float64)."""
+        return self.expand(self.coefficients([p])[0])
+
+    def galerkin(self, drv, x, steps=1):
+        """Reduced Newton steps in the span of the first r_galerkin modes:
+        solve (U^T J U) da = -U^T R(x) and set x += U da, with J and R at x
+        from the mixed driver ``drv`` (element-operator Jacobian).  x is
+        updated in place and returned."""
+        import torch
+        lib = nat.load()
+        rg = self.r_galerkin
+        N, n = drv.d['num_el'], drv.asm.num_dofs
+        H = np.empty(rg * rg)
+        g = np.empty(rg)
+        for _ in range(steps):
+            drv.residual(x)
+            nat.check(lib.sn_pod_reduced_jacobian(N, rg, nat.ptr(self._W), nat.ptr(drv.Mr),
+                                                  H.ctypes.data_as(nat._vp), nat.stream_handle()))
+            nat.check(lib.sn_pod_project(n, rg, nat.ptr(self._U), nat.ptr(drv.R),
+                                         g.ctypes.data_as(nat._vp), nat.stream_handle()))
+            Hm = H.reshape(rg, rg)
+            Hm = 0.5 * (Hm + Hm.T)
+            try:
+                da = np.linalg.solve(Hm, -g)
+            except np.linalg.LinAlgError:
+                da = np.linalg.lstsq(Hm, -g, rcond=None)[0]
+            if not np.all(np.isfinite(da)):
+                break
+            c = torch.from_numpy(da.astype(np.float32)).to(device())
+            nat.check(lib.sn_pod_expand(n, rg, nat.ptr(self._U), nat.ptr(c), nat.ptr(x),
+                                        nat.ptr(x), nat.stream_handle()))
+        return x
+
+    def start(self, solver, p, coeffs=None, galerkin_steps=1, lanes_per_row=8):
+        """Start vector for ``solver`` (a solver on the family topology whose
+        current variant has parameters p): the network's prediction, one
+        reduced Newton step, and the residual judge -- the start is kept only
+        if ||R(x0)|| < ||R(0)||.  Returns (x0 or None, info)."""
+        a = self.coefficients([p])[0] if coeffs is None else coeffs
+        drv = solver.newton_driver('mixed', lanes_per_row, 32, 'ams')
+        drv.retarget(solver.source_fields_device())
+        self.to_device(solver.curl_curl_factory)
+        x = self.expand(a)
+        if galerkin_steps and self.r_galerkin:
+            self.galerkin(drv, x, galerkin_steps)
+        r = drv.residual(x) / drv.r0()
+        use = bool(np.isfinite(r) and r < 1.0)
+        return (x if use else None), {'r_start_rel': float(r), 'used': use}
 
     def prepare(self, dp):
         """Host set-up of the transfer onto dipole ``dp``'s mesh (once per mesh)."""
@@ -271,9 +512,9 @@ class WarmStart:
         if key not in self._transfer:
             p = {k: dp.p[k] for k in PARAM_NAMES}
             src = copy.copy(self.family.mesh)
-            src.nodes = self.family.nodes(dict(DEFAULT, **p))
+            src.nodes = self.family.nodes(p)
             from .mesh import StructuredGrid
-            src.grid = StructuredGrid(*self.family.axes(dict(DEFAULT, **p)))
+            src.grid = StructuredGrid(*self.family.axes(p))
             idx, w = transfer_operator(src, dp.mesh)
             self._transfer[key] = (as_dev_i32(idx.reshape(-1)), as_dev_f64(w.reshape(-1)), p)
         return self._transfer[key]
@@ -318,13 +559,15 @@ def load_default(dp=None):
     return WarmStart.load(path)
 
 
-def solve_with_warm_start(solver, ws, dp, precision='mixed', rtol=1e-8, log=None):
-    """Newton from the network's prediction, or from A = 0 when the
-    prediction's residual is not below the residual at A = 0 (fallback)."""
-    x0 = ws.predict(dp)
-    drv = solver.newton_driver(precision)
-    drv.retarget(solver.source_fields_device())
-    r_pred = drv.residual(x0)
-    use = r_pred < drv.r0()
-    solver.stats_warm = {'r_pred_rel': r_pred / drv.r0(), 'used': bool(use)}
-    return solver.solve_newton(precision, rtol=rtol, x0=x0 if use else None, log=log)
+def solve_with_warm_start(solver, ws, p, rtol=1e-8, log=None, galerkin_steps=1, coeffs=None,
+                          lanes_per_row=8):
+    """Newton (mixed precision, auxiliary-space preconditioner) from the
+    learned start of ``ws`` for the variant with parameters p on the family
+    topology, or from A = 0 when the start's residual is not below the
+    residual at A = 0 (plain Newton is the fallback)."""
+    x0, info = ws.start(solver, p, coeffs=coeffs, galerkin_steps=galerkin_steps,
+                        lanes_per_row=lanes_per_row)
+    x = solver.solve_newton('mixed', rtol=rtol, x0=x0, log=log, precond='ams',
+                            lanes_per_row=lanes_per_row)
+    solver.stats['warm_start'] = info
+    return x

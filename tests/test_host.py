@@ -42,3 +42,48 @@ def test_anchor_rejects_variants_that_differ_beyond_the_current():
             ConcurrentSolver([a, other], warm_start='anchor')
     with pytest.raises(ValueError, match='share the mesh'):
         ConcurrentSolver([a, Dipole(h=0.25, J=2.0e6)], warm_start='anchor')
+
+
+def test_morph_family_contains_its_base_mesh_and_stays_valid():
+    from snoopy_amd.magnets import Dipole
+    from snoopy_amd.warm_start import MorphFamily, sample_params
+    dp = Dipole(h=0.2)
+    fam = MorphFamily(base=dp)
+    np.testing.assert_array_equal(fam.nodes(fam.p0), dp.mesh.nodes)
+    for seed in range(20):
+        p = sample_params(seed)
+        X = fam.nodes(p)[dp.mesh.cells]
+        E = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+        assert np.linalg.det(E).min() > 0
+        # the yoke's faces are grid lines of the variant: iron elements fill
+        # exactly the yoke box minus the aperture
+        vol = np.abs(np.linalg.det(E)) / 6.0
+        iron = vol[dp.mesh.tags == 1].sum()
+        expect = (p['W'] * p['H'] - p['ax'] * p['ay']) * p['L']
+        assert abs(iron - expect) < 1e-9 * expect
+
+
+def test_energy_pod_is_blind_to_the_null_space():
+    """POD in the inner product of a positive semi-definite K: adding null
+    vectors of K (the solver's gauge) to the snapshots changes neither the
+    coefficients nor the captured energy, and the modes are K-orthonormal."""
+    from snoopy_amd.warm_start import POD
+    rng = np.random.default_rng(3)
+    n, m, S = 60, 10, 40
+    B = rng.standard_normal((n - m, n))
+    K = B.T @ B                                    # null space of dimension m
+    N = np.linalg.svd(B)[2][n - m:].T              # basis of the null space
+    Z = rng.standard_normal((S, 3)) @ rng.standard_normal((3, n))   # rank-3 family
+    Z = Z + 0.01 * rng.standard_normal((S, n))
+    junk = (rng.standard_normal((S, m)) @ N.T) * 5.0
+    p1 = POD(Z, 8, Z @ K)
+    p2 = POD(Z + junk, 8, (Z + junk) @ K)
+    # the three family modes exactly; the noise modes to the float32 Gram's
+    # rounding (the null-space part is 500x the noise in amplitude)
+    np.testing.assert_allclose(p1.lam[:3], p2.lam[:3], rtol=1e-4)
+    np.testing.assert_allclose(p1.lam, p2.lam, rtol=1e-2)
+    np.testing.assert_allclose(np.abs(p1.coeffs[:, :3]), np.abs(p2.coeffs[:, :3]), rtol=1e-3,
+                               atol=1e-4)
+    G = p2.U.T.astype(np.float64) @ K @ p2.U.astype(np.float64)
+    np.testing.assert_allclose(G, np.eye(G.shape[0]), atol=1e-3)
+    np.testing.assert_allclose(p2.project((Z + junk)[5])[:3], p2.coeffs[5, :3], rtol=1e-3, atol=1e-4)

@@ -33,3 +33,18 @@ def test_binding_covers_every_declared_symbol():
     lib = nat.load()
     assert lib.sn_version() >= 1
     assert isinstance(lib.sn_last_error(), bytes)
+
+
+def test_strided_tensors_are_refused_at_the_abi():
+    """The kernels read dense row-major arrays: a transposed view, a column
+    slice or a tensor built from a Fortran-ordered numpy array must not reach
+    them (its storage order differs from its index order)."""
+    import numpy as np
+    import pytest
+    import torch
+    from snoopy_amd import _native as nat
+    a = torch.zeros(4, 3)
+    assert nat.ptr(a).value == a.data_ptr()
+    for bad in (a.t(), a[:, :2], torch.from_numpy(np.asfortranarray(np.zeros((4, 3))))):
+        with pytest.raises(ValueError, match='strided'):
+            nat.ptr(bad)
