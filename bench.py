@@ -53,6 +53,8 @@ def parse():
     ap.add_argument('--tets', type=float, default=1.0e6)
     ap.add_argument('--rtol', type=float, default=1e-8)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--no-extra', action='store_true',
+                    help='skip the 64-variant batch and the 6-yoke chain in the default line')
     ap.add_argument('--cpu-cg-iters', type=int, default=600)
     ap.add_argument('--cpu-threads', type=int, default=16)
     ap.add_argument('--lanes', type=int, default=0, help='CG lanes per row (0 = autotune)')
@@ -85,6 +87,10 @@ def parse():
     ap.add_argument('--ws-test', type=int, default=256, help='warm start: held-out variants')
     ap.add_argument('--ws-modes', type=int, default=256, help='warm start: POD modes')
     ap.add_argument('--ws-epochs', type=int, default=2000, help='warm start: training epochs')
+    ap.add_argument('--ws-galerkin', type=int, default=128,
+                    help='warm start: modes of the reduced Newton step (32, 64 or 128)')
+    ap.add_argument('--ws-galerkin-steps', type=int, default=1,
+                    help='warm start: reduced Newton steps before the full Newton')
     ap.add_argument('--copy-calib', action='store_true',
                     help='only run a plain 256 MiB device copy 5 times (HBM counter calibration)')
     ap.add_argument('--quick', action='store_true',
@@ -198,7 +204,8 @@ def main():
         ws = WarmStart.train(fam, n_train=args.ws_train, r=args.ws_modes, epochs=args.ws_epochs,
                              rtol=args.rtol, workers=4, log=log)
         ws.fsolver = ws.fsolvers = None          # the training solvers are not needed again
-        ws.to_device(asm)
+        ws.to_device(asm, r_galerkin=args.ws_galerkin)
+        ws.galerkin_steps = args.ws_galerkin_steps
         torch.cuda.synchronize()
         ws_stats = {k: (float(v) if isinstance(v, (int, float, np.floating)) else v)
                     for k, v in ws.train_stats.items()}
@@ -285,6 +292,8 @@ def main():
             if mode == 'learned':
                 batch_runs[mode]['start_residual'] = [
                     o[1].get('warm_start', {}).get('r_start_rel') for o in res]
+                batch_runs[mode]['cg_per_step_first'] = res[0][1]['cg_iterations']
+                batch_runs[mode]['residual_history_first'] = res[0][1]['residual_history']
             log('batch (%s): %d variants on %d streams, %.3f s per batch (%.1f solves/s), cg %s'
                 % (mode, args.batch, args.streams, tb, args.batch / tb, its))
             if mode == args.batch_warm:
@@ -464,7 +473,101 @@ def main():
     }
     if cpu:
         out['speedup_vs_cpu'] = value / cpu['value']
+    if not args.no_extra:
+        # the batched and multi-magnet workloads on the same box, same run
+        del cs, bout, bdps
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        out['batch64'] = measure_batch64(args, steps=2)
+        out['chain6'] = measure_chain6(args)
     print(json.dumps(out), flush=True)
+
+
+def measure_batch64(args, steps=None):
+    """The batched workload (magnets.batch_variants: 64 seeded dipoles of
+    20k-400k tets) solved concurrently on 4 HIP streams, mixed precision with
+    the auxiliary-space preconditioner, each to 1e-8 of its own |R(0)|: set-up
+    time (meshes, patterns, preconditioner plans), timed batch, and every
+    variant's B at 1000 aperture probes against its own FP64 + AMS solve."""
+    import torch
+    from snoopy_amd.batch import ConcurrentSolver
+    from snoopy_amd.magnets import batch_variants
+    steps = steps or args.steps
+    t0 = time.perf_counter()
+    dps = batch_variants(64)
+    t_mesh = time.perf_counter() - t0
+    cs = ConcurrentSolver(dps, workers=4, precision='mixed', precond='ams')
+    cs.solve(rtol=args.rtol)                  # first solve: device set-up and AMS plans
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t = time.perf_counter()
+    for _ in range(steps):
+        res = cs.solve(rtol=args.rtol)
+    torch.cuda.synchronize()
+    t_solve = (time.perf_counter() - t) / steps
+    err = 0.0
+    for k, (d, s, (x, _)) in enumerate(zip(dps, cs.solvers, res)):
+        xr = s.solve_newton('fp64', rtol=1e-10, precond='ams')
+        pts = d.probes(1000, seed=k)
+        err = max(err, b_error(probe_B(d, s, x, pts), probe_B(d, s, xr, pts))[0])
+    tets = [d.num_tets for d in dps]
+    ams_setup = float(sum(r[1].get('ams_setup_s') or 0.0 for r in res))
+    out = {'variants': len(dps), 'tets_min': min(tets), 'tets_max': max(tets),
+           'tets_total': int(sum(tets)), 'streams': 4, 'solve_s': t_solve,
+           'solves_per_s': len(dps) / t_solve,
+           'setup_s': t_setup - t_solve, 'mesh_s': t_mesh, 'ams_plan_s_total': ams_setup,
+           'all_converged': all(bool(r[1]['converged']) for r in res),
+           'worst_final_residual': float(max(r[1]['residual_history'][-1] for r in res)),
+           'cg_total': int(sum(sum(r[1]['cg_iterations']) for r in res)),
+           'B_rel_err_max_vs_fp64_ams': err}
+    log('batch64: %s' % out)
+    del cs, res
+    torch.cuda.empty_cache()
+    return out
+
+
+def measure_chain6(args, jacobi=False):
+    """The 6-yoke chain (magnets.chain_dipole, about 6M tets): the mixed solve
+    (AMS) timed, the FP64 + AMS solve timed and used as the B reference at
+    5000 beam-axis points, optionally the FP64 + Jacobi solve."""
+    import torch
+    from snoopy_amd.magnets import chain_dipole
+    t0 = time.perf_counter()
+    dp = chain_dipole(6.0e6)
+    s = dp.solver(quad_order=2, verbose=False)
+    s.curl_curl_factory._device_state()
+    t_mesh = time.perf_counter() - t0
+    out = {'tets': dp.num_tets, 'edges': s.curl_curl_factory.num_dofs,
+           'nnz': s.curl_curl_factory.nnz, 'mesh_s': t_mesh}
+    xs = {}
+    forms = [('mixed', 'ams'), ('fp64', 'ams')] + ([('fp64', 'jacobi')] if jacobi else [])
+    for prec, pc in forms:
+        s._src = None
+        s.solve_newton(prec, rtol=args.rtol, precond=pc)      # untimed: set-up
+        torch.cuda.synchronize()
+        s._src = None
+        t = time.perf_counter()
+        xs[(prec, pc)] = s.solve_newton(prec, rtol=args.rtol, precond=pc)
+        torch.cuda.synchronize()
+        st = dict(s.stats)
+        out['%s_%s_s' % (prec, pc)] = time.perf_counter() - t
+        out['%s_%s_cg' % (prec, pc)] = st['cg_iterations']
+        if prec == 'mixed':
+            out['mixed_converged'] = bool(st['converged'])
+            out['mixed_final_residual'] = st['residual_history'][-1]
+            out['ams_plan_s'] = st.get('ams_setup_s')
+    xr = s.solve_newton('fp64', rtol=1e-10, precond='ams')
+    axis = dp.beam_axis(5000)
+    Bref = probe_B(dp, s, xr, axis)
+    out['B_axis_rel_err_mixed_vs_fp64_ams'] = b_error(probe_B(dp, s, xs[('mixed', 'ams')], axis),
+                                                      Bref)[0]
+    out['solves_per_s'] = 1.0 / out['mixed_ams_s']
+    out['speedup_vs_fp64_ams'] = out['fp64_ams_s'] / out['mixed_ams_s']
+    log('chain6: %s' % out)
+    del s, xs, xr
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_batch(args):
@@ -555,6 +658,8 @@ def bench_warmstart(args):
     t0 = time.perf_counter()
     ws = WarmStart.train(fam, n_train=args.ws_train, r=args.ws_modes, epochs=args.ws_epochs,
                          rtol=args.rtol, workers=4, log=log)
+    ws.to_device(ws.fsolver.asm, r_galerkin=args.ws_galerkin)
+    ws.galerkin_steps = args.ws_galerkin_steps
     t_setup = time.perf_counter() - t0
     log('trained in %.1f s: %s' % (t_setup, ws.train_stats))
     fs = ws.fsolver

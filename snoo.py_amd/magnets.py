@@ -185,3 +185,25 @@ def chain_mesh_size_for(target_tets, yokes, pad=0.5, ratio=1.2):
         else:
             hi = mid
     return hi, count(hi)
+
+
+def batch_variants(n=64, seed=12345, tets_lo=2e4, tets_hi=4e5):
+    """The batched workload: dipoles with parameters ``sample_variant(k)``,
+    k = 0..n-1, each meshed to a tetrahedron count drawn log-uniformly from
+    [tets_lo, tets_hi] with generator ``seed``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        v = sample_variant(k)
+        target = float(np.exp(rng.uniform(np.log(tets_lo), np.log(tets_hi))))
+        h, _ = mesh_size_for(target, **v)
+        out.append(Dipole(h=h, **v))
+    return out
+
+
+def chain_dipole(target_tets=6.0e6, n=6, seed=7):
+    """The multi-magnet chain: ``chain_yokes(n, seed)`` in one air box,
+    meshed to about ``target_tets`` tetrahedra."""
+    yokes = chain_yokes(n, seed=seed)
+    h, _ = chain_mesh_size_for(target_tets, yokes)
+    return Dipole(h=h, yokes=yokes)

@@ -489,7 +489,9 @@ class WarmStart:
                                         nat.ptr(x), nat.stream_handle()))
         return x
 
-    def start(self, solver, p, coeffs=None, galerkin_steps=1, lanes_per_row=8):
+    galerkin_steps = 1
+
+    def start(self, solver, p, coeffs=None, galerkin_steps=None, lanes_per_row=8):
         """Start vector for ``solver`` (a solver on the family topology whose
         current variant has parameters p): the network's prediction, one
         reduced Newton step, and the residual judge -- the start is kept only
@@ -499,6 +501,8 @@ class WarmStart:
         drv.retarget(solver.source_fields_device())
         self.to_device(solver.curl_curl_factory)
         x = self.expand(a)
+        if galerkin_steps is None:
+            galerkin_steps = self.galerkin_steps
         if galerkin_steps and self.r_galerkin:
             self.galerkin(drv, x, galerkin_steps)
         r = drv.residual(x) / drv.r0()
@@ -559,7 +563,7 @@ def load_default(dp=None):
     return WarmStart.load(path)
 
 
-def solve_with_warm_start(solver, ws, p, rtol=1e-8, log=None, galerkin_steps=1, coeffs=None,
+def solve_with_warm_start(solver, ws, p, rtol=1e-8, log=None, galerkin_steps=None, coeffs=None,
                           lanes_per_row=8):
     """Newton (mixed precision, auxiliary-space preconditioner) from the
     learned start of ``ws`` for the variant with parameters p on the family
