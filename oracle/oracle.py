@@ -290,11 +290,36 @@ def pcg(A, b, x0, mask, rtol=1e-5, atol=0.0, maxiter=-1):
     return x, it, info.value
 
 
+def scipy_pcg(A, b, x0, mask, rtol=1e-5, atol=0.0, maxiter=-1):
+    """The reference's own linear solve: scipy's cg with the Jacobi
+    preconditioner on J[mask][:, mask] (snoopy/red_mvp_solver.py:195-205);
+    returns (x, iterations, info) like pcg."""
+    from scipy.sparse import diags
+    from scipy.sparse.linalg import cg
+    A = A.tocsr()
+    m = np.asarray(mask, dtype=bool)
+    Ac = A[m][:, m]
+    M = diags(1.0 / Ac.diagonal(), shape=Ac.shape)
+    count = [0]
+
+    def cb(_):
+        count[0] += 1
+    kw = {} if maxiter < 0 else {'maxiter': int(maxiter)}
+    xc, info = cg(Ac, np.asarray(b, dtype=np.float64)[m], x0=_f64(x0)[m], rtol=rtol, atol=atol,
+                  M=M, callback=cb, **kw)
+    x = _f64(x0).copy()
+    x[m] = xc
+    return x, count[0], info
+
+
 class OracleProblem:
     """Everything the CPU solvers need, extracted from a product solver object
-    (mesh arrays, tables, models, source fields, Dirichlet mask)."""
+    (mesh arrays, tables, models, source fields, Dirichlet mask).  ``csr``:
+    "oracle" converts the element triplets with or_coo_to_csr, "scipy" with
+    scipy's csr_array((v, (i, j))) as fem_c/interface_c.pyx:367-368 does."""
 
-    def __init__(self, solver, source_fields=None, threads=1):
+    def __init__(self, solver, source_fields=None, threads=1, csr='oracle'):
+        self.csr = csr
         asm = solver.curl_curl_factory
         self.asm = asm
         self.n = asm.num_dofs
@@ -328,8 +353,12 @@ class OracleProblem:
             r, c, vK, vdK, rh = element_triplets(inp['nodes'], inp['cells'], inp['gids'],
                                                  inp['orient'], inp['curls'], inp['d_phi'],
                                                  inp['w'], self.models[d], x, bs)
-            K = K + coo_to_csr(n, r, c, vK)
-            dK = dK + coo_to_csr(n, r, c, vdK)
+            if self.csr == 'scipy':
+                K = K + csr_array((vK, (r, c)), shape=(n, n))
+                dK = dK + csr_array((vdK, (r, c)), shape=(n, n))
+            else:
+                K = K + coo_to_csr(n, r, c, vK)
+                dK = dK + coo_to_csr(n, r, c, vdK)
             rhs += rh
         return K, dK, rhs
 
@@ -365,8 +394,10 @@ def residual_norm(prob, x):
 
 
 def residual_newton(prob, rtol=1e-8, x0=None, max_newton=40, eta_max=0.1, eta_min=1e-6,
-                    line_search=True):
-    """Inexact Newton on R(x) = K(x)x - rhs(x) with the product's forcing rule."""
+                    line_search=True, linear=None):
+    """Inexact Newton on R(x) = K(x)x - rhs(x) with the product's forcing rule;
+    ``linear`` is the Jacobi-PCG used (pcg, the restatement, by default)."""
+    linear = linear or pcg
     n = prob.n
     r0, _, _, _ = residual_norm(prob, np.zeros(n))
     x = np.zeros(n) if x0 is None else np.where(prob.mask, _f64(x0), 0.0)
@@ -380,7 +411,7 @@ def residual_newton(prob, rtol=1e-8, x0=None, max_newton=40, eta_max=0.1, eta_mi
             eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
         eta = max(eta, min(eta_max, 0.5 * rtol * r0 / rn))
         J = dK + K
-        dx, it, info = pcg(J, -R, np.zeros(n), prob.mask, rtol=eta, atol=0.0)
+        dx, it, info = linear(J, -R, np.zeros(n), prob.mask, rtol=eta, atol=0.0)
         its.append(it)
         x_try = x + np.where(prob.mask, dx, 0.0)
         rn_try, K2, dK2, R2 = residual_norm(prob, x_try)

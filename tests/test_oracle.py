@@ -527,6 +527,11 @@ def test_pcg_matches_scipy_cg(small_problem):
     assert info == info_s == 0
     assert it == len(its)
     np.testing.assert_allclose(x[m], xs, rtol=1e-9, atol=1e-12 * np.abs(xs).max())
+    # the bench's CPU leg calls scipy itself through scipy_pcg
+    x2, it2, info2 = orc.scipy_pcg(J, rhs, np.zeros(prob.n), m, rtol=1e-5, atol=1e-4)
+    assert (it2, info2) == (it, 0)
+    np.testing.assert_array_equal(x2[m], xs)
+    assert not np.any(x2[~m])
 
 
 def test_coo_to_csr_matches_scipy():
