@@ -66,7 +66,7 @@ def parse():
     ap.add_argument('--ams-passes', type=int, default=None)
     ap.add_argument('--ams-coarse', type=int, default=None)
     ap.add_argument('--workload', default='dipole1m',
-                    choices=['dipole1m', 'batch64', 'chain6', 'warmstart'],
+                    choices=['dipole1m', 'batch64', 'chain6', 'warmstart', 'gemm'],
                     help='dipole1m: the metric; batch64: 64 seeded variants 20k-400k tets, '
                          'batched vs one at a time; chain6: 6-yoke chain')
     ap.add_argument('--variants', type=int, default=64)
@@ -89,6 +89,8 @@ def parse():
     ap.add_argument('--ws-epochs', type=int, default=2000, help='warm start: training epochs')
     ap.add_argument('--ws-galerkin', type=int, default=128,
                     help='warm start: modes of the reduced Newton step (32, 64 or 128)')
+    ap.add_argument('--ws-workers', type=int, default=4,
+                    help='warm start: concurrent snapshot solvers (one stream each)')
     ap.add_argument('--ws-galerkin-steps', type=int, default=1,
                     help='warm start: reduced Newton steps before the full Newton')
     ap.add_argument('--copy-calib', action='store_true',
@@ -201,6 +203,8 @@ def main():
         return bench_chain(args)
     if args.workload == 'warmstart':
         return bench_warmstart(args)
+    if args.workload == 'gemm':
+        return bench_gemm(args)
     from snoopy_amd import _native as nat
     from snoopy_amd.magnets import Dipole, mesh_size_for
     from snoopy_amd.red_mvp_solver import _CG
@@ -226,7 +230,7 @@ def main():
         t = time.perf_counter()
         fam = MorphFamily(base=dp)
         ws = WarmStart.train(fam, n_train=args.ws_train, r=args.ws_modes, epochs=args.ws_epochs,
-                             rtol=args.rtol, workers=4, log=log)
+                             rtol=args.rtol, workers=args.ws_workers, log=log)
         ws.fsolver = ws.fsolvers = None          # the training solvers are not needed again
         ws.to_device(asm, r_galerkin=args.ws_galerkin)
         ws.galerkin_steps = args.ws_galerkin_steps
@@ -681,7 +685,7 @@ def bench_warmstart(args):
     log('warm start family: %d tets, %d edges' % (fam.mesh.num_elements, fam.mesh.num_edges))
     t0 = time.perf_counter()
     ws = WarmStart.train(fam, n_train=args.ws_train, r=args.ws_modes, epochs=args.ws_epochs,
-                         rtol=args.rtol, workers=4, log=log)
+                         rtol=args.rtol, workers=args.ws_workers, log=log)
     ws.to_device(ws.fsolver.asm, r_galerkin=args.ws_galerkin)
     ws.galerkin_steps = args.ws_galerkin_steps
     t_setup = time.perf_counter() - t0
@@ -773,6 +777,38 @@ def gemm_rates(shapes, reps=50):
         out.append({'M': M, 'N': N, 'K': K, 'avg_us': ms * 1e3, 'TFLOP/s': tf,
                     'frac_of_peak': tf / BF16_PEAK_TFLOPS})
     return out
+
+
+def bench_gemm(args):
+    """The learned warm start's matrix-core work alone: the BF16 GEMM at the
+    network's shapes and larger ones, and training epochs of the network of
+    the learned warm start (8 -> 4x512 GELU -> 256, batch 2048, forward +
+    backward + Adam) on random data."""
+    import torch
+    from snoopy_amd.mlp import MLP
+    shapes = [(2048, 512, 512), (2048, 256, 512), (8192, 512, 512), (8192, 2048, 2048),
+              (16384, 4096, 4096)]
+    gemm = gemm_rates(shapes)
+    for g in gemm:
+        log('gemm %dx%dx%d: %.1f us, %.1f TFLOP/s' % (g['M'], g['N'], g['K'], g['avg_us'],
+                                                     g['TFLOP/s']))
+    B = 2048
+    X = torch.rand(B, 8, device='cuda') * 2 - 1
+    T = torch.randn(B, 256, device='cuda')
+    mlp = MLP((8, 512, 512, 512, 512, 256), seed=0)
+    mlp.train(X, T, epochs=5)
+    torch.cuda.synchronize()
+    epochs = 200
+    t = time.perf_counter()
+    mlp.train(X, T, epochs=epochs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / epochs
+    flop = 3 * 2 * B * (8 * 512 + 3 * 512 * 512 + 512 * 256)   # forward + 2x backward
+    print(json.dumps({'metric': 'BF16 MFMA GEMM and network training on one MI355X',
+                      'gemm_bf16': gemm, 'mlp_epoch_ms': dt * 1e3,
+                      'mlp_epoch_TFLOP/s': flop / dt / 1e12, 'mlp_batch': B,
+                      'mlp_sizes': list(mlp.sizes), 'peak_TFLOP/s': BF16_PEAK_TFLOPS}),
+          flush=True)
 
 
 def bench_chain(args):

@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
     int num_el, const float *__restrict__ W, const float *__restrict__ Mr,
     float *__restrict__ part) {
     constexpr int TK = R / 16;
-    __shared__ float sW[kGalEB][3][R];
-    __shared__ float sZ[kGalEB][3][R];
+    __shared__ __align__(16) float sW[kGalEB][3][R];
+    __shared__ __align__(16) float sZ[kGalEB][3][R];
     __shared__ float sM[kGalEB][6];
     const int tid = threadIdx.x;
     const int kb = (tid / 16) * TK, lb = (tid % 16) * TK;
@@ -69,10 +69,12 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
     const int b0 = blockIdx.x * per, b1 = min(nb, b0 + per);
     for (int bt = b0; bt < b1; ++bt) {
         const int e0 = bt * kGalEB;
-        for (int v = tid; v < kGalEB * 3 * R; v += kGalBlock) {
-            const int le = v / (3 * R);
-            const int e = e0 + le;
-            (&sW[0][0][0])[v] = (e < num_el) ? W[(size_t)e0 * 3 * R + v] : 0.0f;
+        // the batch's W rows are contiguous: float4 loads (3 R is a multiple of 4)
+        const float4 *src = reinterpret_cast<const float4 *>(W + (size_t)e0 * 3 * R);
+        float4 *dstW = reinterpret_cast<float4 *>(&sW[0][0][0]);
+        for (int v = tid; v < kGalEB * 3 * R / 4; v += kGalBlock) {
+            const int e = e0 + (4 * v) / (3 * R);
+            dstW[v] = (e < num_el) ? src[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         if (tid < kGalEB * 6) {
             const int le = tid / 6, u = tid % 6, e = e0 + le;
@@ -94,9 +96,13 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
             for (int a = 0; a < 3; ++a) {
                 float wk[TK], zl[TK];
 #pragma unroll
-                for (int q = 0; q < TK; ++q) {
-                    wk[q] = sW[le][a][kb + q];
-                    zl[q] = sZ[le][a][lb + q];
+                for (int q = 0; q < TK; q += 2) {     // 8-byte LDS reads (TK is even)
+                    const float2 w2 = *reinterpret_cast<const float2 *>(&sW[le][a][kb + q]);
+                    const float2 z2 = *reinterpret_cast<const float2 *>(&sZ[le][a][lb + q]);
+                    wk[q] = w2.x;
+                    wk[q + 1] = w2.y;
+                    zl[q] = z2.x;
+                    zl[q + 1] = z2.y;
                 }
 #pragma unroll
                 for (int p = 0; p < TK; ++p)
@@ -112,26 +118,61 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
         for (int q = 0; q < TK; ++q) dst[(kb + p) * R + lb + q] = acc[p][q];
 }
 
-__global__ void sum_partials_kernel(int G, int m, const float *__restrict__ part,
-                                    double *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+// out[i] = sum_g part[g*m + i] (float partials, double sums): 64 entries per
+// workgroup, 4 groups of 64 lanes (partial g goes to group g % 4, each group
+// sums in order), the group sums added in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void sum_partials_kernel(int G, int m,
+                                                           const T *__restrict__ part,
+                                                           double *__restrict__ out) {
+    __shared__ double acc[4][64];
+    const int c = threadIdx.x % 64, grp = threadIdx.x / 64;
+    const int i = blockIdx.x * 64 + c;
     double s = 0.0;
-    for (int g = 0; g < G; ++g) s += (double)part[(size_t)g * m + i];   // fixed order
-    out[i] = s;
+    if (i < m)
+        for (int g = grp; g < G; g += 4) s += (double)part[(size_t)g * m + i];
+    acc[grp][c] = s;
+    __syncthreads();
+    if (grp == 0 && i < m) out[i] = ((acc[0][c] + acc[1][c]) + acc[2][c]) + acc[3][c];
 }
 
-// g[k] = sum_i U[k*n + i] v[i], one workgroup per mode
-__global__ __launch_bounds__(256) void pod_project_kernel(int n, const float *__restrict__ U,
+// partial g[k] over one chunk of kProjPer * 256 entries per workgroup: the
+// workgroup's v values stay in registers, every mode k streams its slice of
+// U once (coalesced), wave sums go through LDS; part[b*r + k]
+constexpr int kProjPer = 8;
+__global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
+                                                          const float *__restrict__ U,
                                                           const double *__restrict__ v,
-                                                          double *__restrict__ g) {
-    __shared__ double lds[256 / kWave];
-    const int k = blockIdx.x;
-    const float *u = U + (size_t)k * n;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += (double)u[i] * v[i];
-    s = block_sum(s, lds);
-    if (threadIdx.x == 0) g[k] = s;
+                                                          double *__restrict__ part) {
+    __shared__ double ws[4][128];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const long long base = (long long)blockIdx.x * 256 * kProjPer + threadIdx.x;
+    double vv[kProjPer];
+#pragma unroll
+    for (int j = 0; j < kProjPer; ++j) {
+        const long long i = base + 256LL * j;
+        vv[j] = i < n ? v[i] : 0.0;
+    }
+    for (int k0 = 0; k0 < r; k0 += 128) {
+        const int kc = min(128, r - k0);
+        for (int kk = 0; kk < kc; ++kk) {
+            const float *u = U + (size_t)(k0 + kk) * n;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < kProjPer; ++j) {
+                const long long i = base + 256LL * j;
+                if (i < n) s += (double)u[i] * vv[j];
+            }
+            s = wave_sum(s);
+            if (lane == 0) ws[wid][kk] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < kc)
+            part[(size_t)blockIdx.x * r + k0 + threadIdx.x] =
+                ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) +
+                ws[3][threadIdx.x];
+        __syncthreads();
+    }
 }
 
 }  // namespace sn
@@ -158,7 +199,7 @@ int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, 
                "sn_pod_reduced_jacobian: r must be 32, 64 or 128 (r=%d)", r);
     hipStream_t s = as_stream(stream);
     const int nb = (num_el + kGalEB - 1) / kGalEB;
-    const int G = nb < 1024 ? nb : 1024;
+    const int G = nb < 512 ? nb : 512;
     const int m = r * r;
     float *part = nullptr;
     double *H = nullptr;
@@ -175,8 +216,8 @@ int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, 
                            num_el, W, Mr, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(sum_partials_kernel, dim3((m + 255) / 256), dim3(256), 0, s, G, m,
-                           part, H);
+        hipLaunchKernelGGL(sum_partials_kernel<float>, dim3((m + 63) / 64), dim3(256), 0, s, G,
+                           m, part, H);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(H_host, H, (size_t)m * sizeof(double),
@@ -194,14 +235,21 @@ int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, 
 int sn_pod_project(int n, int r, const float *U, const double *v, double *g_host, void *stream) {
     SN_REQUIRE(n > 0 && r > 0 && U && v && g_host, "sn_pod_project: bad argument");
     hipStream_t s = as_stream(stream);
-    double *g = nullptr;
+    const int G = (int)((n + 256LL * kProjPer - 1) / (256LL * kProjPer));
+    double *part = nullptr, *g = nullptr;
+    SN_HIP(hipMallocAsync((void **)&part, (size_t)G * r * sizeof(double), s));
     SN_HIP(hipMallocAsync((void **)&g, (size_t)r * sizeof(double), s));
-    hipLaunchKernelGGL(pod_project_kernel, dim3(r), dim3(256), 0, s, n, U, v, g);
+    hipLaunchKernelGGL(pod_project_kernel, dim3(G), dim3(256), 0, s, n, r, U, v, part);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sum_partials_kernel<double>, dim3((r + 63) / 64), dim3(256), 0, s, G,
+                           r, part, g);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(g_host, g, (size_t)r * sizeof(double),
                                             hipMemcpyDeviceToHost, s);
-    hipError_t f = hipFreeAsync(g, s);
-    if (e == hipSuccess) e = f;
+    hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(g, s);
+    if (e == hipSuccess) e = f1 != hipSuccess ? f1 : f2;
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         set_error("sn_pod_project: %s", hipGetErrorString(e));

@@ -250,13 +250,28 @@ __global__ void mse_grad_kernel(int n, const float *__restrict__ Y, const float 
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// db[n] = sum_m dZ[m][n], fixed order per column
-__global__ void colsum_kernel(int M, int N, const float *__restrict__ dZ, float *__restrict__ db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// db[n] = sum_m dZ[m][n]: 64 columns per workgroup, 4 row groups of 64
+// lanes (row m goes to group m % 4, each group sums its rows in order), the
+// four group sums added in a fixed order -- deterministic, and every load
+// of a wave reads 64 consecutive floats of one row
+constexpr int kColW = 64, kColG = 4;
+__global__ __launch_bounds__(kColW * kColG) void colsum_kernel(int M, int N,
+                                                               const float *__restrict__ dZ,
+                                                               float *__restrict__ db) {
+    __shared__ float part[kColG][kColW];
+    const int c = threadIdx.x % kColW, g = threadIdx.x / kColW;
+    const int n = blockIdx.x * kColW + c;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int m = g; m < M; m += kColG) s += dZ[(size_t)m * N + n];
+    part[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) {
+        float t = part[0][c];
+#pragma unroll
+        for (int k = 1; k < kColG; ++k) t += part[k][c];
+        db[n] = t;
+    }
 }
 
 // Adam on FP32 master weights; refreshes the BF16 copy
@@ -384,8 +399,8 @@ int sn_mse_grad(int n, const float *Y, const float *T, float scale, float *dYf, 
 
 int sn_colsum_f32(int M, int N, const float *dZ, float *db, void *stream) {
     SN_REQUIRE(M > 0 && N > 0 && dZ && db, "sn_colsum_f32: bad argument");
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256), dim3(256), 0, as_stream(stream), M,
-                       N, dZ, db);
+    hipLaunchKernelGGL(colsum_kernel, dim3((N + kColW - 1) / kColW), dim3(kColW * kColG), 0,
+                       as_stream(stream), M, N, dZ, db);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
