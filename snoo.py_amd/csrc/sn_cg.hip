@@ -19,7 +19,10 @@
 // stop flag once per chunk of iterations; each chunk is captured once as a
 // hipGraph and replayed.
 #include <climits>
+#include <map>
 #include <mutex>
+#include <thread>
+#include <tuple>
 #include <cstring>
 #include <vector>
 
@@ -1381,19 +1384,37 @@ int sn_spmv_f64(const sn_pattern *P, const double *vals, const unsigned char *ma
     return SN_OK;
 }
 
+// Partial-sum buffer of the host reductions, one per (device, stream, host
+// thread), made on the first reduction of that combination and kept for the
+// life of the process: a call reads its partials back before it returns, and
+// the calls of one host thread on one stream cannot overlap, so the buffer is
+// never shared by two calls in flight.
+static std::mutex g_part_mu;
+static std::map<std::tuple<int, hipStream_t, std::thread::id>, double *> g_part;
+
+static int reduce_partials(hipStream_t s, double **out) {
+    int dev = 0;
+    SN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_part_mu);
+    double *&p = g_part[std::make_tuple(dev, s, std::this_thread::get_id())];
+    if (!p) SN_HIP(hipMalloc((void **)&p, 1024 * sizeof(double)));
+    *out = p;
+    return SN_OK;
+}
+
 static int reduce_host(int n, const double *a, const double *b, const unsigned char *mask, int op,
                        double *out, void *stream) {
     SN_REQUIRE(a && out && (op != 0 || b), "reduction: null argument");
     hipStream_t s = as_stream(stream);
     const int G = grid_for(n, kRowBlock, 1024);
     double *part = nullptr;
-    SN_HIP(hipMallocAsync(&part, G * sizeof(double), s));
+    int rc = reduce_partials(s, &part);
+    if (rc) return rc;
     hipLaunchKernelGGL(dot_partials_kernel, dim3(G), dim3(kRowBlock), 0, s, n, a, b, mask, op,
                        part);
     SN_CHECK_LAUNCH();
     std::vector<double> h(G);
     SN_HIP(hipMemcpyAsync(h.data(), part, G * sizeof(double), hipMemcpyDeviceToHost, s));
-    SN_HIP(hipFreeAsync(part, s));
     SN_HIP(hipStreamSynchronize(s));
     double r = 0.0;
     for (int k = 0; k < G; ++k) r = (op == 0) ? r + h[k] : (h[k] > r ? h[k] : r);
