@@ -5,6 +5,7 @@
 // mesh transfer of edge DOFs).  No counterpart in the reference: its solve
 // always starts from A = 0 or a caller-supplied x_0
 // (snoopy/red_mvp_solver.py:146-147,174); this fills x_0.
+#include <algorithm>
 #include <cstdint>
 
 #include "sn_common.hpp"
@@ -41,7 +42,8 @@ template <int TA, int TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     int M, int N, int K, const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
     int ldb, const float *__restrict__ bias, const float *__restrict__ Zin,
-    float *__restrict__ Zout, float *__restrict__ Cf, __bf16 *__restrict__ Cb, int ldc) {
+    float *__restrict__ Zout, float *__restrict__ Cf, __bf16 *__restrict__ Cb, int ldc,
+    int kslice) {
     // LDS tiles with k contiguous per row / column (16 bf16 = 32 B each)
     __shared__ __attribute__((aligned(16))) __bf16 As[kBM][kBK];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN][kBK];
@@ -50,8 +52,12 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
     const int wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    // split K: workgroup layer z sums k in [kz0, kz1) into its own partial
+    // output, Cf + z * M * ldc (one layer: the whole K, straight into Cf)
+    const int kz0 = blockIdx.z * kslice, kz1 = min(K, kz0 + kslice);
+    Cf += (size_t)blockIdx.z * M * ldc;
     f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += kBK) {
+    for (int k0 = kz0; k0 < kz1; k0 += kBK) {
         // each thread stages 4 elements of A and 4 of B (64*16 = 1024 each)
         for (int e = tid; e < kBM * kBK; e += 256) {
             int m, k;
@@ -59,7 +65,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
             else { k = e / kBM; m = e % kBM; }
             const int gm = m0 + m, gk = k0 + k;
             __bf16 v = to_bf16(0.0f);
-            if (gm < M && gk < K) v = TA == 0 ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
+            if (gm < M && gk < kz1) v = TA == 0 ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
             As[m][k] = v;
         }
         for (int e = tid; e < kBN * kBK; e += 256) {
@@ -68,7 +74,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
             else { n = e / kBK; k = e % kBK; }
             const int gn = n0 + n, gk = k0 + k;
             __bf16 v = to_bf16(0.0f);
-            if (gn < N && gk < K) v = TB == 0 ? B[(size_t)gk * ldb + gn] : B[(size_t)gn * ldb + gk];
+            if (gn < N && gk < kz1) v = TB == 0 ? B[(size_t)gk * ldb + gn] : B[(size_t)gn * ldb + gk];
             Bs[n][k] = v;
         }
         __syncthreads();
@@ -101,6 +107,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(
         if (Cf) Cf[o] = v;
         if (Cb) Cb[o] = to_bf16(v);
     }
+}
+
+// C[m][n] = sum_z part[z][m][n] over the split-K layers, in layer order
+__global__ void splitk_sum_kernel(int M, int N, int ldc, int S, const float *__restrict__ part,
+                                  float *__restrict__ C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)M * N) return;
+    const size_t o = (size_t)(t / N) * ldc + (size_t)(t % N);
+    float s = 0.0f;
+    for (int z = 0; z < S; ++z) s += part[(size_t)z * M * ldc + o];
+    C[o] = s;
 }
 
 // Large-tile variant: 128x128 tile, BK = 32, 4 waves in 2x2, each wave a
@@ -351,11 +368,28 @@ int sn_gemm_bf16(int ta, int tb, int epi, int M, int N, int K, const void *A, in
     const dim3 grid = big ? dim3((N + tBN - 1) / tBN, (M + tBM - 1) / tBM)
                           : dim3((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
     const dim3 block(256);
+    // a plain product with few output tiles and a long K (the weight
+    // gradients A^T dZ of the network) is split along K over several
+    // workgroup layers, summed afterwards in layer order
+    const int tiles = (int)grid.x * (int)grid.y;
+    int S = 1, kslice = K;
+    float *Cdst = Cf;
+    float *part = nullptr;
+    if (!big && epi == 0 && Cf && !Cb && !bias && 2 * tiles <= num_cus() && K >= 512) {
+        S = std::min(K / 256, std::max(1, num_cus() / tiles));
+        kslice = ((K + S - 1) / S + kBK - 1) / kBK * kBK;
+        S = (K + kslice - 1) / kslice;
+    }
+    if (S > 1) {
+        SN_HIP(hipMallocAsync((void **)&part, (size_t)S * M * ldc * sizeof(float), s));
+        Cdst = part;
+    }
+    const dim3 gridz(grid.x, grid.y, S);
 #define SN_G(TA, TB, E)                                                                          \
     do {                                                                                         \
         if (!big)                                                                                \
-            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E>), grid, block, 0, s, M, N, K, a,     \
-                               lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                       \
+            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E>), gridz, block, 0, s, M, N, K, a,    \
+                               lda, b, ldb, bias, Zin, Zout, Cdst, cb, ldc, kslice);             \
         else if (vec)                                                                            \
             hipLaunchKernelGGL((gemm_bf16_big_kernel<TA, TB, E, true>), grid, block, 0, s, M, N,  \
                                K, a, lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                 \
@@ -372,6 +406,13 @@ int sn_gemm_bf16(int ta, int tb, int epi, int M, int N, int K, const void *A, in
 #undef SN_GE
 #undef SN_G
     SN_CHECK_LAUNCH();
+    if (S > 1) {
+        const long long mn = (long long)M * N;
+        hipLaunchKernelGGL(splitk_sum_kernel, dim3(grid_for(mn, 256, 1 << 30)), dim3(256), 0, s, M,
+                           N, ldc, S, part, Cf);
+        SN_CHECK_LAUNCH();
+        SN_HIP(hipFreeAsync(part, s));
+    }
     return SN_OK;
 }
 

@@ -47,6 +47,28 @@ def test_gemm_bf16_matches_float64(ta, tb, M, N, K):
     assert np.abs(C.cpu().numpy() - want).max() / np.abs(want).max() < 1e-5
 
 
+@pytest.mark.parametrize('ta,tb', [(1, 0), (0, 0), (1, 1)])
+@pytest.mark.parametrize('M,N,K', [(512, 512, 2048), (512, 256, 2048), (200, 130, 1037)])
+def test_gemm_bf16_split_k_matches_float64(ta, tb, M, N, K):
+    """Plain products with few output tiles and a long K (the network's
+    weight gradients) take the split-K path: layers of K summed in order."""
+    import torch
+    from snoopy_amd import _native as nat
+    rng = np.random.default_rng(M * N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    ref = mo.bf16(A) @ mo.bf16(B)
+    dA = torch.from_numpy(A.T.copy() if ta else A).cuda().to(torch.bfloat16)
+    dB = torch.from_numpy(B.T.copy() if tb else B).cuda().to(torch.bfloat16)
+    C = torch.full((M, N), float('nan'), dtype=torch.float32, device='cuda')
+    nat.check(nat.load().sn_gemm_bf16(ta, tb, 0, M, N, K, nat.ptr(dA), M if ta else K,
+                                      nat.ptr(dB), K if tb else N, None, None, None, nat.ptr(C),
+                                      None, N, nat.stream_handle()))
+    got = C.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-5
+
+
 def test_mlp_forward_backward_match_oracle():
     import torch
     from snoopy_amd.mlp import MLP
