@@ -538,13 +538,23 @@ class WarmStart:
     def save(self, path):
         st = self.mlp.state()
         np.savez(path, counts=np.array(self.family.counts), mean=self.pod.mean, U=self.pod.U,
-                 coeff_std=self.coeff_std, **{'mlp_' + k: v for k, v in st.items()})
+                 coeff_std=self.coeff_std, based=np.array(self.family.base is not None),
+                 **{'mlp_' + k: v for k, v in st.items()})
 
     @classmethod
     def load(cls, path, family=None):
+        """A saved model.  A model of a family built on a dipole's own mesh
+        (MorphFamily(base=...)) needs that family passed back in; the mode
+        count must match the family's edge count either way."""
         from .mlp import MLP
         z = np.load(path)
+        if family is None and 'based' in z.files and bool(z['based']):
+            raise ValueError('WarmStart.load: this model was trained on MorphFamily(base=...); '
+                             'pass family=MorphFamily(base=<the same dipole>)')
         fam = family or MorphFamily(tuple(int(c) for c in z['counts']))
+        if z['U'].shape[0] != fam.mesh.num_edges:
+            raise ValueError('WarmStart.load: the model has %d edge DOFs, the family mesh %d'
+                             % (z['U'].shape[0], fam.mesh.num_edges))
         pod = POD.__new__(POD)
         pod.mean, pod.U = z['mean'], z['U']
         mlp = MLP.from_state({k[4:]: z[k] for k in z.files if k.startswith('mlp_')})

@@ -105,3 +105,51 @@ def test_learned_start_converges_to_the_same_field(trained):
               sum(s1['cg_iterations']), 'B diff %.1e' % err, flush=True)
         assert err < 1e-6
     print('total CG from zero %d, from the learned start %d' % tuple(work), flush=True)
+
+
+def _family_variant(fam, p):
+    """A Dipole of parameters p on its own copy of the family mesh."""
+    m = fam.private_model()
+    m.tet_mesh.nodes[:] = fam.nodes(dict(fam.p0, **p))
+    return fam.variant(p, m)
+
+
+def test_concurrent_learned_batch_matches_the_batch_from_zero(trained):
+    """ConcurrentSolver with warm_start='learned' on 2 streams: every variant
+    converges to 1e-8 and to the field of the same batch solved from A = 0."""
+    from snoopy_amd.batch import ConcurrentSolver
+    from snoopy_amd.warm_start import sample_params
+    fam, ws = trained
+    dps = [_family_variant(fam, sample_params(s)) for s in (6000, 6001, 6002, 6003)]
+    print('case: learned batch of 4 on 2 streams', flush=True)
+    cs = ConcurrentSolver(dps, workers=2, precision='mixed', precond='ams',
+                          warm_start='learned', model=ws)
+    warm = cs.solve(rtol=1e-8)
+    xs_warm = [x.cpu().numpy() for x, _ in warm]
+    st_warm = [st for _, st in warm]
+    cs.set_warm_start(None)
+    cold = cs.solve(rtol=1e-8)
+    for k, ((xc, sc), xw, sw, s) in enumerate(zip(cold, xs_warm, st_warm, cs.solvers)):
+        assert sc['converged'] and sw['converged'], k
+        assert sw['residual_history'][-1] <= 1e-8
+        assert 'warm_start' in sw and sw['warm_start']['r_start_rel'] > 0
+        asm = s.curl_curl_factory
+        Bc, Bw = asm.element_B(xc.cpu().numpy()), asm.element_B(xw)
+        err = np.abs(Bc - Bw).max() / np.abs(Bc).max()
+        print('variant', k, 'cg', sum(sc['cg_iterations']), '->', sum(sw['cg_iterations']),
+              'start %.2e' % sw['warm_start']['r_start_rel'], 'B diff %.1e' % err, flush=True)
+        assert err < 1e-6
+
+
+def test_saved_model_predicts_the_same_coefficients(trained, tmp_path):
+    from snoopy_amd.warm_start import MorphFamily, WarmStart, sample_params
+    fam, ws = trained
+    path = str(tmp_path / 'ws.npz')
+    ws.save(path)
+    ps = [sample_params(s) for s in (7000, 7001, 7002)]
+    a = ws.coefficients(ps)
+    for fam2 in (fam, None):
+        ws2 = WarmStart.load(path, family=fam2)
+        np.testing.assert_array_equal(ws2.coefficients(ps), a)
+    with pytest.raises(ValueError):
+        WarmStart.load(path, family=MorphFamily(counts=(3, 3, 4, 3, 3, 5)))
