@@ -15,8 +15,6 @@ namespace sn {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
-
 __device__ __forceinline__ __bf16 to_bf16(float v) { return (__bf16)v; }
 __device__ __forceinline__ float from_bf16(__bf16 v) { return (float)v; }
 
@@ -28,85 +26,6 @@ __device__ __forceinline__ float gelu_grad(float z) {
     const float Phi = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
     const float phi = 0.39894228040143268f * expf(-0.5f * z * z);
     return Phi + z * phi;
-}
-
-// C[M,N] = op(A)[M,K] op(B)[K,N]
-//   TA = 0: A stored [M][lda] (row m, k contiguous); TA = 1: A stored [K][lda] (A^T)
-//   TB = 0: B stored [K][ldb];                       TB = 1: B stored [N][ldb] (B^T)
-// epilogue: v = acc (+ bias[n]); EPI 1: Z = v stored (fp32), v = gelu(v);
-//           EPI 2: v *= gelu'(Zin[m][n]) (backward through GELU).
-// Outputs Cf (fp32) and/or Cb (bf16), both [M][ldc].
-// Block: 256 threads = 4 waves in a 2x2 arrangement, 64x64 tile; each wave
-// owns a 32x32 sub-tile and issues one 32x32x16 MFMA per k-step.
-template <int TA, int TB, int EPI>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(
-    int M, int N, int K, const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
-    int ldb, const float *__restrict__ bias, const float *__restrict__ Zin,
-    float *__restrict__ Zout, float *__restrict__ Cf, __bf16 *__restrict__ Cb, int ldc,
-    int kslice) {
-    // LDS tiles with k contiguous per row / column (16 bf16 = 32 B each)
-    __shared__ __attribute__((aligned(16))) __bf16 As[kBM][kBK];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN][kBK];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-    // split K: workgroup layer z sums k in [kz0, kz1) into its own partial
-    // output, Cf + z * M * ldc (one layer: the whole K, straight into Cf)
-    const int kz0 = blockIdx.z * kslice, kz1 = min(K, kz0 + kslice);
-    Cf += (size_t)blockIdx.z * M * ldc;
-    f32x16 acc = {};
-    for (int k0 = kz0; k0 < kz1; k0 += kBK) {
-        // each thread stages 4 elements of A and 4 of B (64*16 = 1024 each)
-        for (int e = tid; e < kBM * kBK; e += 256) {
-            int m, k;
-            if (TA == 0) { m = e / kBK; k = e % kBK; }
-            else { k = e / kBM; m = e % kBM; }
-            const int gm = m0 + m, gk = k0 + k;
-            __bf16 v = to_bf16(0.0f);
-            if (gm < M && gk < kz1) v = TA == 0 ? A[(size_t)gm * lda + gk] : A[(size_t)gk * lda + gm];
-            As[m][k] = v;
-        }
-        for (int e = tid; e < kBN * kBK; e += 256) {
-            int n, k;
-            if (TB == 0) { k = e / kBN; n = e % kBN; }
-            else { n = e / kBK; k = e % kBK; }
-            const int gn = n0 + n, gk = k0 + k;
-            __bf16 v = to_bf16(0.0f);
-            if (gn < N && gk < kz1) v = TB == 0 ? B[(size_t)gk * ldb + gn] : B[(size_t)gn * ldb + gk];
-            Bs[n][k] = v;
-        }
-        __syncthreads();
-        // lane l: row/column l % 32, k = 8 * (l / 32) .. +7
-        const int r = lane & 31, kh = (lane >> 5) * 8;
-        bf16x8 a, b;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            a[i] = As[wm + r][kh + i];
-            b[i] = Bs[wn + r][kh + i];
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        __syncthreads();
-    }
-    // D layout: lane l, register i -> column l % 32, row 8*(i/4) + 4*(l/32) + i%4
-    const int col = n0 + wn + (lane & 31);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        if (row >= M || col >= N) continue;
-        float v = acc[i];
-        if (bias) v += bias[col];
-        const size_t o = (size_t)row * ldc + col;
-        if (EPI == 1) {
-            if (Zout) Zout[o] = v;
-            v = gelu(v);
-        } else if (EPI == 2) {
-            v *= gelu_grad(Zin[o]);
-        }
-        if (Cf) Cf[o] = v;
-        if (Cb) Cb[o] = to_bf16(v);
-    }
 }
 
 // C[m][n] = sum_z part[z][m][n] over the split-K layers, in layer order
@@ -250,6 +169,120 @@ __global__ __launch_bounds__(256) void gemm_bf16_big_kernel(
         }
 }
 
+// C[M,N] = op(A)[M,K] op(B)[K,N]
+//   TA = 0: A stored [M][lda] (row m, k contiguous); TA = 1: A stored [K][lda] (A^T)
+//   TB = 0: B stored [K][ldb];                       TB = 1: B stored [N][ldb] (B^T)
+// epilogue: v = acc (+ bias[n]); EPI 1: Z = v stored (fp32), v = gelu(v);
+//           EPI 2: v *= gelu'(Zin[m][n]) (backward through GELU).
+// Outputs Cf (fp32) and/or Cb (bf16), both [M][ldc].
+// Small-tile form for products with few output tiles (the network's own
+// layers, M <= a few thousand, N, K <= 512): 64x64 tile, BK = 64, 4 waves in
+// 2x2, each wave a 32x32 sub-tile (4 MFMAs 32x32x16 per k-tile); 16-byte
+// global loads along the contiguous dimension staged through registers into
+// double-buffered LDS, the next k-tile's loads in flight during the current
+// tile's MFMAs, one barrier per k-tile.  Split K: workgroup layer z sums k
+// in [z kslice, (z+1) kslice) into its own partial output Cf + z M ldc.
+constexpr int kBM = 64, kBN = 64, kBK = 64, kPad = 8;
+
+template <int TA, int TB, int EPI, bool VEC>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(
+    int M, int N, int K, const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
+    int ldb, const float *__restrict__ bias, const float *__restrict__ Zin,
+    float *__restrict__ Zout, float *__restrict__ Cf, __bf16 *__restrict__ Cb, int ldc,
+    int kslice) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[2][kBM][kBK + kPad];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][kBN][kBK + kPad];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    const int kz0 = blockIdx.z * kslice, kz1 = min(K, kz0 + kslice);
+    if (Cf) Cf += (size_t)blockIdx.z * M * ldc;
+    f32x16 acc = {};
+    bf16x8 ra[2], rb[2];
+    auto clamp8 = [](int n) { return n < 0 ? 0 : (n > 8 ? 8 : n); };
+    // 512 vectors of 8 per 64x64 operand tile, two per thread
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int v = tid + 256 * s;
+            if (TA == 0) {           // rows m, k contiguous
+                const int m = v >> 3, kv = (v & 7) * 8, gm = m0 + m, gk = k0 + kv;
+                ra[s] = load8<VEC>(A, (size_t)gm * lda + gk, gm < M ? clamp8(kz1 - gk) : 0);
+            } else {                 // rows k, m contiguous
+                const int k = v >> 3, mv = (v & 7) * 8, gk = k0 + k, gm = m0 + mv;
+                ra[s] = load8<VEC>(A, (size_t)gk * lda + gm, gk < kz1 ? clamp8(M - gm) : 0);
+            }
+            if (TB == 1) {           // rows n, k contiguous
+                const int n = v >> 3, kv = (v & 7) * 8, gn = n0 + n, gk = k0 + kv;
+                rb[s] = load8<VEC>(B, (size_t)gn * ldb + gk, gn < N ? clamp8(kz1 - gk) : 0);
+            } else {                 // rows k, n contiguous
+                const int k = v >> 3, nv = (v & 7) * 8, gk = k0 + k, gn = n0 + nv;
+                rb[s] = load8<VEC>(B, (size_t)gk * ldb + gn, gk < kz1 ? clamp8(N - gn) : 0);
+            }
+        }
+    };
+    auto store_tile = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int v = tid + 256 * s;
+            if (TA == 0) {
+                *reinterpret_cast<bf16x8 *>(&As[buf][v >> 3][(v & 7) * 8]) = ra[s];
+            } else {
+                const int k = v >> 3, mv = (v & 7) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) As[buf][mv + i][k] = ra[s][i];
+            }
+            if (TB == 1) {
+                *reinterpret_cast<bf16x8 *>(&Bs[buf][v >> 3][(v & 7) * 8]) = rb[s];
+            } else {
+                const int k = v >> 3, nv = (v & 7) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) Bs[buf][nv + i][k] = rb[s][i];
+            }
+        }
+    };
+    const int nk = kz1 > kz0 ? (kz1 - kz0 + kBK - 1) / kBK : 0;
+    if (nk > 0) {
+        load_tile(kz0);
+        store_tile(0);
+    }
+    __syncthreads();
+    // lane l: row/column l % 32, k = 8 * (l / 32) .. +7 of each 16-wide step
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    for (int t = 0; t < nk; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < nk) load_tile(kz0 + (t + 1) * kBK);
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[cur][wm + r][kk + kh]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn + r][kk + kh]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        if (t + 1 < nk) store_tile(cur ^ 1);
+        __syncthreads();
+    }
+    // D layout: lane l, register i -> column l % 32, row 8*(i/4) + 4*(l/32) + i%4
+    const int col = n0 + wn + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int row = m0 + wm + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (row >= M || col >= N) continue;
+        float v = acc[i];
+        if (bias) v += bias[col];
+        const size_t o = (size_t)row * ldc + col;
+        if (EPI == 1) {
+            if (Zout) Zout[o] = v;
+            v = gelu(v);
+        } else if (EPI == 2) {
+            v *= gelu_grad(Zin[o]);
+        }
+        if (Cf) Cf[o] = v;
+        if (Cb) Cb[o] = to_bf16(v);
+    }
+}
+
 // dY = 2 (Y - T) * scale, loss partial sums of (Y - T)^2 per block
 __global__ void mse_grad_kernel(int n, const float *__restrict__ Y, const float *__restrict__ T,
                                 float scale, float *__restrict__ dYf, __bf16 *__restrict__ dYb,
@@ -267,28 +300,46 @@ __global__ void mse_grad_kernel(int n, const float *__restrict__ Y, const float 
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// db[n] = sum_m dZ[m][n]: 64 columns per workgroup, 4 row groups of 64
-// lanes (row m goes to group m % 4, each group sums its rows in order), the
-// four group sums added in a fixed order -- deterministic, and every load
-// of a wave reads 64 consecutive floats of one row
-constexpr int kColW = 64, kColG = 4;
-__global__ __launch_bounds__(kColW * kColG) void colsum_kernel(int M, int N,
-                                                               const float *__restrict__ dZ,
-                                                               float *__restrict__ db) {
+// db[n] = sum_m dZ[m][n]: 8 columns per workgroup, each thread four
+// adjacent columns of every 128th row (row m goes to group m % 128, summed
+// in row order), the 128 group sums combined by a fixed pairwise tree --
+// deterministic.  The layers' N of a few hundred gives N / 8 workgroups,
+// each thread a short run of independent loads.
+constexpr int kColW = 8, kColG = 128;
+template <bool VEC>
+__global__ __launch_bounds__(256) void colsum_kernel(int M, int N, const float *__restrict__ dZ,
+                                                     float *__restrict__ db) {
     __shared__ float part[kColG][kColW];
-    const int c = threadIdx.x % kColW, g = threadIdx.x / kColW;
-    const int n = blockIdx.x * kColW + c;
-    float s = 0.0f;
-    if (n < N)
-        for (int m = g; m < M; m += kColG) s += dZ[(size_t)m * N + n];
-    part[g][c] = s;
-    __syncthreads();
-    if (g == 0 && n < N) {
-        float t = part[0][c];
+    const int q = threadIdx.x & 1, g = threadIdx.x >> 1;
+    const int n = blockIdx.x * kColW + 4 * q;
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (VEC && n + 4 <= N) {
+        for (int m = g; m < M; m += kColG) {
+            const float4 v = *reinterpret_cast<const float4 *>(dZ + (size_t)m * N + n);
+            s[0] += v.x;
+            s[1] += v.y;
+            s[2] += v.z;
+            s[3] += v.w;
+        }
+    } else {
+        for (int m = g; m < M; m += kColG)
 #pragma unroll
-        for (int k = 1; k < kColG; ++k) t += part[k][c];
-        db[n] = t;
+            for (int j = 0; j < 4; ++j)
+                if (n + j < N) s[j] += dZ[(size_t)m * N + n + j];
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[g][4 * q + j] = s[j];
+    __syncthreads();
+    for (int h = kColG / 2; h > 0; h >>= 1) {
+        if (g < h)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) part[g][4 * q + j] += part[g + h][4 * q + j];
+        __syncthreads();
+    }
+    if (g == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (n + j < N) db[n + j] = part[0][4 * q + j];
 }
 
 // Adam on FP32 master weights; refreshes the BF16 copy
@@ -387,9 +438,12 @@ int sn_gemm_bf16(int ta, int tb, int epi, int M, int N, int K, const void *A, in
     const dim3 gridz(grid.x, grid.y, S);
 #define SN_G(TA, TB, E)                                                                          \
     do {                                                                                         \
-        if (!big)                                                                                \
-            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E>), gridz, block, 0, s, M, N, K, a,    \
-                               lda, b, ldb, bias, Zin, Zout, Cdst, cb, ldc, kslice);             \
+        if (!big && vec)                                                                         \
+            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E, true>), gridz, block, 0, s, M, N, K,  \
+                               a, lda, b, ldb, bias, Zin, Zout, Cdst, cb, ldc, kslice);          \
+        else if (!big)                                                                           \
+            hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, E, false>), gridz, block, 0, s, M, N, K, \
+                               a, lda, b, ldb, bias, Zin, Zout, Cdst, cb, ldc, kslice);          \
         else if (vec)                                                                            \
             hipLaunchKernelGGL((gemm_bf16_big_kernel<TA, TB, E, true>), grid, block, 0, s, M, N,  \
                                K, a, lda, b, ldb, bias, Zin, Zout, Cf, cb, ldc);                 \
@@ -440,8 +494,13 @@ int sn_mse_grad(int n, const float *Y, const float *T, float scale, float *dYf, 
 
 int sn_colsum_f32(int M, int N, const float *dZ, float *db, void *stream) {
     SN_REQUIRE(M > 0 && N > 0 && dZ && db, "sn_colsum_f32: bad argument");
-    hipLaunchKernelGGL(colsum_kernel, dim3((N + kColW - 1) / kColW), dim3(kColW * kColG), 0,
-                       as_stream(stream), M, N, dZ, db);
+    const dim3 grid((N + kColW - 1) / kColW);
+    if (N % 4 == 0 && reinterpret_cast<uintptr_t>(dZ) % 16 == 0)
+        hipLaunchKernelGGL(colsum_kernel<true>, grid, dim3(256), 0, as_stream(stream), M, N, dZ,
+                           db);
+    else
+        hipLaunchKernelGGL(colsum_kernel<false>, grid, dim3(256), 0, as_stream(stream), M, N, dZ,
+                           db);
     SN_CHECK_LAUNCH();
     return SN_OK;
 }

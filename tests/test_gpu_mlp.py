@@ -102,3 +102,20 @@ def test_mlp_training_reduces_loss():
     net = MLP((8, 128, 128, 32), seed=1)
     hist = net.train(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda(), epochs=300, lr=3e-3)
     assert hist[-1] < 0.05 * hist[0], (hist[0], hist[-1])
+
+
+@pytest.mark.parametrize('M,N', [(2048, 512), (2048, 256), (333, 70), (5, 3)])
+def test_colsum_matches_float64(M, N):
+    """The bias gradient db = column sums of dZ (vector path for N % 4 == 0,
+    element path otherwise; ragged row and column counts)."""
+    import torch
+    from snoopy_amd import _native as nat
+    rng = np.random.default_rng(M + 7 * N)
+    Z = rng.standard_normal((M, N)).astype(np.float32)
+    dZ = torch.from_numpy(Z).cuda()
+    db = torch.full((N,), float('nan'), dtype=torch.float32, device='cuda')
+    nat.check(nat.load().sn_colsum_f32(M, N, nat.ptr(dZ), nat.ptr(db), nat.stream_handle()))
+    want = Z.astype(np.float64).sum(axis=0)
+    got = db.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(Z).sum(axis=0).max()
