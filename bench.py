@@ -79,6 +79,8 @@ def parse():
                          'anchor (batch of J-only variants: median-J variant from zero, the '
                          'others from its scaled solution); the other two batch starts are '
                          'timed beside it')
+    ap.add_argument('--batch-only', action='store_true',
+                    help='dipole1m: time the chosen batch start only (for traces)')
     ap.add_argument('--streams', type=int, default=None,
                     help='concurrent HIP streams (default 8 for dipole1m, 4 for batch64)')
     ap.add_argument('--ws-train', type=int, default=None,
@@ -235,10 +237,15 @@ def main():
         ws.to_device(asm, r_galerkin=args.ws_galerkin)
         ws.galerkin_steps = args.ws_galerkin_steps
         torch.cuda.synchronize()
+        t_c = time.perf_counter()
+        ws.precompute_constant(solver, lanes_per_row=args.lanes)
+        torch.cuda.synchronize()
+        t_const = time.perf_counter() - t_c
         ws_stats = {k: (float(v) if isinstance(v, (int, float, np.floating)) else v)
                     for k, v in ws.train_stats.items()}
         ws_stats['t_setup_total'] = time.perf_counter() - t
         ws_stats['galerkin_modes'] = ws.r_galerkin
+        ws_stats['t_constant_H_per_mesh'] = t_const
         p0 = {k: dp.p[k] for k in PARAM_NAMES}
         log('learned warm start trained: %s' % ws_stats)
 
@@ -310,7 +317,8 @@ def main():
         bdps = [Dipole(h=h, J=float(J)) for J in np.linspace(1.2e6, 1.8e6, args.batch)]
         cs = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
                               precond=args.precond, lanes_per_row=args.lanes)
-        modes = [args.batch_warm] + [m for m in ('none', 'anchor') if m != args.batch_warm]
+        modes = [args.batch_warm] + ([] if args.batch_only else
+                                     [m for m in ('none', 'anchor') if m != args.batch_warm])
         for mode in modes:
             cs.set_warm_start(None if mode == 'none' else mode, ws if mode == 'learned' else None)
             tb, res = timed_steps(lambda: cs.solve(args.rtol))

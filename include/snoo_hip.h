@@ -442,12 +442,18 @@ int sn_f32_to_bf16(int n, const float *x_dev, void *y_dev, void *stream);
  * sn_pod_element_modes: W [num_el][3][r] (float32), once per model and mesh.
  * sn_pod_reduced_jacobian: H (float64, r*r, row-major) to the host; r in
  *   {32, 64, 128}; synchronises the stream.
+ * sn_pod_reduced_jacobian_range: the same sum over the elements [e0, e1) of
+ *   the num_el elements only (H is additive over element ranges: a domain
+ *   whose Mr does not depend on the state, constant reluctivity, can be
+ *   summed once per geometry and added to the other domains' sums).
  * sn_pod_project: g[k] = sum_i U[k*n + i] v[i] (float64) to the host;
  *   synchronises the stream. */
 int sn_pod_element_modes(int num_el, int n, int r, const float *U_dev, const int *glob_ids_dev,
                          const int *orient_dev, float *W_dev, void *stream);
 int sn_pod_reduced_jacobian(int num_el, int r, const float *W_dev, const float *Mr_dev,
                             double *H_host, void *stream);
+int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float *W_dev,
+                                  const float *Mr_dev, double *H_host, void *stream);
 int sn_pod_project(int n, int r, const float *U_dev, const double *v_dev, double *g_host,
                    void *stream);
 /* out[i] = mean[i] + sum_j U[j*n + i] c[j]  (mean may be NULL) */

@@ -65,6 +65,8 @@ SIGNATURES = {
     'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_element_modes': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_reduced_jacobian': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pod_reduced_jacobian_range': (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                                               _vp]),
     'sn_pod_project': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_edge_transfer': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_ams_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,

@@ -308,6 +308,7 @@ class ConcurrentSolver:
         self.set_warm_start(warm_start, model)      # checks the variants first
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
+        self._prepare_learned()
 
     def set_warm_start(self, warm_start, model=None):
         """Choose the start of the next solves: None (A = 0), 'anchor' or
@@ -342,6 +343,16 @@ class ConcurrentSolver:
                     raise ValueError('anchor warm start: the variants may differ in the coil '
                                      'current density J only')
         self.warm_start = warm_start
+        self._prepare_learned()
+
+    def _prepare_learned(self):
+        """Set-up of the learned start on every variant's mesh (the modes'
+        face circulations and the state-independent part of the reduced
+        Jacobian), so that the solves do none of it."""
+        if self.warm_start != 'learned' or not hasattr(self, 'solvers'):
+            return
+        for s in self.solvers:
+            self.model.precompute_constant(s, lanes_per_row=self.kw['lanes_per_row'])
 
     def _run(self, w, idx, rtol, out, x0s=None, coeffs=None):
         import torch

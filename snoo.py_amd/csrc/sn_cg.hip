@@ -17,8 +17,14 @@
 // fences: on a multi-XCD part those would write back L2 per block).  Kernel
 // boundaries order the partial writes before their reads.  The host polls the
 // stop flag once per chunk of iterations; each chunk is captured once as a
-// hipGraph and replayed.
+// hipGraph and replayed.  Two chunk graphs are kept, the full chunk and a
+// quarter of it: a solve starts with the short one, and later replays take
+// the short one whenever the residual's geometric rate so far puts the stop
+// within three short chunks, so a solve that stops early in a chunk does not
+// replay most of a full one as launches that exit at entry.
+#include <algorithm>
 #include <climits>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -790,8 +796,11 @@ struct sn_cg {
     float *yloc = nullptr;       // [6*n_el] element products, slot j*n_el + e
     CgState *st = nullptr;       // [nseg]
     CgState *st_host = nullptr;  // [nseg], pinned
-    // cached graph of one chunk (2*chunk kernels), keyed by its pointers
+    // cached graphs of one chunk and of a short chunk (chunk_short
+    // iterations), keyed by their pointers
     hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph_short = nullptr;
+    int chunk_short = 8;
     const void *g_vals = nullptr;
     const unsigned char *g_mask = nullptr;
     void *g_x = nullptr;
@@ -967,10 +976,22 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
     }
 }
 
+static hipError_t drop_graphs(sn_cg *C) {
+    hipError_t e = hipSuccess;
+    for (hipGraphExec_t *g : {&C->graph, &C->graph_short}) {
+        if (*g) {
+            const hipError_t f = hipGraphExecDestroy(*g);
+            if (e == hipSuccess) e = f;
+            *g = nullptr;
+        }
+    }
+    return e;
+}
+
 static void launch_chunk(sn_cg *C, const void *vals, const unsigned char *mask, void *xdst,
-                         hipStream_t s) {
+                         int count, hipStream_t s) {
     // an even number of iterations keeps the p buffers and rho slots in step
-    for (int k = 0; k < C->chunk; ++k) {
+    for (int k = 0; k < count; ++k) {
         if (C->el_mode)
             launch_elem_iteration(C, (const float *)vals, mask, xdst, k & 1, s);
         else if (C->precision == 64 && !C->mat32)
@@ -1041,22 +1062,22 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
 
     // (re)capture the chunk graph when the pointers or stream changed
     if (!C->graph || C->g_vals != vals || C->g_mask != mask || C->g_x != xdst || C->g_stream != s) {
-        if (C->graph) {
-            SN_HIP(hipGraphExecDestroy(C->graph));
-            C->graph = nullptr;
-        }
+        SN_HIP(drop_graphs(C));
         hipStream_t cap = s;
         bool own = false;
         if (cap == nullptr) {
             SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             own = true;
         }
-        hipGraph_t g;
-        SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        launch_chunk(C, vals, mask, xdst, cap);
-        SN_HIP(hipStreamEndCapture(cap, &g));
-        SN_HIP(hipGraphInstantiate(&C->graph, g, nullptr, nullptr, 0));
-        SN_HIP(hipGraphDestroy(g));
+        for (int which = 0; which < 2; ++which) {
+            hipGraph_t g;
+            SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            launch_chunk(C, vals, mask, xdst, which ? C->chunk_short : C->chunk, cap);
+            SN_HIP(hipStreamEndCapture(cap, &g));
+            SN_HIP(hipGraphInstantiate(which ? &C->graph_short : &C->graph, g, nullptr, nullptr,
+                                       0));
+            SN_HIP(hipGraphDestroy(g));
+        }
         if (own) SN_HIP(hipStreamDestroy(cap));
         C->g_vals = vals;
         C->g_mask = mask;
@@ -1066,11 +1087,25 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
     const size_t sb = C->nseg * sizeof(CgState);
     SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
     SN_HIP(hipStreamSynchronize(s));
+    // ||r|| of every segment at iteration 0, for the rate estimate
+    std::vector<double> r_start(C->nseg);
+    for (int v = 0; v < C->nseg; ++v) r_start[v] = sqrt(C->st_host[v].rr);
     for (;;) {
-        bool running = false;
-        for (int v = 0; v < C->nseg; ++v) running |= (C->st_host[v].done == 0);
+        bool running = false, full = false;
+        for (int v = 0; v < C->nseg; ++v) {
+            const CgState &h = C->st_host[v];
+            if (h.done != 0) continue;
+            running = true;
+            if (h.it <= 0) continue;   // the first replay is a short one
+            // iterations still needed at the mean rate so far; no progress
+            // (or no estimate) takes the full chunk
+            const double r = sqrt(h.rr), r0 = r_start[v];
+            const double rate = (r > 0.0 && r0 > 0.0 && r < r0) ? log(r / r0) / h.it : 0.0;
+            const double need = rate < 0.0 ? log(h.tol / r) / rate : 1e30;
+            if (!(need <= 3.0 * C->chunk_short)) full = true;
+        }
         if (!running) break;
-        SN_HIP(hipGraphLaunch(C->graph, s));
+        SN_HIP(hipGraphLaunch(full ? C->graph : C->graph_short, s));
         SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
         SN_HIP(hipStreamSynchronize(s));
     }
@@ -1124,6 +1159,8 @@ int sn_cg_create_batched(sn_cg **out, const sn_pattern *P, int precision, int lp
     C->mat32 = mat32;
     C->lpr = lpr;
     C->chunk = chunk;
+    // a quarter chunk, even, at least 2
+    C->chunk_short = std::max(2, (chunk / 4) & ~1);
     C->nseg = nseg;
     C->row0_h = row0;
     const int cus = num_cus();
@@ -1167,10 +1204,7 @@ int sn_cg_create(sn_cg **out, const sn_pattern *P, int precision, int lpr, int c
 
 int sn_cg_set_schedule(sn_cg *C, int sched) {
     SN_REQUIRE(C && sched >= 0 && sched <= 3, "sn_cg_set_schedule: schedule in [0, 3]");
-    if (C->graph) {
-        SN_HIP(hipGraphExecDestroy(C->graph));
-        C->graph = nullptr;
-    }
+    SN_HIP(drop_graphs(C));
     C->sched = sched;
     return SN_OK;
 }
@@ -1178,10 +1212,7 @@ int sn_cg_set_schedule(sn_cg *C, int sched) {
 int sn_cg_set_col_offsets(sn_cg *C, int enable) {
     SN_REQUIRE(C, "sn_cg_set_col_offsets: null workspace");
     SN_REQUIRE(!enable || C->dcol, "sn_cg_set_col_offsets: some |column - row| exceeds 32767");
-    if (C->graph) {
-        SN_HIP(hipGraphExecDestroy(C->graph));
-        C->graph = nullptr;
-    }
+    SN_HIP(drop_graphs(C));
     C->use_dcol = enable != 0;
     return SN_OK;
 }
@@ -1192,10 +1223,7 @@ int sn_cg_set_ams(sn_cg *C, sn_ams *A) {
     SN_REQUIRE(C, "sn_cg_set_ams: null workspace");
     SN_REQUIRE(!A || A->n_edges == C->pat->n, "sn_cg_set_ams: %d edges, matrix has %d rows",
                A ? A->n_edges : 0, C->pat->n);
-    if (C->graph) {
-        SN_HIP(hipGraphExecDestroy(C->graph));
-        C->graph = nullptr;
-    }
+    SN_HIP(drop_graphs(C));
     C->ams = A;
     return SN_OK;
 }
@@ -1213,10 +1241,7 @@ int sn_cg_set_element_operator(sn_cg *C, const int *gids, const int *orient, int
         SN_REQUIRE((long long)C->pat->num_el * 6 < (1LL << 31),
                    "sn_cg_set_element_operator: too many elements");
     }
-    if (C->graph) {
-        SN_HIP(hipGraphExecDestroy(C->graph));
-        C->graph = nullptr;
-    }
+    SN_HIP(drop_graphs(C));
     if (!enable) {
         C->el_mode = false;
         return SN_OK;
@@ -1267,7 +1292,7 @@ int sn_element_apply_f64(const sn_pattern *P, const int *gids, const int *orient
 
 void sn_cg_destroy(sn_cg *C) {
     if (!C) return;
-    if (C->graph) (void)hipGraphExecDestroy(C->graph);
+    (void)drop_graphs(C);
     void *bufs[] = {C->r,    C->w0,       C->w1, C->q,     C->dinv,  C->xs,
                     C->partA, C->partB,   C->partbb, C->rtol_dev, C->st, C->row0, C->blkA0,
                     C->blkB0, C->segA,    C->segB, C->dcol, C->el_dofs, C->yloc};

@@ -49,9 +49,11 @@ constexpr int kGalEB = 16;   // elements staged in LDS per step
 
 // Partial H of a contiguous element range per workgroup; thread t owns the
 // TK x TK block of rows (t / 16) * TK.. and columns (t % 16) * TK.. of H.
+// W and Mr start at the first element of the range; Mr value u of element e
+// is Mr[u * mr_stride + e] (the element arrays' own stride).
 template <int R>
 __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
-    int num_el, const float *__restrict__ W, const float *__restrict__ Mr,
+    int num_el, int mr_stride, const float *__restrict__ W, const float *__restrict__ Mr,
     float *__restrict__ part) {
     constexpr int TK = R / 16;
     __shared__ __align__(16) float sW[kGalEB][3][R];
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
         }
         if (tid < kGalEB * 6) {
             const int le = tid / 6, u = tid % 6, e = e0 + le;
-            sM[le][u] = (e < num_el) ? Mr[(size_t)u * num_el + e] : 0.0f;
+            sM[le][u] = (e < num_el) ? Mr[(size_t)u * mr_stride + e] : 0.0f;
         }
         __syncthreads();
         // Z_e = Mr_e W_e (Mr upper triangle 00 01 02 11 12 22)
@@ -138,8 +140,9 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(int G, int m,
 
 // partial g[k] over one chunk of kProjPer * 256 entries per workgroup: the
 // workgroup's v values stay in registers, every mode k streams its slice of
-// U once (coalesced), wave sums go through LDS; part[b*r + k]
-constexpr int kProjPer = 8;
+// U once (coalesced), kProjModes modes at a time so that their loads are in
+// flight together; wave sums go through LDS; part[b*r + k]
+constexpr int kProjPer = 8, kProjModes = 8;
 __global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
                                                           const float *__restrict__ U,
                                                           const double *__restrict__ v,
@@ -155,16 +158,30 @@ __global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
     }
     for (int k0 = 0; k0 < r; k0 += 128) {
         const int kc = min(128, r - k0);
-        for (int kk = 0; kk < kc; ++kk) {
-            const float *u = U + (size_t)(k0 + kk) * n;
-            double s = 0.0;
+        for (int kk = 0; kk < kc; kk += kProjModes) {
+            float u[kProjModes][kProjPer];
 #pragma unroll
-            for (int j = 0; j < kProjPer; ++j) {
-                const long long i = base + 256LL * j;
-                if (i < n) s += (double)u[i] * vv[j];
+            for (int m = 0; m < kProjModes; ++m) {
+                const float *um = U + (size_t)(k0 + kk + m) * n;
+#pragma unroll
+                for (int j = 0; j < kProjPer; ++j) {
+                    const long long i = base + 256LL * j;
+                    u[m][j] = (kk + m < kc && i < n) ? um[i] : 0.0f;
+                }
             }
-            s = wave_sum(s);
-            if (lane == 0) ws[wid][kk] = s;
+            double s[kProjModes];
+#pragma unroll
+            for (int m = 0; m < kProjModes; ++m) {
+                s[m] = 0.0;
+#pragma unroll
+                for (int j = 0; j < kProjPer; ++j) s[m] += (double)u[m][j] * vv[j];
+            }
+#pragma unroll
+            for (int m = 0; m < kProjModes; ++m) s[m] = wave_sum(s[m]);
+            if (lane == 0)
+#pragma unroll
+                for (int m = 0; m < kProjModes; ++m)
+                    if (kk + m < kc) ws[wid][kk + m] = s[m];
         }
         __syncthreads();
         if (threadIdx.x < kc)
@@ -194,11 +211,21 @@ int sn_pod_element_modes(int num_el, int n, int r, const float *U, const int *gi
 
 int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, double *H_host,
                             void *stream) {
+    return sn_pod_reduced_jacobian_range(num_el, 0, num_el, r, W, Mr, H_host, stream);
+}
+
+int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float *W,
+                                  const float *Mr, double *H_host, void *stream) {
     SN_REQUIRE(num_el > 0 && W && Mr && H_host, "sn_pod_reduced_jacobian: bad argument");
+    SN_REQUIRE(0 <= e0 && e0 < e1 && e1 <= num_el,
+               "sn_pod_reduced_jacobian: element range [%d, %d) outside [0, %d)", e0, e1, num_el);
     SN_REQUIRE(r == 32 || r == 64 || r == 128,
                "sn_pod_reduced_jacobian: r must be 32, 64 or 128 (r=%d)", r);
     hipStream_t s = as_stream(stream);
-    const int nb = (num_el + kGalEB - 1) / kGalEB;
+    const int ne = e1 - e0;
+    const float *Wr = W + (size_t)e0 * 3 * r;
+    const float *Mrr = Mr + e0;
+    const int nb = (ne + kGalEB - 1) / kGalEB;
     const int G = nb < 512 ? nb : 512;
     const int m = r * r;
     float *part = nullptr;
@@ -207,13 +234,13 @@ int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, 
     SN_HIP(hipMallocAsync((void **)&H, (size_t)m * sizeof(double), s));
     if (r == 32)
         hipLaunchKernelGGL(pod_reduced_jacobian_kernel<32>, dim3(G), dim3(kGalBlock), 0, s,
-                           num_el, W, Mr, part);
+                           ne, num_el, Wr, Mrr, part);
     else if (r == 64)
         hipLaunchKernelGGL(pod_reduced_jacobian_kernel<64>, dim3(G), dim3(kGalBlock), 0, s,
-                           num_el, W, Mr, part);
+                           ne, num_el, Wr, Mrr, part);
     else
         hipLaunchKernelGGL(pod_reduced_jacobian_kernel<128>, dim3(G), dim3(kGalBlock), 0, s,
-                           num_el, W, Mr, part);
+                           ne, num_el, Wr, Mrr, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(sum_partials_kernel<float>, dim3((m + 63) / 64), dim3(256), 0, s, G,

@@ -178,6 +178,7 @@ class FamilySolver:
         nodes = self.family.nodes(d.p)
         self.asm.tet_mesh.nodes[:] = nodes
         self.asm._dev['nodes'].copy_(torch.from_numpy(nodes.reshape(-1)).to(device()))
+        self.asm.geometry_version = getattr(self.asm, 'geometry_version', 0) + 1
         self.solver.coil_list = d.coils
         self.solver.curl_curl_factory.material_list[0] = d.reluctance
         self.solver._src = None
@@ -459,6 +460,73 @@ class WarmStart:
         """Predicted edge DOFs on the family topology (device, float64)."""
         return self.expand(self.coefficients([p])[0])
 
+    @staticmethod
+    def _domain_ranges(asm):
+        """Element ranges [e0, e1) of the device element order, split into
+        those whose element matrices depend on the state (nonlinear or
+        diluted reluctivity) and those that do not (constant reluctivity):
+        ([(e0, e1), ...], [(n, e0, e1), ...])."""
+        from .materials import ConstantReluctance
+        offs = asm._device_state()['offsets']
+        var, const = [], []
+        for n in range(min(len(asm.material_list), len(offs) - 1)):
+            e0, e1 = int(offs[n]), int(offs[n + 1])
+            if e1 <= e0:
+                continue
+            if isinstance(asm.material_list[n], ConstantReluctance) and asm._dilution(n) is None:
+                const.append((n, e0, e1))
+            else:
+                var.append((e0, e1))
+        return var, const
+
+    def _range_H(self, drv, e0, e1):
+        H = np.empty(self.r_galerkin ** 2)
+        nat.check(nat.load().sn_pod_reduced_jacobian_range(
+            drv.d['num_el'], e0, e1, self.r_galerkin, nat.ptr(self._W), nat.ptr(drv.Mr),
+            H.ctypes.data_as(nat._vp), nat.stream_handle()))
+        return H
+
+    def reduced_jacobian(self, drv):
+        """H = U^T J U over the first r_galerkin modes, J the element
+        Jacobian of drv's last residual() call.  The domains of constant
+        reluctivity contribute a state-independent term, summed once per
+        geometry of the assembler (asm.geometry_version, bumped where node
+        coordinates are changed in place) and kept with the assembler; the
+        other domains are summed on every call."""
+        import weakref
+        asm = drv.asm
+        var, const = self._domain_ranges(asm)
+        H = np.zeros(self.r_galerkin ** 2)
+        for e0, e1 in var:
+            H += self._range_H(drv, e0, e1)
+        if const:
+            key = (getattr(asm, 'geometry_version', 0), self.r_galerkin,
+                   tuple((e0, e1, id(asm.material_list[n])) for n, e0, e1 in const))
+            cache = asm.__dict__.setdefault('_galerkin_const', weakref.WeakKeyDictionary())
+            hit = cache.get(self)
+            if hit is None or hit[0] != key:
+                Hc = np.zeros(self.r_galerkin ** 2)
+                for _, e0, e1 in const:
+                    Hc += self._range_H(drv, e0, e1)
+                hit = (key, Hc)
+                cache[self] = hit
+            H += hit[1]
+        return H.reshape(self.r_galerkin, self.r_galerkin)
+
+    def precompute_constant(self, solver, lanes_per_row=8):
+        """Form the state-independent part of the reduced Jacobian for
+        ``solver``'s geometry now (set-up), so that no start pays for it."""
+        import torch
+        self.to_device(solver.curl_curl_factory)
+        if not self.r_galerkin:
+            return self
+        drv = solver.newton_driver('mixed', lanes_per_row, 32, 'ams')
+        drv.retarget(solver.source_fields_device())
+        drv.residual(torch.zeros(solver.curl_curl_factory.num_dofs, dtype=torch.float64,
+                                 device=device()))
+        self.reduced_jacobian(drv)
+        return self
+
     def galerkin(self, drv, x, steps=1):
         """Reduced Newton steps in the span of the first r_galerkin modes:
         solve (U^T J U) da = -U^T R(x) and set x += U da, with J and R at x
@@ -467,16 +535,13 @@ class WarmStart:
         import torch
         lib = nat.load()
         rg = self.r_galerkin
-        N, n = drv.d['num_el'], drv.asm.num_dofs
-        H = np.empty(rg * rg)
+        n = drv.asm.num_dofs
         g = np.empty(rg)
         for _ in range(steps):
             drv.residual(x)
-            nat.check(lib.sn_pod_reduced_jacobian(N, rg, nat.ptr(self._W), nat.ptr(drv.Mr),
-                                                  H.ctypes.data_as(nat._vp), nat.stream_handle()))
+            Hm = self.reduced_jacobian(drv)
             nat.check(lib.sn_pod_project(n, rg, nat.ptr(self._U), nat.ptr(drv.R),
                                          g.ctypes.data_as(nat._vp), nat.stream_handle()))
-            Hm = H.reshape(rg, rg)
             Hm = 0.5 * (Hm + Hm.T)
             try:
                 da = np.linalg.solve(Hm, -g)

@@ -59,6 +59,58 @@ def test_reduced_jacobian_matches_oracle(r):
     assert np.abs(g - gref).max() <= 1e-12 * np.abs(gref).max()
 
 
+def test_reduced_jacobian_reuses_the_constant_domains():
+    """WarmStart.reduced_jacobian sums the air (constant reluctivity) once per
+    geometry and the iron on every call: at a second, saturating state it
+    still equals the full sum of the kernel over all elements (which then
+    differs from the first state's), and a new geometry version re-forms
+    the cached part."""
+    import torch
+    from snoopy_amd import _native as nat
+    from snoopy_amd.magnets import Dipole
+    from snoopy_amd.warm_start import WarmStart
+    r = 32
+    print('case: cached constant part of the reduced jacobian, dipole h=0.2', flush=True)
+    dp = Dipole(h=0.2)
+    sol = dp.solver(quad_order=2, verbose=False)
+    asm = sol.curl_curl_factory
+    d = asm._device_state()
+    N, n = d['num_el'], asm.num_dofs
+    rng = np.random.default_rng(5)
+    free = sol.make_boundary_mask()
+    U = rng.standard_normal((r, n)).astype(np.float32)
+    U[:, ~free] = 0.0
+    Ud = torch.from_numpy(U).cuda()
+    W = torch.empty(N * 3 * r, dtype=torch.float32, device='cuda')
+    lib = nat.load()
+    nat.check(lib.sn_pod_element_modes(N, n, r, nat.ptr(Ud), nat.ptr(d['gids']),
+                                       nat.ptr(d['orient']), nat.ptr(W), nat.stream_handle()))
+    ws = WarmStart.__new__(WarmStart)
+    ws._W, ws.r_galerkin = W, r
+    drv = sol.newton_driver('mixed', 8, 32, 'ams')
+    drv.retarget(sol.source_fields_device())
+    var, const = ws._domain_ranges(asm)
+    assert len(var) == 1 and len(const) == 1
+    Hs = []
+    for scale in (1e-4, 3e-2):
+        x = rng.standard_normal(n) * scale
+        x[~free] = 0.0
+        drv.residual(torch.from_numpy(x).cuda())
+        H = ws.reduced_jacobian(drv)
+        full = np.empty(r * r)
+        nat.check(lib.sn_pod_reduced_jacobian(N, r, nat.ptr(W), nat.ptr(drv.Mr),
+                                              full.ctypes.data_as(ctypes.c_void_p),
+                                              nat.stream_handle()))
+        full = full.reshape(r, r)
+        assert np.abs(H - full).max() <= 1e-5 * np.abs(full).max()
+        Hs.append(full)
+    assert np.abs(Hs[1] - Hs[0]).max() > 1e-3 * np.abs(Hs[0]).max()
+    key0 = asm._galerkin_const[ws][0]
+    asm.geometry_version = getattr(asm, 'geometry_version', 0) + 1
+    ws.reduced_jacobian(drv)
+    assert asm._galerkin_const[ws][0] != key0
+
+
 @pytest.fixture(scope='module')
 def trained():
     from snoopy_amd.warm_start import MorphFamily, WarmStart
