@@ -316,7 +316,8 @@ def main():
         from snoopy_amd.batch import ConcurrentSolver
         bdps = [Dipole(h=h, J=float(J)) for J in np.linspace(1.2e6, 1.8e6, args.batch)]
         cs = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
-                              precond=args.precond, lanes_per_row=args.lanes)
+                              precond=args.precond, lanes_per_row=args.lanes,
+                              ams_options=ams_opts)
         modes = [args.batch_warm] + ([] if args.batch_only else
                                      [m for m in ('none', 'anchor') if m != args.batch_warm])
         for mode in modes:

@@ -300,14 +300,21 @@ class ConcurrentSolver:
     Newton then runs to the same tolerance."""
 
     def __init__(self, dipoles, workers=4, quad_order=2, precision='mixed', precond='ams',
-                 lanes_per_row=8, warm_start=None, model=None):
+                 lanes_per_row=8, warm_start=None, model=None, ams_options=None):
         import torch
         self.dipoles = list(dipoles)
         self.workers = max(1, int(workers))
         self.kw = dict(precision=precision, precond=precond, lanes_per_row=lanes_per_row)
+        self.ams_options = dict(ams_options or {})
         self.set_warm_start(warm_start, model)      # checks the variants first
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
+        if self.ams_options and precond == 'ams':
+            # the options of the auxiliary-space preconditioner that each
+            # variant's driver builds on its first Newton step
+            for s in self.solvers:
+                s.newton_driver(precision, lanes_per_row, 32, 'ams').ams_options = \
+                    dict(self.ams_options)
         self._prepare_learned()
 
     def set_warm_start(self, warm_start, model=None):
