@@ -79,6 +79,8 @@ def parse():
                          'anchor (batch of J-only variants: median-J variant from zero, the '
                          'others from its scaled solution); the other two batch starts are '
                          'timed beside it')
+    ap.add_argument('--tune-grids', type=int, default=1,
+                    help='dipole1m: autotune the element-operator CG launch shapes (1) or not (0)')
     ap.add_argument('--batch-only', action='store_true',
                     help='dipole1m: time the chosen batch start only (for traces)')
     ap.add_argument('--streams', type=int, default=None,
@@ -223,6 +225,9 @@ def main():
     if args.lanes == 0:
         args.lanes = solver.tuned_lanes('mixed')
         log('autotuned lanes per row: %s' % solver.stats['autotune_lanes'])
+    if args.tune_grids:
+        solver.tuned_element_grids(args.lanes)
+        log('autotuned element-operator CG launch shapes: %s' % solver.stats['autotune_grids'])
     # learned warm start: a family of this dipole's mesh (same topology, grid
     # lines moved with the 8 parameters), snapshots from the solver, energy
     # POD and the BF16 network -- all set-up, outside the timed regions
@@ -465,6 +470,8 @@ def main():
                    'warm_start': ws is not None,
                    'batch_warm_start': start_desc[args.batch_warm],
                    'parallelism': 'single GPU'},
+        'autotune': {k: solver.stats[k] for k in ('autotune_lanes', 'autotune_grids')
+                     if k in solver.stats},
         'single_solve_per_s': 1.0 / t_mixed,
         'single_cold_solves_per_s': 1.0 / t_cold,
         'single_cold_cg_iterations': st_cold['cg_iterations'],

@@ -401,6 +401,16 @@ int sn_cg_set_element_operator(sn_cg *cg, const int *glob_ids_dev, const int *or
                                int enable, void *stream);
 /* 1 when the workspace multiplies with the element operator, else 0. */
 int sn_cg_element_operator(const sn_cg *cg);
+/* Launch shapes of the CG kernels (tuning knobs, no counterpart in the
+ * reference; results identical up to the summation order of the
+ * reductions).  grid_el: workgroups of the element-operator pass (0: one
+ * per 256 elements; capped at the SpMV grid, rounded down to a multiple of
+ * 8); vec_per_cu: workgroups of 1024 threads per CU for the DOF pass, the
+ * initialisation and the preconditioner's last kernel (default 1).
+ * Synchronises the stream; the next solve recaptures its graphs.
+ * sn_cg_grids reports the current shapes. */
+int sn_cg_set_grids(sn_cg *cg, int grid_el, int vec_per_cu, void *stream);
+int sn_cg_grids(const sn_cg *cg, int *grid_el, int *grid_vec);
 /* y = J x with the element operator in float64 (Mr_dev float64, layout of
  * sn_element_newton mode 2; rows and columns with mask == 0 removed, mask may
  * be NULL); work_dev [6*num_el] receives the element products.  No

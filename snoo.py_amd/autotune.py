@@ -1,11 +1,19 @@
 """Launch-shape autotuner for the CG kernels.
 
-For a given pattern and precision, times the CG SpMV kernel (``sn_cg_time_spmv``,
-HIP events on the solver's stream) for every lanes-per-row candidate on the
-actual matrix values and keeps the fastest.  Results are cached per
-(rows, nonzeros, precision) for the life of the process.
+* ``tune_lanes``: for a given pattern and precision, times the CG SpMV kernel
+  (``sn_cg_time_spmv``, HIP events on the solver's stream) for every
+  lanes-per-row candidate on the actual matrix values and keeps the fastest.
+  Cached per (rows, nonzeros, precision).
+* ``tune_element_grids``: for the mixed solver's element-operator CG, times
+  whole CG iterations (element pass, DOF pass and the auxiliary-space
+  preconditioner) on the driver's current Jacobian for candidate launch
+  shapes (``sn_cg_set_grids``: element-pass workgroups, DOF-pass workgroups
+  per CU) and keeps the fastest.  Cached per (elements, DOFs); drivers made
+  later for a mesh of that size start with it.
+Results live for the life of the process.
 """
 import ctypes
+import time
 
 from . import _native as nat
 
@@ -35,3 +43,59 @@ def tune_lanes(pattern, n, nnz, vals, mask, precision, candidates=(4, 8, 16), re
     best = min(times, key=times.get)
     _CACHE[key] = (best, times)
     return best, times
+
+
+_GRIDS = {}
+
+
+def element_grid_candidates(num_el):
+    """(grid_el, vec_per_cu) pairs: the default element grid (one workgroup
+    per 256 elements) and 1/2 and 1/4 of it, each with 1 or 2 DOF-pass
+    workgroups per CU."""
+    base = (int(num_el) + 255) // 256
+    els = [0] + [g for g in (base // 2 // 8 * 8, base // 4 // 8 * 8) if g >= 64]
+    return [(g, v) for g in els for v in (1, 2)]
+
+
+def cached_element_grids(num_el, num_dofs):
+    """The tuned (grid_el, vec_per_cu) of a mesh of this size, or None."""
+    hit = _GRIDS.get((int(num_el), int(num_dofs)))
+    return hit[0] if hit else None
+
+
+def tune_element_grids(drv, candidates=None, iters=64, reps=2):
+    """Fastest (grid_el, vec_per_cu) of the mixed driver's element-operator
+    CG and the measured ms per iteration of each candidate.  ``drv`` must
+    hold a Jacobian (residual() called) and its preconditioner; the CG runs
+    ``iters`` iterations (tolerance 0) from A = 0 on b = -R, ``reps`` times
+    per candidate after one untimed solve that also captures the graphs.
+    The winner is applied to drv's workspace."""
+    import torch
+    key = (int(drv.d['num_el']), int(drv.asm.num_dofs))
+    lib = nat.load()
+    cg = drv.cg_el
+    if key in _GRIDS:
+        g, v = _GRIDS[key][0]
+        nat.check(lib.sn_cg_set_grids(cg.h, g, v, nat.stream_handle()))
+        return _GRIDS[key]
+    torch.neg(drv.R, out=drv.b)
+    x = torch.zeros_like(drv.b)
+    times = {}
+    for g, v in (candidates or element_grid_candidates(key[0])):
+        nat.check(lib.sn_cg_set_grids(cg.h, g, v, nat.stream_handle()))
+        try:
+            cg.solve(drv.Mr, drv.mask, drv.b, x, rtol=0.0, atol=0.0, maxiter=iters)
+            torch.cuda.current_stream().synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                cg.solve(drv.Mr, drv.mask, drv.b, x, rtol=0.0, atol=0.0, maxiter=iters)
+            torch.cuda.current_stream().synchronize()
+        except RuntimeError:
+            continue            # a shape whose solve failed is not a candidate
+        times[(g, v)] = (time.perf_counter() - t) * 1e3 / (reps * iters)
+    if not times:
+        raise RuntimeError('tune_element_grids: no launch shape completed a solve')
+    best = min(times, key=times.get)
+    nat.check(lib.sn_cg_set_grids(cg.h, best[0], best[1], nat.stream_handle()))
+    _GRIDS[key] = (best, times)
+    return _GRIDS[key]

@@ -1274,6 +1274,42 @@ int sn_cg_set_element_operator(sn_cg *C, const int *gids, const int *orient, int
     return SN_OK;
 }
 
+int sn_cg_set_grids(sn_cg *C, int grid_el, int vec_per_cu, void *stream) {
+    SN_REQUIRE(C, "sn_cg_set_grids: null workspace");
+    SN_REQUIRE(grid_el >= 0, "sn_cg_set_grids: grid_el must be >= 0 (0: default)");
+    SN_REQUIRE(vec_per_cu >= 1 && vec_per_cu <= 8, "sn_cg_set_grids: vec_per_cu in [1, 8]");
+    hipStream_t s = as_stream(stream);
+    SN_HIP(drop_graphs(C));
+    // no kernel of this workspace may still read the buffers replaced here
+    SN_HIP(hipStreamSynchronize(s));
+    std::vector<int> blkB0, segB;
+    share_blocks(C->row0_h, vec_per_cu * num_cus(), kVecBlock, blkB0, segB);
+    for (void *p : {(void *)C->blkB0, (void *)C->segB, (void *)C->partB, (void *)C->partbb})
+        if (p) SN_HIP(hipFree(p));
+    C->blkB0 = C->segB = nullptr;
+    C->partB = C->partbb = nullptr;
+    int rc;
+    if ((rc = upload_ints(&C->blkB0, blkB0, s))) return rc;
+    if ((rc = upload_ints(&C->segB, segB, s))) return rc;
+    C->grid_vec = blkB0[C->nseg];
+    SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * sizeof(double)));
+    if (C->el_mode) {
+        int g = grid_el > 0 ? grid_el : grid_for(C->n_el, kRowBlock);
+        if (g > C->grid_rows) g = C->grid_rows;
+        if (g >= 8) g = g / 8 * 8;
+        C->grid_el = g;
+    }
+    return SN_OK;
+}
+
+int sn_cg_grids(const sn_cg *C, int *grid_el, int *grid_vec) {
+    SN_REQUIRE(C, "sn_cg_grids: null workspace");
+    if (grid_el) *grid_el = C->grid_el;
+    if (grid_vec) *grid_vec = C->grid_vec;
+    return SN_OK;
+}
+
 int sn_cg_element_operator(const sn_cg *C) { return (C && C->el_mode) ? 1 : 0; }
 
 int sn_element_apply_f64(const sn_pattern *P, const int *gids, const int *orient,

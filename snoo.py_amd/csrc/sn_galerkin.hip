@@ -138,58 +138,40 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(int G, int m,
     if (grp == 0 && i < m) out[i] = ((acc[0][c] + acc[1][c]) + acc[2][c]) + acc[3][c];
 }
 
-// partial g[k] over one chunk of kProjPer * 256 entries per workgroup: the
-// workgroup's v values stay in registers, every mode k streams its slice of
-// U once (coalesced), kProjModes modes at a time so that their loads are in
-// flight together; wave sums go through LDS; part[b*r + k]
-constexpr int kProjPer = 8, kProjModes = 8;
+// partial g[k] over one chunk of kProjPer * 256 entries and one group of
+// kProjModes modes per workgroup (grid: chunks x mode groups): the chunk's v
+// values stay in registers, every mode k of the group streams its slice of U
+// once (coalesced), wave sums go through LDS; part[b*r + k]
+constexpr int kProjPer = 8, kProjModes = 32;
 __global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
                                                           const float *__restrict__ U,
                                                           const double *__restrict__ v,
                                                           double *__restrict__ part) {
-    __shared__ double ws[4][128];
+    __shared__ double ws[4][kProjModes];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const long long base = (long long)blockIdx.x * 256 * kProjPer + threadIdx.x;
+    const int k0 = blockIdx.y * kProjModes, kc = min(kProjModes, r - k0);
     double vv[kProjPer];
 #pragma unroll
     for (int j = 0; j < kProjPer; ++j) {
         const long long i = base + 256LL * j;
         vv[j] = i < n ? v[i] : 0.0;
     }
-    for (int k0 = 0; k0 < r; k0 += 128) {
-        const int kc = min(128, r - k0);
-        for (int kk = 0; kk < kc; kk += kProjModes) {
-            float u[kProjModes][kProjPer];
+    for (int kk = 0; kk < kc; ++kk) {
+        const float *u = U + (size_t)(k0 + kk) * n;
+        double s = 0.0;
 #pragma unroll
-            for (int m = 0; m < kProjModes; ++m) {
-                const float *um = U + (size_t)(k0 + kk + m) * n;
-#pragma unroll
-                for (int j = 0; j < kProjPer; ++j) {
-                    const long long i = base + 256LL * j;
-                    u[m][j] = (kk + m < kc && i < n) ? um[i] : 0.0f;
-                }
-            }
-            double s[kProjModes];
-#pragma unroll
-            for (int m = 0; m < kProjModes; ++m) {
-                s[m] = 0.0;
-#pragma unroll
-                for (int j = 0; j < kProjPer; ++j) s[m] += (double)u[m][j] * vv[j];
-            }
-#pragma unroll
-            for (int m = 0; m < kProjModes; ++m) s[m] = wave_sum(s[m]);
-            if (lane == 0)
-#pragma unroll
-                for (int m = 0; m < kProjModes; ++m)
-                    if (kk + m < kc) ws[wid][kk + m] = s[m];
+        for (int j = 0; j < kProjPer; ++j) {
+            const long long i = base + 256LL * j;
+            if (i < n) s += (double)u[i] * vv[j];
         }
-        __syncthreads();
-        if (threadIdx.x < kc)
-            part[(size_t)blockIdx.x * r + k0 + threadIdx.x] =
-                ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) +
-                ws[3][threadIdx.x];
-        __syncthreads();
+        s = wave_sum(s);
+        if (lane == 0) ws[wid][kk] = s;
     }
+    __syncthreads();
+    if (threadIdx.x < kc)
+        part[(size_t)blockIdx.x * r + k0 + threadIdx.x] =
+            ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) + ws[3][threadIdx.x];
 }
 
 }  // namespace sn
@@ -266,7 +248,8 @@ int sn_pod_project(int n, int r, const float *U, const double *v, double *g_host
     double *part = nullptr, *g = nullptr;
     SN_HIP(hipMallocAsync((void **)&part, (size_t)G * r * sizeof(double), s));
     SN_HIP(hipMallocAsync((void **)&g, (size_t)r * sizeof(double), s));
-    hipLaunchKernelGGL(pod_project_kernel, dim3(G), dim3(256), 0, s, n, r, U, v, part);
+    hipLaunchKernelGGL(pod_project_kernel, dim3(G, (r + kProjModes - 1) / kProjModes), dim3(256),
+                       0, s, n, r, U, v, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(sum_partials_kernel<double>, dim3((r + 63) / 64), dim3(256), 0, s, G,

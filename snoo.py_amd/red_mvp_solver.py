@@ -189,6 +189,28 @@ def _tuned_lanes(solver, precision):
 RedMVPSolver.tuned_lanes = _tuned_lanes
 
 
+def _tuned_element_grids(solver, lanes_per_row=8):
+    """Launch shapes of the mixed solver's element-operator CG, timed on
+    the Jacobian at x = 0 with the auxiliary-space preconditioner; applied
+    to this solver's driver and to drivers made later for meshes of the
+    same size."""
+    import torch
+    from .autotune import tune_element_grids
+    asm = solver.curl_curl_factory
+    drv = solver.newton_driver('mixed', lanes_per_row, 32, 'ams')
+    drv.retarget(solver.source_fields_device())
+    drv.residual(torch.zeros(asm.num_dofs, dtype=torch.float64, device=device()))
+    drv._refresh_preconditioner()
+    best, times = tune_element_grids(drv)
+    solver.stats['autotune_grids'] = {'best': list(best),
+                                      'ms_per_iteration': {'%d/%d' % k: v
+                                                           for k, v in times.items()}}
+    return best
+
+
+RedMVPSolver.tuned_element_grids = _tuned_element_grids
+
+
 def _absmax(t):
     out = ctypes.c_double()
     nat.check(nat.load().sn_absmax_f64(int(t.numel()), nat.ptr(t), None, ctypes.byref(out),
@@ -289,6 +311,7 @@ class NewtonDriver:
             self.locJ = self.J = self.cg = None
             self.Mr = torch.empty(6 * N, dtype=torch.float32, device=dev)
             self.cg_el = _CG(asm, 32, lanes_per_row, chunk, element_operator=True)
+            self._tuned_grids(self.cg_el)
         else:
             self.locJ = torch.empty(21 * N, dtype=jt, device=dev)
             self.J = torch.empty(nnz, dtype=jt, device=dev)
@@ -307,6 +330,13 @@ class NewtonDriver:
             # stalls it; tighter forcing terms go to the tight solver from
             # the start
             self.eta_fp64 = 4.9e-3
+
+    def _tuned_grids(self, cg):
+        """Launch shapes tuned for a mesh of this size (autotune), if any."""
+        from .autotune import cached_element_grids
+        best = cached_element_grids(self.d['num_el'], self.asm.num_dofs)
+        if best is not None:
+            nat.check(nat.load().sn_cg_set_grids(cg.h, best[0], best[1], nat.stream_handle()))
 
     def _refresh_preconditioner(self):
         """Auxiliary-space preconditioner for the Jacobian of the last
@@ -382,6 +412,7 @@ class NewtonDriver:
             N = self.d['num_el']
             self.Mr = torch.empty(6 * N, dtype=torch.float32, device=device())
             self.cg_el = _CG(self.asm, 32, self.lanes_per_row, self.chunk, element_operator=True)
+            self._tuned_grids(self.cg_el)
             if self.ams is not None:
                 nat.check(nat.load().sn_cg_set_ams(self.cg_el.h, self.ams.handle))
         if not self.el_all:

@@ -189,6 +189,54 @@ def test_element_cg_matches_fp64_csr_cg(solver, precond):
     assert nat.load().sn_cg_element_operator(cg_el.h) == 0
 
 
+def test_element_cg_launch_shapes_give_the_same_solution(solver):
+    """sn_cg_set_grids (the autotuner's knobs) changes only the launch shapes:
+    the element-operator CG with the auxiliary-space preconditioner gives
+    the same iteration count (within 2) and solution (float32 summation
+    order) for every candidate shape, and the autotuner keeps one of them."""
+    import ctypes
+    torch = _torch()
+    from snoopy_amd import _native as nat
+    from snoopy_amd.autotune import element_grid_candidates, tune_element_grids
+    from snoopy_amd.red_mvp_solver import _CG
+    asm = solver.curl_curl_factory
+    mask = solver.mask_device()
+    drv = solver.newton_driver('mixed', 8, 32, 'ams')
+    drv.retarget(solver.source_fields_device())
+    x0 = torch.zeros(asm.num_dofs, dtype=torch.float64, device='cuda')
+    drv.residual(x0)
+    drv._refresh_preconditioner()
+    b = -drv.R
+    cg = _CG(asm, 32, 8, 32, element_operator=True)
+    nat.check(nat.load().sn_cg_set_ams(cg.h, drv.ams.handle))
+    ref = None
+    cands = element_grid_candidates(asm._device_state()['num_el'])
+    assert len(cands) >= 2
+    for g, v in cands:
+        nat.check(nat.load().sn_cg_set_grids(cg.h, g, v, None))
+        ge, gv = ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.load().sn_cg_grids(cg.h, ctypes.byref(ge), ctypes.byref(gv)))
+        assert ge.value % 8 == 0 and gv.value >= v
+        x = torch.zeros_like(x0)
+        it, _, rc = cg.solve(drv.Mr, mask, b, x, rtol=1e-6, atol=0.0, maxiter=3000)
+        assert rc == 0
+        print('shape', (g, v), '->', (ge.value, gv.value), 'iterations', it, flush=True)
+        if ref is None:
+            ref = (it, x.clone())
+        else:
+            assert abs(it - ref[0]) <= 2
+            assert float((x - ref[1]).norm() / ref[1].norm()) < 1e-4
+    from snoopy_amd import autotune
+    try:
+        best, times = tune_element_grids(drv, iters=16, reps=1)
+        assert best in times and best in cands
+    finally:
+        # later tests start from the default shapes
+        autotune._GRIDS.pop((asm._device_state()['num_el'], asm.num_dofs), None)
+    with pytest.raises(RuntimeError):
+        nat.check(nat.load().sn_cg_set_grids(cg.h, 0, 0, None))
+
+
 def test_element_operator_needs_a_float32_workspace(solver):
     from snoopy_amd import _native as nat
     from snoopy_amd.red_mvp_solver import _CG
