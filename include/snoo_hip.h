@@ -369,6 +369,13 @@ int sn_ams_update(sn_ams *ams, const double *nodes_dev, const int *cells_dev,
 int sn_ams_apply(sn_ams *ams, const unsigned char *mask_dev, const double *r_dev,
                  const double *dinv_dev, double *z_dev, void *stream);
 int sn_ams_num_levels(const sn_ams *ams);
+/* Lanes per row (4, 8 or 16; default 8) of the restriction, the up steps and
+ * the tail of the apply: a launch-shape knob for the autotuner (results
+ * identical up to the summation order).  A CG workspace that holds the
+ * handle recaptures its graphs on its next solve only when its own pointers
+ * change, so set the lanes before the first solve, or call sn_cg_set_grids
+ * (which drops the graphs) afterwards. */
+int sn_ams_set_lanes(sn_ams *ams, int lanes);
 /* Host set-up helper: greedy aggregation of the strength graph of a CSR
  * matrix (|a_ij| >= theta sqrt|a_ii a_jj|); agg_host [n] receives aggregate
  * ids 0..n_agg-1 (deterministic, index order). */

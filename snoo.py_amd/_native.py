@@ -89,6 +89,7 @@ SIGNATURES = {
     'sn_cg_set_element_operator': (_c_int, [_vp, _vp, _vp, _c_int, _vp]),
     'sn_cg_element_operator': (_c_int, [_vp]),
     'sn_cg_set_grids': (_c_int, [_vp, _c_int, _c_int, _vp]),
+    'sn_ams_set_lanes': (_c_int, [_vp, _c_int]),
     'sn_cg_grids': (_c_int, [_vp, _vp, _vp]),
     'sn_element_apply_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }

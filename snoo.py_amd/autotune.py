@@ -8,7 +8,8 @@
   whole CG iterations (element pass, DOF pass and the auxiliary-space
   preconditioner) on the driver's current Jacobian for candidate launch
   shapes (``sn_cg_set_grids``: element-pass workgroups, DOF-pass workgroups
-  per CU) and keeps the fastest.  Cached per (elements, DOFs); drivers made
+  per CU; ``sn_ams_set_lanes``: lanes per row of the preconditioner's
+  restriction, up and tail kernels) and keeps the fastest.  Cached per (elements, DOFs); drivers made
   later for a mesh of that size start with it.
 Results live for the life of the process.
 """
@@ -48,41 +49,50 @@ def tune_lanes(pattern, n, nnz, vals, mask, precision, candidates=(4, 8, 16), re
 _GRIDS = {}
 
 
-def element_grid_candidates(num_el):
-    """(grid_el, vec_per_cu) pairs: the default element grid (one workgroup
-    per 256 elements) and 1/2 and 1/4 of it, each with 1 or 2 DOF-pass
-    workgroups per CU."""
+def element_grid_candidates(num_el, ams=True):
+    """(grid_el, vec_per_cu, ams_lanes) triples: the default element grid (one
+    workgroup per 256 elements) with 1 or 2 DOF-pass workgroups per CU and
+    half the element grid with 1, each with 4, 8 or 16 preconditioner lanes
+    (8 only without the preconditioner)."""
     base = (int(num_el) + 255) // 256
-    els = [0] + [g for g in (base // 2 // 8 * 8, base // 4 // 8 * 8) if g >= 64]
-    return [(g, v) for g in els for v in (1, 2)]
+    half = base // 2 // 8 * 8
+    shapes = [(0, 1), (0, 2)] + ([(half, 1)] if half >= 64 else [])
+    return [(g, v, l) for g, v in shapes for l in ((4, 8, 16) if ams else (8,))]
 
 
 def cached_element_grids(num_el, num_dofs):
-    """The tuned (grid_el, vec_per_cu) of a mesh of this size, or None."""
+    """The tuned (grid_el, vec_per_cu, ams_lanes) of a mesh of this size, or None."""
     hit = _GRIDS.get((int(num_el), int(num_dofs)))
     return hit[0] if hit else None
 
 
+def _apply(lib, drv, shape):
+    g, v, l = shape
+    if drv.ams is not None:
+        nat.check(lib.sn_ams_set_lanes(drv.ams.handle, l))
+    # (also drops the captured graphs, so the next solve captures the new shape)
+    nat.check(lib.sn_cg_set_grids(drv.cg_el.h, g, v, nat.stream_handle()))
+
+
 def tune_element_grids(drv, candidates=None, iters=64, reps=2):
-    """Fastest (grid_el, vec_per_cu) of the mixed driver's element-operator
-    CG and the measured ms per iteration of each candidate.  ``drv`` must
-    hold a Jacobian (residual() called) and its preconditioner; the CG runs
-    ``iters`` iterations (tolerance 0) from A = 0 on b = -R, ``reps`` times
-    per candidate after one untimed solve that also captures the graphs.
-    The winner is applied to drv's workspace."""
+    """Fastest (grid_el, vec_per_cu, ams_lanes) of the mixed driver's
+    element-operator CG and the measured ms per iteration of each candidate.
+    ``drv`` must hold a Jacobian (residual() called) and its preconditioner;
+    the CG runs ``iters`` iterations (tolerance 0) from A = 0 on b = -R,
+    ``reps`` times per candidate after one untimed solve that also captures
+    the graphs.  The winner is applied to drv's workspace."""
     import torch
     key = (int(drv.d['num_el']), int(drv.asm.num_dofs))
     lib = nat.load()
     cg = drv.cg_el
     if key in _GRIDS:
-        g, v = _GRIDS[key][0]
-        nat.check(lib.sn_cg_set_grids(cg.h, g, v, nat.stream_handle()))
+        _apply(lib, drv, _GRIDS[key][0])
         return _GRIDS[key]
     torch.neg(drv.R, out=drv.b)
     x = torch.zeros_like(drv.b)
     times = {}
-    for g, v in (candidates or element_grid_candidates(key[0])):
-        nat.check(lib.sn_cg_set_grids(cg.h, g, v, nat.stream_handle()))
+    for shape in (candidates or element_grid_candidates(key[0], drv.ams is not None)):
+        _apply(lib, drv, shape)
         try:
             cg.solve(drv.Mr, drv.mask, drv.b, x, rtol=0.0, atol=0.0, maxiter=iters)
             torch.cuda.current_stream().synchronize()
@@ -92,10 +102,10 @@ def tune_element_grids(drv, candidates=None, iters=64, reps=2):
             torch.cuda.current_stream().synchronize()
         except RuntimeError:
             continue            # a shape whose solve failed is not a candidate
-        times[(g, v)] = (time.perf_counter() - t) * 1e3 / (reps * iters)
+        times[shape] = (time.perf_counter() - t) * 1e3 / (reps * iters)
     if not times:
         raise RuntimeError('tune_element_grids: no launch shape completed a solve')
     best = min(times, key=times.get)
-    nat.check(lib.sn_cg_set_grids(cg.h, best[0], best[1], nat.stream_handle()))
+    _apply(lib, drv, best)
     _GRIDS[key] = (best, times)
     return _GRIDS[key]

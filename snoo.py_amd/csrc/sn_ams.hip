@@ -563,15 +563,14 @@ __global__ __launch_bounds__(kAmsEdgeBlock) void ams_edge_prolong_kernel(
     if (threadIdx.x == 0 && partials) partials[(long long)stride * blockIdx.x] = sum;
 }
 
-template <typename V>
+template <typename V, int LPR>
 static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned char *mask, V *z,
                            int zs, const V *pin, double *partials, int stride, int grid,
                            const AmsSegments *seg, const int *done, int xcd, hipStream_t s) {
     const int L = (int)A->lv.size();
     AmsLevel &l0 = A->lv[0];
-    constexpr int kIncLanes = 8;
-    hipLaunchKernelGGL((ams_edge_restrict_kernel<V, kIncLanes>),
-                       dim3(ams_grid((long long)A->n_free * kIncLanes, kAmsBlock, xcd)),
+    hipLaunchKernelGGL((ams_edge_restrict_kernel<V, LPR>),
+                       dim3(ams_grid((long long)A->n_free * LPR, kAmsBlock, xcd)),
                        dim3(kAmsBlock), 0, s, A->n_free, A->inc_ptr, A->inc, A->xyz, r, l0.dinv,
                        A->omega, l0.b, l0.x, done, xcd);
     const int waves_per_block = kAmsBlock / kWave;
@@ -594,14 +593,14 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
         AmsLevel &f = A->lv[l];
         float4 *x_next = l > 0 ? A->lv[l - 1].x : (float4 *)nullptr;
         if (tail && l == L - 2) {
-            hipLaunchKernelGGL((ams_tail_kernel<8>),
-                               dim3(ams_grid((long long)f.n * 8, kAmsBlock, xcd)),
+            hipLaunchKernelGGL((ams_tail_kernel<LPR>),
+                               dim3(ams_grid((long long)f.n * LPR, kAmsBlock, xcd)),
                                dim3(kAmsBlock), 0, s, f.n, lc.n, f.indptr, f.indices, f.vals,
                                f.dinv, f.b, f.x, A->omega, lc.b, lc.agg_ptr, A->cinv,
                                f.agg_ptr, x_next, A->yx, done, xcd);
             continue;
         }
-        hipLaunchKernelGGL((ams_up_kernel<8>), dim3(ams_grid((long long)f.n * 8, kAmsBlock, xcd)),
+        hipLaunchKernelGGL((ams_up_kernel<LPR>), dim3(ams_grid((long long)f.n * LPR, kAmsBlock, xcd)),
                            dim3(kAmsBlock), 0, s, f.n, f.indptr, f.indices, f.vals, f.dinv, f.b,
                            f.x, A->omega, f.agg_ptr, x_next, A->yx, done, xcd);
     }
@@ -619,14 +618,21 @@ int ams_launch_apply(sn_ams *A, bool v64, const void *r, const void *dinv,
     SN_REQUIRE(zstride >= 1, "ams_launch_apply: zstride must be >= 1");
     SN_REQUIRE(!pin || zstride == 2, "ams_launch_apply: pairs need zstride 2");
     const int xcd = (sched & 2) ? 1 : 0;
-    if (v64)
-        launch_apply_t<double>(A, (const double *)r, (const double *)dinv, mask, (double *)z,
-                               zstride, (const double *)pin, partials, stride, grid, seg, done,
-                               xcd, s);
-    else
-        launch_apply_t<float>(A, (const float *)r, (const float *)dinv, mask, (float *)z,
-                              zstride, (const float *)pin, partials, stride, grid, seg, done,
-                              xcd, s);
+#define SN_AMS_APPLY(LPR)                                                                      \
+    do {                                                                                       \
+        if (v64)                                                                               \
+            launch_apply_t<double, LPR>(A, (const double *)r, (const double *)dinv, mask,      \
+                                        (double *)z, zstride, (const double *)pin, partials,   \
+                                        stride, grid, seg, done, xcd, s);                      \
+        else                                                                                   \
+            launch_apply_t<float, LPR>(A, (const float *)r, (const float *)dinv, mask,         \
+                                       (float *)z, zstride, (const float *)pin, partials,      \
+                                       stride, grid, seg, done, xcd, s);                       \
+    } while (0)
+    if (A->lanes == 4) SN_AMS_APPLY(4);
+    else if (A->lanes == 16) SN_AMS_APPLY(16);
+    else SN_AMS_APPLY(8);
+#undef SN_AMS_APPLY
     SN_CHECK_LAUNCH();
     return SN_OK;
 }
@@ -846,6 +852,14 @@ int sn_ams_level_values(const sn_ams *A, int level, float *vals, void *stream) {
 }
 
 int sn_ams_num_levels(const sn_ams *A) { return A ? (int)A->lv.size() : 0; }
+
+int sn_ams_set_lanes(sn_ams *A, int lanes) {
+    SN_REQUIRE(A, "sn_ams_set_lanes: null handle");
+    SN_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16,
+               "sn_ams_set_lanes: lanes must be 4, 8 or 16 (%d)", lanes);
+    A->lanes = lanes;
+    return SN_OK;
+}
 
 // Greedy aggregation on the strength graph (host): j is a strong neighbour
 // of i when |a_ij| >= theta sqrt(|a_ii a_jj|).  Pass 1 makes an aggregate of

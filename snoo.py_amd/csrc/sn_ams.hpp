@@ -43,6 +43,7 @@ struct sn_ams {
     float omega = 0.6f;          // damped-Jacobi weight of the nodal smoother
     float weight = 1.0f;         // weight of the auxiliary correction
     bool ready = false;          // numeric values present
+    int lanes = 8;               // lanes per row of the restriction, up and tail kernels
 };
 
 namespace sn {

@@ -203,7 +203,7 @@ def _tuned_element_grids(solver, lanes_per_row=8):
     drv._refresh_preconditioner()
     best, times = tune_element_grids(drv)
     solver.stats['autotune_grids'] = {'best': list(best),
-                                      'ms_per_iteration': {'%d/%d' % k: v
+                                      'ms_per_iteration': {'%d/%d/%d' % k: v
                                                            for k, v in times.items()}}
     return best
 
@@ -331,12 +331,17 @@ class NewtonDriver:
             # the start
             self.eta_fp64 = 4.9e-3
 
-    def _tuned_grids(self, cg):
-        """Launch shapes tuned for a mesh of this size (autotune), if any."""
+    def _tuned_grids(self, cg=None):
+        """Launch shapes tuned for a mesh of this size (autotune), if any:
+        the element-operator workspace's grids, the preconditioner's lanes."""
         from .autotune import cached_element_grids
         best = cached_element_grids(self.d['num_el'], self.asm.num_dofs)
-        if best is not None:
+        if best is None:
+            return
+        if cg is not None:
             nat.check(nat.load().sn_cg_set_grids(cg.h, best[0], best[1], nat.stream_handle()))
+        elif self.ams is not None and self.el_all:
+            nat.check(nat.load().sn_ams_set_lanes(self.ams.handle, best[2]))
 
     def _refresh_preconditioner(self):
         """Auxiliary-space preconditioner for the Jacobian of the last
@@ -347,6 +352,7 @@ class NewtonDriver:
             from .ams import AmsPreconditioner
             free = self.mask.cpu().numpy().astype(bool)
             self.ams = AmsPreconditioner(self.asm, free, self.nu_el, **self.ams_options)
+            self._tuned_grids()
             for cg in (self.cg, self.cg_safe, self.cg_el):
                 if cg is not None:
                     nat.check(nat.load().sn_cg_set_ams(cg.h, self.ams.handle))

@@ -190,10 +190,11 @@ def test_element_cg_matches_fp64_csr_cg(solver, precond):
 
 
 def test_element_cg_launch_shapes_give_the_same_solution(solver):
-    """sn_cg_set_grids (the autotuner's knobs) changes only the launch shapes:
-    the element-operator CG with the auxiliary-space preconditioner gives
-    the same iteration count (within 2) and solution (float32 summation
-    order) for every candidate shape, and the autotuner keeps one of them."""
+    """sn_cg_set_grids and sn_ams_set_lanes (the autotuner's knobs) change
+    only the launch shapes: the element-operator CG with the auxiliary-space
+    preconditioner gives the same iteration count (within 2) and solution
+    (float32 summation order) for every candidate shape, and the autotuner
+    keeps one of them."""
     import ctypes
     torch = _torch()
     from snoopy_amd import _native as nat
@@ -212,7 +213,8 @@ def test_element_cg_launch_shapes_give_the_same_solution(solver):
     ref = None
     cands = element_grid_candidates(asm._device_state()['num_el'])
     assert len(cands) >= 2
-    for g, v in cands:
+    for g, v, lanes in cands:
+        nat.check(nat.load().sn_ams_set_lanes(drv.ams.handle, lanes))
         nat.check(nat.load().sn_cg_set_grids(cg.h, g, v, None))
         ge, gv = ctypes.c_int(), ctypes.c_int()
         nat.check(nat.load().sn_cg_grids(cg.h, ctypes.byref(ge), ctypes.byref(gv)))
@@ -220,7 +222,7 @@ def test_element_cg_launch_shapes_give_the_same_solution(solver):
         x = torch.zeros_like(x0)
         it, _, rc = cg.solve(drv.Mr, mask, b, x, rtol=1e-6, atol=0.0, maxiter=3000)
         assert rc == 0
-        print('shape', (g, v), '->', (ge.value, gv.value), 'iterations', it, flush=True)
+        print('shape', (g, v, lanes), '->', (ge.value, gv.value), 'iterations', it, flush=True)
         if ref is None:
             ref = (it, x.clone())
         else:
@@ -235,6 +237,9 @@ def test_element_cg_launch_shapes_give_the_same_solution(solver):
         autotune._GRIDS.pop((asm._device_state()['num_el'], asm.num_dofs), None)
     with pytest.raises(RuntimeError):
         nat.check(nat.load().sn_cg_set_grids(cg.h, 0, 0, None))
+    with pytest.raises(RuntimeError):
+        nat.check(nat.load().sn_ams_set_lanes(drv.ams.handle, 5))
+    nat.check(nat.load().sn_ams_set_lanes(drv.ams.handle, 8))
 
 
 def test_element_operator_needs_a_float32_workspace(solver):
