@@ -362,9 +362,19 @@ class ConcurrentSolver:
             self.model.precompute_constant(s, lanes_per_row=self.kw['lanes_per_row'])
 
     def _run(self, w, idx, rtol, out, x0s=None, coeffs=None):
+        """Worker w: solve on its stream the variants ``idx`` (a list, or a
+        callable handing out the next variant, None when none is left)."""
         import torch
+        if callable(idx):
+            pull = idx
+        else:
+            it = iter([int(i) for i in idx])
+            pull = lambda: next(it, None)    # noqa: E731
         with torch.cuda.stream(self.streams[w]):
-            for i in idx:
+            while True:
+                i = pull()
+                if i is None:
+                    break
                 s = self.solvers[i]
                 s._src = None
                 x0, info = None if x0s is None else x0s.get(int(i)), None
@@ -387,7 +397,9 @@ class ConcurrentSolver:
         import threading
         import torch
         out = [None] * len(self.dipoles)
-        # largest variants first, dealt round-robin, so the streams finish together
+        # one queue, largest variants first: a worker takes the next variant
+        # as soon as its last one has converged, so the streams stay busy
+        # and finish together
         order = np.argsort([-d.num_tets for d in self.dipoles], kind='stable')
         x0s = None
         if self.warm_start == 'anchor' and len(self.dipoles) > 1:
@@ -408,9 +420,14 @@ class ConcurrentSolver:
             coeffs = self.model.coefficients([{k: d.p[k] for k in PARAM_NAMES}
                                               for d in self.dipoles])
             self.model.to_device(self.solvers[0].curl_curl_factory)
-        parts = [order[w::self.workers] for w in range(self.workers)]
+        pending = [int(i) for i in order]
+        lock = threading.Lock()
+
+        def pull():
+            with lock:
+                return pending.pop(0) if pending else None
         torch.cuda.synchronize()
-        threads = [threading.Thread(target=self._run, args=(w, parts[w], rtol, out, x0s, coeffs))
+        threads = [threading.Thread(target=self._run, args=(w, pull, rtol, out, x0s, coeffs))
                    for w in range(self.workers)]
         for t in threads:
             t.start()
