@@ -9,8 +9,19 @@ turns such a file into a ``TetModel``: linear tetrahedra (element type 4)
 with the physical tag of their volume, and the triangles (type 2) of the
 "Dirichlet Boundary" group -- or, when the file has no such group, the outer
 boundary faces.  Node ids are renumbered 0..N-1 in ascending gmsh tag order
-and negatively oriented tetrahedra are reoriented.  Edge DOFs are numbered by
-this build's rule (lexicographic node pairs), not by gmsh's edge tags.
+and negatively oriented tetrahedra are reoriented.
+
+Edge DOFs are numbered by this build's rule (lexicographic node pairs) by
+default.  ``edge_numbering='gmsh'`` numbers them as gmsh's ``createEdges``
+tags minus one, the reference's DOF ids (snoopy/mesh_tools.py:32-36,201-203,
+snoopy/curl_curl_assembler.py:125-129): gmsh numbers an edge when it first
+meets it, walking the model's entities by dimension and then tag (curves,
+surfaces, volumes), each entity's elements in file order, and each
+element's edges in gmsh's local order -- line (0,1); triangle (0,1), (1,2),
+(2,0); tetrahedron (0,1), (1,2), (2,0), (3,0), (3,2), (3,1) -- with the
+element's nodes as the file lists them.  This restates gmsh's public
+createEdges; gmsh is not in this image, so the numbering is not checked
+against gmsh itself.
 """
 import numpy as np
 
@@ -60,8 +71,56 @@ def _orient(nodes, cells):
     return cells
 
 
-def read_msh(path):
-    """TetModel of a gmsh ASCII mesh file (format 4.1 or 2.2)."""
+# gmsh's local edge order of lines, triangles and tetrahedra
+_GMSH_EDGES = {1: [(0, 1)], 2: [(0, 1), (1, 2), (2, 0)],
+               4: [(0, 1), (1, 2), (2, 0), (3, 0), (3, 2), (3, 1)]}
+_GMSH_NODES = {1: 2, 2: 3, 4: 4}
+
+
+def gmsh_edge_order(blocks, num_nodes):
+    """Node-pair keys (lo * num_nodes + hi, 0-based node ids) in the order
+    gmsh's createEdges numbers them: ``blocks`` = [((dim, entity tag),
+    element type, node array [m, k]) ...] in any order; entities are walked
+    by (dim, tag), elements in the given order."""
+    seq = []
+    for _, etype, conn in sorted(blocks, key=lambda b: b[0]):
+        if etype not in _GMSH_EDGES or len(conn) == 0:
+            continue
+        c = np.asarray(conn, dtype=np.int64)
+        pairs = np.stack([np.stack([c[:, a], c[:, b]], axis=1)
+                          for a, b in _GMSH_EDGES[etype]], axis=1).reshape(-1, 2)
+        seq.append(pairs.min(axis=1) * num_nodes + pairs.max(axis=1))
+    keys = np.concatenate(seq) if seq else np.zeros(0, dtype=np.int64)
+    uk, first = np.unique(keys, return_index=True)
+    return uk[np.argsort(first, kind='stable')]
+
+
+def renumber_edges(mesh, key_order):
+    """Give the edges of ``mesh`` the ids of their node-pair keys' positions
+    in ``key_order`` (in place): edges, glob_ids and boundary_edges."""
+    n = mesh.num_nodes
+    key_order = np.asarray(key_order, dtype=np.int64)
+    ekey = mesh.edges[:, 0].astype(np.int64) * n + mesh.edges[:, 1]
+    sorter = np.argsort(key_order, kind='stable')
+    new_id = sorter[np.clip(np.searchsorted(key_order, ekey, sorter=sorter), 0,
+                            max(len(key_order) - 1, 0))] if len(key_order) else ekey * 0
+    if len(key_order) != mesh.num_edges or not np.array_equal(key_order[new_id], ekey):
+        raise ValueError('edge numbering: the numbered edges are not the edges of the '
+                         'tetrahedra (%d numbered, %d in the mesh)'
+                         % (len(key_order), mesh.num_edges))
+    edges = np.empty_like(mesh.edges)
+    edges[new_id] = mesh.edges
+    mesh.edges = edges
+    mesh.glob_ids = new_id[mesh.glob_ids].astype(np.int32)
+    mesh.boundary_edges = new_id[mesh.boundary_edges].astype(np.int32)
+    return mesh
+
+
+def read_msh(path, edge_numbering='sorted'):
+    """TetModel of a gmsh ASCII mesh file (format 4.1 or 2.2); edge DOFs
+    numbered 'sorted' (this build's rule) or 'gmsh' (createEdges order)."""
+    if edge_numbering not in ('sorted', 'gmsh'):
+        raise ValueError("edge_numbering must be 'sorted' or 'gmsh'")
     with open(path) as fh:
         sec = _sections(fh.read())
     if 'MeshFormat' not in sec:
@@ -76,6 +135,7 @@ def read_msh(path):
         names[(int(t[0]), int(t[1]))] = t[2].strip().strip('"')
     dir_phys = {tag for (dim, tag), nm in names.items() if dim == 2 and nm == DIRICHLET}
     tets, tet_tag, tris = [], [], []
+    blocks = []          # ((dim, entity), element type, nodes) for gmsh's edge order
     if version >= 4.0:
         ent_phys = {}
         if 'Entities' in sec:
@@ -107,6 +167,8 @@ def read_msh(path):
             k += 1
             rows = [_ints(el[k + i]) for i in range(nin)]
             k += nin
+            if etype in _GMSH_NODES and nin:
+                blocks.append(((dim, etag), etype, [r[1:1 + _GMSH_NODES[etype]] for r in rows]))
             phys = ent_phys.get((dim, etag), [])
             if etype == 4:
                 tets += [r[1:5] for r in rows]
@@ -124,6 +186,10 @@ def read_msh(path):
             etype, ntag = r[1], r[2]
             phys = r[3] if ntag > 0 else 0
             conn = r[3 + ntag:]
+            if etype in _GMSH_NODES:
+                dim = {1: 1, 2: 2, 4: 3}[etype]
+                ent = r[4] if ntag > 1 else phys
+                blocks.append(((dim, ent), etype, [conn[:_GMSH_NODES[etype]]]))
             if etype == 4:
                 tets.append(conn[:4])
                 tet_tag.append(phys)
@@ -139,6 +205,13 @@ def read_msh(path):
     else:
         faces = outer_faces(cells)
     mesh = TetMesh(nodes, cells, np.asarray(tet_tag, dtype=np.int32), boundary_faces=faces)
+    if edge_numbering == 'gmsh':
+        # the element nodes as the file lists them (before any reorientation)
+        merged = {}
+        for ent, etype, conn in blocks:
+            merged.setdefault((ent, etype), []).extend(conn)
+        renumber_edges(mesh, gmsh_edge_order(
+            [(ent, etype, remap(conn)) for (ent, etype), conn in merged.items()], len(nodes)))
     return TetModel(mesh)
 
 

@@ -393,16 +393,20 @@ def test_diluted_assembly_and_solve(dipole):
     assert _rel(Bd, B0) > 1e-4      # the dilution changes the field
 
 
-def test_msh_mesh_solves_like_the_original(dipole, solver, tmp_path):
+@pytest.mark.parametrize('numbering', ['sorted', 'gmsh'])
+def test_msh_mesh_solves_like_the_original(dipole, solver, tmp_path, numbering):
     """A mesh written to and read back from a gmsh 4.1 file (elements
-    regrouped by volume, point location by element bins) gives the original
+    regrouped by volume, point location by element bins), with this build's
+    edge numbering or gmsh's createEdges order, gives the original
     solution's B at probe points."""
     from snoopy_amd.evaluators import evaluate_fem_solution_node_interpolation
     from snoopy_amd.msh_io import read_msh, write_msh
     from snoopy_amd.red_mvp_solver import RedMVPSolver
     p = tmp_path / 'dipole.msh'
     write_msh(str(p), dipole.mesh)
-    model = read_msh(str(p))
+    model = read_msh(str(p), edge_numbering=numbering)
+    if numbering == 'gmsh':
+        assert not np.array_equal(model.tet_mesh.edges, dipole.mesh.edges)
     s2 = RedMVPSolver(model, dipole.coils, [1], [2], [dipole.reluctance], quad_order=2,
                       verbose=False)
     x1 = solver.solve_newton('fp64', rtol=1e-10).cpu().numpy()

@@ -77,3 +77,59 @@ def test_binned_point_location_matches_structured():
     # points strictly inside an element are found in that element
     ok = a >= 0
     np.testing.assert_array_equal(a[ok], b[ok])
+
+
+def test_gmsh_edge_numbering_of_the_two_tetrahedra_file(tmp_path):
+    """createEdges order worked out by hand for MSH22 (0-based node ids 0..4
+    for tags 10, 11, 12, 13, 20): entities by (dim, tag) -- triangle entity
+    5 (0 1 2), triangle entity 6 (1 2 4), tetrahedron entity 1 (0 1 2 3),
+    tetrahedron entity 2 (1 2 4 3, the file's order although it is
+    reoriented) -- each element's edges in gmsh's local order, an edge
+    numbered when first met."""
+    import importlib
+    mio = importlib.import_module('snoopy_amd.msh_io')
+    p = tmp_path / 'two.msh'
+    p.write_text(MSH22)
+    g = mio.read_msh(str(p), edge_numbering='gmsh').tet_mesh
+    s = mio.read_msh(str(p)).tet_mesh
+    want = [(0, 1), (1, 2), (0, 2), (2, 4), (1, 4), (0, 3), (2, 3), (1, 3), (3, 4)]
+    assert [tuple(e) for e in g.edges.tolist()] == want
+    # the same elements, orientations and boundary edges, under the new ids
+    np.testing.assert_array_equal(g.cells, s.cells)
+    np.testing.assert_array_equal(g.orient, s.orient)
+    np.testing.assert_array_equal(g.edges[g.glob_ids], s.edges[s.glob_ids])
+    assert {tuple(e) for e in g.edges[g.boundary_edges]} == {(0, 1), (0, 2), (1, 2)}
+
+
+def test_gmsh_edge_numbering_is_a_relabelling(tmp_path):
+    """On a dipole mesh written to 4.1 and read back, the gmsh order is a
+    permutation of the sorted edge ids: every element's local edges, the
+    Dirichlet edges and the orientations describe the same node pairs."""
+    import importlib
+    mg = importlib.import_module('snoopy_amd.magnets')
+    mio = importlib.import_module('snoopy_amd.msh_io')
+    d = mg.Dipole(h=0.15)
+    p = tmp_path / 'dipole.msh'
+    mio.write_msh(str(p), d.mesh)
+    s = mio.read_msh(str(p)).tet_mesh
+    g = mio.read_msh(str(p), edge_numbering='gmsh').tet_mesh
+    assert g.num_edges == s.num_edges
+    assert not np.array_equal(g.edges, s.edges)
+    np.testing.assert_array_equal(np.sort(g.edges, axis=0), np.sort(s.edges, axis=0))
+    np.testing.assert_array_equal(g.edges[g.glob_ids], s.edges[s.glob_ids])
+    np.testing.assert_array_equal(g.orient, s.orient)
+    key = lambda m: {tuple(e) for e in m.edges[m.boundary_edges].tolist()}   # noqa: E731
+    assert key(g) == key(s)
+    # the first edges are those of the Dirichlet triangles, the file's first block
+    first = {tuple(e) for e in g.edges[:3].tolist()}
+    assert first <= key(g)
+
+
+def test_edge_numbering_option_is_checked(tmp_path):
+    import importlib
+    import pytest
+    mio = importlib.import_module('snoopy_amd.msh_io')
+    p = tmp_path / 'two.msh'
+    p.write_text(MSH22)
+    with pytest.raises(ValueError):
+        mio.read_msh(str(p), edge_numbering='lexicographic')
