@@ -570,7 +570,8 @@ class WarmStart:
             galerkin_steps = self.galerkin_steps
         if galerkin_steps and self.r_galerkin:
             self.galerkin(drv, x, galerkin_steps)
-        r = drv.residual(x) / drv.r0()
+        r0 = drv.r0()              # before the judged residual, which Newton then reuses
+        r = drv.residual_masked(x) / r0
         use = bool(np.isfinite(r) and r < 1.0)
         return (x if use else None), {'r_start_rel': float(r), 'used': use}
 
