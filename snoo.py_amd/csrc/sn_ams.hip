@@ -202,9 +202,9 @@ __global__ void ams_dinv_kernel(int n, const int *__restrict__ diagpos,
 // A coarsest matrix of up to kAmsLdsCoarse rows is eliminated in LDS (one
 // pivot step costs a few LDS round trips instead of L2 ones); larger ones
 // in the global scratch `a_glob`.  Same arithmetic either way.
-// Dynamic LDS: the matrix (when it fits, 80^2 doubles = 50 KB) followed by
-// two stages of pivot column and row (4 x 80 doubles), within the 64 KB a
-// workgroup may allocate; otherwise the staged pivot column and row only
+// Dynamic LDS: the matrix (when it fits, 80^2 doubles = 50 KB) followed by the
+// staged pivot column and row (2 nc doubles), within the 64 KB a workgroup
+// may allocate
 constexpr int kAmsLdsCoarse = 80;
 
 // One pivot step on entry (i, j), from the old pivot row and column staged
@@ -235,41 +235,13 @@ __global__ __launch_bounds__(kAmsInvThreads) void ams_coarse_invert_kernel(
     for (int i = t; i < nc; i += T)
         for (int k = indptr[i]; k < indptr[i + 1]; ++k) a[(size_t)i * nc + indices[k]] = vals[k];
     __syncthreads();
-    if (in_lds) {
-        // every thread keeps its entries (row-major index t + m T) in
-        // registers; per pivot k the owners of column k and row k stage
-        // them, and one barrier separates the staging from the updates:
-        // the staging buffers alternate between pivots, and a thread that
-        // passed pivot k's barrier has finished pivot k-1's updates, the
-        // last reads of the buffer that pivot k+1 overwrites
-        int ei[kAmsLdsEntries], ej[kAmsLdsEntries];
-        double v[kAmsLdsEntries];
+    // a thread's entries of an LDS-resident matrix: (row, column) formed once
+    int ei[kAmsLdsEntries], ej[kAmsLdsEntries];
 #pragma unroll
-        for (int m = 0; m < kAmsLdsEntries; ++m) {
-            const int idx = t + m * T;
-            const bool own = idx < nc * nc;
-            ei[m] = own ? idx / nc : -1;
-            ej[m] = own ? idx % nc : -1;
-            v[m] = own ? a[idx] : 0.0;
-        }
-        double *stage[2] = {col, col + 2 * kAmsLdsCoarse};
-        for (int k = 0; k < nc; ++k) {
-            double *c = stage[k & 1], *r = c + kAmsLdsCoarse;
-#pragma unroll
-            for (int m = 0; m < kAmsLdsEntries; ++m) {
-                if (ej[m] == k) c[ei[m]] = v[m];
-                if (ei[m] == k) r[ej[m]] = v[m];
-            }
-            __syncthreads();
-            const double pk = 1.0 / r[k];
-#pragma unroll
-            for (int m = 0; m < kAmsLdsEntries; ++m)
-                if (ei[m] >= 0) v[m] = gj_entry(v[m], ei[m], ej[m], k, pk, c, r);
-        }
-#pragma unroll
-        for (int m = 0; m < kAmsLdsEntries; ++m)
-            if (ei[m] >= 0) cinv[t + m * T] = (float)v[m];
-        return;
+    for (int m = 0; m < kAmsLdsEntries; ++m) {
+        const int idx = t + m * T;
+        ei[m] = (in_lds && idx < nc * nc) ? idx / nc : -1;
+        ej[m] = (in_lds && idx < nc * nc) ? idx % nc : 0;
     }
     // two barriers per pivot: stage the old pivot row and column, then
     // update every entry from them
@@ -280,8 +252,17 @@ __global__ __launch_bounds__(kAmsInvThreads) void ams_coarse_invert_kernel(
         }
         __syncthreads();
         const double pk = 1.0 / row[k];
-        for (int idx = t; idx < nc * nc; idx += T)
-            a[idx] = gj_entry(a[idx], idx / nc, idx % nc, k, pk, col, row);
+        if (in_lds) {
+#pragma unroll
+            for (int m = 0; m < kAmsLdsEntries; ++m) {
+                if (ei[m] < 0) continue;
+                const int idx = t + m * T;
+                a[idx] = gj_entry(a[idx], ei[m], ej[m], k, pk, col, row);
+            }
+        } else {
+            for (int idx = t; idx < nc * nc; idx += T)
+                a[idx] = gj_entry(a[idx], idx / nc, idx % nc, k, pk, col, row);
+        }
         __syncthreads();
     }
     for (int k = t; k < nc * nc; k += T) cinv[k] = (float)a[k];
@@ -838,9 +819,8 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
                            v.diagpos, v.vals, v.dinv);
     }
     AmsLevel &c = A->lv[L - 1];
-    // LDS-resident: the matrix and two stages of pivot column and row
-    const size_t lds = ((nc <= kAmsLdsCoarse) ? (size_t)nc * nc + 4 * kAmsLdsCoarse
-                                              : 2 * (size_t)nc) * sizeof(double);
+    const size_t lds = ((nc <= kAmsLdsCoarse) ? (size_t)nc * nc + 2 * nc : 2 * (size_t)nc) *
+                       sizeof(double);
     hipLaunchKernelGGL(ams_coarse_invert_kernel, dim3(1), dim3(kAmsInvThreads), lds, s, nc,
                        c.indptr,
                        c.indices, c.vals, A->dense, A->cinv);
