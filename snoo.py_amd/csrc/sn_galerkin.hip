@@ -157,16 +157,33 @@ __global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
         const long long i = base + 256LL * j;
         vv[j] = i < n ? v[i] : 0.0;
     }
-    for (int kk = 0; kk < kc; ++kk) {
-        const float *u = U + (size_t)(k0 + kk) * n;
-        double s = 0.0;
+    // two modes per trip: both modes' loads in flight before either sum
+    for (int kk = 0; kk < kc; kk += 2) {
+        const bool two = kk + 1 < kc;
+        const float *u0 = U + (size_t)(k0 + kk) * n;
+        const float *u1 = U + (size_t)(k0 + kk + (two ? 1 : 0)) * n;
+        float a0[kProjPer], a1[kProjPer];
 #pragma unroll
         for (int j = 0; j < kProjPer; ++j) {
             const long long i = base + 256LL * j;
-            if (i < n) s += (double)u[i] * vv[j];
+            a0[j] = i < n ? u0[i] : 0.0f;
+            a1[j] = i < n ? u1[i] : 0.0f;
         }
-        s = wave_sum(s);
-        if (lane == 0) ws[wid][kk] = s;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < kProjPer; ++j) {
+            const long long i = base + 256LL * j;
+            if (i < n) {
+                s0 += (double)a0[j] * vv[j];
+                s1 += (double)a1[j] * vv[j];
+            }
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if (lane == 0) {
+            ws[wid][kk] = s0;
+            if (two) ws[wid][kk + 1] = s1;
+        }
     }
     __syncthreads();
     if (threadIdx.x < kc)

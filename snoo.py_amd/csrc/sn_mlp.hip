@@ -371,7 +371,16 @@ __global__ void pod_expand_kernel(int n, int r, const float *__restrict__ U,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = mean ? mean[i] : 0.0;
-    for (int j = 0; j < r; ++j) s += (double)U[(size_t)j * n + i] * (double)c[j];
+    // eight modes' loads issued together, summed in mode order
+    int j = 0;
+    for (; j + 8 <= r; j += 8) {
+        float u[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) u[m] = U[(size_t)(j + m) * n + i];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) s += (double)u[m] * (double)c[j + m];
+    }
+    for (; j < r; ++j) s += (double)U[(size_t)j * n + i] * (double)c[j];
     out[i] = s;
 }
 
