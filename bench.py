@@ -523,9 +523,65 @@ def main():
         import gc
         gc.collect()
         torch.cuda.empty_cache()
+        if ws is not None:
+            out['family8'] = measure_family8(args, ws, steps=2)
         out['batch64'] = measure_batch64(args, steps=2)
         out['chain6'] = measure_chain6(args)
     print(json.dumps(out), flush=True)
+
+
+def measure_family8(args, ws, steps=2):
+    """8 seeded variants of the learned model's own family at the bench's
+    size: all 8 parameters drawn (yoke and aperture sizes, length, J, B_sat,
+    mu_r), each variant on its own moved copy of the bench mesh, solved
+    concurrently from the learned start and from zero on the same solver
+    objects, each to 1e-8 of its own |R(0)|.  Set-up (the moved meshes,
+    preconditioner plans, the learned start's per-mesh terms) is timed
+    apart."""
+    import torch
+    from snoopy_amd.batch import ConcurrentSolver
+    from snoopy_amd.warm_start import sample_params
+    fam = ws.family
+    t0 = time.perf_counter()
+    dps = []
+    for k in range(8):
+        p = sample_params(9100 + k)
+        m = fam.private_model()
+        m.tet_mesh.nodes[:] = fam.nodes(dict(fam.p0, **p))
+        dps.append(fam.variant(p, m))
+    cs = ConcurrentSolver(dps, workers=args.streams, precision='mixed', precond='ams',
+                          lanes_per_row=args.lanes, warm_start='learned', model=ws)
+    cs.solve(rtol=args.rtol)                  # first solve: device set-up and AMS plans
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    runs = {}
+    for mode in ('learned', None):
+        cs.set_warm_start(mode, ws if mode else None)
+        cs.solve(rtol=args.rtol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            res = cs.solve(rtol=args.rtol)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        runs[mode or 'none'] = {
+            's': dt, 'solves_per_s': len(dps) / dt,
+            'cg': [int(sum(o[1]['cg_iterations'])) for o in res],
+            'converged': all(bool(o[1]['converged']) for o in res),
+            'worst_final_residual': max(float(o[1]['residual_history'][-1]) for o in res)}
+        if mode:
+            runs['learned']['start_residual'] = [o[1].get('warm_start', {}).get('r_start_rel')
+                                                 for o in res]
+    out = {'variants': len(dps), 'tets': dps[0].num_tets, 'streams': args.streams,
+           'params_seeds': [9100, 9107], 'setup_s': t_setup, 'runs': runs,
+           'learned_solves_per_s': runs['learned']['solves_per_s'],
+           'cold_solves_per_s': runs['none']['solves_per_s']}
+    log('family8: %s' % out)
+    del cs
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
 
 
 def measure_batch64(args, steps=None):
