@@ -90,12 +90,18 @@ def aggregate(A, theta):
     return agg, int(na.value)
 
 
-# strength threshold of the aggregation: 0.25 gives fewer, larger aggregates
-# than 0.08 (1M-tet dipole: 156 555 / 6 120 / 240 / 12 rows against
-# 156 555 / 24 671 / 3 833 / 557 / 74).  The cycle costs 2 fewer launches,
-# CG takes 5-10 % more iterations, and solves are 3-6 % faster (8 learned
-# 1M-tet variants, 64 variants of 20k-335k tets, a 6M-tet chain).
-AMS_THETA = 0.25
+# strength threshold of the aggregation.  Measured on one MI355X (1M-tet
+# dipole: levels 156 555 / 24 671 / 3 847 / 556 / 75 at 0.08, 156 555 /
+# 6 120 / 240 / 12 at 0.25):
+#   0.08: 8 learned-start variants 97.4 solves/s, CG 77-79 each; from zero
+#         46.5; 64 variants of 20k-335k tets 226; 6M-tet chain 0.583 s
+#   0.12, 0.18: 94.7 and 84.1; 44.5 and 38.6; 204 and 218; 0.584 and 0.648 s
+#   0.25: 81.4, CG 146 for half the variants and 85 for the others (its
+#         coarser levels leave a plateau in the CG near a relative residual
+#         of 4e-3); 47.9; 227; 0.557 s
+# 0.08 is as fast elsewhere within 5 % and keeps the learned start's CG
+# counts even.
+AMS_THETA = 0.08
 
 
 class AmsPlan:

@@ -194,7 +194,9 @@ class BatchedSolver:
         while not conv.all() and k < max_newton:
             if rn_prev is not None:
                 eta = np.minimum(eta_max, np.maximum(0.9 * (rn / rn_prev) ** 2, eta_min))
-            eta = np.maximum(eta, np.minimum(eta_max, 0.5 * rtol * r0 / np.maximum(rn, 1e-300)))
+            # the step that can reach the target asks for just that (with the
+            # margin of NewtonDriver.solve)
+            eta = np.maximum(eta, np.minimum(eta_max, 0.9 * rtol * r0 / np.maximum(rn, 1e-300)))
             active = self._segment_mask(~conv)
             torch.mul(self.R, active, out=self.b)
             self.b.neg_()
@@ -306,6 +308,8 @@ class ConcurrentSolver:
         self.workers = max(1, int(workers))
         self.kw = dict(precision=precision, precond=precond, lanes_per_row=lanes_per_row)
         self.ams_options = dict(ams_options or {})
+        # extra keyword arguments of every variant's solve_newton (forcing terms)
+        self.newton_options = {}
         self.set_warm_start(warm_start, model)      # checks the variants first
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
@@ -385,7 +389,7 @@ class ConcurrentSolver:
                                                 lanes_per_row=self.kw['lanes_per_row'])
                 x = s.solve_newton(self.kw['precision'], rtol=rtol, x0=x0,
                                    lanes_per_row=self.kw['lanes_per_row'],
-                                   precond=self.kw['precond'])
+                                   precond=self.kw['precond'], **self.newton_options)
                 st = dict(s.stats)
                 if info is not None:
                     st['warm_start'] = info

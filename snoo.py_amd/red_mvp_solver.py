@@ -15,6 +15,7 @@ residual R(x) = K(x)x - rhs(x) (matrix-free, per element), stopped on
 iterative refinement: the FP64 residual is recomputed at the new iterate.
 """
 import ctypes
+import math
 import time
 
 import numpy as np
@@ -143,7 +144,8 @@ class RedMVPSolver:
 
     def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
                      eta_min=None, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True,
-                     log=None, precond='jacobi', tight='element', jacobian=None):
+                     log=None, precond='jacobi', tight='element', jacobian=None, eta_first=None,
+                     eta_margin=0.9):
         """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
 
         Returns the device solution (float64) and fills ``self.stats`` with the
@@ -155,7 +157,7 @@ class RedMVPSolver:
         drv.retarget(self.source_fields_device())
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
-                         stats=self.stats, log=log)
+                         stats=self.stats, log=log, eta_first=eta_first, eta_margin=eta_margin)
 
     def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32, precond='jacobi',
                       tight='element', jacobian=None):
@@ -209,6 +211,18 @@ def _tuned_element_grids(solver, lanes_per_row=8):
 
 
 RedMVPSolver.tuned_element_grids = _tuned_element_grids
+
+
+def planned_forcing(rel, rtol, eta_min, eta_max):
+    """Forcing term of the first Newton step from a start whose residual is
+    ``rel`` |R(0)|: the fewest steps that reach ``rtol`` when each removes at
+    most a factor 1/eta_min, and the equal factor per step that does it
+    (a start at 1.6e-3 |R(0)|, rtol 1e-8, eta_min 1e-3: 2 steps of 2.5e-3)."""
+    digits = math.log(rel / rtol)
+    if digits <= 0.0:
+        return eta_max
+    steps = max(1, math.ceil(digits / math.log(1.0 / eta_min) - 1e-9))
+    return min(eta_max, max(eta_min, math.exp(-digits / steps)))
 
 
 def _absmax(t):
@@ -451,7 +465,8 @@ class NewtonDriver:
         return self._r0
 
     def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=None, max_cg=-1,
-              line_search=True, stats=None, log=None, stall_steps=3):
+              line_search=True, stats=None, log=None, stall_steps=3, eta_first=None,
+              eta_margin=0.9, close_start=1e-2):
         """Newton steps until ||R|| <= rtol ||R(0)||, ``max_newton`` steps, or
         ``stall_steps`` consecutive stalled steps: full steps that reduce ||R||
         by less than 10%, or steps that no length of the line search makes
@@ -461,7 +476,16 @@ class NewtonDriver:
         space preconditioner) and, if it stalls there, the step is redone by
         the tight solver (float32 on the element operator, then float64 if
         that stalls too), as are later steps with a forcing term at least as
-        small."""
+        small.
+
+        Forcing terms: Eisenstat-Walker choice 2 (0.9 (|R_k|/|R_k-1|)^2 within
+        [eta_min, eta_max]), never below ``eta_margin`` times the distance
+        still to go, so the step that can reach the target asks for just
+        that.  The first step takes ``eta_first``: by default eta_max, except
+        from a start x0 whose residual is already at most ``close_start``
+        |R(0)|, where Newton is in its linear regime; that step takes the
+        forcing term of the fewest equal steps to the target
+        (``planned_forcing``)."""
         import torch
         stats = stats if stats is not None else {}
         if eta_min is None:
@@ -482,7 +506,9 @@ class NewtonDriver:
         # still those of residual_masked(x0)
         rn = last[2] if reuse else self.residual(x)
         hist, cg_its, etas, modes = [rn / r0], [], [], []
-        eta = eta_max
+        eta = eta_max if eta_first is None else eta_first
+        if eta_first is None and x0 is not None and 0.0 < rn <= close_start * r0:
+            eta = planned_forcing(rn / r0, rtol, eta_min, eta_max)
         rn_prev = None
         k = 0
         stalls = 0
@@ -492,7 +518,7 @@ class NewtonDriver:
             # asking for more than the remaining distance to the target
             if rn_prev is not None:
                 eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
-            eta = max(eta, min(eta_max, 0.5 * rtol * r0 / rn))
+            eta = max(eta, min(eta_max, eta_margin * rtol * r0 / rn))
             etas.append(eta)
             torch.neg(self.R, out=self.b)
             self._refresh_preconditioner()

@@ -87,3 +87,22 @@ def test_energy_pod_is_blind_to_the_null_space():
     G = p2.U.T.astype(np.float64) @ K @ p2.U.astype(np.float64)
     np.testing.assert_allclose(G, np.eye(G.shape[0]), atol=1e-3)
     np.testing.assert_allclose(p2.project((Z + junk)[5])[:3], p2.coeffs[5, :3], rtol=1e-3, atol=1e-4)
+
+
+def test_planned_forcing_splits_the_remaining_digits_into_equal_steps():
+    """A start at 1.6e-3 |R(0)| with rtol 1e-8 and at most 3 digits per
+    float32 step needs 2 steps of 10^-2.6 each; starts that 3 digits or
+    fewer separate from the target take one step; a start already at the
+    target takes eta_max."""
+    import math
+    from snoopy_amd.red_mvp_solver import planned_forcing
+    eta = planned_forcing(1.6e-3, 1e-8, 1e-3, 0.1)
+    assert math.isclose(eta, math.sqrt(1e-8 / 1.6e-3), rel_tol=1e-12)
+    assert math.isclose(planned_forcing(5e-6, 1e-8, 1e-3, 0.1), 2e-3, rel_tol=1e-12)
+    assert math.isclose(planned_forcing(1e-5, 1e-8, 1e-3, 0.1), 1e-3, rel_tol=1e-9)
+    # 6 digits exactly: two steps of 1e-3
+    assert math.isclose(planned_forcing(1e-2, 1e-8, 1e-3, 0.1), 1e-3, rel_tol=1e-9)
+    # 9 digits: three steps of 1e-3, never below eta_min
+    assert math.isclose(planned_forcing(10.0, 1e-8, 1e-3, 0.1), 1e-3, rel_tol=1e-9)
+    assert planned_forcing(5e-9, 1e-8, 1e-3, 0.1) == 0.1
+    assert planned_forcing(2e-8, 1e-8, 1e-3, 0.1) == 0.1     # 0.5 capped at eta_max
