@@ -10,13 +10,13 @@ A solve is complete: source field at the iron quadrature points
 (Biot-Savart) plus Newton from the start vector to ||R|| <= 1e-8 ||R(0)||.
 Mesh, CSR pattern and element-to-nonzero map are set up before the timed
 region (inputs resident in HBM).  One step is one batch: --batch variants of
-the dipole (coil current density 1.2e6..1.8e6 A/m^2) solved on --streams HIP
-streams in the chosen form (mixed precision with the AMS preconditioner) with
-the anchor warm start (--batch-warm): the middle variant is solved from zero,
-then the others concurrently from its solution scaled by the current-density
-ratio, each to ||R|| <= 1e-8 ||R(0)|| of its own; value = solves per second
-of the batch.  The same batch with every variant from zero is timed beside
-it (batch_cold_solves_per_s).  With --batch 0 a step is
+the dipole (coil current density 1.2e6..1.8e6 A/m^2) solved in the chosen
+form (mixed precision with the AMS preconditioner, the learned warm start,
+Newton in lock step with one variant-minor CG for all variants; --batch-warm,
+--batch-solver), each to ||R|| <= 1e-8 ||R(0)|| of its own; value = solves per
+second of the batch.  Beside it: the same batch from zero
+(batch_cold_solves_per_s), and on concurrent HIP streams with the learned,
+zero and anchor starts.  With --batch 0 a step is
 one solve.  The single solve of the chosen form and the plain FP64 GPU solve
 are timed beside it, and the relative B error at 10k aperture probes is
 reported against FP64 solves at 1e-11 (for the single solve and for every
@@ -81,6 +81,11 @@ def parse():
                          'timed beside it')
     ap.add_argument('--tune-grids', type=int, default=1,
                     help='dipole1m: autotune the element-operator CG launch shapes (1) or not (0)')
+    ap.add_argument('--batch-solver', default='variant', choices=['variant', 'concurrent'],
+                    help='dipole1m: the chosen batch form -- variant (lock-step Newton, one '
+                         'variant-minor CG for all variants, at most 8) or concurrent (one '
+                         'solver per variant on --streams HIP streams); the other is timed '
+                         'beside it')
     ap.add_argument('--batch-only', action='store_true',
                     help='dipole1m: time the chosen batch start only (for traces)')
     ap.add_argument('--streams', type=int, default=None,
@@ -313,33 +318,55 @@ def main():
 
     # batched: variants of the dipole (coil current density spread over
     # 1.2e6..1.8e6 A/m^2, the metric's 1.5e6 in the middle), each a complete
-    # solve, run concurrently on --streams HIP streams; the chosen start
-    # (learned), from zero, and the anchor start are timed on the same solvers
-    t_batch, bout, bdps, cs = None, None, None, None
+    # solve.  The chosen form (--batch-solver variant) runs them in lock step
+    # through the variant-minor CG: one set of kernel launches per CG
+    # iteration for all variants.  The chosen start and the start from zero
+    # are timed on it; beside it the same variants on --streams concurrent
+    # HIP streams (one solver each), with the learned, zero and anchor starts.
+    t_batch, bout, bdps, bsolver = None, None, None, None
     batch_runs = {}
     if args.batch > 0:
-        from snoopy_amd.batch import ConcurrentSolver
+        from snoopy_amd.batch import ConcurrentSolver, VariantBatchSolver
         bdps = [Dipole(h=h, J=float(J)) for J in np.linspace(1.2e6, 1.8e6, args.batch)]
-        cs = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
-                              precond=args.precond, lanes_per_row=args.lanes,
-                              ams_options=ams_opts)
-        modes = [args.batch_warm] + ([] if args.batch_only else
-                                     [m for m in ('none', 'anchor') if m != args.batch_warm])
-        for mode in modes:
-            cs.set_warm_start(None if mode == 'none' else mode, ws if mode == 'learned' else None)
-            tb, res = timed_steps(lambda: cs.solve(args.rtol))
+
+        def record(key, mode, tb, res, form):
             its = [int(sum(o[1]['cg_iterations'])) for o in res]
-            batch_runs[mode] = {'s': tb, 'solves_per_s': args.batch / tb, 'cg': its,
-                                'converged': all(bool(o[1]['converged']) for o in res)}
+            batch_runs[key] = {'s': tb, 'solves_per_s': args.batch / tb, 'cg': its,
+                               'form': form,
+                               'converged': all(bool(o[1]['converged']) for o in res)}
             if mode == 'learned':
-                batch_runs[mode]['start_residual'] = [
+                batch_runs[key]['start_residual'] = [
                     o[1].get('warm_start', {}).get('r_start_rel') for o in res]
-                batch_runs[mode]['cg_per_step_first'] = res[0][1]['cg_iterations']
-                batch_runs[mode]['residual_history_first'] = res[0][1]['residual_history']
-            log('batch (%s): %d variants on %d streams, %.3f s per batch (%.1f solves/s), cg %s'
-                % (mode, args.batch, args.streams, tb, args.batch / tb, its))
-            if mode == args.batch_warm:
-                t_batch, bout = tb, res
+                batch_runs[key]['cg_per_step_first'] = res[0][1]['cg_iterations']
+                batch_runs[key]['residual_history_first'] = res[0][1]['residual_history']
+            log('batch %s (%s): %d variants, %.3f s per batch (%.1f solves/s), cg %s'
+                % (form, mode, args.batch, tb, args.batch / tb, its))
+
+        runs = []
+        if args.batch_solver == 'variant' and args.batch <= 8:
+            vb = VariantBatchSolver(bdps, lanes_per_row=args.lanes, workers=args.streams)
+            vmodes = [m for m in [args.batch_warm] + ([] if args.batch_only else ['none'])
+                      if m != 'anchor']
+            runs += [('variant', m, vb) for m in dict.fromkeys(vmodes)]
+        if args.batch_solver == 'concurrent' or not args.batch_only:
+            cs = ConcurrentSolver(bdps, workers=args.streams, precision='mixed',
+                                  precond=args.precond, lanes_per_row=args.lanes,
+                                  ams_options=ams_opts)
+            cmodes = [args.batch_warm] + ([] if args.batch_only else
+                                          [m for m in ('none', 'anchor') if m != args.batch_warm])
+            runs += [('concurrent', m, cs) for m in cmodes]
+        chosen = 'variant' if any(r[0] == 'variant' for r in runs) else 'concurrent'
+        for form, mode, bs in runs:
+            bs.set_warm_start(None if mode == 'none' else mode, ws if mode == 'learned' else None)
+
+            def one_batch():
+                for s_ in bs.solvers:
+                    s_._src = None           # every solve computes its own sources
+                return bs.solve(args.rtol)
+            tb, res = timed_steps(one_batch)
+            record(mode if form == chosen else 'concurrent_' + mode, mode, tb, res, form)
+            if form == chosen and mode == args.batch_warm:
+                t_batch, bout, bsolver = tb, res, bs
     if args.quick:
         print(json.dumps({'quick': True, 'fp64_s': t_fp64, 'mixed_s': t_mixed,
                           'mixed_cold_s': t_cold, 'fp64_jacobi_s': t_jac, 'batch': batch_runs,
@@ -363,7 +390,7 @@ def main():
     batch_err = None
     if bout is not None:
         batch_err = 0.0
-        for bd, bs, (xb, _) in zip(bdps, cs.solvers, bout):
+        for bd, bs, (xb, _) in zip(bdps, bsolver.solvers, bout):
             xr = bs.solve_newton('fp64', rtol=1e-11, lanes_per_row=args.lanes, precond='ams')
             batch_err = max(batch_err, b_error(probe_B(bd, bs, xb, probes),
                                                probe_B(bd, bs, xr, probes))[0])
@@ -412,13 +439,47 @@ def main():
 
     log('roofline: element operator product %.4f ms (%.0f GB/s), csr spmv %.4f ms, '
         'element kernel %.3f ms' % (ms.value, op_gbs, ms_csr.value, asm_ms))
-    traffic = None
-    tfile = os.path.join(ROOT, 'profiles', 'cg_element_product_dipole1M_traffic.json')
-    # (PMC counts of the two kernels at this mesh; see profiles/)
-    if os.path.exists(tfile):
-        tr = json.load(open(tfile))
-        if tr.get('tets') == N:
-            traffic = tr['hbm_bytes_per_product']
+    def pmc_traffic(name, **match):
+        # HBM bytes per launch pair from the PMC runs summarised in profiles/
+        tfile = os.path.join(ROOT, 'profiles', name)
+        if os.path.exists(tfile):
+            tr = json.load(open(tfile))
+            if all(tr.get(k) == v for k, v in match.items()):
+                return tr['hbm_bytes_per_product']
+        return None
+    traffic = pmc_traffic('cg_element_product_dipole1M_traffic.json', tets=N)
+    roof_single = {'bound': 'hbm', 'achieved': op_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                   'frac': op_gbs / HBM_PEAK_GBS, 'traffic': traffic,
+                   'kernel': 'cg_elem_kernel<float> + cg_edge_kernel<float,float,false> '
+                             '(element operator product with the CG update, one variant)',
+                   'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': op_bytes}
+    roof = roof_single
+    if bsolver is not None and getattr(bsolver, 'h', None) is not None:
+        # the batch's own product: the variant-minor element and DOF passes,
+        # 200 back-to-back pairs for all variants at once
+        V = bsolver.nv
+        Mr_v = torch.stack([bsolver._driver(min(v, bsolver.nvar - 1)).Mr for v in range(V)],
+                           dim=1).contiguous()
+        ms_v = ctypes.c_double()
+        nat.check(lib.sn_vcg_time_pair(bsolver.h, nat.ptr(Mr_v), nat.ptr(drv.mask), 200,
+                                       ctypes.byref(ms_v), nat.stream_handle()))
+        del Mr_v
+        # per element: 6 DOF ids and 6 list entries (shared), per variant 6
+        # matrix values, 6 products written and read back; per DOF: range and
+        # mask (shared), per variant the {z, p} pair in each pass, p written,
+        # r and x read and written
+        v_bytes = N * (24 + 24 + 72 * V) + n * (5 + 36 * V) + 4
+        v_gbs = v_bytes / (ms_v.value * 1e-3) / 1e9
+        log('roofline: variant-minor element pair (%d variants) %.4f ms (%.0f GB/s)'
+            % (V, ms_v.value, v_gbs))
+        roof = {'bound': 'hbm', 'achieved': v_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': v_gbs / HBM_PEAK_GBS,
+                'traffic': pmc_traffic('vcg_element_pair_dipole1M_traffic.json', tets=N,
+                                       variants=V),
+                'kernel': 'vcg_elem_kernel<%d> + vcg_edge_kernel<%d,false> (the batch\'s '
+                          'element operator product with the CG update, %d variants at once)'
+                          % (V, V, V),
+                'avg_ms': ms_v.value, 'launches': 200, 'algorithmic_bytes': v_bytes}
     cpu = None
     if not args.no_cpu:
         log('cpu baseline leg')
@@ -439,11 +500,14 @@ def main():
         'none': 'every variant from A = 0'}
     if t_batch is not None:
         value, step_ms = args.batch / t_batch, 1e3 * t_batch
+        form = ('in lock step, one variant-minor CG for all variants (their own element '
+                'operators, forcing terms and stop tests; one auxiliary-space preconditioner)'
+                if batch_runs.get(args.batch_warm, {}).get('form') == 'variant'
+                else 'on %d concurrent HIP streams' % args.streams)
         workload = ('%d variants (coil current density 1.2e6-1.8e6 A/m^2) of the window-frame '
                     'dipole, %d tets, %d edge DOFs, nnz %d each, Q=4, B-spline-in-B^2 iron, '
-                    'Newton to 1e-8 of each variant\'s own |R(0)|, on %d HIP streams, %s; one '
-                    'step = the batch'
-                    % (args.batch, N, n, nnz, args.streams, start_desc[args.batch_warm]))
+                    'Newton to 1e-8 of each variant\'s own |R(0)|, %s, %s; one step = the batch'
+                    % (args.batch, N, n, nnz, form, start_desc[args.batch_warm]))
     else:
         value, step_ms = 1.0 / t_mixed, 1e3 * t_mixed
         workload = ('window-frame dipole, %d tets, %d edge DOFs, nnz %d, Q=4, '
@@ -480,7 +544,11 @@ def main():
         'batch_B_rel_err_max': batch_err,
         'batch_runs': batch_runs,
         'batch_cold_solves_per_s': batch_runs.get('none', {}).get('solves_per_s'),
-        'batch_anchor_solves_per_s_J_only': batch_runs.get('anchor', {}).get('solves_per_s'),
+        'batch_concurrent_solves_per_s': batch_runs.get('concurrent_' + args.batch_warm,
+                                                        {}).get('solves_per_s'),
+        'batch_anchor_solves_per_s_J_only': (batch_runs.get('anchor') or
+                                             batch_runs.get('concurrent_anchor') or
+                                             {}).get('solves_per_s'),
         'batch_learned_solves_per_s': batch_runs.get('learned', {}).get('solves_per_s'),
         'batch_cg_iterations': ([int(sum(o[1]['cg_iterations'])) for o in bout]
                                 if bout is not None else None),
@@ -502,11 +570,8 @@ def main():
         'cg_iterations': stm['cg_iterations'],
         'newton_steps_fp64': st64['newton_steps'],
         'cg_iterations_fp64': st64['cg_iterations'],
-        'roofline': {'bound': 'hbm', 'achieved': op_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': op_gbs / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': 'cg_elem_kernel<float> + cg_edge_kernel<float,float,false> '
-                               '(element operator product with the CG update)',
-                     'avg_ms': ms.value, 'launches': 200, 'algorithmic_bytes': op_bytes},
+        'roofline': roof,
+        'roofline_single_variant': roof_single,
         'csr_spmv': {'kernel': 'cg_spmv_kernel<float,float,%d,%s>'
                                % (args.lanes, 'short' if ib == 2 else 'int'),
                      'avg_ms': ms_csr.value, 'algorithmic_bytes': csr_bytes,
@@ -519,7 +584,7 @@ def main():
         out['speedup_vs_cpu'] = value / cpu['value']
     if not args.no_extra:
         # the batched and multi-magnet workloads on the same box, same run
-        del cs, bout, bdps
+        bsolver = bout = bdps = runs = vb = cs = None      # noqa: F841
         import gc
         gc.collect()
         torch.cuda.empty_cache()

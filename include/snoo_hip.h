@@ -418,6 +418,53 @@ int sn_cg_element_operator(const sn_cg *cg);
  * sn_cg_grids reports the current shapes. */
 int sn_cg_set_grids(sn_cg *cg, int grid_el, int vec_per_cu, void *stream);
 int sn_cg_grids(const sn_cg *cg, int *grid_el, int *grid_vec);
+
+/* ------------------------------------------------- variant-minor batched CG
+ * No counterpart in the reference, which solves one magnet per call
+ * (snoopy/red_mvp_solver.py:121-224, its scipy cg at :202-205).  nvar systems
+ * on one mesh topology (the pattern's DOF numbering and element lists, one
+ * Dirichlet mask) with their own element operators, right-hand sides and
+ * stop tests, solved by the same kernels and captured graphs: every vector
+ * is stored variant-minor (entry i*nvar + v), the element matrices as
+ * Mr[(k*num_el + e)*nvar + v] (k < 6, the layout of sn_element_newton mode 2
+ * per variant, interleaved), so each index is read once for all systems.
+ * Per system the arithmetic is the float32 element-operator CG of
+ * sn_cg_set_element_operator (scipy's cg rule, float64 inner products), with
+ * per-system reductions in a fixed order.
+ * nvar in {1, 2, 4, 8}; glob_ids_dev [6*num_el], orient_dev [num_el] as
+ * given to the element kernels. */
+typedef struct sn_vcg sn_vcg;
+int sn_vcg_create(sn_vcg **out, const sn_pattern *pat, const int *glob_ids_dev,
+                  const int *orient_dev, int nvar, int chunk, void *stream);
+/* The auxiliary-space preconditioner shared by all systems (NULL: each
+ * system's Jacobi diagonal): its hierarchy, values and coordinates are those
+ * the caller last set with sn_ams_update (one system's, or any symmetric
+ * positive definite choice); the workspace keeps per-system vectors of every
+ * level.  Call again after the hierarchy changes. */
+int sn_vcg_set_ams(sn_vcg *vcg, sn_ams *ams, void *stream);
+/* Blocks grouped per XCD (1, the default) or dealt round-robin (0); results
+ * identical up to the summation order of the reductions. */
+int sn_vcg_set_schedule(sn_vcg *vcg, int xcd);
+/* Solve all systems: Mr_dev float32 [6*num_el*nvar], b_dev float64 [n*nvar];
+ * each CG starts from 0 and its float32 result is ADDED to x_dev (float64
+ * [n*nvar], free rows only).  rtol_host [nvar]; stop when ||r_v|| <
+ * max(atol, rtol_v ||b_v||) at the top of an iteration; at most maxiter
+ * iterations (< 0: 10 n).  A system with b_v = 0 is done at once.  Per-system
+ * iteration counts, final recursive-residual norms and status (SN_OK,
+ * SN_ERR_NOT_CONVERGED, SN_ERR_BREAKDOWN) to the host arrays (may be NULL);
+ * returns the first non-OK status. */
+int sn_vcg_solve(sn_vcg *vcg, const float *Mr_dev, const unsigned char *mask_dev,
+                 const double *b_dev, double *x_dev, const double *rtol_host, double atol,
+                 int maxiter, int *iters_host, double *rnorm_host, int *status_host,
+                 void *stream);
+/* Average duration (ms) of the CG's element pass and DOF pass (the product
+ * with the element operators and the CG update of every system) over `reps`
+ * back-to-back pairs on `stream`, timed with HIP events, every system
+ * running; overwrites the workspace's vectors (run after a solve). */
+int sn_vcg_time_pair(sn_vcg *vcg, const float *Mr_dev, const unsigned char *mask_dev, int reps,
+                     double *avg_ms_host, void *stream);
+int sn_vcg_info(const sn_vcg *vcg, int *nvar, int *grid_el, int *grid_vec);
+void sn_vcg_destroy(sn_vcg *vcg);
 /* y = J x with the element operator in float64 (Mr_dev float64, layout of
  * sn_element_newton mode 2; rows and columns with mask == 0 removed, mask may
  * be NULL); work_dev [6*num_el] receives the element products.  No

@@ -470,3 +470,281 @@ def anchor_usable(stats, x):
     converged and every entry is finite; otherwise they start from zero."""
     import torch
     return bool(stats.get('converged')) and bool(torch.isfinite(x).all().item())
+
+
+def _next_power_of_two(n):
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+class VariantBatchSolver:
+    """Variants that share one mesh topology, solved in lock step by the
+    variant-minor CG (``sn_vcg``): one set of kernel launches per CG iteration
+    for all of them, every index stream read once for the batch.
+
+    The variants may differ in their coils (J, placement), their materials
+    and their node coordinates (moved copies of one mesh: the learned start's
+    family), not in the mesh's cells or Dirichlet edges.  Each variant keeps
+    its own residual, Jacobian, forcing terms and stop test (Newton to
+    ||R_v|| <= rtol ||R_v(0)||, R_v(0) its residual at zero, as
+    ``RedMVPSolver.solve_newton``); the float32 CG of a Newton step runs for
+    all unconverged variants at once, each to its own forcing term.  The
+    auxiliary-space preconditioner is shared: it is the preconditioner of the
+    anchor variant (the middle one of the list), refreshed with the anchor's
+    reluctivities at every step.  A variant whose float32 CG stops at its cap,
+    or whose Newton stalls, is finished by its own ``solve_newton``.
+
+    ``warm_start='learned'`` with ``model`` (a WarmStart of the variants'
+    family topology): one batched forward pass of the network, then each
+    variant's learned start (WarmStart.start) before the lock-step Newton.
+    At most 8 variants; a batch of 3, 5, 6 or 7 pads the CG to 4 or 8 systems
+    with empty right-hand sides."""
+
+    def __init__(self, dipoles, quad_order=2, lanes_per_row=8, chunk=32, warm_start=None,
+                 model=None, anchor=None, workers=8):
+        import torch
+        self.dipoles = list(dipoles)
+        V = len(self.dipoles)
+        if not 1 <= V <= 8:
+            raise ValueError('VariantBatchSolver: 1 to 8 variants (%d given)' % V)
+        m0 = self.dipoles[0].mesh
+        for d in self.dipoles[1:]:
+            m = d.mesh
+            if (m.num_edges != m0.num_edges or m.cells.shape != m0.cells.shape
+                    or not np.array_equal(m.cells, m0.cells)
+                    or not np.array_equal(m.glob_ids, m0.glob_ids)
+                    or not np.array_equal(m.boundary_edges, m0.boundary_edges)):
+                raise ValueError('VariantBatchSolver: every variant must share the mesh topology')
+        self.nvar = V
+        self.nv = _next_power_of_two(V)
+        self.lanes_per_row, self.chunk = lanes_per_row, chunk
+        self.solvers = [d.solver(quad_order=quad_order, verbose=False) for d in self.dipoles]
+        self.anchor = (V - 1) // 2 if anchor is None else int(anchor)
+        # the per-variant work around the batched CG (sources, learned start,
+        # residuals) runs on one HIP stream per variant, from host threads:
+        # variant v always on stream v % workers, so its tensors are
+        # allocated, used and freed on one stream
+        self.workers = max(1, min(int(workers), V))
+        self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
+        self.h = None
+        self.ams = None
+        self.stats = {}
+        self.set_warm_start(warm_start, model)
+
+    def set_warm_start(self, warm_start, model=None):
+        if warm_start not in (None, 'learned'):
+            raise ValueError("warm_start must be None or 'learned'")
+        if warm_start == 'learned':
+            if model is None:
+                raise ValueError('learned warm start: pass the trained model (model=WarmStart)')
+            fm = model.family.mesh
+            for d in self.dipoles:
+                if d.mesh.cells.shape != fm.cells.shape or not np.array_equal(d.mesh.cells,
+                                                                             fm.cells):
+                    raise ValueError('learned warm start: every variant must be on the '
+                                     "model's family topology")
+            for s in self.solvers:
+                model.precompute_constant(s, lanes_per_row=self.lanes_per_row)
+        self.warm_start, self.model = warm_start, model
+
+    def _each(self, fn, idx):
+        """fn(v) for the variants idx, each on its own stream (worker threads);
+        returns once every stream has finished."""
+        import threading
+        import torch
+        idx = list(idx)
+        if not idx:
+            return
+        torch.cuda.synchronize()
+        errors = []
+
+        def work(w):
+            try:
+                with torch.cuda.stream(self.streams[w]):
+                    for v in idx:
+                        if v % self.workers == w:
+                            fn(v)
+                    torch.cuda.current_stream().synchronize()
+            except BaseException as e:   # re-raised in the calling thread
+                errors.append(e)
+        threads = [threading.Thread(target=work, args=(w,)) for w in range(self.workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        torch.cuda.synchronize()
+        if errors:
+            raise errors[0]
+
+    def _driver(self, v):
+        return self.solvers[v].newton_driver('mixed', self.lanes_per_row, self.chunk, 'ams')
+
+    def _setup(self):
+        """The workspace and the shared preconditioner (first solve)."""
+        if self.h is not None:
+            return
+        from .ams import AmsPreconditioner
+        s0 = self.solvers[self.anchor]
+        asm = s0.curl_curl_factory
+        d = asm._device_state()
+        self.h = ctypes.c_void_p()
+        nat.check(nat.load().sn_vcg_create(ctypes.byref(self.h), d['pattern'], nat.ptr(d['gids']),
+                                           nat.ptr(d['orient']), self.nv, self.chunk,
+                                           nat.stream_handle()))
+        drv = self._driver(self.anchor)
+        free = s0.mask_device().cpu().numpy().astype(bool)
+        self.ams = AmsPreconditioner(asm, free, drv.nu_el)
+        nat.check(nat.load().sn_vcg_set_ams(self.h, self.ams.handle, nat.stream_handle()))
+
+    def solve(self, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=1e-3, eta_margin=0.9,
+              close_start=1e-2, max_cg=-1, log=None):
+        """Solutions (device float64) and stats of every variant, in order."""
+        import torch
+        from .red_mvp_solver import next_forcing, planned_forcing
+        V, NV = self.nvar, self.nv
+        lib = nat.load()
+        t0 = time.perf_counter()
+        drvs = [self._driver(v) for v in range(V)]
+        n = self.solvers[0].curl_curl_factory.num_dofs
+        if max_cg < 0:
+            max_cg = min(10 * n, 50000)
+        # starts: the learned one (one batched forward pass of the network)
+        # or zero
+        x0s, infos = [None] * V, [None] * V
+        coeffs = None
+        if self.warm_start == 'learned':
+            from .warm_start import PARAM_NAMES
+            coeffs = self.model.coefficients([{k: d.p[k] for k in PARAM_NAMES}
+                                              for d in self.dipoles])
+        r0, xs, rn = [None] * V, [None] * V, [None] * V
+
+        def prepare(v):
+            # the variant's sources, its learned start, |R(0)| and R at the start
+            drv, s = drvs[v], self.solvers[v]
+            drv.retarget(s.source_fields_device())
+            if coeffs is not None:
+                from .warm_start import PARAM_NAMES
+                p = {k: self.dipoles[v].p[k] for k in PARAM_NAMES}
+                x0s[v], infos[v] = self.model.start(s, p, coeffs=coeffs[v],
+                                                    lanes_per_row=self.lanes_per_row)
+            r0[v] = drv.r0()
+            last = getattr(drv, '_last', None)
+            x0 = x0s[v]
+            if x0 is not None and last is not None and last[0] is x0 and \
+                    x0._version == last[1]:
+                xs[v] = x0.clone()
+                rn[v] = last[2]              # R and the Jacobian at x0 are still current
+            else:
+                x = torch.zeros(n, dtype=torch.float64, device=device()) if x0 is None else \
+                    x0.clone()
+                x[drv.mask == 0] = 0.0
+                xs[v] = x
+                rn[v] = drv.residual(x)
+        self._each(prepare, range(V))
+        self._setup()
+        mask = drvs[0].mask
+        hist = [[rn[v] / r0[v]] for v in range(V)]
+        cg_its, etas = [[] for _ in range(V)], [[] for _ in range(V)]
+        eta = [eta_max] * V
+        for v in range(V):
+            if x0s[v] is not None and 0.0 < rn[v] <= close_start * r0[v]:
+                eta[v] = planned_forcing(rn[v] / r0[v], rtol, eta_min, eta_max)
+        rn_prev = [None] * V
+        conv = [rn[v] <= rtol * r0[v] for v in range(V)]
+        live = [not c for c in conv]          # in the lock step
+        stalls = [0] * V
+        Mr = torch.empty((6 * drvs[0].d['num_el'], NV), dtype=torch.float32, device=device())
+        b = torch.zeros((n, NV), dtype=torch.float64, device=device())
+        dx = torch.empty((n, NV), dtype=torch.float64, device=device())
+        its_h = (ctypes.c_int * NV)()
+        rn_h = (ctypes.c_double * NV)()
+        st_h = (ctypes.c_int * NV)()
+        k = 0
+        while any(live) and k < max_newton:
+            rtol_h = (ctypes.c_double * NV)(*([1.0] * NV))
+            b.zero_()
+            for v in range(NV):
+                src = drvs[min(v, V - 1)]
+                Mr[:, v].copy_(src.Mr)       # padding systems repeat the last (b = 0)
+            cap = 300
+            for v in range(V):
+                if not live[v]:
+                    continue
+                eta[v] = next_forcing(rn[v], rn_prev[v], r0[v], rtol, eta[v], eta_min, eta_max,
+                                      eta_margin)
+                etas[v].append(eta[v])
+                rtol_h[v] = eta[v]
+                b[:, v].copy_(drvs[v].R).neg_()
+                cap = max(cap, 4 * (cg_its[v][-1] if cg_its[v] else 0))
+            cap = min(cap, max_cg)
+            self.ams.update(drvs[self.anchor].nu_el)
+            dx.zero_()
+            rc = lib.sn_vcg_solve(self.h, nat.ptr(Mr), nat.ptr(mask), nat.ptr(b), nat.ptr(dx),
+                                  rtol_h, 0.0, int(cap), its_h, rn_h, st_h, nat.stream_handle())
+            if rc not in (nat.SN_OK, nat.SN_ERR_NOT_CONVERGED, nat.SN_ERR_BREAKDOWN):
+                nat.check(rc)
+            torch.cuda.synchronize()
+
+            def advance(v):
+                # the step of variant v: residual at x + dx, line search, stop test
+                drv = drvs[v]
+                cg_its[v].append(int(its_h[v]))
+                if st_h[v] != nat.SN_OK:
+                    live[v] = False          # finished by its own solver below
+                    return
+                step = dx[:, v]
+                x_try = xs[v] + step
+                rn_try = drv.residual(x_try)
+                lam, tries = 1.0, 0
+                while not (rn_try < rn[v]) and tries < 6:
+                    lam *= 0.5
+                    x_try = xs[v] + lam * step
+                    rn_try = drv.residual(x_try)
+                    tries += 1
+                stalled = rn_try >= rn[v] or (lam == 1.0 and rn_try > 0.9 * rn[v])
+                stalls[v] = stalls[v] + 1 if stalled else 0
+                rn_prev[v], rn[v], xs[v] = rn[v], rn_try, x_try
+                hist[v].append(rn[v] / r0[v])
+                conv[v] = rn[v] <= rtol * r0[v]
+                if conv[v] or stalls[v] >= 3:
+                    live[v] = False
+            self._each(advance, [v for v in range(V) if live[v]])
+            k += 1
+            if log is not None:
+                log('  lock-step newton %d: |R|/|R0| = %s, cg %s'
+                    % (k, ['%.2e' % h[-1] for h in hist], [c[-1] if c else 0 for c in cg_its]))
+        out = []
+        for v in range(V):
+            st = {'residual_history': hist[v], 'cg_iterations': cg_its[v], 'etas': etas[v],
+                  'newton_steps': len(cg_its[v]), 'converged': bool(conv[v]), 'r0': r0[v],
+                  'precision': 'mixed', 'precond': 'ams (shared, variant %d)' % self.anchor,
+                  'fallback': False}
+            if infos[v] is not None:
+                st['warm_start'] = infos[v]
+            x = xs[v]
+            if not conv[v]:
+                # the float32 CG stopped at its cap, or Newton stalled: this
+                # variant's own solver (with its float64 fallback) finishes it
+                s = self.solvers[v]
+                x = s.solve_newton('mixed', rtol=rtol, x0=xs[v], precond='ams',
+                                   lanes_per_row=self.lanes_per_row)
+                st['fallback'] = True
+                st['fallback_stats'] = {k2: s.stats.get(k2) for k2 in
+                                        ('residual_history', 'cg_iterations', 'converged')}
+                st['converged'] = bool(s.stats['converged'])
+                st['cg_iterations'] = cg_its[v] + list(s.stats['cg_iterations'])
+            out.append((x, st))
+        torch.cuda.current_stream().synchronize()
+        self.stats = {'t_solve': time.perf_counter() - t0, 'newton_rounds': k,
+                      'anchor': self.anchor, 'systems': NV}
+        return out
+
+    def __del__(self):
+        try:
+            if self.h and nat._lib is not None:
+                nat._lib.sn_vcg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

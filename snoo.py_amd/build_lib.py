@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, 'libsnoo_hip.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
 SOURCES = ['sn_sources.hip', 'sn_pattern.hip', 'sn_assemble.hip', 'sn_cg.hip', 'sn_ams.hip',
-           'sn_mlp.hip', 'sn_reftable.hip', 'sn_galerkin.hip']
+           'sn_mlp.hip', 'sn_reftable.hip', 'sn_galerkin.hip', 'sn_vcg.hip']
 
 FLAGS = ['-O3', '-std=c++17', '-fPIC', '--offload-arch=gfx950',
          # packed FP32 instructions are kept out of the generated code

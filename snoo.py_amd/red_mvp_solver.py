@@ -213,6 +213,16 @@ def _tuned_element_grids(solver, lanes_per_row=8):
 RedMVPSolver.tuned_element_grids = _tuned_element_grids
 
 
+def next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin):
+    """Forcing term of the next Newton step: Eisenstat-Walker choice 2,
+    0.9 (|R_k| / |R_k-1|)^2 within [eta_min, eta_max] (``eta`` itself on the
+    first step, rn_prev None), and never below ``eta_margin`` times the
+    distance still to go, rtol |R(0)| / |R_k| (capped at eta_max)."""
+    if rn_prev is not None:
+        eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
+    return max(eta, min(eta_max, eta_margin * rtol * r0 / rn))
+
+
 def planned_forcing(rel, rtol, eta_min, eta_max):
     """Forcing term of the first Newton step from a start whose residual is
     ``rel`` |R(0)|: the fewest steps that reach ``rtol`` when each removes at
@@ -516,9 +526,7 @@ class NewtonDriver:
         while not converged and k < max_newton and stalls < stall_steps:
             # forcing term (Eisenstat-Walker choice 2 with safeguards), never
             # asking for more than the remaining distance to the target
-            if rn_prev is not None:
-                eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
-            eta = max(eta, min(eta_max, eta_margin * rtol * r0 / rn))
+            eta = next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin)
             etas.append(eta)
             torch.neg(self.R, out=self.b)
             self._refresh_preconditioner()
