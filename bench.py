@@ -637,12 +637,38 @@ def measure_family8(args, ws, steps=2):
         if mode:
             runs['learned']['start_residual'] = [o[1].get('warm_start', {}).get('r_start_rel')
                                                  for o in res]
+    del cs
+    # the same variants in lock step through the variant-minor CG (one shared
+    # preconditioner: the anchor variant's geometry and reluctivity)
+    from snoopy_amd.batch import VariantBatchSolver
+    vb = VariantBatchSolver(dps, lanes_per_row=args.lanes, workers=args.streams,
+                            warm_start='learned', model=ws)
+    vb.solve(rtol=args.rtol)                  # first solve: the shared preconditioner's plan
+    for mode in ('learned', None):
+        vb.set_warm_start(mode, ws if mode else None)
+        vb.solve(rtol=args.rtol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            for s_ in vb.solvers:
+                s_._src = None
+            res = vb.solve(rtol=args.rtol)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        runs['variant_' + (mode or 'none')] = {
+            's': dt, 'solves_per_s': len(dps) / dt,
+            'cg': [int(sum(o[1]['cg_iterations'])) for o in res],
+            'converged': all(bool(o[1]['converged']) for o in res),
+            'fallback': [bool(o[1]['fallback']) for o in res],
+            'worst_final_residual': max(float(o[1]['residual_history'][-1]) for o in res)}
     out = {'variants': len(dps), 'tets': dps[0].num_tets, 'streams': args.streams,
            'params_seeds': [9100, 9107], 'setup_s': t_setup, 'runs': runs,
            'learned_solves_per_s': runs['learned']['solves_per_s'],
-           'cold_solves_per_s': runs['none']['solves_per_s']}
+           'cold_solves_per_s': runs['none']['solves_per_s'],
+           'variant_learned_solves_per_s': runs['variant_learned']['solves_per_s'],
+           'variant_cold_solves_per_s': runs['variant_none']['solves_per_s']}
     log('family8: %s' % out)
-    del cs
+    del vb
     import gc
     gc.collect()
     torch.cuda.empty_cache()

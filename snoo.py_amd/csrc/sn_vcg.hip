@@ -50,30 +50,10 @@ struct VcgState {
 constexpr int kVElBlock = 256;    // element pass
 constexpr int kVVecBlock = 1024;  // DOF pass, init, prolongation
 constexpr int kVAmsBlock = 256;   // restriction, down, coarse, up
-constexpr int kVMax = 8;
 
 namespace {
 
 __device__ __forceinline__ float4 v4_zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
-__device__ __forceinline__ void v4_fma(float4 &a, float s, const float4 &b) {
-    a.x += s * b.x;
-    a.y += s * b.y;
-    a.z += s * b.z;
-}
-__device__ __forceinline__ float4 v4_add(const float4 &a, const float4 &b) {
-    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, 0.f);
-}
-// sum over the lanes that differ in bits [lo, 64) of the lane id, lo a power
-// of two: the lanes of one variant within a wave
-template <int LO, int HI = kWave>
-__device__ __forceinline__ void v4_reduce(float4 &a) {
-#pragma unroll
-    for (int off = HI / 2; off >= LO; off >>= 1) {
-        a.x += __shfl_xor(a.x, off, kWave);
-        a.y += __shfl_xor(a.y, off, kWave);
-        a.z += __shfl_xor(a.z, off, kWave);
-    }
-}
 __device__ __forceinline__ float4 half_diff4(const float4 &hi, const float4 &lo) {
     return make_float4(0.5f * (hi.x - lo.x), 0.5f * (hi.y - lo.y), 0.5f * (hi.z - lo.z), 0.f);
 }
@@ -526,11 +506,52 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
 }
 
 // ---------------------------------------------------- the preconditioner, V-wide
-// Thread (or lane group) per node, row or edge handling all V variants: each
-// gathered index brings the V variants' float4 values (V*16 contiguous bytes)
-// in one round of loads.  The matrices are shared, so one variant's values
-// never mix with another's; a variant that is done is computed along (its
-// vectors are stale and finite) and its results are not used.
+// Thread (or lane group) per node, row or edge handling all V variants.  The
+// preconditioner's vectors hold three components per node and variant,
+// packed [node][component][variant] (12 V bytes per node, no padding), so
+// each gathered index brings its V variants' values in 3 V contiguous
+// floats.  The matrices are shared, so one variant's values never mix with
+// another's; a variant that is done is computed along (its vectors are
+// stale and finite) and its results are not used.
+
+template <int NV>
+__device__ __forceinline__ void ld3(const float *__restrict__ p, float (&o)[3][NV]) {
+    ldv<NV>(p, o[0]);
+    ldv<NV>(p + NV, o[1]);
+    ldv<NV>(p + 2 * NV, o[2]);
+}
+template <int NV>
+__device__ __forceinline__ void st3(float *__restrict__ p, const float (&o)[3][NV]) {
+    stv<NV>(p, o[0]);
+    stv<NV>(p + NV, o[1]);
+    stv<NV>(p + 2 * NV, o[2]);
+}
+template <int NV>
+__device__ __forceinline__ void zero3(float (&a)[3][NV]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a[c][v] = 0.f;
+}
+// a += s * b
+template <int NV>
+__device__ __forceinline__ void fma3(float (&a)[3][NV], float s, const float (&b)[3][NV]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a[c][v] += s * b[c][v];
+}
+// sum over the lanes that differ in bits [0, log2 G) of the lane id (G lanes
+// of one node, row or aggregate)
+template <int NV, int G>
+__device__ __forceinline__ void reduce3(float (&a)[3][NV]) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) a[c][v] += __shfl_xor(a[c][v], off, kWave);
+}
 
 // all variants done: every kernel of the apply returns at once
 template <int NV>
@@ -546,7 +567,7 @@ template <int NV, int LPR>
 __global__ __launch_bounds__(kVAmsBlock) void vams_restrict_kernel(
     int n_free, const int *__restrict__ inc_ptr, const int2 *__restrict__ inc,
     const float4 *__restrict__ xyz, const float *__restrict__ r, const float *__restrict__ dinv0,
-    float omega, float4 *__restrict__ b0, float4 *__restrict__ x0,
+    float omega, float *__restrict__ b0, float *__restrict__ x0,
     const VcgState *__restrict__ st, int xcd) {
     if (all_done<NV>(st)) return;
     const long long a = (v_block(xcd) * blockDim.x + threadIdx.x) / LPR;
@@ -554,9 +575,8 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_restrict_kernel(
     const bool in = a < n_free;
     const int k0 = in ? inc_ptr[a] : 0, k1 = in ? inc_ptr[a + 1] : 0;
     const float4 xa = in ? xyz[a] : v4_zero();
-    float4 acc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = v4_zero();
+    float acc[3][NV];
+    zero3<NV>(acc);
     for (int k = k0 + lane; k < k1; k += LPR) {
         const int2 ie = inc[k];   // {edge, other end | sign}
         const float4 xb = xyz[ie.y & 0x7fffffff];
@@ -564,18 +584,21 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_restrict_kernel(
         ldv<NV>(r + (long long)ie.x * NV, rv);
         const float4 tg = ie.y < 0 ? half_diff4(xa, xb) : half_diff4(xb, xa);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) v4_fma(acc[v], rv[v], tg);
+        for (int v = 0; v < NV; ++v) {
+            acc[0][v] += rv[v] * tg.x;
+            acc[1][v] += rv[v] * tg.y;
+            acc[2][v] += rv[v] * tg.z;
+        }
     }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) v4_reduce<1, LPR>(acc[v]);
-    if (!in) return;
+    reduce3<NV, LPR>(acc);
+    if (!in || lane != 0) return;
     const float d = omega * dinv0[a];
+    st3<NV>(b0 + a * 3 * NV, acc);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        if (v % LPR != lane) continue;
-        b0[a * NV + v] = acc[v];
-        x0[a * NV + v] = make_float4(d * acc[v].x, d * acc[v].y, d * acc[v].z, 0.f);
-    }
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[c][v] *= d;
+    st3<NV>(x0 + a * 3 * NV, acc);
 }
 
 // SL lanes per aggregate of the next level: b_c = sum_rows b - sum_nnz a x;
@@ -585,50 +608,50 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_restrict_kernel(
 template <int NV, int SL, bool TWO>
 __global__ __launch_bounds__(kVAmsBlock) void vams_down_kernel(
     int n_agg, const int *__restrict__ agg_ptr, const int *__restrict__ agg_kp,
-    const int *__restrict__ indices, const float *__restrict__ vals, const float4 *__restrict__ b,
-    const float4 *__restrict__ x, const float *__restrict__ dinv_c, float omega,
-    float4 *__restrict__ bc, float4 *__restrict__ xc, const VcgState *__restrict__ st, int xcd) {
+    const int *__restrict__ indices, const float *__restrict__ vals, const float *__restrict__ b,
+    const float *__restrict__ x, const float *__restrict__ dinv_c, float omega,
+    float *__restrict__ bc, float *__restrict__ xc, const VcgState *__restrict__ st, int xcd) {
     if (all_done<NV>(st)) return;
     const long long ag = (v_block(xcd) * blockDim.x + threadIdx.x) / SL;
     const int slot = threadIdx.x % SL;
     const bool in = ag < n_agg;
     const int r0 = in ? agg_ptr[ag] : 0, r1 = in ? agg_ptr[ag + 1] : 0;
     const int k0 = in ? agg_kp[ag] : 0, k1 = in ? agg_kp[ag + 1] : 0;
-    float4 s[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) s[v] = v4_zero();
+    float s[3][NV];
+    zero3<NV>(s);
     for (int i = r0 + slot; i < r1; i += SL) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) s[v] = v4_add(s[v], b[(long long)i * NV + v]);
+        float bi[3][NV];
+        ld3<NV>(b + (long long)i * 3 * NV, bi);
+        fma3<NV>(s, 1.f, bi);
     }
     // two nonzeros per trip (k and k + SL, the order of a one-at-a-time
     // loop), both index -> x chains in flight at once
     for (int k = k0 + slot; k < k1; k += (TWO ? 2 : 1) * SL) {
         const bool h2 = TWO && k + SL < k1;
         const int k2 = h2 ? k + SL : k;
-        const float a = vals[k], a2 = h2 ? vals[k2] : 0.f;
-        const long long j = (long long)indices[k] * NV, j2 = (long long)indices[k2] * NV;
-        float4 xa[NV], xb[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            xa[v] = x[j + v];
-            xb[v] = x[j2 + v];
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            v4_fma(s[v], -a, xa[v]);
-            v4_fma(s[v], -a2, xb[v]);
+        const float a = vals[k];
+        float xa[3][NV];
+        ld3<NV>(x + (long long)indices[k] * 3 * NV, xa);
+        if (TWO) {
+            const float a2 = h2 ? vals[k2] : 0.f;
+            float xb[3][NV];
+            ld3<NV>(x + (long long)indices[k2] * 3 * NV, xb);
+            fma3<NV>(s, -a, xa);
+            fma3<NV>(s, -a2, xb);
+        } else {
+            fma3<NV>(s, -a, xa);
         }
     }
+    reduce3<NV, SL>(s);
+    if (!in || slot != 0) return;
+    st3<NV>(bc + ag * 3 * NV, s);
+    if (xc) {
+        const float d = omega * dinv_c[ag];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) v4_reduce<1, SL>(s[v]);
-    if (!in) return;
-    const float d = xc ? omega * dinv_c[ag] : 0.f;
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        if (v % SL != slot) continue;
-        bc[ag * NV + v] = s[v];
-        if (xc) xc[ag * NV + v] = make_float4(d * s[v].x, d * s[v].y, d * s[v].z, 0.f);
+            for (int v = 0; v < NV; ++v) s[c][v] *= d;
+        st3<NV>(xc + ag * 3 * NV, s);
     }
 }
 
@@ -637,35 +660,30 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_down_kernel(
 // level-0 result y when there is one level
 template <int NV>
 __global__ __launch_bounds__(kVAmsBlock) void vams_coarse_kernel(
-    int nc, const float *__restrict__ cinv, const float4 *__restrict__ b,
-    const int *__restrict__ agg_ptr, float4 *__restrict__ x_f, float4 *__restrict__ y,
+    int nc, const float *__restrict__ cinv, const float *__restrict__ b,
+    const int *__restrict__ agg_ptr, float *__restrict__ x_f, float *__restrict__ y,
     const VcgState *__restrict__ st) {
     if (all_done<NV>(st)) return;
     const long long I = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (I >= nc) return;   // uniform over the wave
-    float4 acc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = v4_zero();
+    float acc[3][NV];
+    zero3<NV>(acc);
     for (int j = lane; j < nc; j += kWave) {
-        const float c = cinv[I * nc + j];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) v4_fma(acc[v], c, b[(long long)j * NV + v]);
+        float bj[3][NV];
+        ld3<NV>(b + (long long)j * 3 * NV, bj);
+        fma3<NV>(acc, cinv[I * nc + j], bj);
     }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) v4_reduce<1, kWave>(acc[v]);
+    reduce3<NV, kWave>(acc);
     if (x_f) {
         for (int i = agg_ptr[I] + lane; i < agg_ptr[I + 1]; i += kWave) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const long long iv = (long long)i * NV + v;
-                x_f[iv] = v4_add(x_f[iv], acc[v]);
-            }
+            float xi[3][NV];
+            ld3<NV>(x_f + (long long)i * 3 * NV, xi);
+            fma3<NV>(xi, 1.f, acc);
+            st3<NV>(x_f + (long long)i * 3 * NV, xi);
         }
-    } else {
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if (v == lane) y[I * NV + v] = acc[v];
+    } else if (lane == 0) {
+        st3<NV>(y + I * 3 * NV, acc);
     }
 }
 
@@ -676,58 +694,52 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_coarse_kernel(
 template <int NV, int LPR, bool TWO>
 __global__ __launch_bounds__(kVAmsBlock) void vams_up_kernel(
     int n, const int *__restrict__ indptr, const int *__restrict__ indices,
-    const float *__restrict__ vals, const float *__restrict__ dinv, const float4 *__restrict__ b,
-    const float4 *__restrict__ x, float omega, const int *__restrict__ agg_ptr,
-    float4 *__restrict__ x_f, float4 *__restrict__ y, const VcgState *__restrict__ st, int xcd) {
+    const float *__restrict__ vals, const float *__restrict__ dinv, const float *__restrict__ b,
+    const float *__restrict__ x, float omega, const int *__restrict__ agg_ptr,
+    float *__restrict__ x_f, float *__restrict__ y, const VcgState *__restrict__ st, int xcd) {
     if (all_done<NV>(st)) return;
     const long long row = (v_block(xcd) * blockDim.x + threadIdx.x) / LPR;
     const int lane = threadIdx.x % LPR;
     const bool in = row < n;
     const int k0 = in ? indptr[row] : 0, k1 = in ? indptr[row + 1] : 0;
-    float4 acc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = v4_zero();
+    float acc[3][NV];
+    zero3<NV>(acc);
     // two nonzeros per trip, as in vams_down_kernel
     for (int k = k0 + lane; k < k1; k += (TWO ? 2 : 1) * LPR) {
         const bool h2 = TWO && k + LPR < k1;
         const int k2 = h2 ? k + LPR : k;
-        const float a = vals[k], a2 = h2 ? vals[k2] : 0.f;
-        const long long j = (long long)indices[k] * NV, j2 = (long long)indices[k2] * NV;
-        float4 xa[NV], xb[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            xa[v] = x[j + v];
-            xb[v] = x[j2 + v];
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            v4_fma(acc[v], a, xa[v]);
-            v4_fma(acc[v], a2, xb[v]);
+        const float a = vals[k];
+        float xa[3][NV];
+        ld3<NV>(x + (long long)indices[k] * 3 * NV, xa);
+        if (TWO) {
+            const float a2 = h2 ? vals[k2] : 0.f;
+            float xb[3][NV];
+            ld3<NV>(x + (long long)indices[k2] * 3 * NV, xb);
+            fma3<NV>(acc, a, xa);
+            fma3<NV>(acc, a2, xb);
+        } else {
+            fma3<NV>(acc, a, xa);
         }
     }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) v4_reduce<1, LPR>(acc[v]);
+    reduce3<NV, LPR>(acc);
     if (!in) return;
     const float d = omega * dinv[row];
-    float4 yi[NV];
+    float yi[3][NV], bi[3][NV];
+    ld3<NV>(x + row * 3 * NV, yi);
+    ld3<NV>(b + row * 3 * NV, bi);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const float4 xi = x[row * NV + v], bi = b[row * NV + v];
-        yi[v] = make_float4(xi.x + d * (bi.x - acc[v].x), xi.y + d * (bi.y - acc[v].y),
-                            xi.z + d * (bi.z - acc[v].z), 0.f);
-    }
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) yi[c][v] += d * (bi[c][v] - acc[c][v]);
     if (x_f) {
         for (int i = agg_ptr[row] + lane; i < agg_ptr[row + 1]; i += LPR) {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                const long long iv = (long long)i * NV + v;
-                x_f[iv] = v4_add(x_f[iv], yi[v]);
-            }
+            float xi[3][NV];
+            ld3<NV>(x_f + (long long)i * 3 * NV, xi);
+            fma3<NV>(xi, 1.f, yi);
+            st3<NV>(x_f + (long long)i * 3 * NV, xi);
         }
-    } else {
-#pragma unroll
-        for (int v = 0; v < NV; ++v)
-            if (v % LPR == lane) y[row * NV + v] = yi[v];
+    } else if (lane == 0) {
+        st3<NV>(y + row * 3 * NV, yi);
     }
 }
 
@@ -738,7 +750,7 @@ template <int NV>
 __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
     int n_edges, const unsigned char *__restrict__ mask, const float *__restrict__ r,
     const float *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
-    const float4 *__restrict__ xyz, const float4 *__restrict__ y, float weight,
+    const float4 *__restrict__ xyz, const float *__restrict__ y, float weight,
     float2 *__restrict__ w, const float *__restrict__ pin, double *__restrict__ partB,
     const VcgState *__restrict__ st, int xcd) {
     __shared__ double lds[NV * kVVecBlock / kWave];
@@ -751,16 +763,19 @@ __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
             const bool f = mask ? mask[i] != 0 : true;
             float zi[NV];
             if (f) {
-                float ri[NV], di[NV];
+                float ri[NV], di[NV], ylo[3][NV], yhi[3][NV];
                 ldv<NV>(r + i * NV, ri);
                 ldv<NV>(dinv + i * NV, di);
                 const int2 en = edge_nodes[i];
+                ld3<NV>(y + (long long)en.x * 3 * NV, ylo);
+                ld3<NV>(y + (long long)en.y * 3 * NV, yhi);
                 const float4 t = half_diff4(xyz[en.y], xyz[en.x]);
-                const long long lo = (long long)en.x * NV, hi = (long long)en.y * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
-                    const float4 s = v4_add(y[lo + v], y[hi + v]);
-                    zi[v] = di[v] * ri[v] + weight * (t.x * s.x + t.y * s.y + t.z * s.z);
+                    const float dot = t.x * (ylo[0][v] + yhi[0][v]) +
+                                      t.y * (ylo[1][v] + yhi[1][v]) +
+                                      t.z * (ylo[2][v] + yhi[2][v]);
+                    zi[v] = di[v] * ri[v] + weight * dot;
                     rz[v] += (double)ri[v] * (double)zi[v];
                 }
             } else {
@@ -816,8 +831,8 @@ struct sn_vcg {
     sn::VcgState *st = nullptr, *st_host = nullptr;
     sn_ams *ams = nullptr;
     int ams_lanes = 8;
-    std::vector<float4 *> vb, vx;      // per level [n_l*nv]
-    float4 *vy = nullptr;              // [n_nodes*nv] level-0 result, 0 at fixed nodes
+    std::vector<float *> vb, vx;       // per level [n_l*3*nv], [row][component][variant]
+    float *vy = nullptr;               // [n_nodes*3*nv] level-0 result, 0 at fixed nodes
     hipGraphExec_t graph = nullptr, graph_short = nullptr;
     const float *g_Mr = nullptr;
     const unsigned char *g_mask = nullptr;
@@ -839,9 +854,9 @@ static hipError_t vcg_drop_graphs(sn_vcg *C) {
 }
 
 static void vcg_free_ams_vectors(sn_vcg *C) {
-    for (float4 *p : C->vb)
+    for (float *p : C->vb)
         if (p) (void)hipFree(p);
-    for (float4 *p : C->vx)
+    for (float *p : C->vx)
         if (p) (void)hipFree(p);
     if (C->vy) (void)hipFree(C->vy);
     C->vb.clear();
@@ -872,7 +887,7 @@ static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const f
                        l0.dinv, A->omega, C->vb[0], C->vx[0], C->st, xcd);
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
-        float4 *xc = (l + 2 < L) ? C->vx[l + 1] : (float4 *)nullptr;
+        float *xc = (l + 2 < L) ? C->vx[l + 1] : (float *)nullptr;
         if (c.n <= kVSmallLevel)
             hipLaunchKernelGGL((vams_down_kernel<NV, kWave, true>),
                                dim3(vams_grid((long long)c.n * kWave, kVAmsBlock, xcd)),
@@ -889,10 +904,10 @@ static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const f
     AmsLevel &lc = A->lv[L - 1];
     hipLaunchKernelGGL((vams_coarse_kernel<NV>), dim3(grid_for(lc.n, kVAmsBlock / kWave)),
                        dim3(kVAmsBlock), 0, s, lc.n, A->cinv, C->vb[L - 1], lc.agg_ptr,
-                       L > 1 ? C->vx[L - 2] : (float4 *)nullptr, C->vy, C->st);
+                       L > 1 ? C->vx[L - 2] : (float *)nullptr, C->vy, C->st);
     for (int l = L - 2; l >= 0; --l) {
         AmsLevel &f = A->lv[l];
-        float4 *x_f = l > 0 ? C->vx[l - 1] : (float4 *)nullptr;
+        float *x_f = l > 0 ? C->vx[l - 1] : (float *)nullptr;
         if (f.n <= kVSmallLevel)
             hipLaunchKernelGGL((vams_up_kernel<NV, 16, true>),
                                dim3(vams_grid((long long)f.n * 16, kVAmsBlock, xcd)),
@@ -1049,16 +1064,16 @@ int sn_vcg_set_ams(sn_vcg *C, sn_ams *A, void *stream) {
     C->ams_lanes = A->lanes;
     const size_t nv = (size_t)C->nv;
     for (const AmsLevel &l : A->lv) {
-        float4 *b = nullptr, *x = nullptr;
-        SN_HIP(hipMalloc(&b, (size_t)l.n * nv * sizeof(float4)));
-        SN_HIP(hipMalloc(&x, (size_t)l.n * nv * sizeof(float4)));
+        float *b = nullptr, *x = nullptr;
+        SN_HIP(hipMalloc(&b, (size_t)l.n * 3 * nv * sizeof(float)));
+        SN_HIP(hipMalloc(&x, (size_t)l.n * 3 * nv * sizeof(float)));
         C->vb.push_back(b);
         C->vx.push_back(x);
     }
     // the level-0 result is read at both end points of every edge; fixed
     // nodes (ids >= n_free) keep y = 0 for good
-    SN_HIP(hipMalloc(&C->vy, (size_t)A->n_nodes * nv * sizeof(float4)));
-    SN_HIP(hipMemsetAsync(C->vy, 0, (size_t)A->n_nodes * nv * sizeof(float4), s));
+    SN_HIP(hipMalloc(&C->vy, (size_t)A->n_nodes * 3 * nv * sizeof(float)));
+    SN_HIP(hipMemsetAsync(C->vy, 0, (size_t)A->n_nodes * 3 * nv * sizeof(float), s));
     SN_HIP(hipStreamSynchronize(s));
     return SN_OK;
 }
