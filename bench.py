@@ -364,7 +364,13 @@ def main():
                     s_._src = None           # every solve computes its own sources
                 return bs.solve(args.rtol)
             tb, res = timed_steps(one_batch)
-            record(mode if form == chosen else 'concurrent_' + mode, mode, tb, res, form)
+            key = mode if form == chosen else 'concurrent_' + mode
+            record(key, mode, tb, res, form)
+            if form == 'variant':
+                # the last timed batch: sources, starts and first residuals; the
+                # batched CG; the residuals after each CG
+                batch_runs[key]['phases_s'] = {k: bs.stats[k] for k in
+                                               ('t_prepare', 't_cg', 't_advance')}
             if form == chosen and mode == args.batch_warm:
                 t_batch, bout, bsolver = tb, res, bs
     if args.quick:

@@ -442,9 +442,12 @@ int sn_vcg_create(sn_vcg **out, const sn_pattern *pat, const int *glob_ids_dev,
  * positive definite choice); the workspace keeps per-system vectors of every
  * level.  Call again after the hierarchy changes. */
 int sn_vcg_set_ams(sn_vcg *vcg, sn_ams *ams, void *stream);
-/* Blocks grouped per XCD (1, the default) or dealt round-robin (0); results
- * identical up to the summation order of the reductions. */
-int sn_vcg_set_schedule(sn_vcg *vcg, int xcd);
+/* Schedule flags (tuning knobs): bit 0 = blocks grouped per XCD (else dealt
+ * round-robin; results identical up to the summation order of the
+ * reductions); bit 1 = element products stored by element slot and gathered
+ * through the DOF lists (else stored in DOF-list order and read as one run per
+ * DOF; bit-identical results).  Default 3. */
+int sn_vcg_set_schedule(sn_vcg *vcg, int flags);
 /* Solve all systems: Mr_dev float32 [6*num_el*nvar], b_dev float64 [n*nvar];
  * each CG starts from 0 and its float32 result is ADDED to x_dev (float64
  * [n*nvar], free rows only).  rtol_host [nvar]; stop when ||r_v|| <

@@ -643,6 +643,8 @@ class VariantBatchSolver:
                 xs[v] = x
                 rn[v] = drv.residual(x)
         self._each(prepare, range(V))
+        t_prepare = time.perf_counter() - t0
+        t_cg = t_advance = 0.0
         self._setup()
         mask = drvs[0].mask
         hist = [[rn[v] / r0[v]] for v in range(V)]
@@ -681,11 +683,15 @@ class VariantBatchSolver:
             cap = min(cap, max_cg)
             self.ams.update(drvs[self.anchor].nu_el)
             dx.zero_()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
             rc = lib.sn_vcg_solve(self.h, nat.ptr(Mr), nat.ptr(mask), nat.ptr(b), nat.ptr(dx),
                                   rtol_h, 0.0, int(cap), its_h, rn_h, st_h, nat.stream_handle())
             if rc not in (nat.SN_OK, nat.SN_ERR_NOT_CONVERGED, nat.SN_ERR_BREAKDOWN):
                 nat.check(rc)
             torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            t_cg += t2 - t1
 
             def advance(v):
                 # the step of variant v: residual at x + dx, line search, stop test
@@ -711,6 +717,7 @@ class VariantBatchSolver:
                 if conv[v] or stalls[v] >= 3:
                     live[v] = False
             self._each(advance, [v for v in range(V) if live[v]])
+            t_advance += time.perf_counter() - t2
             k += 1
             if log is not None:
                 log('  lock-step newton %d: |R|/|R0| = %s, cg %s'
@@ -738,7 +745,10 @@ class VariantBatchSolver:
             out.append((x, st))
         torch.cuda.current_stream().synchronize()
         self.stats = {'t_solve': time.perf_counter() - t0, 'newton_rounds': k,
-                      'anchor': self.anchor, 'systems': NV}
+                      'anchor': self.anchor, 'systems': NV,
+                      # sources, starts and the first residuals; the batched CG
+                      # solves; the residuals and line searches after them
+                      't_prepare': t_prepare, 't_cg': t_cg, 't_advance': t_advance}
         return out
 
     def __del__(self):

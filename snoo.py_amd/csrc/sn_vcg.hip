@@ -325,11 +325,15 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_init_state_kernel(
 // from the previous kernel's partials; p = z + beta p at the element's 6 DOFs,
 // face circulations u, m = Mr u, the 6 element products, p.q partials.
 // parity = iteration & 1: reads rho[parity ^ 1], writes rho[parity].
-template <int NV>
+// DM: the products go to their places in the DOF lists (ypos[j*N + e]), so
+// the DOF pass reads each DOF's products as one contiguous run; otherwise to
+// slot j*N + e, which the DOF pass gathers through the lists.
+template <int NV, bool DM>
 __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
-    int n_el, const int *__restrict__ el_dofs, const float *__restrict__ Mr,
-    const float2 *__restrict__ zp, float *__restrict__ yloc, const double *__restrict__ partB,
-    int nbB, double *__restrict__ partE, VcgState *__restrict__ st, int parity, int xcd) {
+    int n_el, const int *__restrict__ el_dofs, const int *__restrict__ ypos,
+    const float *__restrict__ Mr, const float2 *__restrict__ zp, float *__restrict__ yloc,
+    const double *__restrict__ partB, int nbB, double *__restrict__ partE,
+    VcgState *__restrict__ st, int parity, int xcd) {
     __shared__ double lds[2 * NV * kVElBlock / kWave];
     __shared__ double sums[2 * NV];
     const int v = threadIdx.x % NV;
@@ -356,9 +360,12 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
     if (active) {
         const long long step = (long long)gridDim.x * blockDim.x / NV;
         for (long long e = (v_block(xcd) * blockDim.x + threadIdx.x) / NV; e < n_el; e += step) {
-            int d[6];
+            int d[6], pos[6];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) d[j] = el_dofs[j * (long long)n_el + e];
+            for (int j = 0; j < 6; ++j) {
+                d[j] = el_dofs[j * (long long)n_el + e];
+                pos[j] = DM ? ypos[j * (long long)n_el + e] : 0;
+            }
             float a[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) a[k] = Mr[(k * (long long)n_el + e) * NV + v];
@@ -379,7 +386,8 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
             const float yh[6] = {m2 - m1, m0 - m2, m1 - m0, m2, -m1, m0};
 #pragma unroll
             for (int j = 0; j < 6; ++j)
-                yloc[(j * (long long)n_el + e) * NV + v] = d[j] < 0 ? -yh[j] : yh[j];
+                yloc[(DM ? (long long)pos[j] : j * (long long)n_el + e) * NV + v] =
+                    d[j] < 0 ? -yh[j] : yh[j];
         }
     }
     class_sums<NV>(pq, lds, sums);
@@ -390,8 +398,11 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
 // element pass's partials; q_i = fixed-order sum of the element products;
 // x += alpha p, r -= alpha q, r.r partials; JAC: z = dinv r and the {z, p}
 // pair, r.z partials; otherwise p to a plain array (the preconditioner writes
-// the pair).  A variant that is done keeps x, r as they are.
-template <int NV, bool JAC>
+// the pair).  A variant that is done keeps x, r as they are.  DM: the DOF's
+// products are the run [dof_ptr[i], dof_ptr[i+1]) of yloc (vcg_elem_kernel
+// with DM), else gathered through the DOF lists; the same values are summed
+// in the same order either way.
+template <int NV, bool JAC, bool DM>
 __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
     int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const float *__restrict__ yloc, const unsigned char *__restrict__ mask,
@@ -432,7 +443,8 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             const int k0 = dof_ptr[i], k1 = dof_ptr[i + 1];
             int c[kHead];
 #pragma unroll
-            for (int m = 0; m < kHead; ++m) c[m] = (fr && k0 + m < k1) ? dof_contrib[k0 + m] : -1;
+            for (int m = 0; m < kHead; ++m)
+                c[m] = (fr && k0 + m < k1) ? (DM ? k0 + m : dof_contrib[k0 + m]) : -1;
             float zz[NV], pp[NV];
             ld_pairs<NV>(zp + i * NV, zz, pp);
             float q[NV];
@@ -460,7 +472,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             }
             for (int k = k0 + kHead; k < k1; ++k) {
                 float y[NV];
-                ldv<NV>(yloc + (long long)dof_contrib[k] * NV, y);
+                ldv<NV>(yloc + (long long)(DM ? k : dof_contrib[k]) * NV, y);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) q[v] += y[v];
             }
@@ -796,6 +808,15 @@ __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
     if (threadIdx.x < NV) partB[((long long)blockIdx.x * NV + threadIdx.x) * 2] = lds[threadIdx.x];
 }
 
+// place of every element product in the DOF lists: ypos[dof_contrib[k]] = k
+// (every slot j*N + e appears in exactly one list)
+__global__ void vcg_slot_pos_kernel(int n_slots, const int *__restrict__ dof_contrib,
+                                    int *__restrict__ ypos) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_slots) return;
+    ypos[dof_contrib[k]] = k;
+}
+
 // timing state: partials that reduce to rho = ||r||^2 = 1 for every variant,
 // z = 1 on the free rows and p = 0 in the pairs read by parity 1
 __global__ void vcg_fill_timing_kernel(long long n_nv, int nv, int nb,
@@ -824,6 +845,11 @@ struct sn_vcg {
     int grid_el = 0, grid_vec = 0;
     int xcd = 1;
     int *el_dofs = nullptr;            // [6*n_el] signed DOF ids (shared)
+    int *ypos = nullptr;               // [6*n_el] place of slot j*n_el + e in the DOF lists
+    // element products stored in DOF-list order (read as one run per DOF)
+    // instead of by element slot: the DOF pass took 128 -> 119 us at 1M tets x
+    // 8 variants, the element pass's scattered stores 103 -> 161 us, so off
+    bool dm = false;
     float *yloc = nullptr;             // [6*n_el*nv]
     float *r = nullptr, *dinv = nullptr, *pout = nullptr, *xs = nullptr;   // [n*nv]
     float2 *w0 = nullptr, *w1 = nullptr;                                  // [n*nv] {z, p}
@@ -932,27 +958,34 @@ static void vcg_apply(sn_vcg *C, const unsigned char *mask, float2 *w, const flo
     else vcg_apply_t<NV, 4>(C, mask, w, pin, s);
 }
 
-template <int NV>
-static void vcg_iteration(sn_vcg *C, const float *Mr, const unsigned char *mask, int parity,
-                          hipStream_t s) {
+template <int NV, bool DM>
+static void vcg_iteration_t(sn_vcg *C, const float *Mr, const unsigned char *mask, int parity,
+                            hipStream_t s) {
     const sn_pattern *P = C->pat;
     const float2 *wold = parity ? C->w1 : C->w0;
     float2 *wnew = parity ? C->w0 : C->w1;
-    hipLaunchKernelGGL((vcg_elem_kernel<NV>), dim3(C->grid_el), dim3(kVElBlock), 0, s, C->n_el,
-                       C->el_dofs, Mr, wold, C->yloc, C->partB, C->grid_vec, C->partE, C->st,
-                       parity, C->xcd);
+    hipLaunchKernelGGL((vcg_elem_kernel<NV, DM>), dim3(C->grid_el), dim3(kVElBlock), 0, s,
+                       C->n_el, C->el_dofs, C->ypos, Mr, wold, C->yloc, C->partB, C->grid_vec,
+                       C->partE, C->st, parity, C->xcd);
     if (C->ams) {
-        hipLaunchKernelGGL((vcg_edge_kernel<NV, false>), dim3(C->grid_vec), dim3(kVVecBlock), 0, s,
-                           P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold, wnew,
-                           C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st, parity,
-                           C->xcd);
+        hipLaunchKernelGGL((vcg_edge_kernel<NV, false, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
+                           0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
+                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
+                           parity, C->xcd);
         vcg_apply<NV>(C, mask, wnew, C->pout, s);
     } else {
-        hipLaunchKernelGGL((vcg_edge_kernel<NV, true>), dim3(C->grid_vec), dim3(kVVecBlock), 0, s,
-                           P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold, wnew,
-                           C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st, parity,
-                           C->xcd);
+        hipLaunchKernelGGL((vcg_edge_kernel<NV, true, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
+                           0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
+                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
+                           parity, C->xcd);
     }
+}
+
+template <int NV>
+static void vcg_iteration(sn_vcg *C, const float *Mr, const unsigned char *mask, int parity,
+                          hipStream_t s) {
+    if (C->dm) vcg_iteration_t<NV, true>(C, Mr, mask, parity, s);
+    else vcg_iteration_t<NV, false>(C, Mr, mask, parity, s);
 }
 
 template <int NV>
@@ -1044,6 +1077,9 @@ int sn_vcg_create(sn_vcg **out, const sn_pattern *P, const int *gids, const int 
                           hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(vcg_el_dofs_kernel, dim3(grid_for(C->n_el, 256)), dim3(256), 0, s, C->n_el,
                        gids, orient, C->el_dofs);
+    SN_HIP(hipMalloc(&C->ypos, (size_t)6 * C->n_el * sizeof(int)));
+    hipLaunchKernelGGL(vcg_slot_pos_kernel, dim3(grid_for(6LL * C->n_el, 256)), dim3(256), 0, s,
+                       6 * C->n_el, P->dof_contrib, C->ypos);
     SN_CHECK_LAUNCH();
     SN_HIP(hipStreamSynchronize(s));
     *out = C;
@@ -1078,10 +1114,11 @@ int sn_vcg_set_ams(sn_vcg *C, sn_ams *A, void *stream) {
     return SN_OK;
 }
 
-int sn_vcg_set_schedule(sn_vcg *C, int xcd) {
-    SN_REQUIRE(C, "sn_vcg_set_schedule: null workspace");
+int sn_vcg_set_schedule(sn_vcg *C, int flags) {
+    SN_REQUIRE(C && flags >= 0 && flags <= 3, "sn_vcg_set_schedule: flags in [0, 3]");
     SN_HIP(vcg_drop_graphs(C));
-    C->xcd = xcd ? 1 : 0;
+    C->xcd = (flags & 1) ? 1 : 0;
+    C->dm = (flags & 2) == 0;
     return SN_OK;
 }
 
@@ -1195,13 +1232,23 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
     for (int r = 0; r < reps; ++r) {
 #define SN_VCG_PAIR(NV)                                                                        \
     do {                                                                                       \
-        hipLaunchKernelGGL((vcg_elem_kernel<NV>), dim3(C->grid_el), dim3(kVElBlock), 0, s,     \
-                           C->n_el, C->el_dofs, Mr, C->w1, C->yloc, C->partB, C->grid_vec,    \
-                           C->partE, C->st, 1, C->xcd);                                        \
-        hipLaunchKernelGGL((vcg_edge_kernel<NV, false>), dim3(C->grid_vec), dim3(kVVecBlock),  \
-                           0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv,     \
-                           C->w1, C->w0, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, \
-                           C->st, 1, C->xcd);                                                  \
+        if (C->dm) {                                                                            \
+            hipLaunchKernelGGL((vcg_elem_kernel<NV, true>), dim3(C->grid_el), dim3(kVElBlock), 0, \
+                               s, C->n_el, C->el_dofs, C->ypos, Mr, C->w1, C->yloc, C->partB,  \
+                               C->grid_vec, C->partE, C->st, 1, C->xcd);                       \
+            hipLaunchKernelGGL((vcg_edge_kernel<NV, false, true>), dim3(C->grid_vec),          \
+                               dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
+                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
+                               C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
+        } else {                                                                               \
+            hipLaunchKernelGGL((vcg_elem_kernel<NV, false>), dim3(C->grid_el), dim3(kVElBlock),  \
+                               0, s, C->n_el, C->el_dofs, C->ypos, Mr, C->w1, C->yloc,         \
+                               C->partB, C->grid_vec, C->partE, C->st, 1, C->xcd);             \
+            hipLaunchKernelGGL((vcg_edge_kernel<NV, false, false>), dim3(C->grid_vec),         \
+                               dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
+                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
+                               C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
+        }                                                                                      \
     } while (0)
         switch (nv) {
             case 1: SN_VCG_PAIR(1); break;
@@ -1238,7 +1285,7 @@ void sn_vcg_destroy(sn_vcg *C) {
     if (!C) return;
     (void)vcg_drop_graphs(C);
     vcg_free_ams_vectors(C);
-    void *bufs[] = {C->el_dofs, C->yloc, C->r, C->dinv, C->pout, C->xs, C->w0, C->w1,
+    void *bufs[] = {C->el_dofs, C->ypos, C->yloc, C->r, C->dinv, C->pout, C->xs, C->w0, C->w1,
                     C->partE, C->partB, C->partbb, C->rtol_dev, C->st};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
