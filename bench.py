@@ -370,7 +370,8 @@ def main():
                 # the last timed batch: sources, starts and first residuals; the
                 # batched CG; the residuals after each CG
                 batch_runs[key]['phases_s'] = {k: bs.stats[k] for k in
-                                               ('t_prepare', 't_cg', 't_advance')}
+                                               ('t_sources', 't_prepare', 't_cg',
+                                                't_advance')}
             if form == chosen and mode == args.batch_warm:
                 t_batch, bout, bsolver = tb, res, bs
     if args.quick:

@@ -526,6 +526,16 @@ int sn_pod_project(int n, int r, const float *U_dev, const double *v_dev, double
 /* out[i] = mean[i] + sum_j U[j*n + i] c[j]  (mean may be NULL) */
 int sn_pod_expand(int n, int r, const float *U_dev, const float *c_dev, const double *mean_dev,
                   double *out_dev, void *stream);
+/* The same for nvec (1, 2, 4 or 8) vectors in one pass over the modes (the
+ * starts of a batch of variants): c_dev [nvec][r] (r <= 256), out_dev
+ * [nvec][n]; every output bit-identical to sn_pod_expand of its vector. */
+int sn_pod_expand_many(int n, int r, int nvec, const float *U_dev, const float *c_dev,
+                       const double *mean_dev, double *out_dev, void *stream);
+/* sn_pod_project of nvec (1, 2, 4 or 8) vectors v_dev [nvec][n] in one pass
+ * over the modes: g_host [nvec][r], each bit-identical to sn_pod_project of
+ * its vector; synchronises the stream. */
+int sn_pod_project_many(int n, int r, int nvec, const float *U_dev, const double *v_dev,
+                        double *g_host, void *stream);
 /* x_f[i] = sum_k w[6i+k] x_c[idx[6i+k]] (idx < 0 skipped): edge DOFs of one
  * mesh interpolated from another (Whitney interpolation weights). */
 int sn_edge_transfer(int nf, const int *idx_dev, const double *w_dev, const double *xc_dev,

@@ -63,6 +63,8 @@ SIGNATURES = {
                              ctypes.c_float, ctypes.c_float, _c_int, ctypes.c_float, _vp]),
     'sn_f32_to_bf16': (_c_int, [_c_int, _vp, _vp, _vp]),
     'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_pod_expand_many': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
+    'sn_pod_project_many': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_pod_element_modes': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_reduced_jacobian': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_pod_reduced_jacobian_range': (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,

@@ -620,16 +620,27 @@ class VariantBatchSolver:
                                               for d in self.dipoles])
         r0, xs, rn = [None] * V, [None] * V, [None] * V
 
-        def prepare(v):
-            # the variant's sources, its learned start, |R(0)| and R at the start
+        def sources(v):
+            # the variant's sources (Biot-Savart) and |R(0)|
             drv, s = drvs[v], self.solvers[v]
             drv.retarget(s.source_fields_device())
-            if coeffs is not None:
-                from .warm_start import PARAM_NAMES
-                p = {k: self.dipoles[v].p[k] for k in PARAM_NAMES}
-                x0s[v], infos[v] = self.model.start(s, p, coeffs=coeffs[v],
-                                                    lanes_per_row=self.lanes_per_row)
             r0[v] = drv.r0()
+        self._each(sources, range(V))
+        t_sources = time.perf_counter() - t0
+        if coeffs is not None:
+            # the learned starts: expansions and projections of all variants in
+            # one pass over the modes each, residuals and reduced Jacobians per
+            # variant on its stream
+            from .warm_start import PARAM_NAMES
+            starts = self.model.start_many(
+                self.solvers, [{k: d.p[k] for k in PARAM_NAMES} for d in self.dipoles],
+                coeffs=coeffs, each=self._each, lanes_per_row=self.lanes_per_row)
+            for v in range(V):
+                x0s[v], infos[v] = starts[v]
+
+        def prepare(v):
+            # R at the start (kept from the judge of the learned start)
+            drv = drvs[v]
             last = getattr(drv, '_last', None)
             x0 = x0s[v]
             if x0 is not None and last is not None and last[0] is x0 and \
@@ -748,7 +759,8 @@ class VariantBatchSolver:
                       'anchor': self.anchor, 'systems': NV,
                       # sources, starts and the first residuals; the batched CG
                       # solves; the residuals and line searches after them
-                      't_prepare': t_prepare, 't_cg': t_cg, 't_advance': t_advance}
+                      't_prepare': t_prepare, 't_sources': t_sources, 't_cg': t_cg,
+                      't_advance': t_advance}
         return out
 
     def __del__(self):

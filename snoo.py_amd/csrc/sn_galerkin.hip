@@ -191,11 +191,106 @@ __global__ __launch_bounds__(256) void pod_project_kernel(int n, int r,
             ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) + ws[3][threadIdx.x];
 }
 
+// pod_project_kernel for nv vectors v[w*n + i] at once: one pass over the
+// modes serves every vector; part[(b*nv + w)*r + k], each vector's partial
+// formed in the order of pod_project_kernel (bit-identical per vector)
+constexpr int kProjMaxVec = 8;
+template <int NVEC>
+__global__ __launch_bounds__(256) void pod_project_many_kernel(int n, int r,
+                                                               const float *__restrict__ U,
+                                                               const double *__restrict__ v,
+                                                               double *__restrict__ part) {
+    __shared__ double ws[4][kProjModes][NVEC];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const long long base = (long long)blockIdx.x * 256 * kProjPer + threadIdx.x;
+    const int k0 = blockIdx.y * kProjModes, kc = min(kProjModes, r - k0);
+    double vv[NVEC][kProjPer];
+#pragma unroll
+    for (int w = 0; w < NVEC; ++w)
+#pragma unroll
+        for (int j = 0; j < kProjPer; ++j) {
+            const long long i = base + 256LL * j;
+            vv[w][j] = i < n ? v[(long long)w * n + i] : 0.0;
+        }
+    for (int kk = 0; kk < kc; kk += 2) {
+        const bool two = kk + 1 < kc;
+        const float *u0 = U + (size_t)(k0 + kk) * n;
+        const float *u1 = U + (size_t)(k0 + kk + (two ? 1 : 0)) * n;
+        float a0[kProjPer], a1[kProjPer];
+#pragma unroll
+        for (int j = 0; j < kProjPer; ++j) {
+            const long long i = base + 256LL * j;
+            a0[j] = i < n ? u0[i] : 0.0f;
+            a1[j] = i < n ? u1[i] : 0.0f;
+        }
+#pragma unroll
+        for (int w = 0; w < NVEC; ++w) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < kProjPer; ++j) {
+                const long long i = base + 256LL * j;
+                if (i < n) {
+                    s0 += (double)a0[j] * vv[w][j];
+                    s1 += (double)a1[j] * vv[w][j];
+                }
+            }
+            s0 = wave_sum(s0);
+            s1 = wave_sum(s1);
+            if (lane == 0) {
+                ws[wid][kk][w] = s0;
+                if (two) ws[wid][kk + 1][w] = s1;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kc * NVEC; t += blockDim.x) {
+        const int k = t / NVEC, w = t % NVEC;
+        part[((size_t)blockIdx.x * NVEC + w) * r + k0 + k] =
+            ((ws[0][k][w] + ws[1][k][w]) + ws[2][k][w]) + ws[3][k][w];
+    }
+}
+
 }  // namespace sn
 
 using namespace sn;
 
 extern "C" {
+
+int sn_pod_project_many(int n, int r, int nvec, const float *U, const double *v,
+                        double *g_host, void *stream) {
+    SN_REQUIRE(n > 0 && r > 0 && U && v && g_host, "sn_pod_project_many: bad argument");
+    SN_REQUIRE(nvec == 1 || nvec == 2 || nvec == 4 || nvec == 8,
+               "sn_pod_project_many: nvec must be 1, 2, 4 or 8 (%d)", nvec);
+    hipStream_t s = as_stream(stream);
+    const int G = (int)((n + 256LL * kProjPer - 1) / (256LL * kProjPer));
+    const int m = nvec * r;
+    double *part = nullptr, *g = nullptr;
+    SN_HIP(hipMallocAsync((void **)&part, (size_t)G * m * sizeof(double), s));
+    SN_HIP(hipMallocAsync((void **)&g, (size_t)m * sizeof(double), s));
+    const dim3 grid(G, (r + kProjModes - 1) / kProjModes);
+    switch (nvec) {
+        case 1: hipLaunchKernelGGL(pod_project_many_kernel<1>, grid, dim3(256), 0, s, n, r, U, v, part); break;
+        case 2: hipLaunchKernelGGL(pod_project_many_kernel<2>, grid, dim3(256), 0, s, n, r, U, v, part); break;
+        case 4: hipLaunchKernelGGL(pod_project_many_kernel<4>, grid, dim3(256), 0, s, n, r, U, v, part); break;
+        default: hipLaunchKernelGGL(pod_project_many_kernel<8>, grid, dim3(256), 0, s, n, r, U, v, part); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sum_partials_kernel<double>, dim3((m + 63) / 64), dim3(256), 0, s, G,
+                           m, part, g);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(g_host, g, (size_t)m * sizeof(double),
+                                            hipMemcpyDeviceToHost, s);
+    hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(g, s);
+    if (e == hipSuccess) e = f1 != hipSuccess ? f1 : f2;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        set_error("sn_pod_project_many: %s", hipGetErrorString(e));
+        return SN_ERR_HIP;
+    }
+    return SN_OK;
+}
 
 int sn_pod_element_modes(int num_el, int n, int r, const float *U, const int *gids,
                          const int *orient, float *W, void *stream) {

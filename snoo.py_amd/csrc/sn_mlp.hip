@@ -384,6 +384,44 @@ __global__ void pod_expand_kernel(int n, int r, const float *__restrict__ U,
     out[i] = s;
 }
 
+// pod_expand_kernel for nvec coefficient vectors c[w*r + j] at once, out[w*n
+// + i]: one pass over the modes serves every vector (the coefficients staged
+// in LDS); each output summed in the order of pod_expand_kernel
+constexpr int kExpandMaxVec = 8, kExpandMaxModes = 256;
+template <int NVEC>
+__global__ __launch_bounds__(256) void pod_expand_many_kernel(int n, int r,
+                                                              const float *__restrict__ U,
+                                                              const float *__restrict__ c,
+                                                              const double *__restrict__ mean,
+                                                              double *__restrict__ out) {
+    __shared__ float cs[NVEC][kExpandMaxModes];
+    for (int t = threadIdx.x; t < NVEC * r; t += blockDim.x) cs[t / r][t % r] = c[t];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s[NVEC];
+    const double m0 = mean ? mean[i] : 0.0;
+#pragma unroll
+    for (int w = 0; w < NVEC; ++w) s[w] = m0;
+    int j = 0;
+    for (; j + 8 <= r; j += 8) {
+        float u[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) u[m] = U[(size_t)(j + m) * n + i];
+#pragma unroll
+        for (int w = 0; w < NVEC; ++w)
+#pragma unroll
+            for (int m = 0; m < 8; ++m) s[w] += (double)u[m] * (double)cs[w][j + m];
+    }
+    for (; j < r; ++j) {
+        const float u = U[(size_t)j * n + i];
+#pragma unroll
+        for (int w = 0; w < NVEC; ++w) s[w] += (double)u * (double)cs[w][j];
+    }
+#pragma unroll
+    for (int w = 0; w < NVEC; ++w) out[(size_t)w * n + i] = s[w];
+}
+
 // x_f[i] = sum_k w[6i+k] x_c[idx[6i+k]]  (idx < 0: no contribution)
 __global__ void edge_transfer_kernel(int nf, const int *__restrict__ idx,
                                      const double *__restrict__ w, const double *__restrict__ xc,
@@ -538,6 +576,25 @@ int sn_pod_expand(int n, int r, const float *U, const float *c, const double *me
     SN_REQUIRE(n > 0 && r >= 0 && out && (r == 0 || (U && c)), "sn_pod_expand: bad argument");
     hipLaunchKernelGGL(pod_expand_kernel, dim3((n + 255) / 256), dim3(256), 0, as_stream(stream),
                        n, r, U, c, mean, out);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
+}
+
+int sn_pod_expand_many(int n, int r, int nvec, const float *U, const float *c, const double *mean,
+                       double *out, void *stream) {
+    SN_REQUIRE(n > 0 && r >= 0 && out && (r == 0 || (U && c)), "sn_pod_expand_many: bad argument");
+    SN_REQUIRE(nvec == 1 || nvec == 2 || nvec == 4 || nvec == 8,
+               "sn_pod_expand_many: nvec must be 1, 2, 4 or 8 (%d)", nvec);
+    SN_REQUIRE(r <= kExpandMaxModes, "sn_pod_expand_many: at most %d modes (%d)", kExpandMaxModes,
+               r);
+    const dim3 grid((n + 255) / 256);
+    hipStream_t s = as_stream(stream);
+    switch (nvec) {
+        case 1: hipLaunchKernelGGL(pod_expand_many_kernel<1>, grid, dim3(256), 0, s, n, r, U, c, mean, out); break;
+        case 2: hipLaunchKernelGGL(pod_expand_many_kernel<2>, grid, dim3(256), 0, s, n, r, U, c, mean, out); break;
+        case 4: hipLaunchKernelGGL(pod_expand_many_kernel<4>, grid, dim3(256), 0, s, n, r, U, c, mean, out); break;
+        default: hipLaunchKernelGGL(pod_expand_many_kernel<8>, grid, dim3(256), 0, s, n, r, U, c, mean, out); break;
+    }
     SN_CHECK_LAUNCH();
     return SN_OK;
 }

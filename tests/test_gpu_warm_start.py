@@ -193,6 +193,64 @@ def test_concurrent_learned_batch_matches_the_batch_from_zero(trained):
         assert err < 1e-6
 
 
+def test_batched_start_matches_the_single_starts(trained):
+    """WarmStart.start_many (expansions and projections of all variants in
+    one pass over the modes) against WarmStart.start per variant: the same
+    residual judge, starts equal up to the order of the correction's sum."""
+    import torch
+    from snoopy_amd.warm_start import PARAM_NAMES, sample_params
+    fam, ws = trained
+    dps = [_family_variant(fam, sample_params(s)) for s in (6100, 6101, 6102)]
+    print('case: batched learned start of 3 (padded to 4)', flush=True)
+    sols = [d.solver(quad_order=2, verbose=False) for d in dps]
+    for s in sols:
+        ws.precompute_constant(s)
+        drv = s.newton_driver('mixed', 8, 32, 'ams')
+        drv.retarget(s.source_fields_device())
+        drv.r0()
+    params = [{k: d.p[k] for k in PARAM_NAMES} for d in dps]
+    many = ws.start_many(sols, params)
+    for s, p, (xm, im) in zip(sols, params, many):
+        xs, info = ws.start(s, p)
+        assert im['used'] == info['used']
+        assert abs(im['r_start_rel'] - info['r_start_rel']) <= 1e-6 * info['r_start_rel']
+        if xs is not None:
+            err = float((xm - xs).abs().max() / xs.abs().max())
+            assert err < 1e-9, err
+    # the pass over the modes: bit-identical to the single expansion
+    a = ws.coefficients(params)
+    X = ws.expand_many(np.vstack([a, a[:1]]))
+    for v in range(3):
+        assert torch.equal(X[v], ws.expand(a[v]))
+
+
+def test_variant_batch_with_learned_start_matches_the_batch_from_zero(trained):
+    """VariantBatchSolver (lock-step Newton, one variant-minor CG) from the
+    learned start: every variant converges to 1e-8 and to the field of the
+    same batch from A = 0."""
+    from snoopy_amd.batch import VariantBatchSolver
+    from snoopy_amd.warm_start import sample_params
+    fam, ws = trained
+    dps = [_family_variant(fam, sample_params(s)) for s in (6200, 6201, 6202, 6203)]
+    print('case: learned variant-minor batch of 4', flush=True)
+    vb = VariantBatchSolver(dps, warm_start='learned', model=ws)
+    warm = vb.solve(rtol=1e-8)
+    xs_warm = [x.cpu().numpy() for x, _ in warm]
+    st_warm = [st for _, st in warm]
+    vb.set_warm_start(None)
+    cold = vb.solve(rtol=1e-8)
+    for k, ((xc, sc), xw, sw, s) in enumerate(zip(cold, xs_warm, st_warm, vb.solvers)):
+        assert sc['converged'] and sw['converged'], k
+        assert sw['residual_history'][-1] <= 1e-8
+        assert sw['warm_start']['r_start_rel'] > 0
+        asm = s.curl_curl_factory
+        Bc, Bw = asm.element_B(xc.cpu().numpy()), asm.element_B(xw)
+        err = np.abs(Bc - Bw).max() / np.abs(Bc).max()
+        print('variant', k, 'cg', sum(sc['cg_iterations']), '->', sum(sw['cg_iterations']),
+              'start %.2e' % sw['warm_start']['r_start_rel'], 'B diff %.1e' % err, flush=True)
+        assert err < 1e-6
+
+
 def test_saved_model_predicts_the_same_coefficients(trained, tmp_path):
     from snoopy_amd.warm_start import MorphFamily, WarmStart, sample_params
     fam, ws = trained
