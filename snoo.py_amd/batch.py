@@ -503,7 +503,7 @@ class VariantBatchSolver:
     with empty right-hand sides."""
 
     def __init__(self, dipoles, quad_order=2, lanes_per_row=8, chunk=32, warm_start=None,
-                 model=None, anchor=None, workers=8):
+                 model=None, anchor=None, workers=8, batched_start=False):
         import torch
         self.dipoles = list(dipoles)
         V = len(self.dipoles)
@@ -527,6 +527,10 @@ class VariantBatchSolver:
         # variant v always on stream v % workers, so its tensors are
         # allocated, used and freed on one stream
         self.workers = max(1, min(int(workers), V))
+        # learned starts: each variant's own pipeline on its stream (sources,
+        # start, residual), or every variant's sources first, then the starts
+        # with one pass over the modes for all (WarmStart.start_many)
+        self.batched_start = bool(batched_start)
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self.h = None
         self.ams = None
@@ -620,23 +624,8 @@ class VariantBatchSolver:
                                               for d in self.dipoles])
         r0, xs, rn = [None] * V, [None] * V, [None] * V
 
-        def sources(v):
-            # the variant's sources (Biot-Savart) and |R(0)|
-            drv, s = drvs[v], self.solvers[v]
-            drv.retarget(s.source_fields_device())
-            r0[v] = drv.r0()
-        self._each(sources, range(V))
-        t_sources = time.perf_counter() - t0
-        if coeffs is not None:
-            # the learned starts: expansions and projections of all variants in
-            # one pass over the modes each, residuals and reduced Jacobians per
-            # variant on its stream
-            from .warm_start import PARAM_NAMES
-            starts = self.model.start_many(
-                self.solvers, [{k: d.p[k] for k in PARAM_NAMES} for d in self.dipoles],
-                coeffs=coeffs, each=self._each, lanes_per_row=self.lanes_per_row)
-            for v in range(V):
-                x0s[v], infos[v] = starts[v]
+        from .warm_start import PARAM_NAMES
+        params = [{k: d.p[k] for k in PARAM_NAMES} for d in self.dipoles]
 
         def prepare(v):
             # R at the start (kept from the judge of the learned start)
@@ -653,7 +642,30 @@ class VariantBatchSolver:
                 x[drv.mask == 0] = 0.0
                 xs[v] = x
                 rn[v] = drv.residual(x)
-        self._each(prepare, range(V))
+
+        def sources(v):
+            # the variant's sources (Biot-Savart) and |R(0)|; unless the starts
+            # are batched, its learned start and R there right after, on the
+            # same stream
+            drv, s = drvs[v], self.solvers[v]
+            drv.retarget(s.source_fields_device())
+            r0[v] = drv.r0()
+            if self.batched_start and coeffs is not None:
+                return
+            if coeffs is not None:
+                x0s[v], infos[v] = self.model.start(s, params[v], coeffs=coeffs[v],
+                                                    lanes_per_row=self.lanes_per_row)
+            prepare(v)
+        self._each(sources, range(V))
+        t_sources = time.perf_counter() - t0
+        if coeffs is not None and self.batched_start:
+            # expansions and projections of all variants in one pass over the
+            # modes each, residuals and reduced Jacobians per variant
+            starts = self.model.start_many(self.solvers, params, coeffs=coeffs, each=self._each,
+                                           lanes_per_row=self.lanes_per_row)
+            for v in range(V):
+                x0s[v], infos[v] = starts[v]
+            self._each(prepare, range(V))
         t_prepare = time.perf_counter() - t0
         t_cg = t_advance = 0.0
         self._setup()
