@@ -144,31 +144,36 @@ __device__ __forceinline__ void ldv_d(const double *__restrict__ p, double (&o)[
         o[0] = p[0];
     }
 }
-// The CG's z and search direction p of one DOF, V variants: 2V contiguous
-// floats, the V z values then the V p values (w + i*V in float2 units), so
-// the preconditioner writes the z half and the DOF pass the p half, each one
-// whole 4V-byte piece, and a gather reads both halves at once.
+// the V pairs {z_v, p_v} of one index (8V contiguous bytes)
 template <int NV>
 __device__ __forceinline__ void ld_pairs(const float2 *__restrict__ w, float (&z)[NV],
                                          float (&p)[NV]) {
-    const float *f = reinterpret_cast<const float *>(w);
-    ldv<NV>(f, z);
-    ldv<NV>(f + NV, p);
+    if constexpr (NV % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < NV / 2; ++q) {
+            const float4 t = reinterpret_cast<const float4 *>(w)[q];
+            z[2 * q] = t.x;
+            p[2 * q] = t.y;
+            z[2 * q + 1] = t.z;
+            p[2 * q + 1] = t.w;
+        }
+    } else {
+        const float2 t = w[0];
+        z[0] = t.x;
+        p[0] = t.y;
+    }
 }
 template <int NV>
 __device__ __forceinline__ void st_pairs(float2 *__restrict__ w, const float (&z)[NV],
                                          const float (&p)[NV]) {
-    float *f = reinterpret_cast<float *>(w);
-    stv<NV>(f, z);
-    stv<NV>(f + NV, p);
-}
-template <int NV>
-__device__ __forceinline__ void st_z(float2 *__restrict__ w, const float (&z)[NV]) {
-    stv<NV>(reinterpret_cast<float *>(w), z);
-}
-template <int NV>
-__device__ __forceinline__ void st_p(float2 *__restrict__ w, const float (&p)[NV]) {
-    stv<NV>(reinterpret_cast<float *>(w) + NV, p);
+    if constexpr (NV % 2 == 0) {
+#pragma unroll
+        for (int q = 0; q < NV / 2; ++q)
+            reinterpret_cast<float4 *>(w)[q] =
+                make_float4(z[2 * q], p[2 * q], z[2 * q + 1], p[2 * q + 1]);
+    } else {
+        w[0] = make_float2(z[0], p[0]);
+    }
 }
 
 // Per-variant block sums of the threads' values vals[v]: the V sums in
@@ -367,9 +372,8 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
             float xh[6];
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                const float *w = reinterpret_cast<const float *>(zp) +
-                                 (long long)(d[j] & 0x7fffffff) * 2 * NV;
-                const float pj = w[v] + beta * w[NV + v];
+                const float2 w = zp[(long long)(d[j] & 0x7fffffff) * NV + v];
+                const float pj = w.x + beta * w.y;
                 xh[j] = d[j] < 0 ? -pj : pj;
             }
             const float u0 = xh[1] - xh[2] + xh[5];
@@ -393,8 +397,8 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
 // DOF pass (thread per DOF, all variants): every variant's alpha from the
 // element pass's partials; q_i = fixed-order sum of the element products;
 // x += alpha p, r -= alpha q, r.r partials; JAC: z = dinv r and the {z, p}
-// pair, r.z partials; otherwise the p half of the pair (the preconditioner
-// writes the z half).  A variant that is done keeps x, r as they are.  DM: the DOF's
+// pair, r.z partials; otherwise p to a plain array (the preconditioner writes
+// the pair).  A variant that is done keeps x, r as they are.  DM: the DOF's
 // products are the run [dof_ptr[i], dof_ptr[i+1]) of yloc (vcg_elem_kernel
 // with DM), else gathered through the DOF lists; the same values are summed
 // in the same order either way.
@@ -403,7 +407,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
     int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const float *__restrict__ yloc, const unsigned char *__restrict__ mask,
     const float *__restrict__ dinv, const float2 *__restrict__ zp, float2 *__restrict__ wnew,
-    float *__restrict__ r, float *__restrict__ x,
+    float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
     const double *__restrict__ partE, int nbE, double *__restrict__ partB,
     VcgState *__restrict__ st, int parity, int xcd) {
     __shared__ double lds[NV * kVVecBlock / kWave];
@@ -463,7 +467,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             for (int v = 0; v < NV; ++v) zero[v] = 0.f;
             if (!fr) {
                 if (JAC) st_pairs<NV>(wnew + i * NV, zero, zero);
-                else st_p<NV>(wnew + i * NV, zero);
+                else stv<NV>(pout + i * NV, zero);
                 continue;
             }
             for (int k = k0 + kHead; k < k1; ++k) {
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             stv<NV>(x + i * NV, xi);
             stv<NV>(r + i * NV, ri);
             if (JAC) st_pairs<NV>(wnew + i * NV, zi, pi);
-            else st_p<NV>(wnew + i * NV, pi);
+            else stv<NV>(pout + i * NV, pi);
         }
     }
     if (JAC) {
@@ -752,15 +756,15 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_up_kernel(
 }
 
 // thread per edge, all variants: z = D_J^-1 r + w Pi y0 on the free edges (0
-// elsewhere), stored as the z half of the pairs (the DOF pass wrote the p
-// half; at the start p is 0); per-block partials of r.z
+// elsewhere), stored with the CG's p as the pairs {z, p} (pin) or as z alone
+// (the start: p stays 0); per-block partials of r.z
 template <int NV>
 __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
     int n_edges, const unsigned char *__restrict__ mask, const float *__restrict__ r,
     const float *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
     const float4 *__restrict__ xyz, const float *__restrict__ y, float weight,
-    float2 *__restrict__ w, double *__restrict__ partB, const VcgState *__restrict__ st,
-    int xcd) {
+    float2 *__restrict__ w, const float *__restrict__ pin, double *__restrict__ partB,
+    const VcgState *__restrict__ st, int xcd) {
     __shared__ double lds[NV * kVVecBlock / kWave];
     double rz[NV];
 #pragma unroll
@@ -790,7 +794,14 @@ __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
 #pragma unroll
                 for (int v = 0; v < NV; ++v) zi[v] = 0.f;
             }
-            st_z<NV>(w + i * NV, zi);
+            if (pin) {
+                float pi[NV];
+                ldv<NV>(pin + i * NV, pi);
+                st_pairs<NV>(w + i * NV, zi, pi);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[i * NV + v].x = zi[v];
+            }
         }
     }
     block_sums<NV>(rz, lds);
@@ -813,12 +824,7 @@ __global__ void vcg_fill_timing_kernel(long long n_nv, int nv, int nb,
                                        double *__restrict__ partB, float2 *__restrict__ w1) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < 2LL * nb * nv) partB[k] = (k < 2LL * nv) ? 1.0 : 0.0;
-    if (k < 2 * n_nv) {
-        // DOF i holds z[0..nv) then p[0..nv)
-        const long long i = k / (2 * nv);
-        const bool zhalf = (k % (2 * nv)) < nv;
-        reinterpret_cast<float *>(w1)[k] = (zhalf && !(mask && !mask[i])) ? 1.f : 0.f;
-    }
+    if (k < n_nv) w1[k] = make_float2((mask && !mask[k / nv]) ? 0.f : 1.f, 0.f);
 }
 
 // x64[i*V + v] += xs (the float32 solution) on free rows
@@ -845,7 +851,7 @@ struct sn_vcg {
     // 8 variants, the element pass's scattered stores 103 -> 161 us, so off
     bool dm = false;
     float *yloc = nullptr;             // [6*n_el*nv]
-    float *r = nullptr, *dinv = nullptr, *xs = nullptr;   // [n*nv]
+    float *r = nullptr, *dinv = nullptr, *pout = nullptr, *xs = nullptr;   // [n*nv]
     float2 *w0 = nullptr, *w1 = nullptr;                                  // [n*nv] {z, p}
     double *partE = nullptr, *partB = nullptr, *partbb = nullptr, *rtol_dev = nullptr;
     sn::VcgState *st = nullptr, *st_host = nullptr;
@@ -895,7 +901,8 @@ inline int vams_grid(long long threads, int block, int xcd) {
 constexpr int kVSmallLevel = 8192;
 
 template <int NV, int LPR>
-static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, hipStream_t s) {
+static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const float *pin,
+                        hipStream_t s) {
     sn_ams *A = C->ams;
     const int L = (int)A->lv.size();
     const int xcd = C->xcd;
@@ -940,14 +947,15 @@ static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, hipStre
     }
     hipLaunchKernelGGL((vams_prolong_kernel<NV>), dim3(C->grid_vec), dim3(kVVecBlock), 0, s,
                        A->n_edges, mask, C->r, C->dinv, A->edge_nodes, A->xyz, C->vy, A->weight,
-                       w, C->partB, C->st, xcd);
+                       w, pin, C->partB, C->st, xcd);
 }
 
 template <int NV>
-static void vcg_apply(sn_vcg *C, const unsigned char *mask, float2 *w, hipStream_t s) {
-    if (C->ams_lanes >= 16) vcg_apply_t<NV, 16>(C, mask, w, s);
-    else if (C->ams_lanes >= 8) vcg_apply_t<NV, 8>(C, mask, w, s);
-    else vcg_apply_t<NV, 4>(C, mask, w, s);
+static void vcg_apply(sn_vcg *C, const unsigned char *mask, float2 *w, const float *pin,
+                      hipStream_t s) {
+    if (C->ams_lanes >= 16) vcg_apply_t<NV, 16>(C, mask, w, pin, s);
+    else if (C->ams_lanes >= 8) vcg_apply_t<NV, 8>(C, mask, w, pin, s);
+    else vcg_apply_t<NV, 4>(C, mask, w, pin, s);
 }
 
 template <int NV, bool DM>
@@ -962,13 +970,13 @@ static void vcg_iteration_t(sn_vcg *C, const float *Mr, const unsigned char *mas
     if (C->ams) {
         hipLaunchKernelGGL((vcg_edge_kernel<NV, false, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
                            0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
-                           wnew, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
+                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
                            parity, C->xcd);
-        vcg_apply<NV>(C, mask, wnew, s);
+        vcg_apply<NV>(C, mask, wnew, C->pout, s);
     } else {
         hipLaunchKernelGGL((vcg_edge_kernel<NV, true, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
                            0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
-                           wnew, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
+                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
                            parity, C->xcd);
     }
 }
@@ -989,7 +997,7 @@ static void vcg_init(sn_vcg *C, const float *Mr, const unsigned char *mask, cons
                        C->dinv, C->xs, C->partB, C->partbb);
     hipLaunchKernelGGL((vcg_init_state_kernel<NV>), dim3(1), dim3(kVVecBlock), 0, s, C->partbb,
                        C->grid_vec, C->st, C->rtol_dev, atol, maxiter);
-    if (C->ams) vcg_apply<NV>(C, mask, C->w0, s);
+    if (C->ams) vcg_apply<NV>(C, mask, C->w0, nullptr, s);
 }
 
 template <int NV>
@@ -1053,6 +1061,7 @@ int sn_vcg_create(sn_vcg **out, const sn_pattern *P, const int *gids, const int 
     SN_HIP(hipMalloc(&C->yloc, (size_t)6 * C->n_el * nvar * sizeof(float)));
     SN_HIP(hipMalloc(&C->r, nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->dinv, nn * sizeof(float)));
+    SN_HIP(hipMalloc(&C->pout, nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->xs, nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->w0, nn * sizeof(float2)));
     SN_HIP(hipMalloc(&C->w1, nn * sizeof(float2)));
@@ -1209,7 +1218,7 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
     }
     SN_HIP(hipMemcpyAsync(C->st, h.data(), nv * sizeof(VcgState), hipMemcpyHostToDevice, s));
     const long long n_nv = (long long)C->n * nv;
-    const long long cover = std::max(2 * n_nv, 2LL * C->grid_vec * nv);
+    const long long cover = std::max(n_nv, 2LL * C->grid_vec * nv);
     hipLaunchKernelGGL(vcg_fill_timing_kernel, dim3(grid_for(cover, 256)), dim3(256), 0, s, n_nv,
                        nv, C->grid_vec, mask, C->partB, C->w1);
     SN_CHECK_LAUNCH();
@@ -1229,7 +1238,7 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
                                C->grid_vec, C->partE, C->st, 1, C->xcd);                       \
             hipLaunchKernelGGL((vcg_edge_kernel<NV, false, true>), dim3(C->grid_vec),          \
                                dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
-                               C->yloc, mask, C->dinv, C->w1, C->w0, C->r, C->xs,              \
+                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
                                C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
         } else {                                                                               \
             hipLaunchKernelGGL((vcg_elem_kernel<NV, false>), dim3(C->grid_el), dim3(kVElBlock),  \
@@ -1237,7 +1246,7 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
                                C->partB, C->grid_vec, C->partE, C->st, 1, C->xcd);             \
             hipLaunchKernelGGL((vcg_edge_kernel<NV, false, false>), dim3(C->grid_vec),         \
                                dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
-                               C->yloc, mask, C->dinv, C->w1, C->w0, C->r, C->xs,              \
+                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
                                C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
         }                                                                                      \
     } while (0)
@@ -1276,7 +1285,7 @@ void sn_vcg_destroy(sn_vcg *C) {
     if (!C) return;
     (void)vcg_drop_graphs(C);
     vcg_free_ams_vectors(C);
-    void *bufs[] = {C->el_dofs, C->ypos, C->yloc, C->r, C->dinv, C->xs, C->w0, C->w1,
+    void *bufs[] = {C->el_dofs, C->ypos, C->yloc, C->r, C->dinv, C->pout, C->xs, C->w0, C->w1,
                     C->partE, C->partB, C->partbb, C->rtol_dev, C->st};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
