@@ -224,16 +224,18 @@ def test_batched_start_matches_the_single_starts(trained):
         assert torch.equal(X[v], ws.expand(a[v]))
 
 
-def test_variant_batch_with_learned_start_matches_the_batch_from_zero(trained):
+@pytest.mark.parametrize('batched_start', [False, True])
+def test_variant_batch_with_learned_start_matches_the_batch_from_zero(trained, batched_start):
     """VariantBatchSolver (lock-step Newton, one variant-minor CG) from the
-    learned start: every variant converges to 1e-8 and to the field of the
-    same batch from A = 0."""
+    learned start, each variant's start on its own stream or all starts in
+    one pass over the modes: every variant converges to 1e-8 and to the field
+    of the same batch from A = 0."""
     from snoopy_amd.batch import VariantBatchSolver
     from snoopy_amd.warm_start import sample_params
     fam, ws = trained
     dps = [_family_variant(fam, sample_params(s)) for s in (6200, 6201, 6202, 6203)]
-    print('case: learned variant-minor batch of 4', flush=True)
-    vb = VariantBatchSolver(dps, warm_start='learned', model=ws)
+    print('case: learned variant-minor batch of 4, batched start %s' % batched_start, flush=True)
+    vb = VariantBatchSolver(dps, warm_start='learned', model=ws, batched_start=batched_start)
     warm = vb.solve(rtol=1e-8)
     xs_warm = [x.cpu().numpy() for x, _ in warm]
     st_warm = [st for _, st in warm]
