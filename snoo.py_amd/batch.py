@@ -603,7 +603,7 @@ class VariantBatchSolver:
         nat.check(nat.load().sn_vcg_set_ams(self.h, self.ams.handle, nat.stream_handle()))
 
     def solve(self, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=1e-3, eta_margin=0.9,
-              close_start=1e-2, max_cg=-1, log=None):
+              close_start=1e-2, max_cg=-1, log=None, plan_close=True):
         """Solutions (device float64) and stats of every variant, in order."""
         import torch
         from .red_mvp_solver import next_forcing, planned_forcing
@@ -698,7 +698,7 @@ class VariantBatchSolver:
                 if not live[v]:
                     continue
                 eta[v] = next_forcing(rn[v], rn_prev[v], r0[v], rtol, eta[v], eta_min, eta_max,
-                                      eta_margin)
+                                      eta_margin, close_start if plan_close else None)
                 etas[v].append(eta[v])
                 rtol_h[v] = eta[v]
                 b[:, v].copy_(drvs[v].R).neg_()

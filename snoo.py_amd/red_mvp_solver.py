@@ -145,7 +145,7 @@ class RedMVPSolver:
     def solve_newton(self, precision='fp64', rtol=1e-8, x0=None, max_newton=40, eta_max=0.1,
                      eta_min=None, lanes_per_row=8, chunk=32, max_cg=-1, line_search=True,
                      log=None, precond='jacobi', tight='element', jacobian=None, eta_first=None,
-                     eta_margin=0.9):
+                     eta_margin=0.9, plan_close=True):
         """Inexact Newton on R(x) = K(x)x - rhs(x) to ||R|| <= rtol ||R(0)||.
 
         Returns the device solution (float64) and fills ``self.stats`` with the
@@ -157,7 +157,8 @@ class RedMVPSolver:
         drv.retarget(self.source_fields_device())
         return drv.solve(x0=x0, rtol=rtol, max_newton=max_newton, eta_max=eta_max,
                          eta_min=eta_min, max_cg=max_cg, line_search=line_search,
-                         stats=self.stats, log=log, eta_first=eta_first, eta_margin=eta_margin)
+                         stats=self.stats, log=log, eta_first=eta_first, eta_margin=eta_margin,
+                         plan_close=plan_close)
 
     def newton_driver(self, precision='fp64', lanes_per_row=8, chunk=32, precond='jacobi',
                       tight='element', jacobian=None):
@@ -213,25 +214,33 @@ def _tuned_element_grids(solver, lanes_per_row=8):
 RedMVPSolver.tuned_element_grids = _tuned_element_grids
 
 
-def next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin):
+def next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin, plan=None):
     """Forcing term of the next Newton step: Eisenstat-Walker choice 2,
     0.9 (|R_k| / |R_k-1|)^2 within [eta_min, eta_max] (``eta`` itself on the
     first step, rn_prev None), and never below ``eta_margin`` times the
-    distance still to go, rtol |R(0)| / |R_k| (capped at eta_max)."""
+    distance still to go, rtol |R(0)| / |R_k| (capped at eta_max).  With
+    ``plan``: once |R_k| <= plan |R(0)| (Newton in its linear regime) the
+    step takes the planned forcing term instead of Eisenstat-Walker's."""
     if rn_prev is not None:
-        eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
+        if plan is not None and rn <= plan * r0:
+            eta = planned_forcing(rn / r0, rtol, eta_min, eta_max, eta_margin)
+        else:
+            eta = min(eta_max, max(0.9 * (rn / rn_prev) ** 2, eta_min))
     return max(eta, min(eta_max, eta_margin * rtol * r0 / rn))
 
 
-def planned_forcing(rel, rtol, eta_min, eta_max):
-    """Forcing term of the first Newton step from a start whose residual is
-    ``rel`` |R(0)|: the fewest steps that reach ``rtol`` when each removes at
-    most a factor 1/eta_min, and the equal factor per step that does it
-    (a start at 1.6e-3 |R(0)|, rtol 1e-8, eta_min 1e-3: 2 steps of 2.5e-3)."""
+def planned_forcing(rel, rtol, eta_min, eta_max, margin=1.0):
+    """Forcing term of a Newton step from a residual of ``rel`` |R(0)|: the
+    fewest steps that reach ``rtol`` when each removes at most a factor
+    1/eta_min, and the equal factor per step that does it (a start at
+    1.6e-3 |R(0)|, rtol 1e-8, eta_min 1e-3: 2 steps of 2.5e-3); a single
+    step asks for ``margin`` times the distance."""
     digits = math.log(rel / rtol)
     if digits <= 0.0:
         return eta_max
     steps = max(1, math.ceil(digits / math.log(1.0 / eta_min) - 1e-9))
+    if steps == 1:
+        return min(eta_max, margin * math.exp(-digits))
     return min(eta_max, max(eta_min, math.exp(-digits / steps)))
 
 
@@ -476,7 +485,7 @@ class NewtonDriver:
 
     def solve(self, x0=None, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=None, max_cg=-1,
               line_search=True, stats=None, log=None, stall_steps=3, eta_first=None,
-              eta_margin=0.9, close_start=1e-2):
+              eta_margin=0.9, close_start=1e-2, plan_close=True):
         """Newton steps until ||R|| <= rtol ||R(0)||, ``max_newton`` steps, or
         ``stall_steps`` consecutive stalled steps: full steps that reduce ||R||
         by less than 10%, or steps that no length of the line search makes
@@ -495,7 +504,11 @@ class NewtonDriver:
         from a start x0 whose residual is already at most ``close_start``
         |R(0)|, where Newton is in its linear regime; that step takes the
         forcing term of the fewest equal steps to the target
-        (``planned_forcing``)."""
+        (``planned_forcing``).  With ``plan_close`` every later step whose
+        residual is at most ``close_start`` |R(0)| is planned the same way
+        (measured on one MI355X: 1M-tet dipole from zero 31.6 -> 30.7 ms, the
+        6M-tet chain 559 -> 484 ms, the 64 variants of 20k-335k tets 9 % more
+        CG iterations in the same time)."""
         import torch
         stats = stats if stats is not None else {}
         if eta_min is None:
@@ -526,7 +539,8 @@ class NewtonDriver:
         while not converged and k < max_newton and stalls < stall_steps:
             # forcing term (Eisenstat-Walker choice 2 with safeguards), never
             # asking for more than the remaining distance to the target
-            eta = next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin)
+            eta = next_forcing(rn, rn_prev, r0, rtol, eta, eta_min, eta_max, eta_margin,
+                               close_start if plan_close else None)
             etas.append(eta)
             torch.neg(self.R, out=self.b)
             self._refresh_preconditioner()
