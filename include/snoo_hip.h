@@ -466,6 +466,12 @@ int sn_vcg_solve(sn_vcg *vcg, const float *Mr_dev, const unsigned char *mask_dev
  * running; overwrites the workspace's vectors (run after a solve). */
 int sn_vcg_time_pair(sn_vcg *vcg, const float *Mr_dev, const unsigned char *mask_dev, int reps,
                      double *avg_ms_host, void *stream);
+/* Launch shapes (tuning knobs; results identical up to the summation order of
+ * the reductions): grid_el workgroups of the element pass (0: one per 256
+ * element-variant lanes, at most 8 per CU), vec_per_cu workgroups of 1024
+ * threads per CU for the DOF pass, the start and the prolongation (default
+ * 1).  Synchronises the stream; the next solve recaptures its graphs. */
+int sn_vcg_set_grids(sn_vcg *vcg, int grid_el, int vec_per_cu, void *stream);
 int sn_vcg_info(const sn_vcg *vcg, int *nvar, int *grid_el, int *grid_vec);
 void sn_vcg_destroy(sn_vcg *vcg);
 /* y = J x with the element operator in float64 (Mr_dev float64, layout of

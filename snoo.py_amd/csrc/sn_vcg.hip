@@ -1273,6 +1273,30 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
     return SN_OK;
 }
 
+int sn_vcg_set_grids(sn_vcg *C, int grid_el, int vec_per_cu, void *stream) {
+    SN_REQUIRE(C, "sn_vcg_set_grids: null workspace");
+    SN_REQUIRE(grid_el >= 0 && vec_per_cu >= 1 && vec_per_cu <= 8,
+               "sn_vcg_set_grids: grid_el >= 0 (0: default), vec_per_cu in [1, 8]");
+    hipStream_t s = as_stream(stream);
+    SN_HIP(vcg_drop_graphs(C));
+    // no kernel of this workspace may still read the partials replaced here
+    SN_HIP(hipStreamSynchronize(s));
+    const int cus = num_cus();
+    int ge = grid_el > 0 ? grid_el : grid_for((long long)C->n_el * C->nv, kVElBlock, 8 * cus);
+    if (ge >= 8) ge = ge / 8 * 8;
+    int gv = grid_for(C->n, kVVecBlock, vec_per_cu * cus);
+    if (gv >= 8) gv = gv / 8 * 8;
+    for (void *p : {(void *)C->partE, (void *)C->partB, (void *)C->partbb})
+        if (p) SN_HIP(hipFree(p));
+    C->partE = C->partB = C->partbb = nullptr;
+    C->grid_el = ge;
+    C->grid_vec = gv;
+    SN_HIP(hipMalloc(&C->partE, (size_t)C->grid_el * C->nv * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * C->nv * sizeof(double)));
+    SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * C->nv * sizeof(double)));
+    return SN_OK;
+}
+
 int sn_vcg_info(const sn_vcg *C, int *nvar, int *grid_el, int *grid_vec) {
     SN_REQUIRE(C, "sn_vcg_info: null workspace");
     if (nvar) *nvar = C->nv;

@@ -110,8 +110,8 @@ def test_variants_match_their_own_single_cg():
 
 def test_product_layouts_and_schedules_agree():
     """Element products in DOF-list order or by element slot: the same values
-    summed in the same order, so bit-identical iterates; round-robin blocks
-    change only the reduction order."""
+    summed in the same order, so bit-identical iterates; round-robin blocks and
+    other grid sizes change only the reduction order."""
     import torch
     from snoopy_amd import _native as nat
     from snoopy_amd.magnets import Dipole
@@ -127,12 +127,17 @@ def test_product_layouts_and_schedules_agree():
         for flags in (1, 3, 2):
             nat.check(nat.load().sn_vcg_set_schedule(h, flags))
             out[flags] = _solve(h, Mr, d0.mask, b, [1e-3] * 4, 4)
+        # other launch shapes: only the reductions' partial sums change
+        nat.check(nat.load().sn_vcg_set_schedule(h, 3))
+        nat.check(nat.load().sn_vcg_set_grids(h, 24, 2, nat.stream_handle()))
+        out['grids'] = _solve(h, Mr, d0.mask, b, [1e-3] * 4, 4)
     finally:
         nat.load().sn_vcg_destroy(h)
     assert torch.equal(out[1][0], out[3][0]) and out[1][1] == out[3][1]
-    assert all(abs(a - c) <= 2 for a, c in zip(out[1][1], out[2][1])), (out[1][1], out[2][1])
-    err = float((out[1][0] - out[2][0]).norm() / out[1][0].norm())
-    assert err < 2e-3, err
+    for k in (2, 'grids'):
+        assert all(abs(a - c) <= 2 for a, c in zip(out[1][1], out[k][1])), (k, out[1][1], out[k][1])
+        err = float((out[1][0] - out[k][0]).norm() / out[1][0].norm())
+        assert err < 2e-3, (k, err)
 
 
 @pytest.mark.parametrize('nvar', [4, 3])
