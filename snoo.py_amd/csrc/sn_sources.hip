@@ -42,12 +42,37 @@ constexpr int kBsTile = 256;
 // outside that sphere is farther than the core radius from the segment and
 // from both end points, so the core test of the reference is false there and
 // is skipped; inside it the test runs as the reference writes it.
+//
+// Coils are polylines: a segment whose start point is bit-for-bit the end
+// point of the segment before it takes that segment's end-point difference
+// and distance instead of forming them again (the same values: same inputs,
+// same operations), which saves one square root per pair.
+//
+// A workgroup's targets are neighbours (consecutive quadrature points): when
+// the box around them lies outside a segment's sphere, with a relative margin
+// of 1e-12 that covers the rounding of either distance, every target's own
+// sphere test is false and is skipped for the whole workgroup.
 constexpr int kBsSegVals = 14;
+
+struct BsVertex {
+    double dx, dy, dz, l;
+};
+
+__device__ __forceinline__ BsVertex bs_vertex(const double *p, double tx, double ty, double tz) {
+    BsVertex v;
+    v.dx = p[0] - tx;
+    v.dy = p[1] - ty;
+    v.dz = p[2] - tz;
+    v.l = sqrt(v.dx * v.dx + v.dy * v.dy + v.dz * v.dz);
+    return v;
+}
 
 __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
     const double *__restrict__ src, int num_src, const double *__restrict__ tar,
     int num_tar, double scale, double rad, double *__restrict__ B, int accumulate) {
     __shared__ double seg[kBsTile * kBsSegVals];
+    __shared__ int flags[kBsTile];              // bit 0: linked, bit 1: box near the sphere
+    __shared__ double box[6][kBsBlock / kWave];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double tx = 0.0, ty = 0.0, tz = 0.0;
     if (i < num_tar) {
@@ -55,13 +80,44 @@ __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
         ty = tar[3 * i + 1];
         tz = tar[3 * i + 2];
     }
+    {
+        // the workgroup's bounding box: -max(-x) and max(x) per axis; a NaN
+        // coordinate makes the box NaN and every segment "near"
+        const bool has = i < num_tar;
+        double b[6] = {has ? -tx : -INFINITY, has ? -ty : -INFINITY, has ? -tz : -INFINITY,
+                       has ? tx : -INFINITY, has ? ty : -INFINITY, has ? tz : -INFINITY};
+        const int w = threadIdx.x / kWave;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double v = b[k];
+            for (int off = 32; off > 0; off >>= 1) {
+                const double o = __shfl_xor(v, off, kWave);
+                v = (o > v || o != o) ? o : v;
+            }
+            if (threadIdx.x % kWave == 0) box[k][w] = v;
+        }
+    }
+    __syncthreads();
+    double lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double a = box[k][0], c = box[k + 3][0];
+        for (int w = 1; w < kBsBlock / kWave; ++w) {
+            a = (box[k][w] > a || box[k][w] != box[k][w]) ? box[k][w] : a;
+            c = (box[k + 3][w] > c || box[k + 3][w] != box[k + 3][w]) ? box[k + 3][w] : c;
+        }
+        lo[k] = -a;
+        hi[k] = c;
+    }
     double ax = 0.0, ay = 0.0, az = 0.0;
     const double r2 = rad * rad;
+    BsVertex prev = {0.0, 0.0, 0.0, 0.0};       // end point of the previous segment
     for (int base = 0; base < num_src; base += kBsTile) {
         const int cnt = min(kBsTile, num_src - base);
         __syncthreads();
         for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-            const double *g = src + 6 * (long long)(base + j);
+            const long long g_ix = base + j;
+            const double *g = src + 6 * g_ix;
             double *s = &seg[kBsSegVals * j];
             const double x0 = g[0], y0 = g[1], z0 = g[2], x1 = g[3], y1 = g[4], z1 = g[5];
             const double nx = x1 - x0, ny = y1 - y0, nz = z1 - z0;
@@ -74,26 +130,46 @@ __global__ __launch_bounds__(kBsBlock) void biot_savart_kernel(
             s[9] = nn;
             s[10] = 0.5 * (x0 + x1); s[11] = 0.5 * (y0 + y1); s[12] = 0.5 * (z0 + z1);
             s[13] = R * R;
+            // bitwise equality of the coordinates (not ==, which equates 0 and -0)
+            const bool linked = g_ix > 0 &&
+                                __double_as_longlong(g[-3]) == __double_as_longlong(x0) &&
+                                __double_as_longlong(g[-2]) == __double_as_longlong(y0) &&
+                                __double_as_longlong(g[-1]) == __double_as_longlong(z0);
+            // distance from the midpoint to the box; far only when it is a number
+            // above the sphere's radius with the margin
+            const double gx = fmax(fmax(lo[0] - s[10], s[10] - hi[0]), 0.0);
+            const double gy = fmax(fmax(lo[1] - s[11], s[11] - hi[1]), 0.0);
+            const double gz = fmax(fmax(lo[2] - s[12], s[12] - hi[2]), 0.0);
+            const double g2 = gx * gx + gy * gy + gz * gz;
+            const bool far = g2 > s[13] * (1.0 + 1e-12) && lo[0] == lo[0] && lo[1] == lo[1] &&
+                             lo[2] == lo[2] && hi[0] == hi[0] && hi[1] == hi[1] &&
+                             hi[2] == hi[2];
+            flags[j] = (linked ? 1 : 0) | (far ? 0 : 2);
         }
         __syncthreads();
         if (i < num_tar) {
             for (int j = 0; j < cnt; ++j) {
                 const double *s = &seg[kBsSegVals * j];
                 const double nx = s[6], ny = s[7], nz = s[8];
-                const double d1x = s[0] - tx, d1y = s[1] - ty, d1z = s[2] - tz;
-                const double d2x = s[3] - tx, d2y = s[4] - ty, d2z = s[5] - tz;
-                const double l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
-                const double l2 = sqrt(d2x * d2x + d2y * d2y + d2z * d2z);
-                const double mx = tx - s[10], my = ty - s[11], mz = tz - s[12];
+                // the same for every lane: a scalar branch
+                const int fl = __builtin_amdgcn_readfirstlane(flags[j]);
+                const BsVertex v1 = (fl & 1) ? prev : bs_vertex(s, tx, ty, tz);
+                const BsVertex v2 = bs_vertex(s + 3, tx, ty, tz);
+                prev = v2;
+                const double d1x = v1.dx, d1y = v1.dy, d1z = v1.dz, l1 = v1.l;
+                const double d2x = v2.dx, d2y = v2.dy, d2z = v2.dz, l2 = v2.l;
                 bool core = false;
                 double px = 0.0, py = 0.0, pz = 0.0, dd = 0.0;
-                if (mx * mx + my * my + mz * mz <= s[13]) {
-                    const double tp = -1.0 * (d1x * nx + d1y * ny + d1z * nz) / s[9];
-                    px = tx - (s[0] + tp * nx);
-                    py = ty - (s[1] + tp * ny);
-                    pz = tz - (s[2] + tp * nz);
-                    dd = px * px + py * py + pz * pz;
-                    core = (dd < r2 && tp <= 1.0 && tp >= 0.0) || l1 < rad || l2 < rad;
+                if (fl & 2) {
+                    const double mx = tx - s[10], my = ty - s[11], mz = tz - s[12];
+                    if (mx * mx + my * my + mz * mz <= s[13]) {
+                        const double tp = -1.0 * (d1x * nx + d1y * ny + d1z * nz) / s[9];
+                        px = tx - (s[0] + tp * nx);
+                        py = ty - (s[1] + tp * ny);
+                        pz = tz - (s[2] + tp * nz);
+                        dd = px * px + py * py + pz * pz;
+                        core = (dd < r2 && tp <= 1.0 && tp >= 0.0) || l1 < rad || l2 < rad;
+                    }
                 }
                 if (core) {
                     // inside the filament core: mu0 I r/(2 pi R^2) along n x p
