@@ -236,6 +236,12 @@ int sn_dot_f64(int n, const double *a_dev, const double *b_dev,
                const unsigned char *mask_dev, double *out_host, void *stream);
 int sn_absmax_f64(int n, const double *a_dev, const unsigned char *mask_dev,
                   double *out_host, void *stream);
+/* sn_dot_f64 with the result left on the device (*out_dev, the same bits):
+ * nothing waits, so a caller can queue several and read them with one copy.
+ * Calls of one host thread on one stream share a partials buffer in stream
+ * order. */
+int sn_dot_f64_dev(int n, const double *a_dev, const double *b_dev,
+                   const unsigned char *mask_dev, double *out_dev, void *stream);
 /* out_host[v] = sum over rows [row0[v], row0[v+1]) with mask != 0 of a*b. */
 int sn_seg_dot_f64(int nseg, const int *row0_dev, const double *a_dev, const double *b_dev,
                    const unsigned char *mask_dev, double *out_host, void *stream);

@@ -43,6 +43,7 @@ SIGNATURES = {
     'sn_csr_gather_one': (_c_int, [_vp, _c_int, _vp, _vp, _vp]),
     'sn_spmv_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _P_dbl, _vp]),
+    'sn_dot_f64_dev': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_seg_dot_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_seg_dot_f64_noalloc': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_absmax_f64': (_c_int, [_c_int, _vp, _vp, _P_dbl, _vp]),

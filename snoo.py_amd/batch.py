@@ -12,6 +12,7 @@ converged variant is frozen (its right-hand side is zero, so its CG segment
 stops at once).  Residuals are FP64, the Jacobian/CG FP32 ("mixed") or FP64.
 """
 import ctypes
+import math
 import time
 
 import numpy as np
@@ -528,10 +529,11 @@ class VariantBatchSolver:
         # allocated, used and freed on one stream
         self.workers = max(1, min(int(workers), V))
         # learned starts: each variant's own pipeline on its stream (sources,
-        # start, residual), or every variant's sources first, then the starts
-        # with one pass over the modes for all (WarmStart.start_many)
+        # start, residual), or the passes over the modes shared by all
+        # variants and few host waits (_batched_starts)
         self.batched_start = bool(batched_start)
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
+        self._pool = None
         self.h = None
         self.ams = None
         self.stats = {}
@@ -556,34 +558,105 @@ class VariantBatchSolver:
     def _each(self, fn, idx):
         """fn(v) for the variants idx, each on its own stream (worker threads);
         returns once every stream has finished."""
-        import threading
         import torch
+        from concurrent.futures import ThreadPoolExecutor
         idx = list(idx)
         if not idx:
             return
         torch.cuda.synchronize()
-        errors = []
+        if self._pool is None:
+            # kept for the solver's lifetime: no thread start-up per call
+            self._pool = ThreadPoolExecutor(max_workers=self.workers,
+                                            thread_name_prefix='variant-batch')
 
         def work(w):
-            try:
-                with torch.cuda.stream(self.streams[w]):
-                    for v in idx:
-                        if v % self.workers == w:
-                            fn(v)
-                    torch.cuda.current_stream().synchronize()
-            except BaseException as e:   # re-raised in the calling thread
-                errors.append(e)
-        threads = [threading.Thread(target=work, args=(w,)) for w in range(self.workers)]
-        for t in threads:
-            t.start()
-        for t in threads:
-            t.join()
+            with torch.cuda.stream(self.streams[w]):
+                for v in idx:
+                    if v % self.workers == w:
+                        fn(v)
+                torch.cuda.current_stream().synchronize()
+        futures = [self._pool.submit(work, w) for w in range(self.workers)
+                   if any(v % self.workers == w for v in idx)]
+        errors = [f.exception() for f in futures]
         torch.cuda.synchronize()
-        if errors:
-            raise errors[0]
+        for e in errors:
+            if e is not None:
+                raise e          # the first error, in the calling thread
 
     def _driver(self, v):
         return self.solvers[v].newton_driver('mixed', self.lanes_per_row, self.chunk, 'ams')
+
+    def _batched_starts(self, drvs, coeffs, r0, x0s, infos):
+        """Sources, |R(0)| and learned starts of every variant, the same
+        values as WarmStart.start gives each one, with the passes over the
+        modes shared and few host waits:
+
+        1. x = mean + U a of all variants in one pass over U (main stream);
+        2. per variant on its stream: Biot-Savart, R(0) (its norm left on the
+           device), R and the Jacobian at x, H = U^T J U (read back);
+        3. g = U^T R of all variants in one pass, the reduced systems solved
+           on the host, x += U da of all variants in one pass;
+        4. the judged residuals queued on one stream, their norms and the
+           |R(0)| read back together."""
+        import torch
+        V, NV = self.nvar, self.nv
+        ws = self.model
+        rg = ws.r_galerkin if ws.galerkin_steps else 0
+        n = self.solvers[0].curl_curl_factory.num_dofs
+        A = np.zeros((NV, coeffs.shape[1]))
+        A[:V] = coeffs
+        X = ws.expand_many(A)
+        norm2 = torch.zeros(2 * V, dtype=torch.float64, device=device())
+        Hs = [None] * V
+
+        def stage(v):
+            drv, s = drvs[v], self.solvers[v]
+            drv.retarget(s.source_fields_device(wait=False))
+            if drv._r0 is None:
+                drv.residual(torch.zeros(n, dtype=torch.float64, device=device()), norm2[v])
+            if rg:
+                drv.residual(X[v])
+                Hs[v] = ws.reduced_jacobian(drv)
+        self._each(stage, range(V))
+        if rg:
+            R = torch.zeros((NV, n), dtype=torch.float64, device=device())
+            for v in range(V):
+                R[v].copy_(drvs[v].R)
+            G = np.empty((NV, rg))
+            nat.check(nat.load().sn_pod_project_many(n, rg, NV, nat.ptr(ws._U), nat.ptr(R),
+                                                     G.ctypes.data_as(nat._vp),
+                                                     nat.stream_handle()))
+            D = np.zeros((NV, rg))
+            Hm = np.stack([0.5 * (H + H.T) for H in Hs])
+            try:
+                # every variant's reduced system in one call
+                D[:V] = np.linalg.solve(Hm, -G[:V, :, None])[:, :, 0]
+            except np.linalg.LinAlgError:
+                for v in range(V):
+                    try:
+                        D[v] = np.linalg.solve(Hm[v], -G[v])
+                    except np.linalg.LinAlgError:
+                        D[v] = np.linalg.lstsq(Hm[v], -G[v], rcond=None)[0]
+            D[~np.all(np.isfinite(D), axis=1)] = 0.0
+            X += ws.expand_many(D, mean=False)
+        # one tensor per start (rows of X would share one version counter)
+        xs = [X[v].clone() for v in range(V)]
+        for v in range(V):
+            drv = drvs[v]
+            xs[v][drv.mask == 0] = 0.0
+            drv.residual(xs[v], norm2[V + v])
+        n2 = norm2.cpu().tolist()
+        for v in range(V):
+            drv = drvs[v]
+            if drv._r0 is None:
+                drv._r0 = math.sqrt(n2[v])
+            r0[v] = drv._r0
+            rn = math.sqrt(n2[V + v])
+            drv._last = (xs[v], xs[v]._version, rn)     # R and the Jacobian at the start
+            r = rn / r0[v]
+            use = bool(np.isfinite(r) and r < 1.0)
+            x0s[v] = xs[v] if use else None
+            infos[v] = {'r_start_rel': float(r), 'used': use}
 
     def _setup(self):
         """The workspace and the shared preconditioner (first solve)."""
@@ -648,24 +721,20 @@ class VariantBatchSolver:
             # are batched, its learned start and R there right after, on the
             # same stream
             drv, s = drvs[v], self.solvers[v]
-            drv.retarget(s.source_fields_device())
+            drv.retarget(s.source_fields_device(wait=False))
             r0[v] = drv.r0()
-            if self.batched_start and coeffs is not None:
-                return
             if coeffs is not None:
                 x0s[v], infos[v] = self.model.start(s, params[v], coeffs=coeffs[v],
                                                     lanes_per_row=self.lanes_per_row)
             prepare(v)
-        self._each(sources, range(V))
-        t_sources = time.perf_counter() - t0
         if coeffs is not None and self.batched_start:
-            # expansions and projections of all variants in one pass over the
-            # modes each, residuals and reduced Jacobians per variant
-            starts = self.model.start_many(self.solvers, params, coeffs=coeffs, each=self._each,
-                                           lanes_per_row=self.lanes_per_row)
+            self._batched_starts(drvs, coeffs, r0, x0s, infos)
+            t_sources = time.perf_counter() - t0
             for v in range(V):
-                x0s[v], infos[v] = starts[v]
-            self._each(prepare, range(V))
+                prepare(v)
+        else:
+            self._each(sources, range(V))
+            t_sources = time.perf_counter() - t0
         t_prepare = time.perf_counter() - t0
         t_cg = t_advance = 0.0
         self._setup()
@@ -683,6 +752,7 @@ class VariantBatchSolver:
         Mr = torch.empty((6 * drvs[0].d['num_el'], NV), dtype=torch.float32, device=device())
         b = torch.zeros((n, NV), dtype=torch.float64, device=device())
         dx = torch.empty((n, NV), dtype=torch.float64, device=device())
+        norm2 = torch.zeros(V, dtype=torch.float64, device=device())
         its_h = (ctypes.c_int * NV)()
         rn_h = (ctypes.c_double * NV)()
         st_h = (ctypes.c_int * NV)()
@@ -716,20 +786,29 @@ class VariantBatchSolver:
             t2 = time.perf_counter()
             t_cg += t2 - t1
 
-            def advance(v):
-                # the step of variant v: residual at x + dx, line search, stop test
-                drv = drvs[v]
+            # the steps: every live variant's residual at x + dx queued on one
+            # stream (memory-bound kernels that each fill the GPU), one read of
+            # their norms, then line searches where a full step does not
+            # reduce the residual
+            steps = []
+            for v in range(V):
+                if not live[v]:
+                    continue
                 cg_its[v].append(int(its_h[v]))
                 if st_h[v] != nat.SN_OK:
                     live[v] = False          # finished by its own solver below
-                    return
-                step = dx[:, v]
-                x_try = xs[v] + step
-                rn_try = drv.residual(x_try)
+                    continue
+                x_try = xs[v] + dx[:, v]
+                drvs[v].residual(x_try, norm2[v])
+                steps.append((v, x_try))
+            n2 = norm2.cpu().tolist() if steps else []
+            for v, x_try in steps:
+                drv = drvs[v]
+                rn_try = math.sqrt(n2[v])
                 lam, tries = 1.0, 0
                 while not (rn_try < rn[v]) and tries < 6:
                     lam *= 0.5
-                    x_try = xs[v] + lam * step
+                    x_try = xs[v] + lam * dx[:, v]
                     rn_try = drv.residual(x_try)
                     tries += 1
                 stalled = rn_try >= rn[v] or (lam == 1.0 and rn_try > 0.9 * rn[v])
@@ -739,7 +818,6 @@ class VariantBatchSolver:
                 conv[v] = rn[v] <= rtol * r0[v]
                 if conv[v] or stalls[v] >= 3:
                     live[v] = False
-            self._each(advance, [v for v in range(V) if live[v]])
             t_advance += time.perf_counter() - t2
             k += 1
             if log is not None:
@@ -777,6 +855,9 @@ class VariantBatchSolver:
 
     def __del__(self):
         try:
+            if self._pool is not None:
+                self._pool.shutdown(wait=False)
+                self._pool = None
             if self.h and nat._lib is not None:
                 nat._lib.sn_vcg_destroy(self.h)
                 self.h = None

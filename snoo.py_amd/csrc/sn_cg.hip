@@ -762,6 +762,23 @@ __global__ __launch_bounds__(kRowBlock) void dot_partials_kernel(
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// The partials (at most 1024) summed in index order by one thread: the same
+// additions, in the same order, as the host sum of reduce_host, so the same
+// bits.  The workgroup stages them in LDS first, so the serial sum waits on
+// LDS rather than on one global load after another.
+constexpr int kSumInOrderMax = 1024;
+
+__global__ __launch_bounds__(kSumInOrderMax) void sum_in_order_kernel(
+    const double *__restrict__ partials, int G, double *__restrict__ out) {
+    __shared__ double buf[kSumInOrderMax];
+    for (int k = threadIdx.x; k < G; k += blockDim.x) buf[k] = partials[k];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double r = 0.0;
+    for (int k = 0; k < G; ++k) r = r + buf[k];
+    *out = r;
+}
+
 }  // namespace sn
 
 using namespace sn;
@@ -1544,6 +1561,24 @@ int sn_seg_dot_f64_noalloc(int nseg, const int *row0_dev, const double *a, const
 int sn_dot_f64(int n, const double *a, const double *b, const unsigned char *mask, double *out,
                void *stream) {
     return reduce_host(n, a, b, mask, 0, out, stream);
+}
+
+int sn_dot_f64_dev(int n, const double *a, const double *b, const unsigned char *mask,
+                   double *out_dev, void *stream) {
+    SN_REQUIRE(n >= 0 && a && b && out_dev, "sn_dot_f64_dev: null argument");
+    hipStream_t s = as_stream(stream);
+    const int G = grid_for(n, kRowBlock, 1024);
+    double *part = nullptr;
+    int rc = reduce_partials(s, &part);
+    if (rc) return rc;
+    hipLaunchKernelGGL(dot_partials_kernel, dim3(G), dim3(kRowBlock), 0, s, n, a, b, mask, 0,
+                       part);
+    SN_CHECK_LAUNCH();
+    SN_REQUIRE(G <= kSumInOrderMax, "sn_dot_f64_dev: too many partials");
+    hipLaunchKernelGGL(sum_in_order_kernel, dim3(1), dim3(kSumInOrderMax), 0, s, part, G,
+                       out_dev);
+    SN_CHECK_LAUNCH();
+    return SN_OK;
 }
 
 int sn_absmax_f64(int n, const double *a, const unsigned char *mask, double *out, void *stream) {

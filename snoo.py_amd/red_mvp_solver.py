@@ -62,9 +62,11 @@ class RedMVPSolver:
         return mask
 
     # ------------------------------------------------------------------ setup
-    def source_fields_device(self):
+    def source_fields_device(self, wait=True):
         """B_s at the quadrature points of each iron domain; None for air
-        (red_mvp_solver.py:149-164)."""
+        (red_mvp_solver.py:149-164).  ``wait=False`` returns as soon as the
+        kernels are queued on the current stream (stats['t_source'] is then
+        the time to queue them)."""
         if self._src is None:
             t0 = time.perf_counter()
             n_iron = len(self.iron_tags)
@@ -72,8 +74,9 @@ class RedMVPSolver:
                                                                       list(range(n_iron)))
             src = [source_field_device(self.coil_list, p).reshape(-1) for p in pts]
             src += [None] * len(self.air_tags)
-            import torch
-            torch.cuda.current_stream().synchronize()
+            if wait:
+                import torch
+                torch.cuda.current_stream().synchronize()
             self.stats['t_source'] = time.perf_counter() - t0
             self.stats['num_targets'] = int(sum(p.shape[0] for p in pts))
             self._src = src
@@ -411,8 +414,12 @@ class NewtonDriver:
         self._last = (x_dev, x_dev._version, rn)
         return rn
 
-    def residual(self, x_dev):
-        """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2."""
+    def residual(self, x_dev, norm2_dev=None):
+        """R(x) (FP64, device) and the Jacobian at x; returns ||R_free||_2.
+        With ``norm2_dev`` (a float64 device scalar) nothing waits: the
+        kernels are queued, ||R_free||^2 lands in norm2_dev, and None is
+        returned (the square root of the value read back is the same number
+        the waiting form returns)."""
         self._last = None
         lib = nat.load()
         if self.el_all:
@@ -425,6 +432,11 @@ class NewtonDriver:
                                             nat.ptr(self.J), nat.stream_handle()))
         nat.check(lib.sn_dof_gather_f64(self.d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
                                         nat.stream_handle()))
+        if norm2_dev is not None:
+            nat.check(lib.sn_dot_f64_dev(int(self.R.numel()), nat.ptr(self.R), nat.ptr(self.R),
+                                         nat.ptr(self.mask), nat.ptr(norm2_dev),
+                                         nat.stream_handle()))
+            return None
         out = ctypes.c_double()
         nat.check(lib.sn_dot_f64(int(self.R.numel()), nat.ptr(self.R), nat.ptr(self.R),
                                  nat.ptr(self.mask), ctypes.byref(out), nat.stream_handle()))
