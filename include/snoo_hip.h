@@ -466,6 +466,12 @@ int sn_vcg_solve(sn_vcg *vcg, const float *Mr_dev, const unsigned char *mask_dev
                  const double *b_dev, double *x_dev, const double *rtol_host, double atol,
                  int maxiter, int *iters_host, double *rnorm_host, int *status_host,
                  void *stream);
+/* The operands of sn_vcg_solve from per-system arrays (host arrays of nvar
+ * device pointers): Mr_dev[(j*n_el + e)*nvar + v] = Mr_src[v][j*n_el + e]
+ * (each [6*n_el] float32, every entry non-null) and b_dev[i*nvar + v] =
+ * -R_src[v][i] (each [n] float64; a null entry gives b = 0), in one pass. */
+int sn_vcg_pack(sn_vcg *vcg, const float *const *Mr_src, const double *const *R_src,
+                float *Mr_dev, double *b_dev, void *stream);
 /* Average duration (ms) of the CG's element pass and DOF pass (the product
  * with the element operators and the CG update of every system) over `reps`
  * back-to-back pairs on `stream`, timed with HIP events, every system

@@ -101,6 +101,7 @@ SIGNATURES = {
     'sn_vcg_info': (_c_int, [_vp, _P_int, _P_int, _P_int]),
     'sn_vcg_time_pair': (_c_int, [_vp, _vp, _vp, _c_int, _P_dbl, _vp]),
     'sn_vcg_set_grids': (_c_int, [_vp, _c_int, _c_int, _vp]),
+    'sn_vcg_pack': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'sn_vcg_destroy': (None, [_vp]),
     'sn_element_apply_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }

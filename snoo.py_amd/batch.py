@@ -504,7 +504,7 @@ class VariantBatchSolver:
     with empty right-hand sides."""
 
     def __init__(self, dipoles, quad_order=2, lanes_per_row=8, chunk=32, warm_start=None,
-                 model=None, anchor=None, workers=8, batched_start=False):
+                 model=None, anchor=None, workers=8, batched_start=True):
         import torch
         self.dipoles = list(dipoles)
         V = len(self.dipoles)
@@ -528,9 +528,10 @@ class VariantBatchSolver:
         # variant v always on stream v % workers, so its tensors are
         # allocated, used and freed on one stream
         self.workers = max(1, min(int(workers), V))
-        # learned starts: each variant's own pipeline on its stream (sources,
-        # start, residual), or the passes over the modes shared by all
-        # variants and few host waits (_batched_starts)
+        # learned starts: the passes over the modes shared by all variants
+        # and two host waits (_batched_starts; 58.7-59.3 ms per batch of 8 at
+        # 1M tets), or each variant's own pipeline on its stream (sources,
+        # WarmStart.start, residual; 60.8-62.1 ms, same box)
         self.batched_start = bool(batched_start)
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self._pool = None
@@ -759,10 +760,6 @@ class VariantBatchSolver:
         k = 0
         while any(live) and k < max_newton:
             rtol_h = (ctypes.c_double * NV)(*([1.0] * NV))
-            b.zero_()
-            for v in range(NV):
-                src = drvs[min(v, V - 1)]
-                Mr[:, v].copy_(src.Mr)       # padding systems repeat the last (b = 0)
             cap = 300
             for v in range(V):
                 if not live[v]:
@@ -771,9 +768,16 @@ class VariantBatchSolver:
                                       eta_margin, close_start if plan_close else None)
                 etas[v].append(eta[v])
                 rtol_h[v] = eta[v]
-                b[:, v].copy_(drvs[v].R).neg_()
                 cap = max(cap, 4 * (cg_its[v][-1] if cg_its[v] else 0))
             cap = min(cap, max_cg)
+            # Mr and b = -R of the live variants packed variant-minor in one
+            # pass (padding systems repeat the last variant's Mr, with b = 0)
+            mr_src = (ctypes.c_void_p * NV)(*[drvs[min(v, V - 1)].Mr.data_ptr()
+                                                for v in range(NV)])
+            r_src = (ctypes.c_void_p * NV)(*[drvs[v].R.data_ptr() if v < V and live[v] else None
+                                               for v in range(NV)])
+            nat.check(lib.sn_vcg_pack(self.h, mr_src, r_src, nat.ptr(Mr), nat.ptr(b),
+                                      nat.stream_handle()))
             self.ams.update(drvs[self.anchor].nu_el)
             dx.zero_()
             torch.cuda.synchronize()

@@ -827,6 +827,36 @@ __global__ void vcg_fill_timing_kernel(long long n_nv, int nv, int nb,
     if (k < n_nv) w1[k] = make_float2((mask && !mask[k / nv]) ? 0.f : 1.f, 0.f);
 }
 
+// Variant-minor operands from per-variant arrays: Mr[(j*N + e)*V + v] =
+// src_v[j*N + e] and b[i*V + v] = -R_v[i] (0 where R_v is null).  A thread
+// per slot or DOF reads the V sources (each coalesced across the threads) and
+// writes its V values as one contiguous run.
+struct VcgPackSrc {
+    const float *mr[8];
+    const double *r[8];
+};
+
+template <int NV>
+__global__ __launch_bounds__(256) void vcg_pack_kernel(long long n_slots, long long n,
+                                                       VcgPackSrc src, float *__restrict__ Mr,
+                                                       double *__restrict__ b) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_slots) {
+        float m[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) m[v] = src.mr[v][k];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Mr[k * NV + v] = m[v];
+    }
+    if (k < n) {
+        double r[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) r[v] = src.r[v] ? -src.r[v][k] : 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) b[k * NV + v] = r[v];
+    }
+}
+
 // x64[i*V + v] += xs (the float32 solution) on free rows
 __global__ void vcg_finalize_kernel(long long n_nv, int nv, const unsigned char *__restrict__ mask,
                                     const float *__restrict__ xs, double *__restrict__ x) {
@@ -1199,6 +1229,29 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
         }
     }
     return worst;
+}
+
+int sn_vcg_pack(sn_vcg *C, const float *const *Mr_src, const double *const *R_src,
+                float *Mr, double *b, void *stream) {
+    SN_REQUIRE(C && Mr_src && R_src && Mr && b, "sn_vcg_pack: null argument");
+    VcgPackSrc src{};
+    for (int v = 0; v < C->nv; ++v) {
+        SN_REQUIRE(Mr_src[v], "sn_vcg_pack: every system needs its element operators");
+        src.mr[v] = Mr_src[v];
+        src.r[v] = R_src[v];
+    }
+    const long long n_slots = 6LL * C->n_el;
+    const long long cover = std::max<long long>(n_slots, C->n);
+    const dim3 g((unsigned)((cover + 255) / 256)), blk(256);
+    hipStream_t s = as_stream(stream);
+    switch (C->nv) {
+    case 1: hipLaunchKernelGGL((vcg_pack_kernel<1>), g, blk, 0, s, n_slots, (long long)C->n, src, Mr, b); break;
+    case 2: hipLaunchKernelGGL((vcg_pack_kernel<2>), g, blk, 0, s, n_slots, (long long)C->n, src, Mr, b); break;
+    case 4: hipLaunchKernelGGL((vcg_pack_kernel<4>), g, blk, 0, s, n_slots, (long long)C->n, src, Mr, b); break;
+    default: hipLaunchKernelGGL((vcg_pack_kernel<8>), g, blk, 0, s, n_slots, (long long)C->n, src, Mr, b); break;
+    }
+    SN_CHECK_LAUNCH();
+    return SN_OK;
 }
 
 int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int reps,

@@ -165,3 +165,67 @@ def test_lock_step_newton_matches_single_solves(nvar):
     for (x, st), (x2, st2) in zip(out, out2):
         assert st['cg_iterations'] == st2['cg_iterations']
         assert bool((x == x2).all())
+
+
+@pytest.mark.parametrize('n', [1, 1000, 3_000_001])
+def test_device_dot_has_the_bits_of_the_host_dot(n):
+    """sn_dot_f64_dev leaves on the device the same number sn_dot_f64 returns
+    (the same partials, summed in the same order), with and without a mask;
+    several queued on one stream land in their own slots."""
+    import torch
+    from snoopy_amd import _native as nat
+    lib = nat.load()
+    g = torch.Generator(device='cuda').manual_seed(5)
+    a = torch.randn(n, dtype=torch.float64, device='cuda', generator=g) * 1e3
+    b = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    mask = (torch.rand(n, device='cuda', generator=g) > 0.3).to(torch.uint8)
+    out = torch.full((4,), float('nan'), dtype=torch.float64, device='cuda')
+    cases = [(a, b, None), (a, a, None), (a, b, mask), (b, b, mask)]
+    for k, (x, y, m) in enumerate(cases):
+        nat.check(lib.sn_dot_f64_dev(n, nat.ptr(x), nat.ptr(y), nat.ptr(m) if m is not None else None,
+                                     nat.ptr(out[k]), nat.stream_handle()))
+    got = out.cpu().tolist()
+    for k, (x, y, m) in enumerate(cases):
+        ref = ctypes.c_double()
+        nat.check(lib.sn_dot_f64(n, nat.ptr(x), nat.ptr(y), nat.ptr(m) if m is not None else None,
+                                 ctypes.byref(ref), nat.stream_handle()))
+        assert got[k] == ref.value, (k, got[k], ref.value)
+        # and the value itself, against a float64 sum on the host
+        xs, ys = x.cpu().numpy(), y.cpu().numpy()
+        w = np.ones(n, dtype=bool) if m is None else m.cpu().numpy().astype(bool)
+        exact = float(np.sum(xs[w] * ys[w]))
+        assert abs(got[k] - exact) <= 1e-12 * max(1.0, float(np.sum(np.abs(xs[w] * ys[w])))), k
+
+
+def test_pack_matches_the_strided_copies():
+    """sn_vcg_pack gives the variant-minor operands the strided copies give:
+    padding systems repeat the last element operators, null residuals give
+    b = 0."""
+    import torch
+    from snoopy_amd import _native as nat
+    from snoopy_amd.magnets import Dipole
+    dp = Dipole(h=H)
+    s, drv = _single_state(dp)
+    lib = nat.load()
+    N6, n = drv.Mr.numel(), drv.R.numel()
+    g = torch.Generator(device='cuda').manual_seed(3)
+    mrs = [torch.randn(N6, dtype=torch.float32, device='cuda', generator=g) for _ in range(3)]
+    rs = [torch.randn(n, dtype=torch.float64, device='cuda', generator=g) for _ in range(3)]
+    live = [True, False, True]
+    for nv in (4, 8):
+        h = _vcg(s, nv, drv.ams)
+        try:
+            Mr = torch.full((N6, nv), float('nan'), dtype=torch.float32, device='cuda')
+            b = torch.full((n, nv), float('nan'), dtype=torch.float64, device='cuda')
+            mr_src = (ctypes.c_void_p * nv)(*[mrs[min(v, 2)].data_ptr() for v in range(nv)])
+            r_src = (ctypes.c_void_p * nv)(*[rs[v].data_ptr() if v < 3 and live[v] else None
+                                             for v in range(nv)])
+            nat.check(lib.sn_vcg_pack(h, mr_src, r_src, nat.ptr(Mr), nat.ptr(b),
+                                      nat.stream_handle()))
+            torch.cuda.synchronize()
+        finally:
+            lib.sn_vcg_destroy(h)
+        for v in range(nv):
+            assert torch.equal(Mr[:, v], mrs[min(v, 2)]), (nv, v)
+            ref = -rs[v] if v < 3 and live[v] else torch.zeros_like(rs[0])
+            assert torch.equal(b[:, v], ref), (nv, v)
