@@ -120,6 +120,107 @@ __global__ __launch_bounds__(kGalBlock) void pod_reduced_jacobian_kernel(
         for (int q = 0; q < TK; ++q) dst[(kb + p) * R + lb + q] = acc[p][q];
 }
 
+// The same sum for r = 128 on the matrix cores.  Over one element chunk the
+// K = 3 EB rows k = 3 le + a of W and of Z = Mr W are staged in LDS, and
+//     H[p][q] += sum_k W[k][p] Z[k][q]
+// is a 128 x 128 x K product.  H is symmetric, so only its 10 upper 32 x 32
+// tiles are formed, two per wave (five waves), with v_mfma_f32_32x32x2_f32
+// (FP32 operands, FP32 accumulation); the partial writes each off-diagonal
+// tile twice, as itself and transposed.  Lane layouts of the 32x32x2 form:
+// A[i][kk] and B[kk][j] in lane i + 32 kk (resp. j + 32 kk); C[i][j] in lane
+// j, register 8 (q / 4) + 4 (lane / 32) + q % 4 = i.
+typedef __attribute__((ext_vector_type(16))) float gal_f32x16;
+constexpr int kGalMfmaEB = 16;
+static_assert(kGalMfmaEB == kGalEB, "the launch sizes its grid with the chunk of kGalEB elements");
+constexpr int kGalMfmaWaves = 5;
+constexpr int kGalMfmaBlock = 64 * kGalMfmaWaves;
+__constant__ int kGalUpperTiles[10][2] = {{0, 0}, {0, 1}, {0, 2}, {0, 3}, {1, 1},
+                                          {1, 2}, {1, 3}, {2, 2}, {2, 3}, {3, 3}};
+
+__global__ __launch_bounds__(kGalMfmaBlock) void pod_reduced_jacobian_mfma_kernel(
+    int num_el, int mr_stride, const float *__restrict__ W, const float *__restrict__ Mr,
+    float *__restrict__ part) {
+    constexpr int R = 128, EB = kGalMfmaEB, K = 3 * EB;
+    // staging items: (element of the chunk, group of 4 modes), each a thread's
+    // 3 float4 of W and the element's 6 Mr values, loaded one chunk ahead
+    constexpr int ITEMS = EB * R / 4;
+    constexpr int PER = (ITEMS + kGalMfmaBlock - 1) / kGalMfmaBlock;
+    __shared__ __align__(16) float sW[K][R];
+    __shared__ __align__(16) float sZ[K][R];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ti0 = kGalUpperTiles[2 * wave][0], tj0 = kGalUpperTiles[2 * wave][1];
+    const int ti1 = kGalUpperTiles[2 * wave + 1][0], tj1 = kGalUpperTiles[2 * wave + 1][1];
+    const int kh = lane >> 5, c = lane & 31;
+    gal_f32x16 acc0, acc1;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc0[q] = acc1[q] = 0.0f;
+    const int nb = (num_el + EB - 1) / EB;
+    const int per = (nb + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * per, b1 = min(nb, b0 + per);
+    float4 w[PER][3];
+    float m[PER][6];
+    auto load = [&](int bt) {
+        const int e0 = bt * EB;
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int it = tid + t * kGalMfmaBlock;
+            const int le = it / (R / 4), l4 = it % (R / 4), e = e0 + le;
+            const bool ok = it < ITEMS && e < num_el;
+            const float4 *src = reinterpret_cast<const float4 *>(W + (size_t)e * 3 * R) + l4;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                w[t][a] = ok ? src[a * (R / 4)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < 6; ++u) m[t][u] = ok ? Mr[(size_t)u * mr_stride + e] : 0.0f;
+        }
+    };
+    if (b0 < b1) load(b0);
+    for (int bt = b0; bt < b1; ++bt) {
+        // W and Z = Mr W of the chunk into LDS
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int it = tid + t * kGalMfmaBlock;
+            if (it >= ITEMS) continue;
+            const int le = it / (R / 4), l4 = it % (R / 4);
+            const float *mm = m[t];
+            float4 z[3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float x0 = w[t][0][q], x1 = w[t][1][q], x2 = w[t][2][q];
+                z[0][q] = mm[0] * x0 + mm[1] * x1 + mm[2] * x2;
+                z[1][q] = mm[1] * x0 + mm[3] * x1 + mm[4] * x2;
+                z[2][q] = mm[2] * x0 + mm[4] * x1 + mm[5] * x2;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                *reinterpret_cast<float4 *>(&sW[3 * le + a][4 * l4]) = w[t][a];
+                *reinterpret_cast<float4 *>(&sZ[3 * le + a][4 * l4]) = z[a];
+            }
+        }
+        __syncthreads();
+        if (bt + 1 < b1) load(bt + 1);       // in flight during the products
+#pragma unroll 4
+        for (int k = 0; k < K; k += 2) {
+            const float a0 = sW[k + kh][32 * ti0 + c], bz0 = sZ[k + kh][32 * tj0 + c];
+            const float a1 = sW[k + kh][32 * ti1 + c], bz1 = sZ[k + kh][32 * tj1 + c];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bz0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bz1, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *dst = part + (size_t)blockIdx.x * R * R;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int i = 8 * (q >> 2) + 4 * kh + (q & 3);
+        const int r0 = 32 * ti0 + i, c0 = 32 * tj0 + c;
+        dst[r0 * R + c0] = acc0[q];
+        if (ti0 != tj0) dst[c0 * R + r0] = acc0[q];
+        const int r1 = 32 * ti1 + i, c1 = 32 * tj1 + c;
+        dst[r1 * R + c1] = acc1[q];
+        if (ti1 != tj1) dst[c1 * R + r1] = acc1[q];
+    }
+}
+
 // out[i] = sum_g part[g*m + i] (float partials, double sums): 64 entries per
 // workgroup, 4 groups of 64 lanes (partial g goes to group g % 4, each group
 // sums in order), the group sums added in a fixed order
@@ -319,7 +420,7 @@ int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float
     const int ne = e1 - e0;
     const float *Wr = W + (size_t)e0 * 3 * r;
     const float *Mrr = Mr + e0;
-    const int nb = (ne + kGalEB - 1) / kGalEB;
+    const int nb = (ne + kGalEB - 1) / kGalEB;      // kGalMfmaEB is the same chunk
     const int G = nb < 512 ? nb : 512;
     const int m = r * r;
     float *part = nullptr;
@@ -333,7 +434,7 @@ int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float
         hipLaunchKernelGGL(pod_reduced_jacobian_kernel<64>, dim3(G), dim3(kGalBlock), 0, s,
                            ne, num_el, Wr, Mrr, part);
     else
-        hipLaunchKernelGGL(pod_reduced_jacobian_kernel<128>, dim3(G), dim3(kGalBlock), 0, s,
+        hipLaunchKernelGGL(pod_reduced_jacobian_mfma_kernel, dim3(G), dim3(kGalMfmaBlock), 0, s,
                            ne, num_el, Wr, Mrr, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {

@@ -17,7 +17,7 @@ from oracle import oracle as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('r', [32, 64])
+@pytest.mark.parametrize('r', [32, 64, 128])
 def test_reduced_jacobian_matches_oracle(r):
     import torch
     from snoopy_amd import _native as nat
