@@ -554,6 +554,21 @@ int sn_pod_expand_many(int n, int r, int nvec, const float *U_dev, const float *
  * its vector; synchronises the stream. */
 int sn_pod_project_many(int n, int r, int nvec, const float *U_dev, const double *v_dev,
                         double *g_host, void *stream);
+/* Device-output forms, queued on the stream without waiting:
+ * sn_pod_reduced_jacobian_range_dev writes H (float64, r*r) to H_dev and
+ * sn_pod_project_many_dev writes g [nvec][r] to g_dev, the same values as
+ * the host forms. */
+int sn_pod_reduced_jacobian_range_dev(int num_el, int e0, int e1, int r, const float *W_dev,
+                                      const float *Mr_dev, double *H_dev, void *stream);
+int sn_pod_project_many_dev(int n, int r, int nvec, const float *U_dev, const double *v_dev,
+                            double *g_dev, void *stream);
+/* Reduced Newton corrections of nsys systems on the device: d_dev[s] =
+ * -((H_s + H_s^T)/2)^-1 g_dev[s] (float32) by Cholesky factorization, H_dev
+ * [nsys][r][r] and g_dev [nsys][r] float64, r <= 128; d = 0 for a system
+ * whose matrix is not numerically positive definite (a pivot not above
+ * 1e-13 of the largest diagonal entry) or whose solution is not finite. */
+int sn_pod_reduced_solve_many(int nsys, int r, const double *H_dev, const double *g_dev,
+                              float *d_dev, void *stream);
 /* x_f[i] = sum_k w[6i+k] x_c[idx[6i+k]] (idx < 0 skipped): edge DOFs of one
  * mesh interpolated from another (Whitney interpolation weights). */
 int sn_edge_transfer(int nf, const int *idx_dev, const double *w_dev, const double *xc_dev,

@@ -66,6 +66,9 @@ SIGNATURES = {
     'sn_pod_expand': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_expand_many': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_project_many': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pod_reduced_jacobian_range_dev': (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pod_project_many_dev': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'sn_pod_reduced_solve_many': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_pod_element_modes': (_c_int, [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp]),
     'sn_pod_reduced_jacobian': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
     'sn_pod_reduced_jacobian_range': (_c_int, [_c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,

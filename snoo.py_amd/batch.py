@@ -528,10 +528,11 @@ class VariantBatchSolver:
         # variant v always on stream v % workers, so its tensors are
         # allocated, used and freed on one stream
         self.workers = max(1, min(int(workers), V))
-        # learned starts: the passes over the modes shared by all variants
-        # and two host waits (_batched_starts; 58.7-59.3 ms per batch of 8 at
-        # 1M tets), or each variant's own pipeline on its stream (sources,
-        # WarmStart.start, residual; 60.8-62.1 ms, same box)
+        # learned starts: queued from this thread with the passes over the
+        # modes shared by all variants and one host wait (_batched_starts;
+        # 54.9 ms per batch of 8 at 1M tets), or each variant's own pipeline
+        # on its stream from a worker thread (sources, WarmStart.start,
+        # residual; 57.4 ms, same box)
         self.batched_start = bool(batched_start)
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self._pool = None
@@ -588,57 +589,61 @@ class VariantBatchSolver:
         return self.solvers[v].newton_driver('mixed', self.lanes_per_row, self.chunk, 'ams')
 
     def _batched_starts(self, drvs, coeffs, r0, x0s, infos):
-        """Sources, |R(0)| and learned starts of every variant, the same
-        values as WarmStart.start gives each one, with the passes over the
-        modes shared and few host waits:
+        """Sources, |R(0)| and learned starts of every variant, queued from
+        this thread with one host wait at the end:
 
-        1. x = mean + U a of all variants in one pass over U (main stream);
-        2. per variant on its stream: Biot-Savart, R(0) (its norm left on the
-           device), R and the Jacobian at x, H = U^T J U (read back);
+        1. x = mean + U a of all variants in one pass over U (this stream);
+        2. per variant on its own stream: Biot-Savart, R(0) (its norm left on
+           the device), R and the Jacobian at x, H = U^T J U on the device;
         3. g = U^T R of all variants in one pass, the reduced systems solved
-           on the host, x += U da of all variants in one pass;
-        4. the judged residuals queued on one stream, their norms and the
-           |R(0)| read back together."""
+           on the device (sn_pod_reduced_solve_many), x += U d of all
+           variants in one pass;
+        4. the judged residuals, their norms and the |R(0)| read back
+           together.
+
+        The values are those of WarmStart.start for each variant, except
+        the reduced solve (Cholesky on the device; the host solves with LU)
+        and that a system that is not numerically positive definite gets no
+        correction (the host falls back to least squares)."""
         import torch
         V, NV = self.nvar, self.nv
         ws = self.model
         rg = ws.r_galerkin if ws.galerkin_steps else 0
+        lib = nat.load()
         n = self.solvers[0].curl_curl_factory.num_dofs
         A = np.zeros((NV, coeffs.shape[1]))
         A[:V] = coeffs
         X = ws.expand_many(A)
-        norm2 = torch.zeros(2 * V, dtype=torch.float64, device=device())
-        Hs = [None] * V
-
-        def stage(v):
-            drv, s = drvs[v], self.solvers[v]
-            drv.retarget(s.source_fields_device(wait=False))
-            if drv._r0 is None:
-                drv.residual(torch.zeros(n, dtype=torch.float64, device=device()), norm2[v])
-            if rg:
-                drv.residual(X[v])
-                Hs[v] = ws.reduced_jacobian(drv)
-        self._each(stage, range(V))
+        # |R(0)|^2, |R(x0)|^2 of the judges, and |R|^2 at the network's x (unused)
+        norm2 = torch.zeros(3 * V, dtype=torch.float64, device=device())
+        H = torch.zeros((NV, rg, rg), dtype=torch.float64, device=device()) if rg else None
+        main = torch.cuda.current_stream()
+        ready = main.record_event()
+        done = []
+        for v in range(V):
+            st = self.streams[v % self.workers]
+            st.wait_event(ready)
+            with torch.cuda.stream(st):
+                drv, s = drvs[v], self.solvers[v]
+                drv.retarget(s.source_fields_device(wait=False))
+                if drv._r0 is None:
+                    drv.residual(torch.zeros(n, dtype=torch.float64, device=device()), norm2[v])
+                if rg:
+                    drv.residual(X[v], norm2[2 * V + v])
+                    ws.reduced_jacobian_device(drv, H[v])
+                done.append(st.record_event())
+        for e in done:
+            main.wait_event(e)
         if rg:
             R = torch.zeros((NV, n), dtype=torch.float64, device=device())
             for v in range(V):
                 R[v].copy_(drvs[v].R)
-            G = np.empty((NV, rg))
-            nat.check(nat.load().sn_pod_project_many(n, rg, NV, nat.ptr(ws._U), nat.ptr(R),
-                                                     G.ctypes.data_as(nat._vp),
-                                                     nat.stream_handle()))
-            D = np.zeros((NV, rg))
-            Hm = np.stack([0.5 * (H + H.T) for H in Hs])
-            try:
-                # every variant's reduced system in one call
-                D[:V] = np.linalg.solve(Hm, -G[:V, :, None])[:, :, 0]
-            except np.linalg.LinAlgError:
-                for v in range(V):
-                    try:
-                        D[v] = np.linalg.solve(Hm[v], -G[v])
-                    except np.linalg.LinAlgError:
-                        D[v] = np.linalg.lstsq(Hm[v], -G[v], rcond=None)[0]
-            D[~np.all(np.isfinite(D), axis=1)] = 0.0
+            G = torch.zeros((NV, rg), dtype=torch.float64, device=device())
+            nat.check(lib.sn_pod_project_many_dev(n, rg, NV, nat.ptr(ws._U), nat.ptr(R),
+                                                  nat.ptr(G), nat.stream_handle()))
+            D = torch.zeros((NV, rg), dtype=torch.float32, device=device())
+            nat.check(lib.sn_pod_reduced_solve_many(V, rg, nat.ptr(H), nat.ptr(G), nat.ptr(D),
+                                                    nat.stream_handle()))
             X += ws.expand_many(D, mean=False)
         # one tensor per start (rows of X would share one version counter)
         xs = [X[v].clone() for v in range(V)]

@@ -351,23 +351,91 @@ __global__ __launch_bounds__(256) void pod_project_many_kernel(int n, int r,
     }
 }
 
+// Every system's reduced Newton correction d = -(H_s)^-1 g with H_s =
+// (H + H^T)/2: one workgroup per system, Cholesky factorization of H_s in LDS
+// (packed lower triangle, FP64), forward and back substitution.  A pivot that
+// is not above 1e-13 times the largest diagonal entry, or a non-finite
+// result, gives d = 0 (no correction) for that system.
+constexpr int kSolveMaxR = 128;
+constexpr int kSolveBlock = 256;
+
+__global__ __launch_bounds__(kSolveBlock) void reduced_solve_kernel(
+    int r, const double *__restrict__ H, const double *__restrict__ g, float *__restrict__ d) {
+    __shared__ double L[kSolveMaxR * (kSolveMaxR + 1) / 2];
+    __shared__ double y[kSolveMaxR];
+    __shared__ double dmax;
+    __shared__ int fail;
+    const int tid = threadIdx.x, sys = blockIdx.x;
+    const double *Hs = H + (size_t)sys * r * r;
+    auto at = [](int i, int j) { return i * (i + 1) / 2 + j; };     // j <= i
+    for (int t = tid; t < r * r; t += kSolveBlock) {
+        const int i = t / r, j = t % r;
+        if (j <= i) L[at(i, j)] = 0.5 * (Hs[(size_t)i * r + j] + Hs[(size_t)j * r + i]);
+    }
+    for (int i = tid; i < r; i += kSolveBlock) y[i] = -g[(size_t)sys * r + i];
+    __syncthreads();
+    if (tid == 0) {
+        double m = 0.0;
+        for (int i = 0; i < r; ++i) m = fmax(m, fabs(L[at(i, i)]));
+        dmax = m;
+        fail = !(m > 0.0) || !(m < INFINITY);
+    }
+    __syncthreads();
+    for (int k = 0; k < r && !fail; ++k) {
+        if (tid == 0) {
+            const double dk = L[at(k, k)];
+            if (!(dk > 1e-13 * dmax) || !(dk < INFINITY)) fail = 1;
+            else L[at(k, k)] = sqrt(dk);
+        }
+        __syncthreads();
+        if (fail) break;
+        const double lkk = L[at(k, k)];
+        for (int i = k + 1 + tid; i < r; i += kSolveBlock) L[at(i, k)] /= lkk;
+        __syncthreads();
+        for (int i = k + 1 + tid; i < r; i += kSolveBlock) {
+            const double lik = L[at(i, k)];
+            for (int j = k + 1; j <= i; ++j) L[at(i, j)] -= lik * L[at(j, k)];
+        }
+        __syncthreads();
+    }
+    if (!fail) {
+        for (int k = 0; k < r; ++k) {              // L z = y
+            if (tid == 0) y[k] /= L[at(k, k)];
+            __syncthreads();
+            const double yk = y[k];
+            for (int i = k + 1 + tid; i < r; i += kSolveBlock) y[i] -= L[at(i, k)] * yk;
+            __syncthreads();
+        }
+        for (int k = r - 1; k >= 0; --k) {         // L^T x = z
+            if (tid == 0) y[k] /= L[at(k, k)];
+            __syncthreads();
+            const double yk = y[k];
+            for (int i = tid; i < k; i += kSolveBlock) y[i] -= L[at(k, i)] * yk;
+            __syncthreads();
+        }
+        for (int i = tid; i < r; i += kSolveBlock)
+            if (!isfinite(y[i])) fail = 1;        // benign race: every writer stores 1
+        __syncthreads();
+    }
+    for (int i = tid; i < r; i += kSolveBlock)
+        d[(size_t)sys * r + i] = fail ? 0.0f : (float)y[i];
+}
+
 }  // namespace sn
 
 using namespace sn;
 
 extern "C" {
 
-int sn_pod_project_many(int n, int r, int nvec, const float *U, const double *v,
-                        double *g_host, void *stream) {
-    SN_REQUIRE(n > 0 && r > 0 && U && v && g_host, "sn_pod_project_many: bad argument");
+static int project_many_to(int n, int r, int nvec, const float *U, const double *v, double *g,
+                           hipStream_t s) {
+    SN_REQUIRE(n > 0 && r > 0 && U && v && g, "sn_pod_project_many: bad argument");
     SN_REQUIRE(nvec == 1 || nvec == 2 || nvec == 4 || nvec == 8,
                "sn_pod_project_many: nvec must be 1, 2, 4 or 8 (%d)", nvec);
-    hipStream_t s = as_stream(stream);
     const int G = (int)((n + 256LL * kProjPer - 1) / (256LL * kProjPer));
     const int m = nvec * r;
-    double *part = nullptr, *g = nullptr;
+    double *part = nullptr;
     SN_HIP(hipMallocAsync((void **)&part, (size_t)G * m * sizeof(double), s));
-    SN_HIP(hipMallocAsync((void **)&g, (size_t)m * sizeof(double), s));
     const dim3 grid(G, (r + kProjModes - 1) / kProjModes);
     switch (nvec) {
         case 1: hipLaunchKernelGGL(pod_project_many_kernel<1>, grid, dim3(256), 0, s, n, r, U, v, part); break;
@@ -381,16 +449,40 @@ int sn_pod_project_many(int n, int r, int nvec, const float *U, const double *v,
                            m, part, g);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(g_host, g, (size_t)m * sizeof(double),
-                                            hipMemcpyDeviceToHost, s);
-    hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(g, s);
-    if (e == hipSuccess) e = f1 != hipSuccess ? f1 : f2;
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t f = hipFreeAsync(part, s);
+    if (e == hipSuccess) e = f;
     if (e != hipSuccess) {
         set_error("sn_pod_project_many: %s", hipGetErrorString(e));
         return SN_ERR_HIP;
     }
     return SN_OK;
+}
+
+int sn_pod_project_many(int n, int r, int nvec, const float *U, const double *v,
+                        double *g_host, void *stream) {
+    SN_REQUIRE(g_host && r > 0 && nvec > 0, "sn_pod_project_many: bad argument");
+    hipStream_t s = as_stream(stream);
+    const int m = nvec * r;
+    double *g = nullptr;
+    SN_HIP(hipMallocAsync((void **)&g, (size_t)m * sizeof(double), s));
+    int rc = project_many_to(n, r, nvec, U, v, g, s);
+    hipError_t e = hipSuccess;
+    if (rc == SN_OK)
+        e = hipMemcpyAsync(g_host, g, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipError_t f = hipFreeAsync(g, s);
+    if (e == hipSuccess) e = f;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (rc != SN_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("sn_pod_project_many: %s", hipGetErrorString(e));
+        return SN_ERR_HIP;
+    }
+    return SN_OK;
+}
+
+int sn_pod_project_many_dev(int n, int r, int nvec, const float *U, const double *v,
+                            double *g_dev, void *stream) {
+    return project_many_to(n, r, nvec, U, v, g_dev, as_stream(stream));
 }
 
 int sn_pod_element_modes(int num_el, int n, int r, const float *U, const int *gids,
@@ -409,14 +501,13 @@ int sn_pod_reduced_jacobian(int num_el, int r, const float *W, const float *Mr, 
     return sn_pod_reduced_jacobian_range(num_el, 0, num_el, r, W, Mr, H_host, stream);
 }
 
-int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float *W,
-                                  const float *Mr, double *H_host, void *stream) {
-    SN_REQUIRE(num_el > 0 && W && Mr && H_host, "sn_pod_reduced_jacobian: bad argument");
+static int reduced_jacobian_to(int num_el, int e0, int e1, int r, const float *W,
+                               const float *Mr, double *H_dev, hipStream_t s) {
+    SN_REQUIRE(num_el > 0 && W && Mr && H_dev, "sn_pod_reduced_jacobian: bad argument");
     SN_REQUIRE(0 <= e0 && e0 < e1 && e1 <= num_el,
                "sn_pod_reduced_jacobian: element range [%d, %d) outside [0, %d)", e0, e1, num_el);
     SN_REQUIRE(r == 32 || r == 64 || r == 128,
                "sn_pod_reduced_jacobian: r must be 32, 64 or 128 (r=%d)", r);
-    hipStream_t s = as_stream(stream);
     const int ne = e1 - e0;
     const float *Wr = W + (size_t)e0 * 3 * r;
     const float *Mrr = Mr + e0;
@@ -424,9 +515,7 @@ int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float
     const int G = nb < 512 ? nb : 512;
     const int m = r * r;
     float *part = nullptr;
-    double *H = nullptr;
     SN_HIP(hipMallocAsync((void **)&part, (size_t)G * m * sizeof(float), s));
-    SN_HIP(hipMallocAsync((void **)&H, (size_t)m * sizeof(double), s));
     if (r == 32)
         hipLaunchKernelGGL(pod_reduced_jacobian_kernel<32>, dim3(G), dim3(kGalBlock), 0, s,
                            ne, num_el, Wr, Mrr, part);
@@ -439,18 +528,53 @@ int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(sum_partials_kernel<float>, dim3((m + 63) / 64), dim3(256), 0, s, G,
-                           m, part, H);
+                           m, part, H_dev);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(H_host, H, (size_t)m * sizeof(double),
-                                            hipMemcpyDeviceToHost, s);
-    hipError_t f1 = hipFreeAsync(part, s), f2 = hipFreeAsync(H, s);
-    if (e == hipSuccess) e = f1 != hipSuccess ? f1 : f2;
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipError_t f = hipFreeAsync(part, s);
+    if (e == hipSuccess) e = f;
     if (e != hipSuccess) {
         set_error("sn_pod_reduced_jacobian: %s", hipGetErrorString(e));
         return SN_ERR_HIP;
     }
+    return SN_OK;
+}
+
+int sn_pod_reduced_jacobian_range(int num_el, int e0, int e1, int r, const float *W,
+                                  const float *Mr, double *H_host, void *stream) {
+    SN_REQUIRE(H_host && r > 0, "sn_pod_reduced_jacobian: bad argument");
+    hipStream_t s = as_stream(stream);
+    const int m = r * r;
+    double *H = nullptr;
+    SN_HIP(hipMallocAsync((void **)&H, (size_t)m * sizeof(double), s));
+    int rc = reduced_jacobian_to(num_el, e0, e1, r, W, Mr, H, s);
+    hipError_t e = hipSuccess;
+    if (rc == SN_OK)
+        e = hipMemcpyAsync(H_host, H, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipError_t f = hipFreeAsync(H, s);
+    if (e == hipSuccess) e = f;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (rc != SN_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("sn_pod_reduced_jacobian: %s", hipGetErrorString(e));
+        return SN_ERR_HIP;
+    }
+    return SN_OK;
+}
+
+int sn_pod_reduced_jacobian_range_dev(int num_el, int e0, int e1, int r, const float *W,
+                                      const float *Mr, double *H_dev, void *stream) {
+    return reduced_jacobian_to(num_el, e0, e1, r, W, Mr, H_dev, as_stream(stream));
+}
+
+int sn_pod_reduced_solve_many(int nsys, int r, const double *H_dev, const double *g_dev,
+                              float *d_dev, void *stream) {
+    SN_REQUIRE(nsys >= 1 && r >= 1 && r <= kSolveMaxR && H_dev && g_dev && d_dev,
+               "sn_pod_reduced_solve_many: bad argument (nsys=%d, r=%d, r <= %d)", nsys, r,
+               kSolveMaxR);
+    hipLaunchKernelGGL(reduced_solve_kernel, dim3(nsys), dim3(kSolveBlock), 0, as_stream(stream),
+                       r, H_dev, g_dev, d_dev);
+    SN_CHECK_LAUNCH();
     return SN_OK;
 }
 

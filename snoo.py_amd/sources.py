@@ -56,6 +56,23 @@ def brick_field_host(brick, points):
     return out.cpu().numpy()
 
 
+def segments_device(coil, disc_arc=5):
+    """The coil's segments (device float64 [M,6]), kept with the coil and
+    formed again when its key points, levels or ``disc_arc`` change.  Their
+    upload from pageable memory would otherwise wait for the work already
+    queued on the stream (the previous coil's Biot-Savart)."""
+    import torch
+    key = (np.asarray(coil.kp, dtype=np.float64).tobytes(),
+           np.asarray(coil.y, dtype=np.float64).tobytes(), int(disc_arc))
+    hit = getattr(coil, '_segments_dev', None)
+    if hit is None or hit[0] != key:
+        if hit is not None:
+            torch.cuda.synchronize()       # no queued kernel may still read the old array
+        hit = (key, as_dev_f64(coil.get_segments(disc_arc=disc_arc)))
+        coil._segments_dev = hit
+    return hit[1]
+
+
 def source_field_device(coil_list, points_dev, disc_arc=5):
     """Sum of the coil fields at device points [M,3] (float64)."""
     import torch
@@ -64,7 +81,7 @@ def source_field_device(coil_list, points_dev, disc_arc=5):
     out = torch.zeros_like(points_dev)
     for coil in coil_list:
         if isinstance(coil, RacetrackCoil):
-            segs = as_dev_f64(coil.get_segments(disc_arc=disc_arc))
+            segs = segments_device(coil, disc_arc)
             biot_savart_device(segs, points_dev, coil.current, coil.rad, out, accumulate=True)
         elif isinstance(coil, BrickField):
             brick_field_device(coil, points_dev, out, accumulate=True)

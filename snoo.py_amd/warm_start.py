@@ -459,11 +459,15 @@ class WarmStart:
     def expand_many(self, A, mean=True):
         """mean + U a for every row a of A ([V, r], V in 1, 2, 4, 8) in one pass
         over the modes: device float64 [V, n], row v bit-identical to
-        expand(A[v]) (without the mean when ``mean`` is False)."""
+        expand(A[v]) (without the mean when ``mean`` is False).  A may be a
+        host array or a float32 device tensor (then nothing waits)."""
         import torch
         self.to_device()
-        A = np.ascontiguousarray(A, dtype=np.float32)
-        c = torch.from_numpy(A).to(device())
+        if isinstance(A, torch.Tensor):
+            c = A.to(device=device(), dtype=torch.float32).contiguous()
+        else:
+            A = np.ascontiguousarray(A, dtype=np.float32)
+            c = torch.from_numpy(A).to(device())
         n = self.pod.U.shape[0]
         out = torch.empty((A.shape[0], n), dtype=torch.float64, device=device())
         nat.check(nat.load().sn_pod_expand_many(n, int(A.shape[1]), int(A.shape[0]),
@@ -587,6 +591,43 @@ class WarmStart:
                 cache[self] = hit
             H += hit[1]
         return H.reshape(self.r_galerkin, self.r_galerkin)
+
+    def reduced_jacobian_device(self, drv, out):
+        """reduced_jacobian(drv) queued on the current stream into the
+        float64 device tensor ``out`` [r_galerkin, r_galerkin]: the same
+        sums, in the same order, without a host wait (the constant domains'
+        term is the host cache's, uploaded once per geometry; forming that
+        cache, the first time, waits)."""
+        import torch
+        import weakref
+        asm = drv.asm
+        var, const = self._domain_ranges(asm)
+        lib = nat.load()
+        out.zero_()
+        if var:
+            tmp = torch.empty_like(out)
+            for e0, e1 in var:
+                nat.check(lib.sn_pod_reduced_jacobian_range_dev(
+                    drv.d['num_el'], e0, e1, self.r_galerkin, nat.ptr(self._W), nat.ptr(drv.Mr),
+                    nat.ptr(tmp), nat.stream_handle()))
+                out += tmp
+        if const:
+            dev = asm.__dict__.setdefault('_galerkin_const_dev', weakref.WeakKeyDictionary())
+            cache = asm.__dict__.get('_galerkin_const')
+            hit = cache.get(self) if cache is not None else None
+            key = (getattr(asm, 'geometry_version', 0), self.r_galerkin,
+                   tuple((e0, e1, id(asm.material_list[n])) for n, e0, e1 in const))
+            if hit is None or hit[0] != key:
+                # the host form computes (and caches) the constant term; the
+                # state-dependent part it sums as well is dropped here
+                self.reduced_jacobian(drv)
+                hit = asm.__dict__['_galerkin_const'][self]
+            d = dev.get(self)
+            if d is None or d[0] != key:
+                d = (key, torch.from_numpy(hit[1].reshape(out.shape)).to(out.device))
+                dev[self] = d
+            out += d[1]
+        return out
 
     def precompute_constant(self, solver, lanes_per_row=8):
         """Form the state-independent part of the reduced Jacobian for

@@ -447,3 +447,27 @@ def test_msh_mesh_assembly_matches_oracle(dipole, tmp_path):
         np.testing.assert_array_equal(a.indices, b.indices)
         assert _rel(a.data, b.data) < 1e-12
     assert _rel(rhs, rhso) < 1e-12
+
+
+def test_coil_segments_are_kept_until_the_geometry_changes(dipole):
+    """The device segments of a coil are reused while its key points and
+    levels are unchanged, formed again after either changes, and the field
+    follows the new geometry (against the oracle)."""
+    from snoopy_amd.sources import segments_device, source_field_device
+    from snoopy_amd.device import as_dev_f64
+    import copy
+    c = copy.deepcopy(dipole.coils[0])
+    s1 = segments_device(c)
+    assert segments_device(c) is s1
+    np.testing.assert_array_equal(s1.cpu().numpy(), c.get_segments(disc_arc=5))
+    c.kp = c.kp * 1.1                     # a new geometry
+    s2 = segments_device(c)
+    assert s2 is not s1
+    np.testing.assert_array_equal(s2.cpu().numpy(), c.get_segments(disc_arc=5))
+    c.y[0] += 0.01                        # changed in place
+    s3 = segments_device(c)
+    assert s3 is not s2
+    pts = np.random.default_rng(4).uniform(-0.6, 0.6, size=(500, 3))
+    Bg = source_field_device([c], as_dev_f64(pts)).cpu().numpy()
+    Bo = orc.coil_field([c], pts, threads=2)
+    assert _rel(Bg, Bo) < 1e-12

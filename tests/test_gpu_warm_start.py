@@ -111,6 +111,68 @@ def test_reduced_jacobian_reuses_the_constant_domains():
     assert asm._galerkin_const[ws][0] != key0
 
 
+@pytest.mark.parametrize('r', [32, 128])
+def test_device_forms_of_the_reduced_step_match_the_host_forms(r):
+    """The reduced Jacobian (with its cached constant part) and the batched
+    projection left on the device equal their host forms bit for bit; the
+    device Cholesky solve matches numpy's solve of the symmetrised systems,
+    and gives no correction for a system that is not positive definite."""
+    import torch
+    from snoopy_amd import _native as nat
+    from snoopy_amd.magnets import Dipole
+    from snoopy_amd.warm_start import WarmStart
+    print('case: device reduced step r=%d, dipole h=0.2' % r, flush=True)
+    dp = Dipole(h=0.2)
+    sol = dp.solver(quad_order=2, verbose=False)
+    asm = sol.curl_curl_factory
+    d = asm._device_state()
+    N, n = d['num_el'], asm.num_dofs
+    rng = np.random.default_rng(9)
+    free = sol.make_boundary_mask()
+    U = rng.standard_normal((r, n)).astype(np.float32)
+    U[:, ~free] = 0.0
+    Ud = torch.from_numpy(U).cuda()
+    W = torch.empty(N * 3 * r, dtype=torch.float32, device='cuda')
+    lib = nat.load()
+    nat.check(lib.sn_pod_element_modes(N, n, r, nat.ptr(Ud), nat.ptr(d['gids']),
+                                       nat.ptr(d['orient']), nat.ptr(W), nat.stream_handle()))
+    ws = WarmStart.__new__(WarmStart)
+    ws._W, ws.r_galerkin = W, r
+    drv = sol.newton_driver('mixed', 8, 32, 'ams')
+    drv.retarget(sol.source_fields_device())
+    x = rng.standard_normal(n) * 1e-2
+    x[~free] = 0.0
+    drv.residual(torch.from_numpy(x).cuda())
+    Hh = ws.reduced_jacobian(drv)
+    Hd = torch.full((r, r), float('nan'), dtype=torch.float64, device='cuda')
+    ws.reduced_jacobian_device(drv, Hd)
+    assert np.array_equal(Hd.cpu().numpy(), Hh)
+    # projections of 4 vectors
+    V4 = torch.from_numpy(rng.standard_normal((4, n))).cuda()
+    gh = np.empty((4, r))
+    nat.check(lib.sn_pod_project_many(n, r, 4, nat.ptr(Ud), nat.ptr(V4),
+                                      gh.ctypes.data_as(ctypes.c_void_p), nat.stream_handle()))
+    gd = torch.full((4, r), float('nan'), dtype=torch.float64, device='cuda')
+    nat.check(lib.sn_pod_project_many_dev(n, r, 4, nat.ptr(Ud), nat.ptr(V4), nat.ptr(gd),
+                                          nat.stream_handle()))
+    assert np.array_equal(gd.cpu().numpy(), gh)
+    # solves: the reduced Jacobian, a random SPD matrix, a singular one
+    M = rng.standard_normal((r, r))
+    S = np.zeros((r, r))
+    S[:r // 2, :r // 2] = np.eye(r // 2)
+    Hs = np.stack([Hh, M @ M.T + r * np.eye(r), S])
+    g = rng.standard_normal((3, r))
+    D = torch.full((3, r), float('nan'), dtype=torch.float32, device='cuda')
+    nat.check(lib.sn_pod_reduced_solve_many(3, r, nat.ptr(torch.from_numpy(Hs).cuda()),
+                                            nat.ptr(torch.from_numpy(g).cuda()), nat.ptr(D),
+                                            nat.stream_handle()))
+    D = D.cpu().numpy()
+    for k in range(2):
+        ref = np.linalg.solve(0.5 * (Hs[k] + Hs[k].T), -g[k])
+        assert np.abs(D[k] - ref).max() <= 1e-5 * np.abs(ref).max(), k
+    assert np.all(D[2] == 0.0)
+
+
 @pytest.fixture(scope='module')
 def trained():
     from snoopy_amd.warm_start import MorphFamily, WarmStart
