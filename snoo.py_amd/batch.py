@@ -608,12 +608,17 @@ class VariantBatchSolver:
         import torch
         V, NV = self.nvar, self.nv
         ws = self.model
-        rg = ws.r_galerkin if ws.galerkin_steps else 0
         lib = nat.load()
         n = self.solvers[0].curl_curl_factory.num_dofs
-        A = np.zeros((NV, coeffs.shape[1]))
-        A[:V] = coeffs
-        X = ws.expand_many(A)
+        if coeffs is None:
+            # from zero: the residual at zero is R(0), which Newton reuses
+            rg = 0
+            X = torch.zeros((NV, n), dtype=torch.float64, device=device())
+        else:
+            rg = ws.r_galerkin if ws.galerkin_steps else 0
+            A = np.zeros((NV, coeffs.shape[1]))
+            A[:V] = coeffs
+            X = ws.expand_many(A)
         # |R(0)|^2, |R(x0)|^2 of the judges, and |R|^2 at the network's x (unused)
         norm2 = torch.zeros(3 * V, dtype=torch.float64, device=device())
         H = torch.zeros((NV, rg, rg), dtype=torch.float64, device=device()) if rg else None
@@ -626,7 +631,7 @@ class VariantBatchSolver:
             with torch.cuda.stream(st):
                 drv, s = drvs[v], self.solvers[v]
                 drv.retarget(s.source_fields_device(wait=False))
-                if drv._r0 is None:
+                if drv._r0 is None and coeffs is not None:
                     drv.residual(torch.zeros(n, dtype=torch.float64, device=device()), norm2[v])
                 if rg:
                     drv.residual(X[v], norm2[2 * V + v])
@@ -654,11 +659,16 @@ class VariantBatchSolver:
         n2 = norm2.cpu().tolist()
         for v in range(V):
             drv = drvs[v]
-            if drv._r0 is None:
+            rn = math.sqrt(n2[V + v])
+            if coeffs is None:
+                drv._r0 = rn
+            elif drv._r0 is None:
                 drv._r0 = math.sqrt(n2[v])
             r0[v] = drv._r0
-            rn = math.sqrt(n2[V + v])
             drv._last = (xs[v], xs[v]._version, rn)     # R and the Jacobian at the start
+            if coeffs is None:
+                x0s[v] = xs[v]
+                continue
             r = rn / r0[v]
             use = bool(np.isfinite(r) and r < 1.0)
             x0s[v] = xs[v] if use else None
@@ -733,7 +743,7 @@ class VariantBatchSolver:
                 x0s[v], infos[v] = self.model.start(s, params[v], coeffs=coeffs[v],
                                                     lanes_per_row=self.lanes_per_row)
             prepare(v)
-        if coeffs is not None and self.batched_start:
+        if self.batched_start:
             self._batched_starts(drvs, coeffs, r0, x0s, infos)
             t_sources = time.perf_counter() - t0
             for v in range(V):
