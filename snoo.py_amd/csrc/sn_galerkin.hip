@@ -356,14 +356,17 @@ __global__ __launch_bounds__(256) void pod_project_many_kernel(int n, int r,
 // (packed lower triangle, FP64), forward and back substitution.  A pivot that
 // is not above 1e-13 times the largest diagonal entry, or a non-finite
 // result, gives d = 0 (no correction) for that system.
+//
+// Two barriers per factorization step (the column, then the trailing
+// update) and one per substitution step: every thread forms the step's
+// unknown itself from values no thread writes in that step.
 constexpr int kSolveMaxR = 128;
 constexpr int kSolveBlock = 256;
 
 __global__ __launch_bounds__(kSolveBlock) void reduced_solve_kernel(
     int r, const double *__restrict__ H, const double *__restrict__ g, float *__restrict__ d) {
     __shared__ double L[kSolveMaxR * (kSolveMaxR + 1) / 2];
-    __shared__ double y[kSolveMaxR];
-    __shared__ double dmax;
+    __shared__ double y[kSolveMaxR], z[kSolveMaxR], c[kSolveMaxR], cs[kSolveMaxR];
     __shared__ int fail;
     const int tid = threadIdx.x, sys = blockIdx.x;
     const double *Hs = H + (size_t)sys * r * r;
@@ -373,52 +376,57 @@ __global__ __launch_bounds__(kSolveBlock) void reduced_solve_kernel(
         if (j <= i) L[at(i, j)] = 0.5 * (Hs[(size_t)i * r + j] + Hs[(size_t)j * r + i]);
     }
     for (int i = tid; i < r; i += kSolveBlock) y[i] = -g[(size_t)sys * r + i];
+    if (tid == 0) fail = 0;
     __syncthreads();
-    if (tid == 0) {
-        double m = 0.0;
-        for (int i = 0; i < r; ++i) m = fmax(m, fabs(L[at(i, i)]));
-        dmax = m;
-        fail = !(m > 0.0) || !(m < INFINITY);
-    }
-    __syncthreads();
-    for (int k = 0; k < r && !fail; ++k) {
-        if (tid == 0) {
-            const double dk = L[at(k, k)];
-            if (!(dk > 1e-13 * dmax) || !(dk < INFINITY)) fail = 1;
-            else L[at(k, k)] = sqrt(dk);
+    double dmax = 0.0;
+    for (int i = 0; i < r; ++i) dmax = fmax(dmax, fabs(L[at(i, i)]));
+    const bool bad_scale = !(dmax > 0.0) || !(dmax < INFINITY);
+    // factorization, right-looking: column k is copied to c (and c / A_kk to
+    // cs), then the trailing triangle A_ij -= c_i cs_j, 16 x 16 threads over
+    // its rows and columns
+    const int tr = tid / 16, tc = tid % 16;
+    for (int k = 0; k < r && !bad_scale; ++k) {
+        const double dk = L[at(k, k)];
+        if (!(dk > 1e-13 * dmax) || !(dk < INFINITY)) {
+            if (tid == 0) fail = 1;
+            break;                                   // uniform: every thread read the same dk
         }
-        __syncthreads();
-        if (fail) break;
-        const double lkk = L[at(k, k)];
-        for (int i = k + 1 + tid; i < r; i += kSolveBlock) L[at(i, k)] /= lkk;
-        __syncthreads();
+        const double lk = sqrt(dk);
         for (int i = k + 1 + tid; i < r; i += kSolveBlock) {
-            const double lik = L[at(i, k)];
-            for (int j = k + 1; j <= i; ++j) L[at(i, j)] -= lik * L[at(j, k)];
+            const double a = L[at(i, k)];
+            c[i] = a;
+            cs[i] = a / dk;
+            L[at(i, k)] = a / lk;                    // the factor's column
+        }
+        __syncthreads();
+        if (tid == 0) L[at(k, k)] = lk;
+        for (int i = k + 1 + tr; i < r; i += 16) {
+            const double ci = c[i];
+            for (int j = k + 1 + tc; j <= i; j += 16) L[at(i, j)] -= ci * cs[j];
         }
         __syncthreads();
     }
-    if (!fail) {
-        for (int k = 0; k < r; ++k) {              // L z = y
-            if (tid == 0) y[k] /= L[at(k, k)];
-            __syncthreads();
-            const double yk = y[k];
-            for (int i = k + 1 + tid; i < r; i += kSolveBlock) y[i] -= L[at(i, k)] * yk;
-            __syncthreads();
-        }
-        for (int k = r - 1; k >= 0; --k) {         // L^T x = z
-            if (tid == 0) y[k] /= L[at(k, k)];
-            __syncthreads();
-            const double yk = y[k];
-            for (int i = tid; i < k; i += kSolveBlock) y[i] -= L[at(k, i)] * yk;
+    __syncthreads();
+    const bool ok = !bad_scale && !fail;
+    if (ok) {
+        for (int k = 0; k < r; ++k) {                // L z = y
+            const double zk = y[k] / L[at(k, k)];
+            for (int i = k + 1 + tid; i < r; i += kSolveBlock) y[i] -= L[at(i, k)] * zk;
+            if (tid == 0) z[k] = zk;
             __syncthreads();
         }
-        for (int i = tid; i < r; i += kSolveBlock)
-            if (!isfinite(y[i])) fail = 1;        // benign race: every writer stores 1
-        __syncthreads();
+        for (int k = r - 1; k >= 0; --k) {           // L^T x = z
+            const double xk = z[k] / L[at(k, k)];
+            for (int i = tid; i < k; i += kSolveBlock) z[i] -= L[at(k, i)] * xk;
+            if (tid == 0) y[k] = xk;
+            __syncthreads();
+        }
     }
+    bool finite = ok;
+    if (ok)
+        for (int i = 0; i < r; ++i) finite = finite && isfinite(y[i]);
     for (int i = tid; i < r; i += kSolveBlock)
-        d[(size_t)sys * r + i] = fail ? 0.0f : (float)y[i];
+        d[(size_t)sys * r + i] = finite ? (float)y[i] : 0.0f;
 }
 
 }  // namespace sn
