@@ -817,6 +817,7 @@ struct sn_cg {
     // iterations), keyed by their pointers
     hipGraphExec_t graph = nullptr;
     hipGraphExec_t graph_short = nullptr;
+    hipGraphExec_t graph_tail = nullptr;   // two iterations
     int chunk_short = 8;
     const void *g_vals = nullptr;
     const unsigned char *g_mask = nullptr;
@@ -995,7 +996,7 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
 
 static hipError_t drop_graphs(sn_cg *C) {
     hipError_t e = hipSuccess;
-    for (hipGraphExec_t *g : {&C->graph, &C->graph_short}) {
+    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail}) {
         if (*g) {
             const hipError_t f = hipGraphExecDestroy(*g);
             if (e == hipSuccess) e = f;
@@ -1086,13 +1087,14 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
             SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             own = true;
         }
-        for (int which = 0; which < 2; ++which) {
+        hipGraphExec_t *dst[3] = {&C->graph, &C->graph_short, &C->graph_tail};
+        const int len[3] = {C->chunk, C->chunk_short, 2};
+        for (int which = 0; which < 3; ++which) {
             hipGraph_t g;
             SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-            launch_chunk(C, vals, mask, xdst, which ? C->chunk_short : C->chunk, cap);
+            launch_chunk(C, vals, mask, xdst, len[which], cap);
             SN_HIP(hipStreamEndCapture(cap, &g));
-            SN_HIP(hipGraphInstantiate(which ? &C->graph_short : &C->graph, g, nullptr, nullptr,
-                                       0));
+            SN_HIP(hipGraphInstantiate(dst[which], g, nullptr, nullptr, 0));
             SN_HIP(hipGraphDestroy(g));
         }
         if (own) SN_HIP(hipStreamDestroy(cap));
@@ -1107,22 +1109,36 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
     // ||r|| of every segment at iteration 0, for the rate estimate
     std::vector<double> r_start(C->nseg);
     for (int v = 0; v < C->nseg; ++v) r_start[v] = sqrt(C->st_host[v].rr);
+    // Replays: the first a short one; then a full chunk only while some
+    // running segment's estimate of the iterations still needed (at its mean
+    // rate so far) is at least two full chunks, short chunks down to 6 still
+    // needed, two-iteration ones after that (iterations replayed after every
+    // segment stopped still launch kernels that exit at entry).
     for (;;) {
-        bool running = false, full = false;
+        bool running = false, first = false;
+        double need_max = 0.0;
         for (int v = 0; v < C->nseg; ++v) {
             const CgState &h = C->st_host[v];
             if (h.done != 0) continue;
             running = true;
-            if (h.it <= 0) continue;   // the first replay is a short one
-            // iterations still needed at the mean rate so far; no progress
-            // (or no estimate) takes the full chunk
+            if (h.it <= 0) {
+                first = true;
+                continue;
+            }
             const double r = sqrt(h.rr), r0 = r_start[v];
             const double rate = (r > 0.0 && r0 > 0.0 && r < r0) ? log(r / r0) / h.it : 0.0;
-            const double need = rate < 0.0 ? log(h.tol / r) / rate : 1e30;
-            if (!(need <= 3.0 * C->chunk_short)) full = true;
+            // no rate yet: a long solve once it has run two full chunks
+            const double need = rate < 0.0 ? log(h.tol / r) / rate
+                                           : (h.it >= 2 * C->chunk ? 1e30 : C->chunk_short);
+            need_max = std::max(need_max, need);
         }
         if (!running) break;
-        SN_HIP(hipGraphLaunch(full ? C->graph : C->graph_short, s));
+        hipGraphExec_t g = C->graph_short;
+        if (!first) {
+            if (need_max >= 2.0 * C->chunk) g = C->graph;
+            else if (need_max <= 6.0) g = C->graph_tail;
+        }
+        SN_HIP(hipGraphLaunch(g, s));
         SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
         SN_HIP(hipStreamSynchronize(s));
     }

@@ -871,7 +871,7 @@ __global__ void vcg_finalize_kernel(long long n_nv, int nv, const unsigned char 
 struct sn_vcg {
     const sn_pattern *pat = nullptr;
     int nv = 0, n = 0, n_el = 0;
-    int chunk = 32, chunk_short = 8;
+    int chunk = 32, chunk_short = 8, chunk_tail = 2;
     int grid_el = 0, grid_vec = 0;
     int xcd = 1;
     int *el_dofs = nullptr;            // [6*n_el] signed DOF ids (shared)
@@ -889,7 +889,7 @@ struct sn_vcg {
     int ams_lanes = 8;
     std::vector<float *> vb, vx;       // per level [n_l*3*nv], [row][component][variant]
     float *vy = nullptr;               // [n_nodes*3*nv] level-0 result, 0 at fixed nodes
-    hipGraphExec_t graph = nullptr, graph_short = nullptr;
+    hipGraphExec_t graph = nullptr, graph_short = nullptr, graph_tail = nullptr;
     const float *g_Mr = nullptr;
     const unsigned char *g_mask = nullptr;
     hipStream_t g_stream = nullptr;
@@ -899,7 +899,7 @@ namespace sn {
 
 static hipError_t vcg_drop_graphs(sn_vcg *C) {
     hipError_t e = hipSuccess;
-    for (hipGraphExec_t *g : {&C->graph, &C->graph_short}) {
+    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail}) {
         if (*g) {
             const hipError_t f = hipGraphExecDestroy(*g);
             if (e == hipSuccess) e = f;
@@ -1173,12 +1173,14 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
             SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             own = true;
         }
-        for (int which = 0; which < 2; ++which) {
+        hipGraphExec_t *dst[3] = {&C->graph, &C->graph_short, &C->graph_tail};
+        const int len[3] = {C->chunk, C->chunk_short, C->chunk_tail};
+        for (int which = 0; which < 3; ++which) {
             hipGraph_t g;
             SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-            SN_VCG_DISPATCH(C, vcg_chunk, Mr, mask, which ? C->chunk_short : C->chunk, cap);
+            SN_VCG_DISPATCH(C, vcg_chunk, Mr, mask, len[which], cap);
             SN_HIP(hipStreamEndCapture(cap, &g));
-            SN_HIP(hipGraphInstantiate(which ? &C->graph_short : &C->graph, g, nullptr, nullptr, 0));
+            SN_HIP(hipGraphInstantiate(dst[which], g, nullptr, nullptr, 0));
             SN_HIP(hipGraphDestroy(g));
         }
         if (own) SN_HIP(hipStreamDestroy(cap));
@@ -1191,21 +1193,38 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
     SN_HIP(hipStreamSynchronize(s));
     std::vector<double> r_start(nv);
     for (int v = 0; v < nv; ++v) r_start[v] = sqrt(C->st_host[v].rr);
+    // Replays: the first a short one; then a full chunk only while every
+    // running system's estimate of the iterations still needed (at its mean
+    // rate so far) is at least two full chunks, short chunks down to 6
+    // still needed, two-iteration ones after that.  An iteration replayed
+    // after every system stopped still launches its kernels (they exit at
+    // entry, about 60 us at 1M tets x 8 systems), a poll costs about 40 us.
     for (;;) {
-        bool running = false, full = false;
+        bool running = false, first = false;
+        double need_max = 0.0;
         for (int v = 0; v < nv; ++v) {
             const VcgState &h = C->st_host[v];
             if (h.done != 0) continue;
             running = true;
-            if (h.it <= 0) continue;   // the first replay is a short one
-            // iterations still needed at the mean rate so far
+            if (h.it <= 0) {
+                first = true;
+                continue;
+            }
             const double r = sqrt(h.rr), r0 = r_start[v];
             const double rate = (r > 0.0 && r0 > 0.0 && r < r0) ? log(r / r0) / h.it : 0.0;
-            const double need = rate < 0.0 ? log(h.tol / r) / rate : 1e30;
-            if (!(need <= 3.0 * C->chunk_short)) full = true;
+            // no rate yet (the residual has not gone below its start): a
+            // long solve once it has run two full chunks, else poll soon
+            const double need = rate < 0.0 ? log(h.tol / r) / rate
+                                           : (h.it >= 2 * C->chunk ? 1e30 : C->chunk_short);
+            need_max = std::max(need_max, need);
         }
         if (!running) break;
-        SN_HIP(hipGraphLaunch(full ? C->graph : C->graph_short, s));
+        hipGraphExec_t g = C->graph_short;
+        if (!first) {
+            if (need_max >= 2.0 * C->chunk) g = C->graph;
+            else if (need_max <= 6.0) g = C->graph_tail;
+        }
+        SN_HIP(hipGraphLaunch(g, s));
         SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
         SN_HIP(hipStreamSynchronize(s));
     }
