@@ -692,8 +692,11 @@ class VariantBatchSolver:
         nat.check(nat.load().sn_vcg_set_ams(self.h, self.ams.handle, nat.stream_handle()))
 
     def solve(self, rtol=1e-8, max_newton=40, eta_max=0.1, eta_min=1e-3, eta_margin=0.9,
-              close_start=1e-2, max_cg=-1, log=None, plan_close=True):
-        """Solutions (device float64) and stats of every variant, in order."""
+              close_start=1e-2, max_cg=-1, log=None, plan_close=True, lag_preconditioner=True):
+        """Solutions (device float64) and stats of every variant, in order.
+        ``lag_preconditioner``: after the first Newton step, refresh the
+        shared preconditioner only while the anchor variant's residual is
+        above close_start |R(0)|."""
         import torch
         from .red_mvp_solver import next_forcing, planned_forcing
         V, NV = self.nvar, self.nv
@@ -793,7 +796,13 @@ class VariantBatchSolver:
                                                for v in range(NV)])
             nat.check(lib.sn_vcg_pack(self.h, mr_src, r_src, nat.ptr(Mr), nat.ptr(b),
                                       nat.stream_handle()))
-            self.ams.update(drvs[self.anchor].nu_el)
+            a = self.anchor
+            if k == 0 or not (rn[a] <= close_start * r0[a]) or not lag_preconditioner:
+                # once the anchor is within close_start of the solution its
+                # reluctivities barely move: the preconditioner of the step
+                # before is kept (the same CG iterations at 1M tets, from the
+                # learned start and from zero)
+                self.ams.update(drvs[a].nu_el)
             dx.zero_()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
