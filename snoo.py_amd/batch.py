@@ -632,7 +632,7 @@ class VariantBatchSolver:
                 drv, s = drvs[v], self.solvers[v]
                 drv.retarget(s.source_fields_device(wait=False))
                 if drv._r0 is None and coeffs is not None:
-                    drv.residual(torch.zeros(n, dtype=torch.float64, device=device()), norm2[v])
+                    drv.r0_queued(norm2[v])
                 if rg:
                     drv.residual(X[v], norm2[2 * V + v])
                     ws.reduced_jacobian_device(drv, H[v])

@@ -487,6 +487,35 @@ class NewtonDriver:
                                       maxiter=maxiter)
         return its, rc, x_try
 
+    def r0_queued(self, norm2_dev):
+        """||R(0)||^2 queued into the float64 device scalar norm2_dev.  At
+        x = 0 the elements of a domain without source fields see B = 0 and
+        add nothing (their local values are set to zero instead of being
+        formed), so only the domains with sources run the element kernel.
+        The element matrices and reluctivities of the other domains are left
+        as they were: the next residual() forms them."""
+        import torch
+        n = self.asm.num_dofs
+        z = torch.zeros(n, dtype=torch.float64, device=device())
+        with_src = [i for i, b in enumerate(self.src) if b is not None]
+        if not self.el_all or len(with_src) == len(self.src):
+            self.residual(z, norm2_dev)
+            return
+        self._last = None
+        d = self.asm._device_state()
+        offs, N = d['offsets'], d['num_el']
+        loc = self.loc6.view(6, N)
+        for i in range(len(self.src)):
+            if self.src[i] is None:
+                loc[:, int(offs[i]):int(offs[i + 1])].zero_()
+        lib = nat.load()
+        self.asm.newton_local_device(z, self.quad_order, self.src, 2, True, self.Mr, self.loc6,
+                                     domains=with_src, nu_el=self.nu_el)
+        nat.check(lib.sn_dof_gather_f64(self.d['pattern'], nat.ptr(self.loc6), nat.ptr(self.R),
+                                        nat.stream_handle()))
+        nat.check(lib.sn_dot_f64_dev(int(self.R.numel()), nat.ptr(self.R), nat.ptr(self.R),
+                                     nat.ptr(self.mask), nat.ptr(norm2_dev), nat.stream_handle()))
+
     def r0(self):
         """||R(0)|| (the norm of the source term), cached."""
         if self._r0 is None:

@@ -229,3 +229,23 @@ def test_pack_matches_the_strided_copies():
             assert torch.equal(Mr[:, v], mrs[min(v, 2)]), (nv, v)
             ref = -rs[v] if v < 3 and live[v] else torch.zeros_like(rs[0])
             assert torch.equal(b[:, v], ref), (nv, v)
+
+
+def test_source_only_residual_at_zero_has_the_bits_of_the_full_one():
+    """R(0) from the domains with sources only (the others' local values set
+    to zero) equals the full residual at zero bit for bit, also when the
+    local values hold another state before."""
+    import torch
+    from snoopy_amd.magnets import Dipole
+    dp = Dipole(h=H)
+    s, drv = _single_state(dp)
+    n = s.curl_curl_factory.num_dofs
+    full = drv.residual(torch.zeros(n, dtype=torch.float64, device='cuda'))
+    R_full = drv.R.clone()
+    x = torch.from_numpy(np.random.default_rng(2).standard_normal(n) * 1e-2).cuda()
+    x[drv.mask == 0] = 0.0
+    drv.residual(x)                           # other values in every element
+    out = torch.full((1,), float('nan'), dtype=torch.float64, device='cuda')
+    drv.r0_queued(out[0])
+    assert float(np.sqrt(out.item())) == full
+    assert torch.equal(drv.R, R_full)
