@@ -536,6 +536,7 @@ class VariantBatchSolver:
         self.batched_start = bool(batched_start)
         self.streams = [torch.cuda.Stream() for _ in range(self.workers)]
         self._pool = None
+        self._operands = None
         self.h = None
         self.ams = None
         self.stats = {}
@@ -768,9 +769,15 @@ class VariantBatchSolver:
         conv = [rn[v] <= rtol * r0[v] for v in range(V)]
         live = [not c for c in conv]          # in the lock step
         stalls = [0] * V
-        Mr = torch.empty((6 * drvs[0].d['num_el'], NV), dtype=torch.float32, device=device())
-        b = torch.zeros((n, NV), dtype=torch.float64, device=device())
-        dx = torch.empty((n, NV), dtype=torch.float64, device=device())
+        if self._operands is None:
+            # kept for the solver's lifetime: the CG's captured graphs are
+            # keyed by the operand pointers, so the same buffers let every
+            # solve replay them instead of capturing them again
+            self._operands = (
+                torch.empty((6 * drvs[0].d['num_el'], NV), dtype=torch.float32, device=device()),
+                torch.zeros((n, NV), dtype=torch.float64, device=device()),
+                torch.empty((n, NV), dtype=torch.float64, device=device()))
+        Mr, b, dx = self._operands
         norm2 = torch.zeros(V, dtype=torch.float64, device=device())
         its_h = (ctypes.c_int * NV)()
         rn_h = (ctypes.c_double * NV)()
