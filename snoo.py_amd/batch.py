@@ -590,57 +590,60 @@ class VariantBatchSolver:
         return self.solvers[v].newton_driver('mixed', self.lanes_per_row, self.chunk, 'ams')
 
     def _batched_starts(self, drvs, coeffs, r0, x0s, infos):
-        """Sources, |R(0)| and learned starts of every variant, queued from
-        this thread with one host wait at the end:
+        """Sources, |R(0)| and starts of every variant, queued from this
+        thread with one host wait at the end.  With the learned start:
 
         1. x = mean + U a of all variants in one pass over U (this stream);
         2. per variant on its own stream: Biot-Savart, R(0) (its norm left on
            the device), R and the Jacobian at x, H = U^T J U on the device;
         3. g = U^T R of all variants in one pass, the reduced systems solved
            on the device (sn_pod_reduced_solve_many), x += U d of all
-           variants in one pass;
+           variants in one pass (2 and 3 once per reduced Newton step);
         4. the judged residuals, their norms and the |R(0)| read back
            together.
 
-        The values are those of WarmStart.start for each variant, except
-        the reduced solve (Cholesky on the device; the host solves with LU)
-        and that a system that is not numerically positive definite gets no
-        correction (the host falls back to least squares)."""
+        The starts are WarmStart.start's up to the order of the correction's
+        sum (added after the expansion here, inside it there).  From zero:
+        the sources, then R at zero, which is R(0) and Newton's first
+        residual."""
         import torch
         V, NV = self.nvar, self.nv
         ws = self.model
         lib = nat.load()
         n = self.solvers[0].curl_curl_factory.num_dofs
         if coeffs is None:
-            # from zero: the residual at zero is R(0), which Newton reuses
-            rg = 0
+            rg, steps = 0, 0
             X = torch.zeros((NV, n), dtype=torch.float64, device=device())
         else:
-            rg = ws.r_galerkin if ws.galerkin_steps else 0
+            rg = ws.r_galerkin
+            steps = ws.galerkin_steps if rg else 0
             A = np.zeros((NV, coeffs.shape[1]))
             A[:V] = coeffs
             X = ws.expand_many(A)
-        # |R(0)|^2, |R(x0)|^2 of the judges, and |R|^2 at the network's x (unused)
+        # |R(0)|^2, |R(x0)|^2 of the judges, and |R|^2 at the reduced steps' x (unused)
         norm2 = torch.zeros(3 * V, dtype=torch.float64, device=device())
         H = torch.zeros((NV, rg, rg), dtype=torch.float64, device=device()) if rg else None
         main = torch.cuda.current_stream()
-        ready = main.record_event()
-        done = []
-        for v in range(V):
-            st = self.streams[v % self.workers]
-            st.wait_event(ready)
-            with torch.cuda.stream(st):
-                drv, s = drvs[v], self.solvers[v]
-                drv.retarget(s.source_fields_device(wait=False))
-                if drv._r0 is None and coeffs is not None:
-                    drv.r0_queued(norm2[v])
-                if rg:
-                    drv.residual(X[v], norm2[2 * V + v])
-                    ws.reduced_jacobian_device(drv, H[v])
-                done.append(st.record_event())
-        for e in done:
-            main.wait_event(e)
-        if rg:
+        for step in range(max(1, steps)):
+            ready = main.record_event()
+            done = []
+            for v in range(V):
+                st = self.streams[v % self.workers]
+                st.wait_event(ready)
+                with torch.cuda.stream(st):
+                    drv, s = drvs[v], self.solvers[v]
+                    if step == 0:
+                        drv.retarget(s.source_fields_device(wait=False))
+                        if drv._r0 is None and coeffs is not None:
+                            drv.r0_queued(norm2[v])
+                    if steps:
+                        drv.residual(X[v], norm2[2 * V + v])
+                        ws.reduced_jacobian_device(drv, H[v])
+                    done.append(st.record_event())
+            for e in done:
+                main.wait_event(e)
+            if not steps:
+                break
             R = torch.zeros((NV, n), dtype=torch.float64, device=device())
             for v in range(V):
                 R[v].copy_(drvs[v].R)

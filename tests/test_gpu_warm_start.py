@@ -315,6 +315,39 @@ def test_variant_batch_with_learned_start_matches_the_batch_from_zero(trained, b
         assert err < 1e-6
 
 
+@pytest.mark.parametrize('steps', [1, 2])
+def test_batch_starts_match_the_single_starts(trained, steps):
+    """The variant batch's starts (sources, R(0), reduced steps queued for
+    all variants with one host wait) against WarmStart.start per variant,
+    with one and two reduced Newton steps: the same judge, starts equal up to
+    the order of the corrections' sums."""
+    import torch
+    from snoopy_amd.batch import VariantBatchSolver
+    from snoopy_amd.warm_start import PARAM_NAMES, sample_params
+    fam, ws = trained
+    dps = [_family_variant(fam, sample_params(s)) for s in (6300, 6301, 6302)]
+    print('case: batch starts of 3, %d reduced steps' % steps, flush=True)
+    old = ws.galerkin_steps
+    ws.galerkin_steps = steps
+    try:
+        vb = VariantBatchSolver(dps, warm_start='learned', model=ws)
+        drvs = [vb._driver(v) for v in range(3)]
+        params = [{k: d.p[k] for k in PARAM_NAMES} for d in dps]
+        coeffs = ws.coefficients(params)
+        r0, x0s, infos = [None] * 3, [None] * 3, [None] * 3
+        vb._batched_starts(drvs, coeffs, r0, x0s, infos)
+        for v, (s, p) in enumerate(zip(vb.solvers, params)):
+            xs, info = ws.start(s, p, coeffs=coeffs[v])
+            assert infos[v]['used'] == info['used'], v
+            assert abs(infos[v]['r_start_rel'] - info['r_start_rel']) <= 1e-6 * info['r_start_rel']
+            assert r0[v] == s.newton_driver('mixed', 8, 32, 'ams').r0()
+            if xs is not None:
+                err = float((x0s[v] - xs).abs().max() / xs.abs().max())
+                assert err < 1e-9, (v, err)
+    finally:
+        ws.galerkin_steps = old
+
+
 def test_saved_model_predicts_the_same_coefficients(trained, tmp_path):
     from snoopy_amd.warm_start import MorphFamily, WarmStart, sample_params
     fam, ws = trained
