@@ -144,11 +144,13 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_kernel(
 
 // one block per segment: tol, ||b||, state for iteration 0
 __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
-    SegMap sm, const double *__restrict__ partbb, CgState *__restrict__ st,
-    const double *__restrict__ rtol, double atol, int maxiter) {
+    SegMap sm, const double *__restrict__ partbb, const double *__restrict__ partB,
+    CgState *__restrict__ st, const double *__restrict__ rtol, double atol, int maxiter) {
     __shared__ double lds[kVecBlock / kWave];
     const int seg = blockIdx.x;
     const double bb = reduce_range<kVecBlock>(partbb, sm.blkB0[seg], sm.blkB0[seg + 1], 1, 0, lds);
+    // ||r_0||^2 from the init pass's partials (the host's rate estimate)
+    const double rr0 = reduce_range<kVecBlock>(partB, sm.blkB0[seg], sm.blkB0[seg + 1], 2, 1, lds);
     if (threadIdx.x == 0) {
         CgState *S = st + seg;
         const double bnorm = sqrt(bb);
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kVecBlock) void cg_init_state_kernel(
         S->tol = atol > rt ? atol : rt;  // _get_atol_rtol
         S->rho[0] = S->rho[1] = 0.0;
         S->pq = 0.0;
-        S->rr = 0.0;
+        S->rr = rr0;
         S->it = 0;
         S->maxiter = maxiter;
         S->done = (bnorm == 0.0) ? 1 : 0;
@@ -817,7 +819,8 @@ struct sn_cg {
     // iterations), keyed by their pointers
     hipGraphExec_t graph = nullptr;
     hipGraphExec_t graph_short = nullptr;
-    hipGraphExec_t graph_tail = nullptr;   // two iterations
+    hipGraphExec_t graph_tail = nullptr;      // two iterations
+    hipGraphExec_t graph_quarter = nullptr;   // four iterations
     int chunk_short = 8;
     const void *g_vals = nullptr;
     const unsigned char *g_mask = nullptr;
@@ -970,7 +973,7 @@ static void launch_init_lpr(sn_cg *C, const M *vals, const unsigned char *mask, 
         default: launch_init<V, M, 8>(C, vals, mask, b, x0, s); break;
     }
     hipLaunchKernelGGL(cg_init_state_kernel, dim3(C->nseg), dim3(kVecBlock), 0, s, C->map(),
-                       C->partbb, C->st, C->rtol_dev, atol, maxiter);
+                       C->partbb, C->partB, C->st, C->rtol_dev, atol, maxiter);
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, C->precision == 64, C->r, C->dinv, mask, C->w0, 2,
@@ -986,7 +989,7 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
                        (float *)C->w0, (float *)C->w1, (float *)C->dinv, C->xs, C->partB,
                        C->partbb);
     hipLaunchKernelGGL(cg_init_state_kernel, dim3(1), dim3(kVecBlock), 0, s, C->map(), C->partbb,
-                       C->st, C->rtol_dev, atol, maxiter);
+                       C->partB, C->st, C->rtol_dev, atol, maxiter);
     if (C->ams) {
         const AmsSegments seg{C->row0, C->blkB0, C->segB};
         (void)ams_launch_apply(C->ams, false, C->r, C->dinv, mask, C->w0, 2, nullptr, C->partB,
@@ -996,7 +999,7 @@ static void launch_init_elem(sn_cg *C, const float *Mr, const unsigned char *mas
 
 static hipError_t drop_graphs(sn_cg *C) {
     hipError_t e = hipSuccess;
-    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail}) {
+    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail, &C->graph_quarter}) {
         if (*g) {
             const hipError_t f = hipGraphExecDestroy(*g);
             if (e == hipSuccess) e = f;
@@ -1087,9 +1090,9 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
             SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             own = true;
         }
-        hipGraphExec_t *dst[3] = {&C->graph, &C->graph_short, &C->graph_tail};
-        const int len[3] = {C->chunk, C->chunk_short, 2};
-        for (int which = 0; which < 3; ++which) {
+        hipGraphExec_t *dst[4] = {&C->graph, &C->graph_short, &C->graph_quarter, &C->graph_tail};
+        const int len[4] = {C->chunk, C->chunk_short, 4, 2};
+        for (int which = 0; which < 4; ++which) {
             hipGraph_t g;
             SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
             launch_chunk(C, vals, mask, xdst, len[which], cap);
@@ -1107,13 +1110,15 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
     SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
     SN_HIP(hipStreamSynchronize(s));
     // ||r|| of every segment at iteration 0, for the rate estimate
-    std::vector<double> r_start(C->nseg);
-    for (int v = 0; v < C->nseg; ++v) r_start[v] = sqrt(C->st_host[v].rr);
+    std::vector<double> r_start(C->nseg), r_last(C->nseg);
+    std::vector<int> it_last(C->nseg, 0);
+    for (int v = 0; v < C->nseg; ++v) r_start[v] = r_last[v] = sqrt(C->st_host[v].rr);
     // Replays: the first a short one; then a full chunk only while some
-    // running segment's estimate of the iterations still needed (at its mean
-    // rate so far) is at least two full chunks, short chunks down to 6 still
-    // needed, two-iteration ones after that (iterations replayed after every
-    // segment stopped still launch kernels that exit at entry).
+    // running segment's estimate of the iterations still needed (at its rate
+    // over the last replay) is at least two full chunks, short chunks (8) down to 10
+    // still needed, 4 down to 5, two-iteration ones after that (iterations
+    // replayed after every segment stopped still launch kernels that exit at
+    // entry).
     for (;;) {
         bool running = false, first = false;
         double need_max = 0.0;
@@ -1125,9 +1130,16 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
                 first = true;
                 continue;
             }
-            const double r = sqrt(h.rr), r0 = r_start[v];
-            const double rate = (r > 0.0 && r0 > 0.0 && r < r0) ? log(r / r0) / h.it : 0.0;
-            // no rate yet: a long solve once it has run two full chunks
+            // the rate over the last replay, else the mean rate since the start
+            const double r = sqrt(h.rr), r0 = r_start[v], rl = r_last[v];
+            double rate = 0.0;
+            if (h.it > it_last[v] && r > 0.0 && rl > 0.0 && r < rl)
+                rate = log(r / rl) / (h.it - it_last[v]);
+            else if (r > 0.0 && r0 > 0.0 && r < r0)
+                rate = log(r / r0) / h.it;
+            r_last[v] = r;
+            it_last[v] = h.it;
+            // no rate: a long solve once it has run two full chunks
             const double need = rate < 0.0 ? log(h.tol / r) / rate
                                            : (h.it >= 2 * C->chunk ? 1e30 : C->chunk_short);
             need_max = std::max(need_max, need);
@@ -1136,7 +1148,8 @@ static int cg_run(sn_cg *C, const void *vals, const unsigned char *mask, const d
         hipGraphExec_t g = C->graph_short;
         if (!first) {
             if (need_max >= 2.0 * C->chunk) g = C->graph;
-            else if (need_max <= 6.0) g = C->graph_tail;
+            else if (need_max < 5.0) g = C->graph_tail;
+            else if (need_max < 10.0) g = C->graph_quarter;
         }
         SN_HIP(hipGraphLaunch(g, s));
         SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_init_state_kernel(
         S->tol = atol > rt ? atol : rt;
         S->rho[0] = S->rho[1] = 0.0;
         S->pq = 0.0;
-        S->rr = 0.0;
+        S->rr = sums[threadIdx.x];   // ||r_0||^2 = ||b||^2 (x_0 = 0), for the host's rate estimate
         S->beta = 0.0;
         S->it = 0;
         S->maxiter = maxiter;
@@ -889,7 +889,8 @@ struct sn_vcg {
     int ams_lanes = 8;
     std::vector<float *> vb, vx;       // per level [n_l*3*nv], [row][component][variant]
     float *vy = nullptr;               // [n_nodes*3*nv] level-0 result, 0 at fixed nodes
-    hipGraphExec_t graph = nullptr, graph_short = nullptr, graph_tail = nullptr;
+    hipGraphExec_t graph = nullptr, graph_short = nullptr, graph_tail = nullptr,
+                   graph_quarter = nullptr;
     const float *g_Mr = nullptr;
     const unsigned char *g_mask = nullptr;
     hipStream_t g_stream = nullptr;
@@ -899,7 +900,7 @@ namespace sn {
 
 static hipError_t vcg_drop_graphs(sn_vcg *C) {
     hipError_t e = hipSuccess;
-    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail}) {
+    for (hipGraphExec_t *g : {&C->graph, &C->graph_short, &C->graph_tail, &C->graph_quarter}) {
         if (*g) {
             const hipError_t f = hipGraphExecDestroy(*g);
             if (e == hipSuccess) e = f;
@@ -1173,9 +1174,9 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
             SN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
             own = true;
         }
-        hipGraphExec_t *dst[3] = {&C->graph, &C->graph_short, &C->graph_tail};
-        const int len[3] = {C->chunk, C->chunk_short, C->chunk_tail};
-        for (int which = 0; which < 3; ++which) {
+        hipGraphExec_t *dst[4] = {&C->graph, &C->graph_short, &C->graph_quarter, &C->graph_tail};
+        const int len[4] = {C->chunk, C->chunk_short, 4, C->chunk_tail};
+        for (int which = 0; which < 4; ++which) {
             hipGraph_t g;
             SN_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
             SN_VCG_DISPATCH(C, vcg_chunk, Mr, mask, len[which], cap);
@@ -1191,14 +1192,16 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
     const size_t sb = nv * sizeof(VcgState);
     SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
     SN_HIP(hipStreamSynchronize(s));
-    std::vector<double> r_start(nv);
-    for (int v = 0; v < nv; ++v) r_start[v] = sqrt(C->st_host[v].rr);
-    // Replays: the first a short one; then a full chunk only while every
-    // running system's estimate of the iterations still needed (at its mean
-    // rate so far) is at least two full chunks, short chunks down to 6
-    // still needed, two-iteration ones after that.  An iteration replayed
-    // after every system stopped still launches its kernels (they exit at
-    // entry, about 60 us at 1M tets x 8 systems), a poll costs about 40 us.
+    std::vector<double> r_start(nv), r_last(nv);
+    std::vector<int> it_last(nv, 0);
+    for (int v = 0; v < nv; ++v) r_start[v] = r_last[v] = sqrt(C->st_host[v].rr);
+    // Replays: the first a short one; then a full chunk only while the
+    // running systems' estimate of the iterations still needed (at their
+    // rates over the last replay) is at least two full chunks, short chunks (8) down to
+    // 10 still needed, 4 down to 5, two-iteration ones after that.  An
+    // iteration replayed after every system stopped still launches its
+    // kernels (they exit at entry, about 60 us at 1M tets x 8 systems), a
+    // poll costs about 40 us.
     for (;;) {
         bool running = false, first = false;
         double need_max = 0.0;
@@ -1210,10 +1213,17 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
                 first = true;
                 continue;
             }
-            const double r = sqrt(h.rr), r0 = r_start[v];
-            const double rate = (r > 0.0 && r0 > 0.0 && r < r0) ? log(r / r0) / h.it : 0.0;
-            // no rate yet (the residual has not gone below its start): a
-            // long solve once it has run two full chunks, else poll soon
+            // the rate over the last replay, else the mean rate since the start
+            const double r = sqrt(h.rr), r0 = r_start[v], rl = r_last[v];
+            double rate = 0.0;
+            if (h.it > it_last[v] && r > 0.0 && rl > 0.0 && r < rl)
+                rate = log(r / rl) / (h.it - it_last[v]);
+            else if (r > 0.0 && r0 > 0.0 && r < r0)
+                rate = log(r / r0) / h.it;
+            r_last[v] = r;
+            it_last[v] = h.it;
+            // no rate (the residual has not gone below its start): a long
+            // solve once it has run two full chunks, else poll soon
             const double need = rate < 0.0 ? log(h.tol / r) / rate
                                            : (h.it >= 2 * C->chunk ? 1e30 : C->chunk_short);
             need_max = std::max(need_max, need);
@@ -1222,7 +1232,8 @@ int sn_vcg_solve(sn_vcg *C, const float *Mr, const unsigned char *mask, const do
         hipGraphExec_t g = C->graph_short;
         if (!first) {
             if (need_max >= 2.0 * C->chunk) g = C->graph;
-            else if (need_max <= 6.0) g = C->graph_tail;
+            else if (need_max < 5.0) g = C->graph_tail;
+            else if (need_max < 10.0) g = C->graph_quarter;
         }
         SN_HIP(hipGraphLaunch(g, s));
         SN_HIP(hipMemcpyAsync(C->st_host, C->st, sb, hipMemcpyDeviceToHost, s));
