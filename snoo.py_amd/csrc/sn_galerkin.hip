@@ -313,33 +313,69 @@ __global__ __launch_bounds__(256) void pod_project_many_kernel(int n, int r,
             const long long i = base + 256LL * j;
             vv[w][j] = i < n ? v[(long long)w * n + i] : 0.0;
         }
-    for (int kk = 0; kk < kc; kk += 2) {
+    // two modes per trip, the next trip's two modes loaded during this one's
+    // sums
+    float a0[kProjPer], a1[kProjPer];
+    auto load2 = [&](int kk, float *x0, float *x1) {
         const bool two = kk + 1 < kc;
         const float *u0 = U + (size_t)(k0 + kk) * n;
         const float *u1 = U + (size_t)(k0 + kk + (two ? 1 : 0)) * n;
-        float a0[kProjPer], a1[kProjPer];
 #pragma unroll
         for (int j = 0; j < kProjPer; ++j) {
             const long long i = base + 256LL * j;
-            a0[j] = i < n ? u0[i] : 0.0f;
-            a1[j] = i < n ? u1[i] : 0.0f;
+            x0[j] = i < n ? u0[i] : 0.0f;
+            x1[j] = i < n ? u1[i] : 0.0f;
         }
+    };
+    if (kc > 0) load2(0, a0, a1);
+    for (int kk = 0; kk < kc; kk += 2) {
+        const bool two = kk + 1 < kc;
+        float n0[kProjPer], n1[kProjPer];
+        if (kk + 2 < kc) load2(kk + 2, n0, n1);
+        // per mode, the NVEC vectors' sums reduced over the wave together
 #pragma unroll
-        for (int w = 0; w < NVEC; ++w) {
-            double s0 = 0.0, s1 = 0.0;
+        for (int which = 0; which < 2; ++which) {
+            const float *am = which ? a1 : a0;
+            double a[NVEC];
 #pragma unroll
-            for (int j = 0; j < kProjPer; ++j) {
-                const long long i = base + 256LL * j;
-                if (i < n) {
-                    s0 += (double)a0[j] * vv[w][j];
-                    s1 += (double)a1[j] * vv[w][j];
+            for (int w = 0; w < NVEC; ++w) {
+                double sw = 0.0;
+#pragma unroll
+                for (int j = 0; j < kProjPer; ++j) {
+                    const long long i = base + 256LL * j;
+                    if (i < n) sw += (double)am[j] * vv[w][j];
+                }
+                a[w] = sw;
+            }
+            // Butterfly over the wave, lane masks 32, 16, ..., 1 in the order
+            // of wave_sum, so every sum has wave_sum's bits: while more than
+            // one value is left, each lane keeps half of them (by its mask
+            // bit) and adds its partner's partial of that half; then lane l
+            // holds vector l >> (6 - log2 NVEC), summed over the remaining
+            // masks like one value.  NVEC - 1 + 6 - log2 NVEC shuffles
+            // instead of 6 NVEC.
+            int m = 32;
+#pragma unroll
+            for (int c = NVEC; c > 1; c >>= 1, m >>= 1) {
+                const bool hi = (lane & m) != 0;
+#pragma unroll
+                for (int q = 0; q < c / 2; ++q) {
+                    const double keep = hi ? a[q + c / 2] : a[q];
+                    const double send = hi ? a[q] : a[q + c / 2];
+                    a[q] = keep + __shfl_xor(send, m, kWave);
                 }
             }
-            s0 = wave_sum(s0);
-            s1 = wave_sum(s1);
-            if (lane == 0) {
-                ws[wid][kk][w] = s0;
-                if (two) ws[wid][kk + 1][w] = s1;
+            double t = a[0];
+            for (; m > 0; m >>= 1) t += __shfl_xor(t, m, kWave);
+            constexpr int LOGV = NVEC == 1 ? 0 : NVEC == 2 ? 1 : NVEC == 4 ? 2 : 3;
+            if ((lane & ((1 << (6 - LOGV)) - 1)) == 0 && (which == 0 || two))
+                ws[wid][kk + which][lane >> (6 - LOGV)] = t;
+        }
+        if (kk + 2 < kc) {
+#pragma unroll
+            for (int j = 0; j < kProjPer; ++j) {
+                a0[j] = n0[j];
+                a1[j] = n1[j];
             }
         }
     }

@@ -156,6 +156,12 @@ def test_device_forms_of_the_reduced_step_match_the_host_forms(r):
     nat.check(lib.sn_pod_project_many_dev(n, r, 4, nat.ptr(Ud), nat.ptr(V4), nat.ptr(gd),
                                           nat.stream_handle()))
     assert np.array_equal(gd.cpu().numpy(), gh)
+    # each row has the bits of the single-vector projection
+    for w in range(4):
+        g1 = np.empty(r)
+        nat.check(lib.sn_pod_project(n, r, nat.ptr(Ud), nat.ptr(V4[w]),
+                                     g1.ctypes.data_as(ctypes.c_void_p), nat.stream_handle()))
+        assert np.array_equal(g1, gh[w]), w
     # solves: the reduced Jacobian, a random SPD matrix, a singular one
     M = rng.standard_normal((r, r))
     S = np.zeros((r, r))
