@@ -452,7 +452,10 @@ int sn_vcg_set_ams(sn_vcg *vcg, sn_ams *ams, void *stream);
  * round-robin; results identical up to the summation order of the
  * reductions); bit 1 = element products stored by element slot and gathered
  * through the DOF lists (else stored in DOF-list order and read as one run per
- * DOF; bit-identical results).  Default 3. */
+ * DOF; bit-identical results); bit 2 = the preconditioner's down steps form
+ * the fine residual from the pre-smoothed iterate (else they apply each
+ * coarse level's combined restriction P^T (I - w A D^-1); the same operator,
+ * different float rounding).  Default 3. */
 int sn_vcg_set_schedule(sn_vcg *vcg, int flags);
 /* Solve all systems: Mr_dev float32 [6*num_el*nvar], b_dev float64 [n*nvar];
  * each CG starts from 0 and its float32 result is ADDED to x_dev (float64

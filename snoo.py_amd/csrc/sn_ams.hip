@@ -168,6 +168,22 @@ __device__ __forceinline__ float edge_aux(const float4 *__restrict__ yx, int2 en
     return t.x * s.x + t.y * s.y + t.z * s.z;
 }
 
+// R = P^T (I - omega A_f D_f^-1) of a coarse level: entry e (aggregate I,
+// fine row j) = [j in I] - omega dinv_f[j] sum_{i in I} a_ij, the sum over the
+// plan's fine nonzeros (fixed order)
+__global__ void ams_rt_kernel(int n_rt, const int *__restrict__ cols, const int *__restrict__ gptr,
+                              const int *__restrict__ glist, const float *__restrict__ vals_f,
+                              const float *__restrict__ dinv_f, float omega,
+                              float *__restrict__ rt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_rt) return;
+    const int c = cols[e];
+    const int j = c & 0x7fffffff;
+    float a = 0.f;
+    for (int k = gptr[e]; k < gptr[e + 1]; ++k) a += vals_f[glist[k]];
+    rt[e] = (c < 0 ? 1.f : 0.f) - omega * dinv_f[j] * a;
+}
+
 __global__ void ams_gather_kernel(int n_out, const int *__restrict__ gptr,
                                   const int *__restrict__ glist, const float *__restrict__ in,
                                   float *__restrict__ out) {
@@ -767,6 +783,7 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     lv.n = n;
     lv.nnz = nnz;
     lv.indptr_h.assign(indptr, indptr + n + 1);
+    lv.indices_h.assign(indices, indices + nnz);
     std::vector<int> agg_kp;
     if (L > 0) {
         const std::vector<int> &fp = A->lv[L - 1].indptr_h;
@@ -774,6 +791,8 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
         for (int I = 0; I <= n; ++I) agg_kp[I] = fp[agg_ptr[I]];
     }
     int rc;
+    // host sources of the uploads below, alive until the stream is synchronised
+    std::vector<int> rptr, rcols, gptr_h, glist_h;
     if ((rc = upload(&lv.indptr, indptr, (size_t)n + 1, s))) return rc;
     if ((rc = upload(&lv.indices, indices, (size_t)nnz, s))) return rc;
     if ((rc = upload(&lv.diagpos, diag.data(), (size_t)n, s))) return rc;
@@ -782,6 +801,44 @@ int sn_ams_add_level(sn_ams *A, int n, int nnz, const int *indptr, const int *in
     if (L > 0) {
         if ((rc = upload(&lv.agg_ptr, agg_ptr, (size_t)n + 1, s))) return rc;
         if ((rc = upload(&lv.agg_kp, agg_kp.data(), (size_t)n + 1, s))) return rc;
+        // the pattern of R = P^T (I - omega A_f D_f^-1): per aggregate the
+        // sorted distinct fine rows reached by its rows' nonzeros, and for
+        // each entry the fine nonzeros that sum to it, in nonzero order
+        const AmsLevel &f = A->lv[L - 1];
+        rptr.assign((size_t)n + 1, 0);
+        glist_h.resize((size_t)f.nnz);
+        std::vector<int> cnt;
+        gptr_h.push_back(0);
+        for (int I = 0; I < n; ++I) {
+            const int k0 = f.indptr_h[agg_ptr[I]], k1 = f.indptr_h[agg_ptr[I + 1]];
+            std::vector<int> cols(f.indices_h.begin() + k0, f.indices_h.begin() + k1);
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            cnt.assign(cols.size(), 0);
+            for (int k = k0; k < k1; ++k)
+                ++cnt[std::lower_bound(cols.begin(), cols.end(), f.indices_h[k]) - cols.begin()];
+            std::vector<int> pos(cols.size());
+            for (size_t q = 0; q < cols.size(); ++q) {
+                const int j = cols[q];
+                const bool own = j >= agg_ptr[I] && j < agg_ptr[I + 1];
+                rcols.push_back(own ? (int)(j | 0x80000000u) : j);
+                pos[q] = gptr_h.back();
+                gptr_h.push_back(gptr_h.back() + cnt[q]);
+            }
+            for (int k = k0; k < k1; ++k) {
+                const size_t q = std::lower_bound(cols.begin(), cols.end(), f.indices_h[k]) -
+                                 cols.begin();
+                glist_h[pos[q]++] = k;
+            }
+            rptr[I + 1] = (int)rcols.size();
+        }
+        SN_REQUIRE(gptr_h.back() == f.nnz, "sn_ams_add_level: restriction plan does not cover");
+        lv.rt_nnz = (int)rcols.size();
+        if ((rc = upload(&lv.rt_ptr, rptr.data(), rptr.size(), s))) return rc;
+        if ((rc = upload(&lv.rt_cols, rcols.data(), rcols.size(), s))) return rc;
+        if ((rc = upload(&lv.rt_gptr, gptr_h.data(), gptr_h.size(), s))) return rc;
+        if ((rc = upload(&lv.rt_glist, glist_h.data(), glist_h.size(), s))) return rc;
+        SN_HIP(hipMalloc(&lv.rt_vals, (size_t)std::max(1, lv.rt_nnz) * sizeof(float)));
     }
     SN_HIP(hipMalloc(&lv.vals, (size_t)nnz * sizeof(float)));
     SN_HIP(hipMalloc(&lv.dinv, (size_t)n * sizeof(float)));
@@ -817,6 +874,13 @@ int sn_ams_update(sn_ams *A, const double *nodes, const int *cells, const float 
                            v.gptr, v.glist, src, v.vals);
         hipLaunchKernelGGL(ams_dinv_kernel, dim3(grid_for(v.n, 256)), dim3(256), 0, s, v.n,
                            v.diagpos, v.vals, v.dinv);
+    }
+    for (int l = 1; l < L; ++l) {
+        AmsLevel &v = A->lv[l], &f = A->lv[l - 1];
+        if (v.rt_vals)
+            hipLaunchKernelGGL(ams_rt_kernel, dim3(grid_for(v.rt_nnz, 256)), dim3(256), 0, s,
+                               v.rt_nnz, v.rt_cols, v.rt_gptr, v.rt_glist, f.vals, f.dinv,
+                               A->omega, v.rt_vals);
     }
     AmsLevel &c = A->lv[L - 1];
     const size_t lds = ((nc <= kAmsLdsCoarse) ? (size_t)nc * nc + 2 * nc : 2 * (size_t)nc) *
@@ -923,7 +987,8 @@ void sn_ams_destroy(sn_ams *A) {
     if (!A) return;
     for (AmsLevel &v : A->lv) {
         void *bufs[] = {v.indptr, v.indices, v.diagpos, v.gptr, v.glist, v.agg_ptr, v.agg_kp,
-                        v.vals, v.dinv, v.b, v.x};
+                        v.vals, v.dinv, v.b, v.x, v.rt_ptr, v.rt_cols, v.rt_gptr, v.rt_glist,
+                        v.rt_vals};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
     }

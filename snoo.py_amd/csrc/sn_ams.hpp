@@ -17,6 +17,17 @@ struct AmsLevel {
     int *agg_ptr = nullptr; // [n+1] rows of level-1 forming each row here (null at level 0)
     int *agg_kp = nullptr;  // [n+1] level-1 indptr at agg_ptr: each aggregate's nonzeros
     std::vector<int> indptr_h;  // host copy of indptr (forms the next level's agg_kp)
+    std::vector<int> indices_h; // host copy of indices (forms the next level's rt pattern)
+    // coarse levels: the restriction of the previous level's pre-smoothed
+    // residual, R = P^T (I - omega A_f D_f^-1), one row per aggregate, so that
+    // b_c = R b_f (the columns: rows of the previous level reached from the
+    // aggregate; sign bit set for the aggregate's own rows, where P^T adds 1)
+    int rt_nnz = 0;
+    int *rt_ptr = nullptr;   // [n+1]
+    int *rt_cols = nullptr;  // [rt_nnz]
+    int *rt_gptr = nullptr;  // [rt_nnz+1] plan over the previous level's nonzeros
+    int *rt_glist = nullptr; // [previous nnz]
+    float *rt_vals = nullptr;// [rt_nnz]
     float *vals = nullptr;  // [nnz]
     float *dinv = nullptr;  // [n]
     float4 *b = nullptr;    // right-hand side of the cycle at this level

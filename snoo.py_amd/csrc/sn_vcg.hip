@@ -667,6 +667,54 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_down_kernel(
     }
 }
 
+// The same step through the combined restriction R = P^T (I - w A_f D_f^-1)
+// of the coarse level: b_c = R b_f, a gather of b_f per entry of R (its
+// distinct fine rows, about half the fine nonzeros of the aggregate on level
+// 0), instead of b summed over the aggregate's rows minus A_f applied to the
+// pre-smoothed x_f = w D_f^-1 b_f; x_c = w D_c^-1 b_c unless the next level
+// is the coarsest.  The values differ from that step in float rounding only.
+template <int NV, int SL, bool TWO>
+__global__ __launch_bounds__(kVAmsBlock) void vams_down_rt_kernel(
+    int n_agg, const int *__restrict__ rt_ptr, const int *__restrict__ rt_cols,
+    const float *__restrict__ rt_vals, const float *__restrict__ b,
+    const float *__restrict__ dinv_c, float omega, float *__restrict__ bc,
+    float *__restrict__ xc, const VcgState *__restrict__ st, int xcd) {
+    if (all_done<NV>(st)) return;
+    const long long ag = (v_block(xcd) * blockDim.x + threadIdx.x) / SL;
+    const int slot = threadIdx.x % SL;
+    const bool in = ag < n_agg;
+    const int k0 = in ? rt_ptr[ag] : 0, k1 = in ? rt_ptr[ag + 1] : 0;
+    float s[3][NV];
+    zero3<NV>(s);
+    for (int k = k0 + slot; k < k1; k += (TWO ? 2 : 1) * SL) {
+        const bool h2 = TWO && k + SL < k1;
+        const int k2 = h2 ? k + SL : k;
+        const float a = rt_vals[k];
+        float ba[3][NV];
+        ld3<NV>(b + (long long)(rt_cols[k] & 0x7fffffff) * 3 * NV, ba);
+        if (TWO) {
+            const float a2 = h2 ? rt_vals[k2] : 0.f;
+            float bb[3][NV];
+            ld3<NV>(b + (long long)(rt_cols[k2] & 0x7fffffff) * 3 * NV, bb);
+            fma3<NV>(s, a, ba);
+            fma3<NV>(s, a2, bb);
+        } else {
+            fma3<NV>(s, a, ba);
+        }
+    }
+    reduce3<NV, SL>(s);
+    if (!in || slot != 0) return;
+    st3<NV>(bc + ag * 3 * NV, s);
+    if (xc) {
+        const float d = omega * dinv_c[ag];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s[c][v] *= d;
+        st3<NV>(xc + ag * 3 * NV, s);
+    }
+}
+
 // wave per coarsest row I: e_I = (C^-1 b)_I, prolongated into the finer
 // level's x (x_f[i] += e_I for the rows i of aggregate I), or stored as the
 // level-0 result y when there is one level
@@ -880,6 +928,9 @@ struct sn_vcg {
     // instead of by element slot: the DOF pass took 128 -> 119 us at 1M tets x
     // 8 variants, the element pass's scattered stores 103 -> 161 us, so off
     bool dm = false;
+    // the down steps through the coarse levels' combined restrictions R
+    // (default; off: the fine residual formed from the pre-smoothed x)
+    bool rt = true;
     float *yloc = nullptr;             // [6*n_el*nv]
     float *r = nullptr, *dinv = nullptr, *pout = nullptr, *xs = nullptr;   // [n*nv]
     float2 *w0 = nullptr, *w1 = nullptr;                                  // [n*nv] {z, p}
@@ -945,7 +996,18 @@ static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const f
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
         float *xc = (l + 2 < L) ? C->vx[l + 1] : (float *)nullptr;
-        if (c.n <= kVSmallLevel)
+        if (C->rt && c.rt_vals) {
+            if (c.n <= kVSmallLevel)
+                hipLaunchKernelGGL((vams_down_rt_kernel<NV, kWave, true>),
+                                   dim3(vams_grid((long long)c.n * kWave, kVAmsBlock, xcd)),
+                                   dim3(kVAmsBlock), 0, s, c.n, c.rt_ptr, c.rt_cols, c.rt_vals,
+                                   C->vb[l], c.dinv, A->omega, C->vb[l + 1], xc, C->st, xcd);
+            else
+                hipLaunchKernelGGL((vams_down_rt_kernel<NV, 16, false>),
+                                   dim3(vams_grid((long long)c.n * 16, kVAmsBlock, xcd)),
+                                   dim3(kVAmsBlock), 0, s, c.n, c.rt_ptr, c.rt_cols, c.rt_vals,
+                                   C->vb[l], c.dinv, A->omega, C->vb[l + 1], xc, C->st, xcd);
+        } else if (c.n <= kVSmallLevel)
             hipLaunchKernelGGL((vams_down_kernel<NV, kWave, true>),
                                dim3(vams_grid((long long)c.n * kWave, kVAmsBlock, xcd)),
                                dim3(kVAmsBlock), 0, s, c.n, c.agg_ptr, c.agg_kp, f.indices,
@@ -1146,10 +1208,11 @@ int sn_vcg_set_ams(sn_vcg *C, sn_ams *A, void *stream) {
 }
 
 int sn_vcg_set_schedule(sn_vcg *C, int flags) {
-    SN_REQUIRE(C && flags >= 0 && flags <= 3, "sn_vcg_set_schedule: flags in [0, 3]");
+    SN_REQUIRE(C && flags >= 0 && flags <= 7, "sn_vcg_set_schedule: flags in [0, 7]");
     SN_HIP(vcg_drop_graphs(C));
     C->xcd = (flags & 1) ? 1 : 0;
     C->dm = (flags & 2) == 0;
+    C->rt = (flags & 4) == 0;
     return SN_OK;
 }
 

@@ -111,7 +111,8 @@ def test_variants_match_their_own_single_cg():
 def test_product_layouts_and_schedules_agree():
     """Element products in DOF-list order or by element slot: the same values
     summed in the same order, so bit-identical iterates; round-robin blocks and
-    other grid sizes change only the reduction order."""
+    other grid sizes change only the reduction order, the preconditioner's
+    down steps without the combined restrictions only float rounding."""
     import torch
     from snoopy_amd import _native as nat
     from snoopy_amd.magnets import Dipole
@@ -127,6 +128,11 @@ def test_product_layouts_and_schedules_agree():
         for flags in (1, 3, 2):
             nat.check(nat.load().sn_vcg_set_schedule(h, flags))
             out[flags] = _solve(h, Mr, d0.mask, b, [1e-3] * 4, 4)
+        # down steps through the fine residual of the pre-smoothed iterate
+        # instead of the combined restrictions: the same preconditioner up to
+        # float rounding
+        nat.check(nat.load().sn_vcg_set_schedule(h, 7))
+        out['no rt'] = _solve(h, Mr, d0.mask, b, [1e-3] * 4, 4)
         # other launch shapes: only the reductions' partial sums change
         nat.check(nat.load().sn_vcg_set_schedule(h, 3))
         nat.check(nat.load().sn_vcg_set_grids(h, 24, 2, nat.stream_handle()))
@@ -134,7 +140,7 @@ def test_product_layouts_and_schedules_agree():
     finally:
         nat.load().sn_vcg_destroy(h)
     assert torch.equal(out[1][0], out[3][0]) and out[1][1] == out[3][1]
-    for k in (2, 'grids'):
+    for k in (2, 'grids', 'no rt'):
         assert all(abs(a - c) <= 2 for a, c in zip(out[1][1], out[k][1])), (k, out[1][1], out[k][1])
         err = float((out[1][0] - out[k][0]).norm() / out[1][0].norm())
         assert err < 2e-3, (k, err)
