@@ -96,6 +96,7 @@ SIGNATURES = {
     'sn_cg_element_operator': (_c_int, [_vp]),
     'sn_cg_set_grids': (_c_int, [_vp, _c_int, _c_int, _vp]),
     'sn_ams_set_lanes': (_c_int, [_vp, _c_int]),
+    'sn_ams_set_restriction': (_c_int, [_vp, _c_int]),
     'sn_cg_grids': (_c_int, [_vp, _vp, _vp]),
     'sn_vcg_create': (_c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _c_int, _c_int, _vp]),
     'sn_vcg_set_ams': (_c_int, [_vp, _vp, _vp]),

@@ -362,6 +362,40 @@ __global__ __launch_bounds__(kAmsBlock) void ams_down_kernel(
     }
 }
 
+// The same step through the coarse level's combined restriction
+// R = P^T (I - omega A_f D_f^-1): b_c = R b_f, one gather of b_f per distinct
+// fine row reached from the aggregate (on level 0 about half the aggregate's
+// fine nonzeros), the same operator up to float rounding
+__global__ __launch_bounds__(kAmsBlock) void ams_down_rt_kernel(
+    int n_agg, const int *__restrict__ rt_ptr, const int *__restrict__ rt_cols,
+    const float *__restrict__ rt_vals, const float4 *__restrict__ b,
+    const float *__restrict__ dinv_c, float omega, float4 *__restrict__ bc,
+    float4 *__restrict__ xc, const int *__restrict__ done, int xcd) {
+    const int wid =
+        (int)((ams_block(xcd, blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (wid >= n_agg) return;  // uniform over the wave
+    const int k0 = rt_ptr[wid], k1 = rt_ptr[wid + 1];
+    if (done && *done) return;
+    float4 s = f4_zero();
+    for (int k = k0 + lane; k < k1; k += 2 * kWave) {
+        const bool h2 = k + kWave < k1;
+        const int k2 = h2 ? k + kWave : k;
+        const float v = rt_vals[k], v2 = rt_vals[k2];
+        const float4 b1 = b[rt_cols[k] & 0x7fffffff], b2 = b[rt_cols[k2] & 0x7fffffff];
+        f4_fma(s, v, b1);
+        if (h2) f4_fma(s, v2, b2);
+    }
+    f4_reduce<kWave>(s);
+    if (lane == 0) {
+        bc[wid] = s;
+        if (xc) {
+            const float d = omega * dinv_c[wid];
+            xc[wid] = make_float4(d * s.x, d * s.y, d * s.z, 0.f);
+        }
+    }
+}
+
 // wave per row I of the coarsest level: e_I = (C^-1 b)_I, then prolongation
 // into the finer level's pre-smoothed iterate, x_f[i] += e_I for the rows i
 // of aggregate I (or, with a single level, y = e, stored as y[2I] of the
@@ -596,10 +630,16 @@ static void launch_apply_t(sn_ams *A, const V *r, const V *dinv, const unsigned 
     const bool tail = L >= 2 && lc.n <= kAmsLdsCoarse && A->lv[L - 2].n <= kAmsTailRows;
     for (int l = 0; l + 1 < L; ++l) {
         AmsLevel &f = A->lv[l], &c = A->lv[l + 1];
-        hipLaunchKernelGGL(ams_down_kernel, dim3(ams_grid(c.n, waves_per_block, xcd)),
-                           dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, c.agg_kp, f.indices, f.vals,
-                           f.b, f.x, c.dinv, A->omega, c.b,
-                           (l + 2 < L) ? c.x : (float4 *)nullptr, done, xcd);
+        if (A->rt && c.rt_vals)
+            hipLaunchKernelGGL(ams_down_rt_kernel, dim3(ams_grid(c.n, waves_per_block, xcd)),
+                               dim3(kAmsBlock), 0, s, c.n, c.rt_ptr, c.rt_cols, c.rt_vals, f.b,
+                               c.dinv, A->omega, c.b, (l + 2 < L) ? c.x : (float4 *)nullptr,
+                               done, xcd);
+        else
+            hipLaunchKernelGGL(ams_down_kernel, dim3(ams_grid(c.n, waves_per_block, xcd)),
+                               dim3(kAmsBlock), 0, s, c.n, c.agg_ptr, c.agg_kp, f.indices,
+                               f.vals, f.b, f.x, c.dinv, A->omega, c.b,
+                               (l + 2 < L) ? c.x : (float4 *)nullptr, done, xcd);
     }
     if (!tail)
         hipLaunchKernelGGL(ams_coarse_kernel, dim3(grid_for(lc.n, waves_per_block)),
@@ -916,6 +956,12 @@ int sn_ams_level_values(const sn_ams *A, int level, float *vals, void *stream) {
 }
 
 int sn_ams_num_levels(const sn_ams *A) { return A ? (int)A->lv.size() : 0; }
+
+int sn_ams_set_restriction(sn_ams *A, int combined) {
+    SN_REQUIRE(A, "sn_ams_set_restriction: null handle");
+    A->rt = combined != 0;
+    return SN_OK;
+}
 
 int sn_ams_set_lanes(sn_ams *A, int lanes) {
     SN_REQUIRE(A, "sn_ams_set_lanes: null handle");

@@ -55,6 +55,7 @@ struct sn_ams {
     float weight = 1.0f;         // weight of the auxiliary correction
     bool ready = false;          // numeric values present
     int lanes = 8;               // lanes per row of the restriction, up and tail kernels
+    bool rt = true;              // down steps through the combined restrictions R
 };
 
 namespace sn {
