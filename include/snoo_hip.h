@@ -382,12 +382,14 @@ int sn_ams_num_levels(const sn_ams *ams);
  * change, so set the lanes before the first solve, or call sn_cg_set_grids
  * (which drops the graphs) afterwards. */
 int sn_ams_set_lanes(sn_ams *ams, int lanes);
-/* Down steps of the apply (combined != 0, the default) through each coarse
- * level's combined restriction P^T (I - w A D^-1), a gather per distinct fine
- * row, or (0) through the fine residual of the pre-smoothed iterate: the same
- * operator up to float rounding.  Like the lanes, set it before the solves
- * whose graphs capture the apply, or call sn_cg_set_ams again (which drops
- * them). */
+/* Down steps of the single-system apply through each coarse level's
+ * combined restriction P^T (I - w A D^-1), a gather per distinct fine row
+ * (combined != 0), or through the fine residual of the pre-smoothed iterate
+ * (0, the default: one system's step is bound by its dependent loads, not
+ * their number): the same operator up to float rounding.  The batched CG has
+ * its own switch (sn_vcg_set_schedule bit 2) and uses them by default.  Like
+ * the lanes, set it before the solves whose graphs capture the apply, or
+ * call sn_cg_set_ams again (which drops them). */
 int sn_ams_set_restriction(sn_ams *ams, int combined);
 /* Host set-up helper: greedy aggregation of the strength graph of a CSR
  * matrix (|a_ij| >= theta sqrt|a_ii a_jj|); agg_host [n] receives aggregate

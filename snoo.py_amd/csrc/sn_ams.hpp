@@ -55,7 +55,10 @@ struct sn_ams {
     float weight = 1.0f;         // weight of the auxiliary correction
     bool ready = false;          // numeric values present
     int lanes = 8;               // lanes per row of the restriction, up and tail kernels
-    bool rt = true;              // down steps through the combined restrictions R
+    // down steps of the single-system apply through the combined restrictions
+    // R: off by default (one variant's level-0 step is latency-bound, 9.4 us
+    // with R against 9.7 us without, at 1M tets); the batched CG uses R
+    bool rt = false;
 };
 
 namespace sn {

@@ -56,8 +56,8 @@ def test_level_matrices_match_oracle(case):
 @pytest.mark.parametrize('combined', [1, 0])
 def test_apply_matches_oracle(case, combined):
     """One application against the oracle, with the down steps through the
-    coarse levels' combined restrictions P^T (I - w A D^-1) (the default) and
-    through the fine residual of the pre-smoothed iterate."""
+    coarse levels' combined restrictions P^T (I - w A D^-1) and through the
+    fine residual of the pre-smoothed iterate (the default)."""
     import torch
     from snoopy_amd import _native as nat
     d, solver, drv = case
@@ -83,7 +83,7 @@ def test_apply_matches_oracle(case, combined):
     print('apply max rel diff %.2e' % err, flush=True)
     assert err < 1e-6
     assert np.all(z[~free] == 0.0)
-    nat.check(nat.load().sn_ams_set_restriction(drv.ams.handle, 1))
+    nat.check(nat.load().sn_ams_set_restriction(drv.ams.handle, 0))
 
 
 def _spmv(solver, vals, mask, x):
