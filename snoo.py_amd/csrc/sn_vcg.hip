@@ -144,36 +144,20 @@ __device__ __forceinline__ void ldv_d(const double *__restrict__ p, double (&o)[
         o[0] = p[0];
     }
 }
-// the V pairs {z_v, p_v} of one index (8V contiguous bytes)
+// the record of index i in a pair array: z of the V variants, then p (2V
+// contiguous floats); each half is one contiguous run of 4V bytes, so a kernel
+// that writes only z or only p stores whole 16-byte pieces
 template <int NV>
-__device__ __forceinline__ void ld_pairs(const float2 *__restrict__ w, float (&z)[NV],
+__device__ __forceinline__ void ld_pairs(const float *w, long long i, float (&z)[NV],
                                          float (&p)[NV]) {
-    if constexpr (NV % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < NV / 2; ++q) {
-            const float4 t = reinterpret_cast<const float4 *>(w)[q];
-            z[2 * q] = t.x;
-            p[2 * q] = t.y;
-            z[2 * q + 1] = t.z;
-            p[2 * q + 1] = t.w;
-        }
-    } else {
-        const float2 t = w[0];
-        z[0] = t.x;
-        p[0] = t.y;
-    }
+    ldv<NV>(w + i * 2 * NV, z);
+    ldv<NV>(w + i * 2 * NV + NV, p);
 }
 template <int NV>
-__device__ __forceinline__ void st_pairs(float2 *__restrict__ w, const float (&z)[NV],
+__device__ __forceinline__ void st_pairs(float *w, long long i, const float (&z)[NV],
                                          const float (&p)[NV]) {
-    if constexpr (NV % 2 == 0) {
-#pragma unroll
-        for (int q = 0; q < NV / 2; ++q)
-            reinterpret_cast<float4 *>(w)[q] =
-                make_float4(z[2 * q], p[2 * q], z[2 * q + 1], p[2 * q + 1]);
-    } else {
-        w[0] = make_float2(z[0], p[0]);
-    }
+    stv<NV>(w + i * 2 * NV, z);
+    stv<NV>(w + i * 2 * NV + NV, p);
 }
 
 // Per-variant block sums of the threads' values vals[v]: the V sums in
@@ -229,8 +213,8 @@ template <int NV>
 __global__ __launch_bounds__(kVVecBlock) void vcg_init_kernel(
     int n, int n_el, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const float *__restrict__ Mr, const unsigned char *__restrict__ mask,
-    const double *__restrict__ b, float *__restrict__ r, float2 *__restrict__ w,
-    float2 *__restrict__ w_other, float *__restrict__ dinv, float *__restrict__ xs,
+    const double *__restrict__ b, float *__restrict__ r, float *__restrict__ w,
+    float *__restrict__ w_other, float *__restrict__ dinv, float *__restrict__ xs,
     double *__restrict__ partB, double *__restrict__ partbb) {
     __shared__ double lds[NV * kVVecBlock / kWave];
     double bb[NV], rr[NV], rz[NV];
@@ -285,8 +269,8 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_init_kernel(
         stv<NV>(r + i * NV, ri);
         stv<NV>(dinv + i * NV, di);
         stv<NV>(xs + i * NV, zero);
-        st_pairs<NV>(w + i * NV, zi, zero);
-        st_pairs<NV>(w_other + i * NV, zero, zero);
+        st_pairs<NV>(w, i, zi, zero);
+        st_pairs<NV>(w_other, i, zero, zero);
     }
     block_sums<NV>(bb, lds);
     if (threadIdx.x < NV) partbb[(long long)blockIdx.x * NV + threadIdx.x] = lds[threadIdx.x];
@@ -331,7 +315,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_init_state_kernel(
 template <int NV, bool DM>
 __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
     int n_el, const int *__restrict__ el_dofs, const int *__restrict__ ypos,
-    const float *__restrict__ Mr, const float2 *__restrict__ zp, float *__restrict__ yloc,
+    const float *__restrict__ Mr, const float *__restrict__ zp, float *__restrict__ yloc,
     const double *__restrict__ partB, int nbB, double *__restrict__ partE,
     VcgState *__restrict__ st, int parity, int xcd) {
     __shared__ double lds[2 * NV * kVElBlock / kWave];
@@ -372,8 +356,8 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
             float xh[6];
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
-                const float2 w = zp[(long long)(d[j] & 0x7fffffff) * NV + v];
-                const float pj = w.x + beta * w.y;
+                const float *w = zp + (long long)(d[j] & 0x7fffffff) * 2 * NV;
+                const float pj = w[v] + beta * w[NV + v];
                 xh[j] = d[j] < 0 ? -pj : pj;
             }
             const float u0 = xh[1] - xh[2] + xh[5];
@@ -397,8 +381,9 @@ __global__ __launch_bounds__(kVElBlock) void vcg_elem_kernel(
 // DOF pass (thread per DOF, all variants): every variant's alpha from the
 // element pass's partials; q_i = fixed-order sum of the element products;
 // x += alpha p, r -= alpha q, r.r partials; JAC: z = dinv r and the {z, p}
-// pair, r.z partials; otherwise p to a plain array (the preconditioner writes
-// the pair).  A variant that is done keeps x, r as they are.  DM: the DOF's
+// record of wnew, r.z partials; otherwise p into the p half of the record it
+// has read from zp (the preconditioner then writes the z half).  A variant
+// that is done keeps x, r as they are.  DM: the DOF's
 // products are the run [dof_ptr[i], dof_ptr[i+1]) of yloc (vcg_elem_kernel
 // with DM), else gathered through the DOF lists; the same values are summed
 // in the same order either way.
@@ -406,8 +391,8 @@ template <int NV, bool JAC, bool DM>
 __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
     int n, const int *__restrict__ dof_ptr, const int *__restrict__ dof_contrib,
     const float *__restrict__ yloc, const unsigned char *__restrict__ mask,
-    const float *__restrict__ dinv, const float2 *__restrict__ zp, float2 *__restrict__ wnew,
-    float *__restrict__ pout, float *__restrict__ r, float *__restrict__ x,
+    const float *__restrict__ dinv, float *__restrict__ zp, float *__restrict__ wnew,
+    float *__restrict__ r, float *__restrict__ x,
     const double *__restrict__ partE, int nbE, double *__restrict__ partB,
     VcgState *__restrict__ st, int parity, int xcd) {
     __shared__ double lds[NV * kVVecBlock / kWave];
@@ -446,7 +431,7 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             for (int m = 0; m < kHead; ++m)
                 c[m] = (fr && k0 + m < k1) ? (DM ? k0 + m : dof_contrib[k0 + m]) : -1;
             float zz[NV], pp[NV];
-            ld_pairs<NV>(zp + i * NV, zz, pp);
+            ld_pairs<NV>(zp, i, zz, pp);
             float q[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) q[v] = 0.f;
@@ -466,8 +451,8 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
 #pragma unroll
             for (int v = 0; v < NV; ++v) zero[v] = 0.f;
             if (!fr) {
-                if (JAC) st_pairs<NV>(wnew + i * NV, zero, zero);
-                else stv<NV>(pout + i * NV, zero);
+                if (JAC) st_pairs<NV>(wnew, i, zero, zero);
+                else stv<NV>(zp + i * 2 * NV + NV, zero);
                 continue;
             }
             for (int k = k0 + kHead; k < k1; ++k) {
@@ -496,8 +481,8 @@ __global__ __launch_bounds__(kVVecBlock) void vcg_edge_kernel(
             }
             stv<NV>(x + i * NV, xi);
             stv<NV>(r + i * NV, ri);
-            if (JAC) st_pairs<NV>(wnew + i * NV, zi, pi);
-            else stv<NV>(pout + i * NV, pi);
+            if (JAC) st_pairs<NV>(wnew, i, zi, pi);
+            else stv<NV>(zp + i * 2 * NV + NV, pi);
         }
     }
     if (JAC) {
@@ -804,14 +789,15 @@ __global__ __launch_bounds__(kVAmsBlock) void vams_up_kernel(
 }
 
 // thread per edge, all variants: z = D_J^-1 r + w Pi y0 on the free edges (0
-// elsewhere), stored with the CG's p as the pairs {z, p} (pin) or as z alone
-// (the start: p stays 0); per-block partials of r.z
+// elsewhere), stored in the z half of the edge's record in w (the p half holds
+// the CG's p, written by the DOF pass, or 0 at the start); per-block partials
+// of r.z
 template <int NV>
 __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
     int n_edges, const unsigned char *__restrict__ mask, const float *__restrict__ r,
     const float *__restrict__ dinv, const int2 *__restrict__ edge_nodes,
     const float4 *__restrict__ xyz, const float *__restrict__ y, float weight,
-    float2 *__restrict__ w, const float *__restrict__ pin, double *__restrict__ partB,
+    float *__restrict__ w, double *__restrict__ partB,
     const VcgState *__restrict__ st, int xcd) {
     __shared__ double lds[NV * kVVecBlock / kWave];
     double rz[NV];
@@ -842,14 +828,7 @@ __global__ __launch_bounds__(kVVecBlock) void vams_prolong_kernel(
 #pragma unroll
                 for (int v = 0; v < NV; ++v) zi[v] = 0.f;
             }
-            if (pin) {
-                float pi[NV];
-                ldv<NV>(pin + i * NV, pi);
-                st_pairs<NV>(w + i * NV, zi, pi);
-            } else {
-#pragma unroll
-                for (int v = 0; v < NV; ++v) w[i * NV + v].x = zi[v];
-            }
+            stv<NV>(w + i * 2 * NV, zi);
         }
     }
     block_sums<NV>(rz, lds);
@@ -866,13 +845,17 @@ __global__ void vcg_slot_pos_kernel(int n_slots, const int *__restrict__ dof_con
 }
 
 // timing state: partials that reduce to rho = ||r||^2 = 1 for every variant,
-// z = 1 on the free rows and p = 0 in the pairs read by parity 1
+// z = 1 on the free rows and p = 0 in the records read by parity 1
 __global__ void vcg_fill_timing_kernel(long long n_nv, int nv, int nb,
                                        const unsigned char *__restrict__ mask,
-                                       double *__restrict__ partB, float2 *__restrict__ w1) {
+                                       double *__restrict__ partB, float *__restrict__ w1) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < 2LL * nb * nv) partB[k] = (k < 2LL * nv) ? 1.0 : 0.0;
-    if (k < n_nv) w1[k] = make_float2((mask && !mask[k / nv]) ? 0.f : 1.f, 0.f);
+    if (k < n_nv) {
+        const long long i = k / nv, v = k - i * nv;
+        w1[i * 2 * nv + v] = (mask && !mask[i]) ? 0.f : 1.f;
+        w1[i * 2 * nv + nv + v] = 0.f;
+    }
 }
 
 // Variant-minor operands from per-variant arrays: Mr[(j*N + e)*V + v] =
@@ -932,8 +915,12 @@ struct sn_vcg {
     // (default; off: the fine residual formed from the pre-smoothed x)
     bool rt = true;
     float *yloc = nullptr;             // [6*n_el*nv]
-    float *r = nullptr, *dinv = nullptr, *pout = nullptr, *xs = nullptr;   // [n*nv]
-    float2 *w0 = nullptr, *w1 = nullptr;                                  // [n*nv] {z, p}
+    float *r = nullptr, *dinv = nullptr, *xs = nullptr;   // [n*nv]
+    // [n][2*nv] records {z of the variants, p of the variants}; with the
+    // preconditioner every iteration reads and rewrites w0 in place (the DOF
+    // pass its p half, the preconditioner its z half), without it (Jacobi)
+    // the iterations alternate between w0 and w1
+    float *w0 = nullptr, *w1 = nullptr;
     double *partE = nullptr, *partB = nullptr, *partbb = nullptr, *rtol_dev = nullptr;
     sn::VcgState *st = nullptr, *st_host = nullptr;
     sn_ams *ams = nullptr;
@@ -983,8 +970,7 @@ inline int vams_grid(long long threads, int block, int xcd) {
 constexpr int kVSmallLevel = 8192;
 
 template <int NV, int LPR>
-static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const float *pin,
-                        hipStream_t s) {
+static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float *w, hipStream_t s) {
     sn_ams *A = C->ams;
     const int L = (int)A->lv.size();
     const int xcd = C->xcd;
@@ -1040,37 +1026,37 @@ static void vcg_apply_t(sn_vcg *C, const unsigned char *mask, float2 *w, const f
     }
     hipLaunchKernelGGL((vams_prolong_kernel<NV>), dim3(C->grid_vec), dim3(kVVecBlock), 0, s,
                        A->n_edges, mask, C->r, C->dinv, A->edge_nodes, A->xyz, C->vy, A->weight,
-                       w, pin, C->partB, C->st, xcd);
+                       w, C->partB, C->st, xcd);
 }
 
 template <int NV>
-static void vcg_apply(sn_vcg *C, const unsigned char *mask, float2 *w, const float *pin,
-                      hipStream_t s) {
-    if (C->ams_lanes >= 16) vcg_apply_t<NV, 16>(C, mask, w, pin, s);
-    else if (C->ams_lanes >= 8) vcg_apply_t<NV, 8>(C, mask, w, pin, s);
-    else vcg_apply_t<NV, 4>(C, mask, w, pin, s);
+static void vcg_apply(sn_vcg *C, const unsigned char *mask, float *w, hipStream_t s) {
+    if (C->ams_lanes >= 16) vcg_apply_t<NV, 16>(C, mask, w, s);
+    else if (C->ams_lanes >= 8) vcg_apply_t<NV, 8>(C, mask, w, s);
+    else vcg_apply_t<NV, 4>(C, mask, w, s);
 }
 
 template <int NV, bool DM>
 static void vcg_iteration_t(sn_vcg *C, const float *Mr, const unsigned char *mask, int parity,
                             hipStream_t s) {
     const sn_pattern *P = C->pat;
-    const float2 *wold = parity ? C->w1 : C->w0;
-    float2 *wnew = parity ? C->w0 : C->w1;
+    // with the preconditioner the records of w0 are rewritten in place
+    float *wold = (parity && !C->ams) ? C->w1 : C->w0;
+    float *wnew = (parity || C->ams) ? C->w0 : C->w1;
     hipLaunchKernelGGL((vcg_elem_kernel<NV, DM>), dim3(C->grid_el), dim3(kVElBlock), 0, s,
                        C->n_el, C->el_dofs, C->ypos, Mr, wold, C->yloc, C->partB, C->grid_vec,
                        C->partE, C->st, parity, C->xcd);
     if (C->ams) {
         hipLaunchKernelGGL((vcg_edge_kernel<NV, false, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
                            0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
-                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
-                           parity, C->xcd);
-        vcg_apply<NV>(C, mask, wnew, C->pout, s);
+                           nullptr, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st, parity,
+                           C->xcd);
+        vcg_apply<NV>(C, mask, wnew, s);
     } else {
         hipLaunchKernelGGL((vcg_edge_kernel<NV, true, DM>), dim3(C->grid_vec), dim3(kVVecBlock),
                            0, s, P->n, P->dof_ptr, P->dof_contrib, C->yloc, mask, C->dinv, wold,
-                           wnew, C->pout, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st,
-                           parity, C->xcd);
+                           wnew, C->r, C->xs, C->partE, C->grid_el, C->partB, C->st, parity,
+                           C->xcd);
     }
 }
 
@@ -1090,7 +1076,7 @@ static void vcg_init(sn_vcg *C, const float *Mr, const unsigned char *mask, cons
                        C->dinv, C->xs, C->partB, C->partbb);
     hipLaunchKernelGGL((vcg_init_state_kernel<NV>), dim3(1), dim3(kVVecBlock), 0, s, C->partbb,
                        C->grid_vec, C->st, C->rtol_dev, atol, maxiter);
-    if (C->ams) vcg_apply<NV>(C, mask, C->w0, nullptr, s);
+    if (C->ams) vcg_apply<NV>(C, mask, C->w0, s);
 }
 
 template <int NV>
@@ -1154,10 +1140,9 @@ int sn_vcg_create(sn_vcg **out, const sn_pattern *P, const int *gids, const int 
     SN_HIP(hipMalloc(&C->yloc, (size_t)6 * C->n_el * nvar * sizeof(float)));
     SN_HIP(hipMalloc(&C->r, nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->dinv, nn * sizeof(float)));
-    SN_HIP(hipMalloc(&C->pout, nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->xs, nn * sizeof(float)));
-    SN_HIP(hipMalloc(&C->w0, nn * sizeof(float2)));
-    SN_HIP(hipMalloc(&C->w1, nn * sizeof(float2)));
+    SN_HIP(hipMalloc(&C->w0, 2 * nn * sizeof(float)));
+    SN_HIP(hipMalloc(&C->w1, 2 * nn * sizeof(float)));
     SN_HIP(hipMalloc(&C->partE, (size_t)C->grid_el * nvar * sizeof(double)));
     SN_HIP(hipMalloc(&C->partB, (size_t)2 * C->grid_vec * nvar * sizeof(double)));
     SN_HIP(hipMalloc(&C->partbb, (size_t)C->grid_vec * nvar * sizeof(double)));
@@ -1372,8 +1357,9 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
     SN_HIP(hipEventCreate(&e0));
     SN_HIP(hipEventCreate(&e1));
     SN_HIP(hipEventRecord(e0, s));
-    // parity 1 reads the pairs of w1 (p zero) and rho[0], writes rho[1] and p
-    // elsewhere: every repetition sees the same state (x and r move)
+    // parity 1 reads the records of w1 (z = 1, p = 0 at first) and rho[0],
+    // writes rho[1] and p in place: beta = 1, so p grows by z per repetition
+    // (values only; the kernels' work is the same in every repetition)
     const sn_pattern *P = C->pat;
     for (int r = 0; r < reps; ++r) {
 #define SN_VCG_PAIR(NV)                                                                        \
@@ -1384,7 +1370,7 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
                                C->grid_vec, C->partE, C->st, 1, C->xcd);                       \
             hipLaunchKernelGGL((vcg_edge_kernel<NV, false, true>), dim3(C->grid_vec),          \
                                dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
-                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
+                               C->yloc, mask, C->dinv, C->w1, nullptr, C->r, C->xs,            \
                                C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
         } else {                                                                               \
             hipLaunchKernelGGL((vcg_elem_kernel<NV, false>), dim3(C->grid_el), dim3(kVElBlock),  \
@@ -1392,7 +1378,7 @@ int sn_vcg_time_pair(sn_vcg *C, const float *Mr, const unsigned char *mask, int 
                                C->partB, C->grid_vec, C->partE, C->st, 1, C->xcd);             \
             hipLaunchKernelGGL((vcg_edge_kernel<NV, false, false>), dim3(C->grid_vec),         \
                                dim3(kVVecBlock), 0, s, P->n, P->dof_ptr, P->dof_contrib,       \
-                               C->yloc, mask, C->dinv, C->w1, C->w0, C->pout, C->r, C->xs,     \
+                               C->yloc, mask, C->dinv, C->w1, nullptr, C->r, C->xs,            \
                                C->partE, C->grid_el, C->partB, C->st, 1, C->xcd);              \
         }                                                                                      \
     } while (0)
@@ -1455,7 +1441,7 @@ void sn_vcg_destroy(sn_vcg *C) {
     if (!C) return;
     (void)vcg_drop_graphs(C);
     vcg_free_ams_vectors(C);
-    void *bufs[] = {C->el_dofs, C->ypos, C->yloc, C->r, C->dinv, C->pout, C->xs, C->w0, C->w1,
+    void *bufs[] = {C->el_dofs, C->ypos, C->yloc, C->r, C->dinv, C->xs, C->w0, C->w1,
                     C->partE, C->partB, C->partbb, C->rtol_dev, C->st};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
